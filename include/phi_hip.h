@@ -1,0 +1,155 @@
+/*
+ * phi_hip.h - C ABI of the MI355X HEOM integrator (drop-in for PHI's hot path).
+ *
+ * Every entry point returns 0 on success or a negative status (PHI_HIP_E*),
+ * never throws, takes plain pointers and sizes, and is single-threaded per
+ * handle.  Complex arrays are interleaved doubles (re, im); matrices are
+ * column-major (element (i, j) at index i + j*N), as in PHI.  ADO state
+ * arrays hold every node in PHI's id order (hierarchy.cc:45-155), N*N complex
+ * per node.
+ *
+ * Reference interfaces replaced (file:line in strj/phi):
+ *   phi_hip_main                   main / PhiParameters::ReadCommandLine /
+ *                                  HierarchyIntegrator::Launch
+ *                                  (phi.cc:4-10, phi_parameters.cc:93-146,
+ *                                  hierarchy_integrator.cc:113-175)
+ *   phi_hip_params_*               struct PhiParameters and its reader
+ *                                  (phi_parameters.h:32-122,
+ *                                  phi_parameters.cc:152-888, 934-1118)
+ *   phi_hip_integrator_*           class HierarchyIntegrator: constructor,
+ *                                  Launch(), density_matrix()
+ *                                  (hierarchy_integrator.h:25-33,
+ *                                  hierarchy_integrator.cc:49-191)
+ *   phi_hip_create / _destroy      Hierarchy + AllocateMemory +
+ *                                  LinkHierarchyNodes + InitializePrefactors
+ *                                  (hierarchy.cc:45-155,
+ *                                  hierarchy_integrator.cc:870-1036,
+ *                                  1204-1466)
+ *   phi_hip_rhs                    HierarchyUpdater::Get over all nodes
+ *                                  (hierarchy_updater.h:36-52)
+ *   phi_hip_rk4_steps              the body of Rk4Integrate's loop
+ *                                  (hierarchy_integrator.cc:2677-2857)
+ *   phi_hip_rkf45_attempt/_accept  one attempted step of Rkf45Integrate
+ *                                  (hierarchy_integrator.cc:1998-2438)
+ *   phi_hip_steady_state           BicgstabSteadyState
+ *                                  (hierarchy_integrator.cc:3495-3783)
+ *   phi_hip_get_rho0               density_matrix()
+ *                                  (hierarchy_integrator.cc:177-191)
+ */
+#ifndef PHI_HIP_H_
+#define PHI_HIP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHI_HIP_OK 0
+#define PHI_HIP_EINVAL (-1)
+#define PHI_HIP_EPARAMS (-2)
+#define PHI_HIP_EUNSUPPORTED (-3)
+#define PHI_HIP_EHIP (-4)
+#define PHI_HIP_EIO (-5)
+#define PHI_HIP_ENOTCONVERGED (-6)
+#define PHI_HIP_ENODEVICE (-7)
+
+typedef struct phi_params phi_params;
+typedef struct phi_integrator phi_integrator;
+typedef struct phi_ctx phi_ctx;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* phi_hip_last_error(void);
+/* ABI version (major * 100 + minor). */
+int phi_hip_version(void);
+
+/* The `phi <paramfile> <method> [threads]` program. */
+int phi_hip_main(int argc, const char* const* argv);
+
+/* ---- Parameters (PhiParameters) ------------------------------------------ */
+/* Defaults of PhiParameters::SetDefaults. */
+int phi_hip_params_new(phi_params** out);
+/* ReadCommandLine semantics: method is the CLI word ("rk4", "rkf45",
+ * "bicgstab", ...), threads <= 0 keeps the default.  Invalid input still
+ * returns a handle, with are_parameters_valid = 0 and run_method BROKENINPUT
+ * where PHI would set them; the call returns PHI_HIP_EPARAMS then. */
+int phi_hip_params_read(const char* prm_path, const char* method, int threads,
+                        phi_params** out);
+void phi_hip_params_free(phi_params* p);
+/* Field access by PhiParameters member name (e.g. "temperature",
+ * "num_states", "gamma", "time_indepenent_hamiltonian", "run_method"). */
+int phi_hip_params_get_double(const phi_params* p, const char* name, double* out);
+int phi_hip_params_set_double(phi_params* p, const char* name, double v);
+int phi_hip_params_get_int(const phi_params* p, const char* name, int* out);
+int phi_hip_params_set_int(phi_params* p, const char* name, int v);
+/* Real arrays (gamma, lambda, kappa, bath_coupling_op, fields,
+ * multiple_independent_bath_indices): *n receives the length; out may be
+ * NULL to query it. */
+int phi_hip_params_get_array(const phi_params* p, const char* name, double* out, size_t cap,
+                             size_t* n);
+int phi_hip_params_set_array(phi_params* p, const char* name, const double* v, size_t n);
+/* Complex matrices (time_indepenent_hamiltonian, initial_density_matrix, ...):
+ * n complex values, interleaved. */
+int phi_hip_params_get_matrix(const phi_params* p, const char* name, double* out, size_t cap,
+                              size_t* n);
+int phi_hip_params_set_matrix(phi_params* p, const char* name, const double* v, size_t n);
+int phi_hip_params_get_string(const phi_params* p, const char* name, char* out, size_t cap);
+int phi_hip_params_set_string(phi_params* p, const char* name, const char* v);
+/* run_method by CLI word, then FixIncompatibleCombinations. */
+int phi_hip_params_fix(phi_params* p, const char* method);
+/* The output-file header PHI writes before the dynamics (PhiParameters::Write
+ * + "#Total Number of matrices" + memory table + separator) for a hierarchy
+ * of num_nodes nodes.  *len receives the full length. */
+int phi_hip_params_header(const phi_params* p, int num_nodes, char* out, size_t cap, size_t* len);
+/* Value checks of PHI's ParameterInput (kind: "int", "float", "bool",
+ * "reals", "complexes"); *valid = 1 when accepted. */
+int phi_hip_validate(const char* kind, const char* text, int* valid);
+
+/* ---- HierarchyIntegrator ----------------------------------------------------- */
+int phi_hip_integrator_create(phi_params* p, phi_integrator** out);
+/* Launch(): runs params' run_method, writes the output file.  Returns the
+ * run status (0, or PHI_HIP_E*). */
+int phi_hip_integrator_launch(phi_integrator* it);
+/* density_matrix(): rho_0 after Launch (N*N complex). */
+int phi_hip_integrator_density_matrix(phi_integrator* it, double* out, size_t cap);
+/* Whole ADO state after Launch (num_nodes*N*N complex). */
+int phi_hip_integrator_state(phi_integrator* it, double* out, size_t cap);
+/* Run counters: out[0] steps/attempts, [1] accepted, [2] final time,
+ * [3] final dt, [4] wall seconds, [5] BiCGSTAB iterations, [6] residual,
+ * [7] nodes. */
+int phi_hip_integrator_info(phi_integrator* it, double* out, size_t cap);
+void phi_hip_integrator_destroy(phi_integrator* it);
+
+/* ---- Device hierarchy context (the hot path) ---------------------------------- */
+/* precision: 0 = FP64, 1 = FP32 bulk with FP64 state. */
+int phi_hip_create(const phi_params* p, int precision, phi_ctx** out);
+void phi_hip_destroy(phi_ctx* c);
+int phi_hip_num_nodes(const phi_ctx* c);
+int phi_hip_num_modes(const phi_ctx* c);
+/* Multi-indices (num_nodes * D ints), PHI id order. */
+int phi_hip_get_index(const phi_ctx* c, int* out, size_t cap);
+int phi_hip_set_state(phi_ctx* c, const double* state, size_t n_complex);
+int phi_hip_get_state(phi_ctx* c, double* state, size_t n_complex);
+int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n_complex);
+/* out = L(in) for every node (PHI's Get, any complex input); host arrays. */
+int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
+/* nsteps RK4 steps of size dt on the device state. */
+int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
+/* nsteps RK4 steps with a HIP event around every stage kernel (on the
+ * context's stream); ms[s] = total milliseconds of stage s. */
+int phi_hip_rk4_stage_times(phi_ctx* c, long long nsteps, double dt, double* ms);
+/* One RKF45 attempt.  coefs: 15 b's (b10, b20, b21, ..., b54), 4 c's of y4
+ * (c50, c52, c53, c54) and 5 of y5 (c60, c62, c63, c64, c65), all already
+ * multiplied by dt.  *err = max |y4 - y5|.  Accept keeps y5. */
+int phi_hip_rkf45_attempt(phi_ctx* c, const double* coefs, double* err);
+int phi_hip_rkf45_accept(phi_ctx* c);
+/* Steady state from the current state: BiCGSTAB on L x = 0 with the
+ * (0,0) row of node 0 replaced by trace(rho_0) = 1. */
+int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, double* resid);
+int phi_hip_synchronize(phi_ctx* c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHI_HIP_H_ */

@@ -1,0 +1,810 @@
+"""Oracle restatement of PHI's `.prm` reader and output-file header.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ (and by bench.py's cpu_baseline
+leg) to check the product's C++ parser and header writer.  phi_amd/ never
+imports it.
+
+Restates (reference file:line):
+  * PhiParameters::SetDefaults            phi_parameters.cc:9-88
+  * PhiParameters::ReadInputFile          phi_parameters.cc:152-784
+    (the getline/eof loop, '#' lines skipped, `Key=value` scalars,
+    `Key:` + one row per line matrices, continuation rows, silent unknown keys)
+  * ParameterInput::Get / IsValid*        parameter_input.cc:10-742
+  * PhiParameters::FixIncompatibleCombinations  phi_parameters.cc:786-888
+  * PhiParameters::Write (output header)  phi_parameters.cc:934-1118
+  * GetHierarchyNodeCountEstimate         phi_parameters.cc:1129-1147
+  * PrintMemoryRequirements (file table)  hierarchy_integrator.cc:193-311
+    in 32-bit float arithmetic, as PHI computes it
+  * PrepareForOutput                      hierarchy_integrator.cc:313-319
+
+C++ `ostream << double` at the default precision is printf("%g"), which
+Python's '%g' reproduces; `<< complex` is "(re,im)" with the same format.
+"""
+
+import math
+
+import numpy as np
+
+HBAR = 5.3088354
+K_B = 0.6950344
+
+NUMBERS = "1234567890"
+ARRAY = ".Ee-+,"
+COMPLEX_ARRAY = ".Ee-+,()"
+
+
+class ParseError(Exception):
+    pass
+
+
+def fmt(x):
+    """C++ ostream default formatting of a double (precision 6)."""
+    return "%g" % x
+
+
+def fmt_c(z):
+    z = complex(z)
+    return "(%s,%s)" % (fmt(z.real), fmt(z.imag))
+
+
+class PhiParams:
+    """Field-for-field mirror of struct PhiParameters (phi_parameters.h:32-122)."""
+
+    def __init__(self):
+        self.verbose = 1
+        self.hbar = HBAR
+        self.boltzmann_constant = K_B
+        self.num_states = 0
+        self.hilbert_space_size = 0
+        self.hilbert_space_num_elements = 0
+        self.num_bath_couplings = 0
+        self.num_matsubara_terms = 1
+        self.hierarchy_truncation_level = 1
+        self.matrix_count = 0
+        self.time_indepenent_hamiltonian = None
+        self.initial_annealing_hamiltonian = None
+        self.final_annealing_hamiltonian = None
+        self.initial_hamiltonian_field = None
+        self.final_hamiltonian_field = None
+        self.minimum_field_change = 0.01
+        self.temperature = 300.0
+        self.kappa = None
+        self.gamma = None
+        self.lambda_ = None
+        self.bath_coupling_op = None
+        self.multiple_independent_bath_indices = None
+        self.initial_density_matrix = None
+        self.spectrum_initial_density_matrix = None
+        self.integration_time = 0.0
+        self.timestep = 0.001
+        self.output_minimum_timestep = 0.01
+        self.start_time = 0.0
+        self.restart_output_time = 0.0
+        self.rkf45_minimum_timestep = 1e-6
+        self.rkf45_tolerance = 1e-6
+        self.bicgstab_tolerance = 1e-6
+        self.filter_tolerance = 0.0
+        self.output_filename = ""
+        self.restart_input_filename = ""
+        self.restart_output_filename = ""
+        self.restart_backup_filename = ""
+        self.is_truncated_system = False
+        self.use_time_local_truncation = False
+        self.is_multiple_independent_baths = False
+        self.is_spectrum_calculation = False
+        self.is_diagonal_bath_coupling = False
+        self.is_full_bath_coupling = False
+        self.is_rho_normalized = True
+        self.is_restarted = False
+        self.bicgstab_ell = 2
+        self.num_threads = 1
+        self.stupid_hierarchy_partition = False
+        self.are_parameters_valid = True
+        self.run_method = "NONE"
+
+
+# --- ParameterInput validators (parameter_input.cc:341-731) -----------------
+
+def _at(s, i):
+    if i >= len(s):
+        # std::string::at throws std::out_of_range in PHI (abort); the
+        # restatement reports it as a parse error.
+        raise ParseError("empty value")
+    return s[i]
+
+
+def is_valid_integer(s):
+    i = 0
+    while _at(s, i) == " ":
+        i += 1
+    ok = True
+    ws = False
+    while ok and i < len(s):
+        if s[i] not in NUMBERS or ws:
+            ok = False
+        i += 1
+        while i < len(s) and s[i] == " ":
+            i += 1
+            ws = True
+    return ok
+
+
+def is_valid_float(s):
+    ok = True
+    period = e = ws = False
+    m = 0
+    i = 0
+    while _at(s, i) == " ":
+        i += 1
+    while ok and i < len(s):
+        c = s[i]
+        numi = NUMBERS.find(c)
+        modi = ARRAY.find(c)
+        if (numi < 0 and modi < 0) or (numi >= 0 and ws):
+            ok = False
+        elif modi >= 0:
+            if modi == 0:
+                if period or e or ws:
+                    ok = False
+                period = True
+            elif modi in (1, 2):
+                if m == 0 or e or ws:
+                    ok = False
+                e = True
+            elif modi in (3, 4):
+                if (m != 0 and s[i - 1] not in "eE") or ws:
+                    ok = False
+            else:
+                ok = False
+        i += 1
+        while i < len(s) and s[i] == " ":
+            i += 1
+            ws = True
+        m += 1
+    return ok
+
+
+def is_valid_bool(s):
+    i = 0
+    while _at(s, i) == " ":
+        i += 1
+    ok = _at(s, i) in "01"
+    i += 1
+    while ok and i < len(s):
+        if s[i] != " ":
+            ok = False
+        i += 1
+    return ok
+
+
+def is_valid_array(s):
+    isnum = True
+    period = e = comma = ws = False
+    m = -1
+    i = 0
+    while _at(s, i) == " ":
+        i += 1
+    while isnum and i < len(s):
+        m += 1
+        c = s[i]
+        numi = NUMBERS.find(c)
+        modi = ARRAY.find(c)
+        if numi < 0 and modi < 0:
+            isnum = False
+        elif numi >= 0:
+            comma = False
+            if ws:
+                isnum = False
+        else:
+            if modi != 5:
+                comma = False
+            if modi == 0:
+                if period or e or ws:
+                    isnum = False
+                period = True
+            elif modi in (1, 2):
+                if m == 0 or e or ws:
+                    isnum = False
+                e = True
+            elif modi in (3, 4):
+                if (m != 0 and s[i - 1] not in "eE") or ws:
+                    isnum = False
+            elif modi == 5:
+                if m == 0 or i == len(s) - 1 or comma:
+                    isnum = False
+                period = False
+                e = False
+                comma = True
+                ws = False
+                m = -1
+        i += 1
+        while i < len(s) and s[i] == " ":
+            i += 1
+            if modi != 5:
+                ws = True
+    return isnum
+
+
+def is_valid_complex_array(s):
+    isnum = True
+    period = e = comma = ws = lb = False
+    m = -1
+    i = 0
+    while _at(s, i) == " ":
+        i += 1
+    while isnum and i < len(s):
+        c = s[i]
+        numi = NUMBERS.find(c)
+        modi = COMPLEX_ARRAY.find(c)
+        if numi < 0 and modi < 0:
+            isnum = False
+        elif numi >= 0:
+            m += 1
+            if ws or not lb:
+                isnum = False
+            comma = False
+            ws = False
+        else:
+            if modi == 0:
+                if period or e or ws or not lb:
+                    isnum = False
+                period = True
+            elif modi in (1, 2):
+                if m == 0 or e or ws or not lb:
+                    isnum = False
+                e = True
+                m = 0
+            elif modi in (3, 4):
+                if m != 0 or ws or not lb:
+                    isnum = False
+            elif modi == 5:
+                if m == 0 or i == len(s) - 1 or comma:
+                    isnum = False
+                period = False
+                e = False
+                comma = True
+                m = 0
+            elif modi == 6:
+                if (m > 0 and not comma) or lb:
+                    isnum = False
+                lb = True
+                m = 0
+            elif modi == 7:
+                if not lb or m == 0:
+                    isnum = False
+                lb = False
+            if modi != 5:
+                comma = False
+            ws = False
+        i += 1
+        while isnum and i < len(s) and s[i] == " ":
+            i += 1
+            if modi not in (5, 6, 7):
+                ws = True
+    return isnum
+
+
+def _stream_number(text, as_int):
+    """`std::stringstream >> value` on a validated field."""
+    t = text.strip(" ")
+    if as_int:
+        j = 0
+        while j < len(t) and t[j] in "+-0123456789":
+            j += 1
+        return int(t[:j]) if j else 0
+    try:
+        return float(t)
+    except ValueError:
+        # A field the validator lets through but the stream cannot read
+        # (".", "1e", "-") leaves 0 with failbit set (C++11 num_get).
+        return 0.0
+
+
+def get_scalar(line, kind, rng):
+    if kind == "int":
+        if not is_valid_integer(line):
+            raise ParseError("integer")
+        v = _stream_number(line, True)
+    elif kind == "float":
+        if not is_valid_float(line):
+            raise ParseError("float")
+        v = _stream_number(line, False)
+    else:
+        if not is_valid_bool(line):
+            raise ParseError("bool")
+        return _stream_number(line, True) == 1
+    if rng == "POSITIVE" and v <= 0:
+        raise ParseError("must be positive")
+    if rng == "NON_NEGATIVE" and v < 0:
+        raise ParseError("must be non-negative")
+    return v
+
+
+def get_float_array(line, size, as_int=False):
+    """ParameterInput::Get(line, array, size, j_skip) for int/Float arrays."""
+    if not is_valid_array(line):
+        raise ParseError("array")
+    if line.endswith(","):
+        raise ParseError("trailing comma")
+    if line.count(",") != size - 1:
+        raise ParseError("comma count")
+    parts = line.split(",")
+    out = []
+    for p in parts[:-1]:
+        if p == "":
+            raise ParseError("empty element")
+    for p in parts:
+        out.append(_stream_number(p, as_int))
+    if len(out) != size:
+        raise ParseError("count")
+    return out
+
+
+def get_complex_array(line, size):
+    if line.endswith(","):
+        raise ParseError("trailing comma")
+    commas = line.count(",")
+    if commas != 2 * size - 1 and commas != size - 1:
+        raise ParseError("comma count")
+    lb = line.count("(")
+    if lb != 0 and lb != size:
+        raise ParseError("bracket count")
+    if lb == 0 and commas != size - 1:
+        raise ParseError("comma count")
+    if lb == 0:
+        return [complex(v) for v in get_float_array(line, size)]
+    if not is_valid_complex_array(line):
+        raise ParseError("complex array")
+    out = []
+    il = line.find("(")
+    ic = line.find(",", il)
+    ir = line.find(")", ic)
+    while il >= 0 and ic >= 0:
+        if ic - il <= 1 and ir - ic <= 1:
+            raise ParseError("empty complex element")
+        re = _stream_number(line[il + 1:ic], False)
+        im = _stream_number(line[ic + 1:ir], False)
+        out.append(complex(re, im))
+        il = line.find("(", ir)
+        ic = line.find(",", il + 1) if il >= 0 else -1
+        ir = line.find(")", ic)
+    if len(out) != size:
+        raise ParseError("count")
+    return out
+
+
+# --- ReadInputFile -----------------------------------------------------------
+
+def read_command_line(path, method, threads=None):
+    """PhiParameters::ReadCommandLine (phi_parameters.cc:93-146)."""
+    p = PhiParams()
+    if threads is not None:
+        p.num_threads = int(threads)
+    table = {"rk4spectrum": "RK4SPECTRUM", "rkf45spectrum": "RKF45SPECTRUM",
+             "rk4": "RK4", "rkf45": "RKF45", "print": "PRINTHIERARCHY",
+             "memory": "PRINTMEMORY", "bicgstabl": "BICGSTABL",
+             "bicgstab": "BICGSTABL", "bicgstabu": "BICGSTABU"}
+    if method is not None:
+        p.run_method = table.get(method, "NONE")
+    read_input_file(p, path)
+    return p
+
+
+def read_input_file(p, path):
+    with open(path, "r", newline="") as f:
+        segs = f.read().split("\n")
+    state = {"i": 0, "eof": False}
+
+    def getline():
+        i = state["i"]
+        state["i"] = i + 1
+        if i >= len(segs) - 1:
+            state["eof"] = True
+        return segs[i] if i < len(segs) else ""
+
+    row = 0
+    var_name = ""
+    var = ""
+    line = getline()
+    try:
+        while not state["eof"] and p.are_parameters_valid:
+            while "#" in line:
+                var = ""
+                line = getline()
+            if "=" in line:
+                pos = line.find("=")
+                var_name = line[:pos]
+                var = line[pos + 1:]
+            elif ":" in line:
+                var_name = line[:line.find(":")]
+                row = 0
+                var = getline()
+            elif len(line) > 0:
+                var = line
+            else:
+                var_name = ""
+            line = getline()
+            row = _dispatch(p, var_name, var, row)
+    except ParseError:
+        p.are_parameters_valid = False
+    if not p.are_parameters_valid:
+        return p
+    if p.run_method != "PRINTMEMORY":
+        fix_incompatible_combinations(p)
+    p.matrix_count = node_count_estimate(
+        p.hierarchy_truncation_level, p.num_bath_couplings * p.num_matsubara_terms)
+    return p
+
+
+def _matrix(p, attr, row, var, size):
+    if getattr(p, attr) is None or row == 0:
+        setattr(p, attr, np.zeros((size, size), dtype=complex))
+    vals = get_complex_array(var, size)
+    getattr(p, attr)[row, :] = vals
+
+
+def _dispatch(p, name, var, row):
+    n = name
+    if n in ("fout", "Output", "OutputFile"):
+        p.output_filename = var
+    elif n in ("Ns", "NumStates"):
+        p.num_states = get_scalar(var, "int", "POSITIVE")
+        if p.hilbert_space_size == 0:
+            p.hilbert_space_size = p.num_states
+            p.hilbert_space_num_elements = p.num_states ** 2
+        if p.num_bath_couplings == 0:
+            p.num_bath_couplings = p.num_states
+        p.multiple_independent_bath_indices = list(range(p.num_bath_couplings))
+    elif n == "HilbertSpaceSize":
+        if (p.time_indepenent_hamiltonian is not None
+                or p.final_annealing_hamiltonian is not None):
+            raise ParseError("HilbertSpaceSize after Hamiltonian")
+        if p.is_diagonal_bath_coupling or p.is_full_bath_coupling:
+            raise ParseError("HilbertSpaceSize after bath operators")
+        p.hilbert_space_size = get_scalar(var, "int", "POSITIVE")
+        p.hilbert_space_num_elements = p.hilbert_space_size ** 2
+        if p.hilbert_space_size < p.num_states:
+            p.are_parameters_valid = False
+    elif n in ("M", "NumCouplingTerms"):
+        p.num_bath_couplings = get_scalar(var, "int", "POSITIVE")
+        if p.num_states != 0:
+            p.multiple_independent_bath_indices = list(range(p.num_bath_couplings))
+    elif n in ("Kt", "MatsubaraTerms"):
+        p.num_matsubara_terms = get_scalar(var, "int", "NON_NEGATIVE") + 1
+    elif n in ("Lh", "HierarchyTruncation"):
+        p.hierarchy_truncation_level = get_scalar(var, "int", "POSITIVE")
+    elif n in ("dt", "Timestep"):
+        p.timestep = get_scalar(var, "float", "POSITIVE")
+    elif n in ("t", "Time"):
+        p.integration_time = get_scalar(var, "float", "POSITIVE")
+    elif n == "OutputMinimumTimestep":
+        p.output_minimum_timestep = get_scalar(var, "float", "POSITIVE")
+    elif n in ("filter", "filtered", "FilterTolerance"):
+        p.filter_tolerance = get_scalar(var, "float", "POSITIVE")
+    elif n in ("TL", "TimeLocal"):
+        p.use_time_local_truncation = get_scalar(var, "bool", None)
+    elif n in ("rho_normalized", "rho_normalised"):
+        p.is_rho_normalized = get_scalar(var, "bool", None)
+    elif n in ("H", "Hamiltonian"):
+        if p.num_states <= 0:
+            raise ParseError("NumStates before Hamiltonian")
+        if (p.initial_annealing_hamiltonian is not None
+                or p.final_annealing_hamiltonian is not None
+                or (p.time_indepenent_hamiltonian is not None and row == 0)):
+            raise ParseError("Hamiltonian already specified")
+        if row != 0 and row >= p.hilbert_space_size:
+            raise ParseError("Hamiltonian overflow")
+        _matrix(p, "time_indepenent_hamiltonian", row, var, p.hilbert_space_size)
+        row += 1
+    elif n in ("H_X", "TransverseHamiltonian"):
+        if p.num_states <= 0 or row >= p.hilbert_space_size:
+            raise ParseError(n)
+        _matrix(p, "initial_annealing_hamiltonian", row, var, p.hilbert_space_size)
+        row += 1
+    elif n == "TransverseField":
+        cnt = var.count(",") + 1
+        p.initial_hamiltonian_field = get_float_array(var, cnt)
+    elif n in ("H_Z", "LongitudinalHamiltonian"):
+        if p.num_states <= 0 or row >= p.hilbert_space_size:
+            raise ParseError(n)
+        _matrix(p, "final_annealing_hamiltonian", row, var, p.hilbert_space_size)
+        row += 1
+    elif n == "LongitudinalField":
+        cnt = var.count(",") + 1
+        p.final_hamiltonian_field = get_float_array(var, cnt)
+    elif n == "MinimumFieldChange":
+        p.minimum_field_change = get_scalar(var, "float", "NON_NEGATIVE")
+    elif n in ("rho0", "InitialDensityMatrix"):
+        if p.num_states <= 0:
+            raise ParseError("Ns before rho0")
+        if p.initial_density_matrix is not None and row == 0:
+            raise ParseError("rho0 already specified")
+        if row != 0 and row >= p.hilbert_space_size:
+            raise ParseError("rho0 overflow")
+        _matrix(p, "initial_density_matrix", row, var, p.hilbert_space_size)
+        row += 1
+    elif n == "spectrum_rho0" and p.spectrum_initial_density_matrix is None:
+        if p.num_states <= 0:
+            raise ParseError("Ns before spectrum_rho0")
+        p.spectrum_initial_density_matrix = get_complex_array(var, p.hilbert_space_size)
+    elif n in ("Multibath", "multiBath", "DiagonalCouplingIndices"):
+        if p.num_bath_couplings <= 0:
+            raise ParseError(n)
+        p.multiple_independent_bath_indices = [
+            int(v) for v in get_float_array(var, p.num_bath_couplings, as_int=True)]
+    elif n in ("Vbath", "DiagonalCouplingTerms", "CorrelatedCouplingTerms"):
+        if p.num_bath_couplings <= 0 or p.num_states <= 0:
+            raise ParseError(n)
+        if p.bath_coupling_op is not None and row == 0:
+            raise ParseError("bath operators already specified")
+        if row == 0:
+            p.is_diagonal_bath_coupling = True
+            p.bath_coupling_op = np.zeros(p.num_bath_couplings * p.hilbert_space_size)
+        elif row >= p.num_bath_couplings:
+            raise ParseError("bath overflow")
+        hs = p.hilbert_space_size
+        p.bath_coupling_op[hs * row:hs * (row + 1)] = get_float_array(var, hs)
+        row += 1
+    elif n == "FullBathTerms":
+        if p.num_bath_couplings <= 0 or p.num_states <= 0:
+            raise ParseError(n)
+        if p.bath_coupling_op is not None and row == 0:
+            raise ParseError("bath operators already specified")
+        ne = p.hilbert_space_num_elements
+        if row == 0:
+            p.is_full_bath_coupling = True
+            p.bath_coupling_op = np.zeros(p.num_bath_couplings * ne)
+        elif row >= p.num_bath_couplings:
+            raise ParseError("bath overflow")
+        p.bath_coupling_op[ne * row:ne * (row + 1)] = get_float_array(var, ne)
+        row += 1
+    elif n == "RestartInput":
+        p.restart_input_filename = var.split()[0] if var.split() else ""
+        p.is_restarted = True
+    elif n in ("lambda", "Lambda"):
+        if p.num_bath_couplings <= 0 or p.lambda_ is not None:
+            raise ParseError(n)
+        p.lambda_ = get_float_array(var, p.num_bath_couplings)
+    elif n in ("gamma", "Gamma"):
+        if p.num_bath_couplings <= 0 or p.gamma is not None:
+            raise ParseError(n)
+        p.gamma = get_float_array(var, p.num_bath_couplings)
+    elif n == "kappa":
+        if p.num_states <= 0 or p.kappa is not None:
+            raise ParseError(n)
+        p.kappa = get_float_array(var, p.hilbert_space_size)
+    elif n in ("T", "Temperature"):
+        p.temperature = get_scalar(var, "float", "NON_NEGATIVE")
+    elif n == "verbose":
+        p.verbose = _stream_number(var, True)
+    elif n == "RestartFile":
+        p.restart_output_filename = var.split()[0] if var.split() else ""
+    elif n == "RestartOutputTimeSecs":
+        p.restart_output_time = get_scalar(var, "float", "POSITIVE")
+    elif n in ("tol_low", "RKF45mindt"):
+        p.rkf45_minimum_timestep = get_scalar(var, "float", "POSITIVE")
+        # phi_parameters.cc:688-693: the steady-state alias only takes effect
+        # inside the verbose branch.
+        if p.verbose > 0 and n == "tol_low":
+            p.bicgstab_tolerance = p.rkf45_minimum_timestep
+    elif n in ("tol_high", "RKF45tolerance"):
+        p.rkf45_tolerance = get_scalar(var, "float", "POSITIVE")
+    elif n == "bicgstab_tolerance":
+        p.bicgstab_tolerance = get_scalar(var, "float", "POSITIVE")
+    elif n == "bicgstab":
+        p.bicgstab_ell = get_scalar(var, "int", "NON_NEGATIVE")
+    elif n == "CyclicPartition":
+        p.stupid_hierarchy_partition = get_scalar(var, "bool", None)
+    elif n == "hbar":
+        p.hbar = get_scalar(var, "float", "POSITIVE")
+    elif n in ("kB", "k_B"):
+        p.boltzmann_constant = get_scalar(var, "float", "POSITIVE")
+    return row
+
+
+def fix_incompatible_combinations(p):
+    """phi_parameters.cc:786-888."""
+    p.are_parameters_valid = not (
+        p.time_indepenent_hamiltonian is None
+        and (p.initial_annealing_hamiltonian is None
+             or p.final_annealing_hamiltonian is None))
+    if not p.are_parameters_valid and p.run_method != "NONE":
+        p.run_method = "BROKENINPUT"
+        return
+    if p.initial_annealing_hamiltonian is not None and p.initial_hamiltonian_field is None:
+        p.initial_hamiltonian_field = [1.0, 0.0]
+    if p.initial_annealing_hamiltonian is not None and p.final_hamiltonian_field is None:
+        p.final_hamiltonian_field = [0.0, 1.0]
+    if p.run_method in ("RKF45", "RKF45SPECTRUM", "BICGSTAB", "BICGSTABL",
+                        "BICGSTABU") and not p.is_rho_normalized:
+        p.is_rho_normalized = True
+    if p.run_method in ("BICGSTAB", "BICGSTABL", "BICGSTABU"):
+        p.use_time_local_truncation = False
+    if p.run_method == "BICGSTABL" and p.bicgstab_ell < 1:
+        p.run_method = "BICGSTAB"
+    if p.filter_tolerance > 0 and (p.is_diagonal_bath_coupling or p.is_full_bath_coupling):
+        p.filter_tolerance = 0.0
+    p.is_spectrum_calculation = p.run_method in ("RK4SPECTRUM", "RKF45SPECTRUM")
+    if p.is_spectrum_calculation:
+        if p.initial_density_matrix is not None and p.spectrum_initial_density_matrix is None:
+            p.spectrum_initial_density_matrix = list(np.diag(p.initial_density_matrix))
+        elif p.spectrum_initial_density_matrix is None:
+            p.spectrum_initial_density_matrix = [1.0] * p.hilbert_space_size
+    if p.multiple_independent_bath_indices is None:
+        p.multiple_independent_bath_indices = list(range(p.num_bath_couplings))
+    if p.hilbert_space_size < p.num_states:
+        p.hilbert_space_size = p.num_states
+        p.hilbert_space_num_elements = p.num_states ** 2
+    elif p.hilbert_space_size > p.num_states:
+        p.is_truncated_system = True
+
+
+def node_count_estimate(max_level, terms):
+    """GetHierarchyNodeCountEstimate: sum_l (l+D-1)!/(l!(D-1)!) in double."""
+    def ff(a, b):
+        acc = 1.0
+        for i in range(a, b + 1):
+            acc *= i
+        return acc
+    count = 0.0
+    for l in range(max_level):
+        count += ff(1, l + terms - 1) / ff(1, l) / ff(1, terms - 1)
+    return int(count)
+
+
+# --- Output header -------------------------------------------------------------
+
+def _matrix_rows(out, name, mat, hs):
+    """Write's matrix block: row i prints column-major elements i + j*hs for
+    j < hs-1, then element (i+1)*hs - 1 (phi_parameters.cc:1047-1090)."""
+    flat = np.asarray(mat).T.reshape(-1)      # column-major storage
+    out.append("#%s:\n" % name)
+    for i in range(hs):
+        vals = [fmt_c(flat[i + j * hs]) for j in range(hs - 1)]
+        vals.append(fmt_c(flat[(i + 1) * hs - 1]))
+        out.append("#" + ",".join(vals) + "\n")
+
+
+def write_header(p):
+    """PhiParameters::Write (phi_parameters.cc:934-1118)."""
+    o = []
+    o.append("#PHI %s\n" % fmt(1.1))
+    o.append("#Parameters:\n")
+    o.append("#-----------\n")
+    if p.hbar != HBAR:
+        o.append("#ReducedPlanckConstant=%s\n" % fmt(p.hbar))
+    if p.boltzmann_constant != K_B:
+        o.append("#BoltzmannConstant=%s\n" % fmt(p.boltzmann_constant))
+    o.append("#NumStates=%d\n" % p.num_states)
+    if p.hilbert_space_size != p.num_states:
+        o.append("#HilbertSpaceSize=%d\n" % p.hilbert_space_size)
+    o.append("#MatsubaraTerms=%d\n" % (p.num_matsubara_terms - 1))
+    o.append("#CouplingTerms=%d\n" % p.num_bath_couplings)
+    o.append("#Temperature=%s\n" % fmt(p.temperature))
+    o.append("#HierarchyTruncation=%d\n" % p.hierarchy_truncation_level)
+    o.append("#TimeLocal=%d\n" % int(p.use_time_local_truncation))
+    o.append("#Timestep=%s\n" % fmt(p.timestep))
+    o.append("#Time=%s\n" % fmt(p.integration_time))
+    o.append("#filter=%s\n" % fmt(p.filter_tolerance))
+    o.append("#RKF45tolerance = %s\n" % fmt(p.rkf45_tolerance))
+    o.append("#RKF45mindt = %s\n" % fmt(p.rkf45_minimum_timestep))
+    if p.gamma is not None:
+        o.append("#gamma:\n#" + ",".join(fmt(v) for v in p.gamma) + "\n")
+    if p.lambda_ is not None:
+        o.append("#lambda:\n#" + ",".join(fmt(v) for v in p.lambda_) + "\n")
+    if p.kappa is not None:
+        o.append("#kappa:\n#" + ",".join(fmt(v) for v in p.kappa[:p.num_states]) + "\n")
+    if p.initial_hamiltonian_field:
+        o.append("#TransverseField:\n#" + ",".join(fmt(v) for v in p.initial_hamiltonian_field) + "\n")
+    if p.final_hamiltonian_field:
+        o.append("#LongitudinalField:\n#" + ",".join(fmt(v) for v in p.final_hamiltonian_field) + "\n")
+    if p.is_multiple_independent_baths:
+        o.append("#multiBath:\n#" + ",".join(
+            "%d" % v for v in p.multiple_independent_bath_indices) + "\n")
+    hs = p.hilbert_space_size
+    if p.is_diagonal_bath_coupling:
+        o.append("#CorrelatedCouplingTerms:\n")
+        for m in range(p.num_bath_couplings):
+            o.append("#" + ",".join(fmt(v) for v in p.bath_coupling_op[m * hs:(m + 1) * hs]) + "\n")
+    if p.is_full_bath_coupling:
+        ne = p.hilbert_space_num_elements
+        o.append("#FullBathTerms:\n")
+        for m in range(p.num_bath_couplings):
+            o.append("#" + ",".join(fmt(v) for v in p.bath_coupling_op[m * ne:(m + 1) * ne]) + "\n")
+    if p.initial_annealing_hamiltonian is not None:
+        _matrix_rows(o, "TransverseHamiltonian", p.initial_annealing_hamiltonian, hs)
+    if p.final_annealing_hamiltonian is not None:
+        _matrix_rows(o, "LongitudinalHamiltonian", p.final_annealing_hamiltonian, hs)
+    if p.time_indepenent_hamiltonian is not None:
+        _matrix_rows(o, "Hamiltonian", p.time_indepenent_hamiltonian, hs)
+    if p.initial_density_matrix is not None:
+        _matrix_rows(o, "InitialDensityMatrix", p.initial_density_matrix, hs)
+    if p.restart_input_filename != "":
+        o.append("#RestartInput=%s\n" % p.restart_input_filename)
+    if p.restart_output_time > 0:
+        o.append("#RestartOutputTime=%s\n" % fmt(p.restart_output_time))
+        o.append("#RestartFile=%s\n" % p.restart_output_filename)
+    o.append("#Estimated Number of Matrices=%d\n" % node_count_estimate(
+        p.hierarchy_truncation_level, p.num_bath_couplings * p.num_matsubara_terms))
+    o.append("#-------------\n")
+    if p.run_method in ("RK4", "RKF45", "RK4SPECTRUM", "RKF45SPECTRUM"):
+        o.append("#Threads=%d\n" % p.num_threads)
+        o.append("#Integrator=%s\n" % {"RK4": "rk4", "RKF45": "rkf45",
+                                         "RK4SPECTRUM": "rk4spectrum",
+                                         "RKF45SPECTRUM": "rkf45spectrum"}[p.run_method])
+    elif p.run_method in ("BICGSTAB", "BICGSTABU", "BICGSTABL"):
+        o.append("#Threads=%d\n" % p.num_threads)
+        if p.run_method == "BICGSTABL":
+            o.append("#Calculating steady-state using BiCGSTAB(%d)\n" % p.bicgstab_ell)
+        else:
+            o.append("#Calculating steady-state using BiCGSTAB\n")
+    return "".join(o)
+
+
+def memory_table(p, matrix_count):
+    """PrintMemoryRequirements(output_to_file=true), float32 arithmetic."""
+    f = np.float32
+    n = p.num_states
+    m = p.num_bath_couplings
+    k = p.num_matsubara_terms
+    T = p.num_threads
+    ne = n * n
+    hsne = p.hilbert_space_num_elements
+    hs = p.hilbert_space_size
+    mc = matrix_count
+    C, I, F, TT = 16, 4, 8, 8
+    matrices = f(f(f(C * ne * mc)) / f(1024)) / f(1024)
+    mem = f(0)
+    mem = f(mem + f(I * (m + 4 + m + m + m + m + 5) + C * (1 + m + m + m + m + ne)))
+    mem = f(mem * f(2 * mc))
+    mem = f(mem + f(C * (3 + ne) + I * 6 + TT * 2))
+    if p.use_time_local_truncation:
+        mem = f(mem + f(C * ne * T))
+        mem = f(mem + f(C * m * k * ne))
+    if p.is_diagonal_bath_coupling:
+        mem = f(mem + f(C * m * ne * (k + 2) * T))
+        mem = f(mem + f(C * m * hs))
+    if p.is_full_bath_coupling:
+        mem = f(mem + f(C * m * ne * 3 * T))
+        mem = f(mem + f(C * m * hsne))
+    mem = f(mem + f(C * hsne * T))
+    if p.is_truncated_system:
+        mem = f(mem + f(C * hs * T))
+        mem = f(mem + f(C * hsne * T))
+        mem = f(mem + f(I * 2 * T))
+        mem = f(mem + f(C * ne * T))
+    rk4 = f(f(f(f(C * ne * mc * 4) + mem)) / f(1024)) / f(1024)
+    rkf = f(0)
+    rkf = f(rkf + f(C * ne * mc * 6))
+    rkf = f(rkf + f(C * 31))
+    rkf = f(rkf + f(F * 7 + I * 2))
+    rkf = f(f(f(mem + rkf)) / f(1024)) / f(1024)
+    mem2 = f(0)
+    mem2 = f(mem2 + f(I * (m + 4 + m + m + m + m + 5) + C * (1 + m + m + m + m + n)))
+    mem2 = f(mem2 * f(2 * mc))
+    mem2 = f(mem2 + f(C * (3 + n) + I * 6 + TT * 2))
+    srk4 = f(0)
+    srk4 = f(srk4 + f(C * (n + 1) * mc * 4))
+    srk4 = f(f(f(srk4 + mem2)) / f(1024)) / f(1024)
+    srkf = f(0)
+    srkf = f(srkf + f(C * 3 * n * mc * 6))
+    srkf = f(srkf + f(C * 31))
+    srkf = f(srkf + f(F * 7 + I * 2))
+    srkf = f(f(f(srkf + mem2)) / f(1024)) / f(1024)
+    ss = f(0)
+    ss = f(ss + f(I * (m + 4 + m + m + m + m + 5) + C * (1 + m + m + m + m)))
+    ss = f(ss + f(6 * (C * (n * n))))
+    ss = f(ss * f(mc))
+    ss = f(ss + f(9 * C + 3 * I + 2 * F))
+    ss = f(f(ss / f(1024)) / f(1024))
+    g = lambda v: fmt(float(v))
+    return ("#Memory to store matrices for calculations:\n"
+            "#hierarchy: %s Mb.\n#rk4: %s Mb.\n#rkf45: %s Mb.\n"
+            "#rk4spectrum: %s Mb.\n#rkf45spectrum: %s Mb.\n#steadystate: %s Mb.\n"
+            % (g(matrices), g(rk4), g(rkf), g(srk4), g(srkf), g(ss)))
+
+
+def full_header(p, matrix_count):
+    """PrepareForOutput (hierarchy_integrator.cc:313-319)."""
+    return (write_header(p) + "#Total Number of matrices:%d\n" % matrix_count
+            + memory_table(p, matrix_count) + "#-------------\n")

@@ -1,0 +1,397 @@
+"""Python bindings of the MI355X HEOM integrator (ctypes over libphi_hip.so).
+
+Mirrors PHI's C++ entry points for this path:
+  PhiParameters        struct PhiParameters + ReadCommandLine/ReadInputFile
+                       (phi_parameters.h:32-122, phi_parameters.cc:93-888)
+  HierarchyIntegrator  constructor, Launch(), density_matrix()
+                       (hierarchy_integrator.h:25-33)
+  Context              the device hierarchy: batched Get (rhs), RK4 steps,
+                       RKF45 attempts, steady state (include/phi_hip.h)
+Every call goes through the C ABI; there is no fallback path.  Importing the
+module fails loudly when the library has not been built.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphi_hip.so")
+EXE_PATH = os.path.join(HERE, "phi")
+
+RUN_METHODS = ["BROKENINPUT", "NONE", "RK4", "RKF45", "RK4SPECTRUM", "RKF45SPECTRUM",
+               "BICGSTABL", "BICGSTAB", "BICGSTABU", "PRINTHIERARCHY", "PRINTMEMORY"]
+
+_lib = None
+
+
+class PhiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("phi_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded C-ABI library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("%s is missing: run `python phi_amd/build.py` (or "
+                          "__graft_entry__.build()) first" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    sz, cs = ctypes.c_size_t, ctypes.c_char_p
+    psz = ctypes.POINTER(ctypes.c_size_t)
+    sigs = {
+        "phi_hip_last_error": (cs, []),
+        "phi_hip_version": (ctypes.c_int, []),
+        "phi_hip_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(cs)]),
+        "phi_hip_params_new": (ctypes.c_int, [ctypes.POINTER(vp)]),
+        "phi_hip_params_read": (ctypes.c_int, [cs, cs, ctypes.c_int, ctypes.POINTER(vp)]),
+        "phi_hip_params_free": (None, [vp]),
+        "phi_hip_params_get_double": (ctypes.c_int, [vp, cs, dp]),
+        "phi_hip_params_set_double": (ctypes.c_int, [vp, cs, ctypes.c_double]),
+        "phi_hip_params_get_int": (ctypes.c_int, [vp, cs, ip]),
+        "phi_hip_params_set_int": (ctypes.c_int, [vp, cs, ctypes.c_int]),
+        "phi_hip_params_get_array": (ctypes.c_int, [vp, cs, dp, sz, psz]),
+        "phi_hip_params_set_array": (ctypes.c_int, [vp, cs, dp, sz]),
+        "phi_hip_params_get_matrix": (ctypes.c_int, [vp, cs, dp, sz, psz]),
+        "phi_hip_params_set_matrix": (ctypes.c_int, [vp, cs, dp, sz]),
+        "phi_hip_params_get_string": (ctypes.c_int, [vp, cs, cs, sz]),
+        "phi_hip_params_set_string": (ctypes.c_int, [vp, cs, cs]),
+        "phi_hip_params_fix": (ctypes.c_int, [vp, cs]),
+        "phi_hip_params_header": (ctypes.c_int, [vp, ctypes.c_int, cs, sz, psz]),
+        "phi_hip_validate": (ctypes.c_int, [cs, cs, ip]),
+        "phi_hip_integrator_create": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
+        "phi_hip_integrator_launch": (ctypes.c_int, [vp]),
+        "phi_hip_integrator_density_matrix": (ctypes.c_int, [vp, dp, sz]),
+        "phi_hip_integrator_state": (ctypes.c_int, [vp, dp, sz]),
+        "phi_hip_integrator_info": (ctypes.c_int, [vp, dp, sz]),
+        "phi_hip_integrator_destroy": (None, [vp]),
+        "phi_hip_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp)]),
+        "phi_hip_destroy": (None, [vp]),
+        "phi_hip_num_nodes": (ctypes.c_int, [vp]),
+        "phi_hip_num_modes": (ctypes.c_int, [vp]),
+        "phi_hip_get_index": (ctypes.c_int, [vp, ip, sz]),
+        "phi_hip_set_state": (ctypes.c_int, [vp, dp, sz]),
+        "phi_hip_get_state": (ctypes.c_int, [vp, dp, sz]),
+        "phi_hip_get_rho0": (ctypes.c_int, [vp, dp, sz]),
+        "phi_hip_rhs": (ctypes.c_int, [vp, dp, dp, sz]),
+        "phi_hip_rk4_steps": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double]),
+        "phi_hip_rk4_stage_times": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
+        "phi_hip_rkf45_attempt": (ctypes.c_int, [vp, dp, dp]),
+        "phi_hip_rkf45_accept": (ctypes.c_int, [vp]),
+        "phi_hip_steady_state": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ip, dp]),
+        "phi_hip_synchronize": (ctypes.c_int, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise PhiError(rc, lib().phi_hip_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _colmajor(m):
+    """N x N complex (m[i, j] = element (i, j)) -> interleaved column-major."""
+    m = np.asarray(m, dtype=complex)
+    return np.ascontiguousarray(m.T).view(np.float64).reshape(-1).copy()
+
+
+def _from_colmajor(flat, n):
+    return np.asarray(flat).view(complex).reshape(n, n).T.copy()
+
+
+class PhiParameters:
+    """Handle on a PhiParameters instance (field names as in PHI)."""
+
+    def __init__(self, handle=None):
+        if handle is None:
+            h = ctypes.c_void_p()
+            _check(lib().phi_hip_params_new(ctypes.byref(h)))
+            handle = h
+        self._h = handle
+
+    @classmethod
+    def read(cls, path, method=None, threads=0, check=True):
+        """ReadCommandLine(path, method, threads): parse a `.prm` file."""
+        h = ctypes.c_void_p()
+        rc = lib().phi_hip_params_read(os.fsencode(path),
+                                       method.encode() if method else None, threads,
+                                       ctypes.byref(h))
+        obj = cls(h)
+        if check:
+            _check(rc)
+        obj.read_status = rc
+        return obj
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().phi_hip_params_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def get_double(self, name):
+        v = ctypes.c_double()
+        _check(lib().phi_hip_params_get_double(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_double(self, name, v):
+        _check(lib().phi_hip_params_set_double(self._h, name.encode(), float(v)))
+
+    def get_int(self, name):
+        v = ctypes.c_int()
+        _check(lib().phi_hip_params_get_int(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_int(self, name, v):
+        _check(lib().phi_hip_params_set_int(self._h, name.encode(), int(v)))
+
+    def get_array(self, name):
+        n = ctypes.c_size_t()
+        _check(lib().phi_hip_params_get_array(self._h, name.encode(), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value)
+        if n.value:
+            _check(lib().phi_hip_params_get_array(self._h, name.encode(), _dp(out), n.value, None))
+        return out
+
+    def set_array(self, name, v):
+        a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+        _check(lib().phi_hip_params_set_array(self._h, name.encode(), _dp(a), a.size))
+
+    def get_matrix(self, name):
+        """Complex matrix as an N x N array (m[i, j] = element (i, j)), or None."""
+        n = ctypes.c_size_t()
+        _check(lib().phi_hip_params_get_matrix(self._h, name.encode(), None, 0, ctypes.byref(n)))
+        if n.value == 0:
+            return None
+        out = np.zeros(2 * n.value)
+        _check(lib().phi_hip_params_get_matrix(self._h, name.encode(), _dp(out), n.value, None))
+        k = int(round(np.sqrt(n.value)))
+        return _from_colmajor(out, k)
+
+    def set_matrix(self, name, m):
+        flat = _colmajor(m)
+        _check(lib().phi_hip_params_set_matrix(self._h, name.encode(), _dp(flat), flat.size // 2))
+
+    def get_string(self, name):
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().phi_hip_params_get_string(self._h, name.encode(), buf, 4096))
+        return buf.value.decode()
+
+    def set_string(self, name, v):
+        _check(lib().phi_hip_params_set_string(self._h, name.encode(), v.encode()))
+
+    def fix(self, method=None):
+        """run_method from the CLI word, then FixIncompatibleCombinations."""
+        return lib().phi_hip_params_fix(self._h, method.encode() if method else None)
+
+    @property
+    def run_method(self):
+        return RUN_METHODS[self.get_int("run_method")]
+
+    def header(self, num_nodes):
+        n = ctypes.c_size_t()
+        _check(lib().phi_hip_params_header(self._h, num_nodes, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _check(lib().phi_hip_params_header(self._h, num_nodes, buf, n.value + 1, None))
+        return buf.value.decode()
+
+
+def validate(kind, text):
+    v = ctypes.c_int()
+    _check(lib().phi_hip_validate(kind.encode(), text.encode(), ctypes.byref(v)))
+    return bool(v.value)
+
+
+class HierarchyIntegrator:
+    """PHI's HierarchyIntegrator on one MI355X: Launch() + density_matrix()."""
+
+    def __init__(self, params):
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib().phi_hip_integrator_create(params._h, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().phi_hip_integrator_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def Launch(self, check=True):
+        rc = lib().phi_hip_integrator_launch(self._h)
+        if check:
+            _check(rc)
+        return rc
+
+    def density_matrix(self):
+        n = self.params.get_int("num_states")
+        out = np.zeros(2 * n * n)
+        _check(lib().phi_hip_integrator_density_matrix(self._h, _dp(out), n * n))
+        return _from_colmajor(out, n)
+
+    def info(self):
+        out = np.zeros(8)
+        _check(lib().phi_hip_integrator_info(self._h, _dp(out), 8))
+        keys = ["steps", "accepted", "final_time", "final_dt", "seconds",
+                "bicgstab_iterations", "bicgstab_residual", "num_nodes"]
+        return dict(zip(keys, out.tolist()))
+
+    def state(self):
+        info = self.info()
+        n = self.params.get_int("num_states")
+        nodes = int(info["num_nodes"])
+        out = np.zeros(2 * nodes * n * n)
+        _check(lib().phi_hip_integrator_state(self._h, _dp(out), nodes * n * n))
+        return out.view(complex).reshape(nodes, n, n).transpose(0, 2, 1).copy()
+
+
+class Context:
+    """Device-resident hierarchy (phi_hip_create): the hot path."""
+
+    def __init__(self, params, precision="fp64"):
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib().phi_hip_create(params._h, 1 if precision == "fp32" else 0, ctypes.byref(h)))
+        self._h = h
+        self.num_nodes = lib().phi_hip_num_nodes(h)
+        self.D = lib().phi_hip_num_modes(h)
+        self.N = params.get_int("num_states")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().phi_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_complex(self):
+        return self.num_nodes * self.N * self.N
+
+    def index(self):
+        out = np.zeros(self.num_nodes * self.D, dtype=np.int32)
+        _check(lib().phi_hip_get_index(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       out.size))
+        return out.reshape(self.num_nodes, self.D)
+
+    def _to_flat(self, x):
+        return np.ascontiguousarray(np.asarray(x, dtype=complex).transpose(0, 2, 1)).view(
+            np.float64).reshape(-1)
+
+    def _from_flat(self, f):
+        return f.view(complex).reshape(self.num_nodes, self.N, self.N).transpose(0, 2, 1).copy()
+
+    def set_state(self, rho):
+        f = self._to_flat(rho)
+        _check(lib().phi_hip_set_state(self._h, _dp(f), self.n_complex))
+
+    def state(self):
+        out = np.zeros(2 * self.n_complex)
+        _check(lib().phi_hip_get_state(self._h, _dp(out), self.n_complex))
+        return self._from_flat(out)
+
+    def rho0(self):
+        out = np.zeros(2 * self.N * self.N)
+        _check(lib().phi_hip_get_rho0(self._h, _dp(out), self.N * self.N))
+        return _from_colmajor(out, self.N)
+
+    def rhs(self, x):
+        f = self._to_flat(x)
+        out = np.zeros_like(f)
+        _check(lib().phi_hip_rhs(self._h, _dp(f), _dp(out), self.n_complex))
+        return self._from_flat(out)
+
+    def rk4_steps(self, nsteps, dt):
+        _check(lib().phi_hip_rk4_steps(self._h, int(nsteps), float(dt)))
+
+    def rk4_stage_times(self, nsteps, dt):
+        """Milliseconds per stage kernel (4 values), summed over nsteps."""
+        ms = np.zeros(4)
+        _check(lib().phi_hip_rk4_stage_times(self._h, int(nsteps), float(dt), _dp(ms)))
+        return ms
+
+    def rkf45_attempt(self, coefs):
+        c = np.ascontiguousarray(np.asarray(coefs, dtype=np.float64))
+        assert c.size == 24
+        err = ctypes.c_double()
+        _check(lib().phi_hip_rkf45_attempt(self._h, _dp(c), ctypes.byref(err)))
+        return err.value
+
+    def rkf45_accept(self):
+        _check(lib().phi_hip_rkf45_accept(self._h))
+
+    def steady_state(self, tol, max_iter):
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        rc = lib().phi_hip_steady_state(self._h, tol, max_iter, ctypes.byref(it), ctypes.byref(res))
+        return rc, it.value, res.value
+
+    def synchronize(self):
+        _check(lib().phi_hip_synchronize(self._h))
+
+
+def params_from_problem(H, rho0, gamma, lambda_, temperature, lh, num_terms=1,
+                        hbar=None, kb=None, dt=0.001, t_end=1.0, out_min=0.01,
+                        method="rk4", output=None, precision="fp64", verbose=-2,
+                        full_precision_output=None):
+    """PhiParameters set field by field, as PHI's own tests do
+    (hierarchy_integrator_test.cc:27-44), then FixIncompatibleCombinations."""
+    p = PhiParameters()
+    H = np.asarray(H, dtype=complex)
+    N = H.shape[0]
+    p.set_int("verbose", verbose)
+    if hbar is not None:
+        p.set_double("hbar", hbar)
+    if kb is not None:
+        p.set_double("boltzmann_constant", kb)
+    p.set_int("num_states", N)
+    p.set_int("hilbert_space_size", N)
+    p.set_int("hilbert_space_num_elements", N * N)
+    p.set_int("num_bath_couplings", len(gamma))
+    p.set_int("num_matsubara_terms", num_terms)
+    p.set_int("hierarchy_truncation_level", lh)
+    p.set_double("temperature", temperature)
+    p.set_array("gamma", gamma)
+    p.set_array("lambda", lambda_)
+    p.set_matrix("time_indepenent_hamiltonian", H)
+    p.set_matrix("initial_density_matrix", rho0)
+    p.set_double("timestep", dt)
+    p.set_double("integration_time", t_end)
+    p.set_double("output_minimum_timestep", out_min)
+    p.set_string("output_filename", output or "")
+    if full_precision_output:
+        p.set_string("full_precision_output", full_precision_output)
+    p.set_int("bulk_precision", 1 if precision == "fp32" else 0)
+    rc = p.fix(method)
+    if rc != 0:
+        raise PhiError(rc, lib().phi_hip_last_error().decode())
+    return p
+
+
+def run_cli(args, cwd=None, timeout=None):
+    """Run the `phi` program (returns CompletedProcess)."""
+    import subprocess
+    return subprocess.run([EXE_PATH] + list(args), cwd=cwd, stdout=subprocess.PIPE,
+                          stderr=subprocess.PIPE, text=True, timeout=timeout)

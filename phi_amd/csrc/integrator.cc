@@ -1,0 +1,615 @@
+// HierarchyIntegrator for the MI355X HEOM integrator.  See integrator.h.
+//
+// Host control logic restates PHI's loops (hierarchy_integrator.cc):
+//   RK4   2578-2939: while (time_now < end) { step; time_now += dt }, output
+//         after step i when i % ceil(OutputMinimumTimestep / dt) == 0.
+//   RKF45 1797-2573: Fehlberg weights pre-multiplied by the initial dt and
+//         rescaled in place; accept when err < tol or dt/2 < mindt; grow by
+//         1.1 when err < tol/10; the end clamp replaces the growth factor;
+//         shrink by 0.5 on reject.
+// The arithmetic of each step runs on the GPU (engine.cc / heom_kernels.hip).
+#include "integrator.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+
+namespace phi {
+
+namespace {
+
+double Now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+std::string MemoryTable(const PhiParameters& p, int mc) {
+  // 32-bit float sums in PHI's order; sizes: int 4, complex 16, double 8,
+  // time_t 8.
+  const long long n = p.num_states, m = p.num_bath_couplings, k = p.num_matsubara_terms;
+  const long long T = p.num_threads, ne = n * n, hsne = p.hilbert_space_num_elements,
+                  hs = p.hilbert_space_size;
+  const long long C = 16, I = 4, F = 8, TT = 8;
+  volatile float mem = 0;
+  mem = mem + (float)(I * (m + 4 + m + m + m + m + 5) + C * (1 + m + m + m + m + ne));
+  mem = mem * (float)(2 * mc);
+  mem = mem + (float)(C * (3 + ne) + I * 6 + TT * 2);
+  if (p.use_time_local_truncation) {
+    mem = mem + (float)(C * ne * T);
+    mem = mem + (float)(C * m * k * ne);
+  }
+  if (p.is_diagonal_bath_coupling) {
+    mem = mem + (float)(C * m * ne * (k + 2) * T);
+    mem = mem + (float)(C * m * hs);
+  }
+  if (p.is_full_bath_coupling) {
+    mem = mem + (float)(C * m * ne * 3 * T);
+    mem = mem + (float)(C * m * hsne);
+  }
+  mem = mem + (float)(C * hsne * T);
+  if (p.is_truncated_system) {
+    mem = mem + (float)(C * hs * T);
+    mem = mem + (float)(C * hsne * T);
+    mem = mem + (float)(I * 2 * T);
+    mem = mem + (float)(C * ne * T);
+  }
+  volatile float matrices = (float)(C * ne * mc);
+  matrices = matrices / 1024.0f;
+  matrices = matrices / 1024.0f;
+  volatile float rk4 = (float)(C * ne * mc * 4);
+  rk4 = rk4 + mem;
+  rk4 = rk4 / 1024.0f;
+  rk4 = rk4 / 1024.0f;
+  volatile float rkf = 0;
+  rkf = rkf + (float)(C * ne * mc * 6);
+  rkf = rkf + (float)(C * 31);
+  rkf = rkf + (float)(F * 7 + I * 2);
+  rkf = mem + rkf;
+  rkf = rkf / 1024.0f;
+  rkf = rkf / 1024.0f;
+  volatile float mem2 = 0;
+  mem2 = mem2 + (float)(I * (m + 4 + m + m + m + m + 5) + C * (1 + m + m + m + m + n));
+  mem2 = mem2 * (float)(2 * mc);
+  mem2 = mem2 + (float)(C * (3 + n) + I * 6 + TT * 2);
+  volatile float srk4 = 0;
+  srk4 = srk4 + (float)(C * (n + 1) * mc * 4);
+  srk4 = srk4 + mem2;
+  srk4 = srk4 / 1024.0f;
+  srk4 = srk4 / 1024.0f;
+  volatile float srkf = 0;
+  srkf = srkf + (float)(C * 3 * n * mc * 6);
+  srkf = srkf + (float)(C * 31);
+  srkf = srkf + (float)(F * 7 + I * 2);
+  srkf = srkf + mem2;
+  srkf = srkf / 1024.0f;
+  srkf = srkf / 1024.0f;
+  volatile float ss = 0;
+  ss = ss + (float)(I * (m + 4 + m + m + m + m + 5) + C * (1 + m + m + m + m));
+  ss = ss + (float)(6 * (C * (n * n)));
+  ss = ss * (float)mc;
+  ss = ss + (float)(9 * C + 3 * I + 2 * F);
+  ss = ss / 1024.0f;
+  ss = ss / 1024.0f;
+  std::ostringstream o;
+  o << "#Memory to store matrices for calculations:\n"
+    << "#hierarchy: " << (float)matrices << " Mb.\n"
+    << "#rk4: " << (float)rk4 << " Mb.\n"
+    << "#rkf45: " << (float)rkf << " Mb.\n"
+    << "#rk4spectrum: " << (float)srk4 << " Mb.\n"
+    << "#rkf45spectrum: " << (float)srkf << " Mb.\n"
+    << "#steadystate: " << (float)ss << " Mb.\n";
+  return o.str();
+}
+
+HierarchyIntegrator::HierarchyIntegrator(PhiParameters* p) : p_(p) {
+  if (p_->verbose > 0) {
+    std::cout << "PHI-compatible HEOM integrator on AMD Instinct MI355X (HIP, gfx950).\n"
+              << "Integration of density matrix using hierarchy equations of motion.\n"
+              << "\nPlease cite Strumpfer and Schulten, J. Chem. Theor. Comp. (2012)\n"
+              << "in all publications reporting results obtained with PHI\n\n";
+  }
+}
+
+HierarchyIntegrator::~HierarchyIntegrator() { delete engine_; }
+
+void HierarchyIntegrator::Fail(int code, const std::string& msg) {
+  info_.status = code;
+  error_ = msg;
+  std::cerr << "ERROR: " << msg << "\n";
+}
+
+std::complex<double>* HierarchyIntegrator::density_matrix() {
+  const int N = p_->num_states;
+  std::complex<double>* out = new std::complex<double>[(size_t)N * N];
+  if (have_final_) {
+    for (size_t i = 0; i < (size_t)N * N; ++i) out[i] = final_rho0_[i];
+  } else {
+    for (size_t i = 0; i < (size_t)N * N; ++i) out[i] = 0;
+  }
+  return out;
+}
+
+int HierarchyIntegrator::GetState(std::vector<std::complex<double>>* out) {
+  if (!engine_) return kErrInvalidArgument;
+  out->resize(engine_->state_elems());
+  return engine_->GetState(out->data());
+}
+
+void HierarchyIntegrator::Launch() {
+  info_ = RunInfo();
+  const RunMethod rm = p_->run_method;
+  if (!p_->are_parameters_valid || rm == BROKENINPUT) {
+    if (p_->verbose > -1) std::cout << "Parameter error.\n";
+    info_.status = kErrInvalidParameters;
+    error_ = "invalid parameters";
+  } else if (rm == RK4 || rm == RKF45 || rm == BICGSTAB || rm == BICGSTABL || rm == BICGSTABU) {
+    int rc = Initialize();
+    if (!rc) rc = InitialState();
+    if (!rc) {
+      if (rm == RK4) rc = RunRk4();
+      else if (rm == RKF45) rc = RunRkf45();
+      else rc = RunSteadyState();
+    }
+    if (rc && info_.status == 0) info_.status = rc;
+    // Launch() returns with the output complete on disk.
+    if (out_.is_open()) out_.close();
+    if (full_.is_open()) full_.close();
+  } else if (rm == PRINTMEMORY) {
+    std::cout << "Total number of matrices: " << p_->matrix_count << "\n";
+    std::cout << "Number of elements per matrix: " << p_->num_states * p_->num_states << "\n";
+    std::string t = MemoryTable(*p_, p_->matrix_count);
+    // Console form: no leading '#', indented entries.
+    std::istringstream in(t);
+    std::string line;
+    bool first = true;
+    while (std::getline(in, line)) {
+      if (first) { std::cout << line.substr(1) << "\n"; first = false; }
+      else std::cout << "\t" << line.substr(1) << "\n";
+    }
+  } else if (rm == PRINTHIERARCHY) {
+    HierarchyTables h;
+    std::string msg = BuildHierarchy(*p_, &h);
+    if (!msg.empty()) {
+      Fail(kErrInvalidParameters, msg);
+    } else {
+      std::ostringstream ss;
+      ss << "Hierarchy connections:\n";
+      for (int l = h.Lh - 1; l >= 0; --l) {
+        ss << "Level " << l << ":\n";
+        for (int i = h.tier_start[l + 1] - 1; i >= h.tier_start[l]; --i) {
+          ss << "\t" << i << " ->";
+          for (int j = 0; j < h.D; ++j)
+            if (h.prev[(size_t)i * h.D + j] >= 0) ss << " " << h.prev[(size_t)i * h.D + j];
+          ss << "\n";
+        }
+      }
+      std::cout << ss.str();
+    }
+  } else if (rm == RK4SPECTRUM || rm == RKF45SPECTRUM) {
+    Fail(kErrUnsupported, "spectrum calculations are not part of this integrator");
+  }
+  if (p_->verbose > -2) std::cout << "Done.\n";
+}
+
+int HierarchyIntegrator::Initialize() {
+  const PhiParameters& p = *p_;
+  if (p.time_indepenent_hamiltonian.empty()) {
+    Fail(kErrUnsupported, "time-dependent (annealing) Hamiltonians are not supported; "
+                          "give a Hamiltonian");
+    return info_.status;
+  }
+  if (p.is_diagonal_bath_coupling || p.is_full_bath_coupling) {
+    Fail(kErrUnsupported, "only independent (site projector) baths are supported");
+    return info_.status;
+  }
+  if (p.use_time_local_truncation) {
+    Fail(kErrUnsupported, "time-local truncation is not supported");
+    return info_.status;
+  }
+  if (p.is_truncated_system) {
+    Fail(kErrUnsupported, "HilbertSpaceSize > NumStates is not supported");
+    return info_.status;
+  }
+  if (p.filter_tolerance > 0) {
+    Fail(kErrUnsupported, "FilterTolerance (adaptive truncation) is not supported");
+    return info_.status;
+  }
+  std::string why;
+  int rc = Engine::Create(p, &engine_, &why);
+  if (rc) {
+    Fail(rc, why);
+    return rc;
+  }
+  info_.num_nodes = engine_->num_nodes();
+  if (p.verbose > 0)
+    std::cout << "\nTotal: " << engine_->num_nodes() << " matrices\n\n";
+  WriteHeader();
+  if (!p.full_precision_output.empty()) full_.open(p.full_precision_output.c_str());
+  return kOk;
+}
+
+void HierarchyIntegrator::WriteHeader() {
+  if (p_->output_filename.empty()) return;
+  out_.open(p_->output_filename.c_str());
+  if (!out_.is_open()) return;
+  p_->Write(&out_, engine_->num_nodes());
+  out_ << "#Total Number of matrices:" << engine_->num_nodes() << std::endl;
+  out_ << MemoryTable(*p_, engine_->num_nodes());
+  out_ << "#-------------\n";
+}
+
+void HierarchyIntegrator::FullPrecisionLine(double t, const std::complex<double>* r) {
+  if (!full_.is_open()) return;
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%.17g", t);
+  full_ << buf;
+  const int ne = p_->num_states * p_->num_states;
+  for (int i = 0; i < ne; ++i) {
+    std::snprintf(buf, sizeof(buf), " %.17g %.17g", r[i].real(), r[i].imag());
+    full_ << buf;
+  }
+  full_ << "\n";
+}
+
+int HierarchyIntegrator::InitialState() {
+  const PhiParameters& p = *p_;
+  const int N = p.num_states, NE = N * N;
+  std::vector<std::complex<double>> state(engine_->state_elems(), 0);
+  if (p.is_restarted) {
+    double t = 0, dt = 0;
+    int rc = ReadRestart(&t, &dt);
+    if (rc) return rc;
+    p_->start_time = t;
+    p_->timestep = dt;
+    return kOk;
+  }
+  if ((int)p.initial_density_matrix.size() != NE) {
+    Fail(kErrUnsupported, "InitialDensityMatrix is required");
+    return info_.status;
+  }
+  // ADOs stay Hermitian only from a Hermitian start; the integrator relies
+  // on it (rho_n^dagger obeys the same equations).
+  double scale = 0, asym = 0;
+  for (int a = 0; a < N; ++a)
+    for (int b = 0; b < N; ++b) {
+      const std::complex<double> x = p.initial_density_matrix[a + (size_t)b * N];
+      const std::complex<double> y = p.initial_density_matrix[b + (size_t)a * N];
+      scale = std::max(scale, std::abs(x));
+      asym = std::max(asym, std::abs(x - std::conj(y)));
+    }
+  if (asym > 1e-12 * std::max(1.0, scale)) {
+    Fail(kErrInvalidParameters, "InitialDensityMatrix must be Hermitian");
+    return info_.status;
+  }
+  // InitializeDensityMatrix (hierarchy_integrator.cc:1070-1091): node 0 gets
+  // rho0 / scale with scale = 1 at n = 0; all other ADOs start at zero.
+  for (int e = 0; e < NE; ++e) state[e] = p.initial_density_matrix[e];
+  int rc = engine_->SetState(state.data());
+  if (rc) {
+    Fail(rc, engine_->last_error());
+    return rc;
+  }
+  if (out_.is_open()) {
+    out_ << p.start_time << " ";
+    for (int e = 0; e < NE; ++e) out_ << state[e] << " ";
+    out_ << "\n" << std::flush;
+  }
+  FullPrecisionLine(p.start_time, state.data());
+  return kOk;
+}
+
+void HierarchyIntegrator::WriteRho0Line(const std::complex<double>* r, double t) {
+  if (!out_.is_open()) return;
+  out_ << t;
+  const int ne = p_->num_states * p_->num_states;
+  for (int j = 0; j < ne; ++j) out_ << " " << r[j];
+  out_ << "\n";
+}
+
+void HierarchyIntegrator::WriteFinal(const std::complex<double>* r, double t) {
+  final_rho0_.assign(r, r + (size_t)p_->num_states * p_->num_states);
+  have_final_ = true;
+  if (!out_.is_open()) return;
+  out_ << std::setw(20) << std::setprecision(6) << t << std::setw(6) << std::setprecision(4);
+  const int ne = p_->num_states * p_->num_states;
+  for (int j = 0; j < ne; ++j) out_ << " " << r[j];
+  out_ << "\n";
+}
+
+int HierarchyIntegrator::RunRk4() {
+  const PhiParameters& p = *p_;
+  const int NE = p.num_states * p.num_states;
+  const double dt = p.timestep;
+  const double end_time = p.integration_time;
+  const int output_factor = (int)std::ceil(p.output_minimum_timestep / dt);
+  // Step count and times exactly as PHI accumulates them (F7: time_now +=
+  // dt in double until time_now >= end).
+  std::vector<double> times;
+  {
+    double t = p.start_time;
+    while (t < end_time) {
+      t += dt;
+      times.push_back(t);
+    }
+  }
+  const long long steps = (long long)times.size();
+  if (p.verbose > -1) std::cout << "\tIntegrating using RK4.\n";
+  const double t0 = Now();
+  // Chunks sized so the node-0 history never overflows.
+  const long long chunk = (long long)output_factor * 2048;
+  std::vector<std::complex<double>> hist;
+  for (long long s = 0; s < steps; s += chunk) {
+    const long long n = std::min(chunk, steps - s);
+    int rc = engine_->Rk4Steps(n, dt, s, output_factor);
+    if (!rc) rc = engine_->DrainHistory(&hist);
+    if (rc < 0) {
+      Fail(rc, engine_->last_error());
+      return rc;
+    }
+    const double el = Now() - t0;
+    long long k = 0;
+    for (long long i = s; i < s + n; ++i) {
+      if (i % output_factor != 0) continue;
+      if (out_.is_open() && i % 100 == 0) {
+        const double per = el / (double)(s + n);
+        out_ << "#Time for 1st timestep: " << per << " s.\n"
+             << "#Estimated time for 1 time unit: " << per / dt / 60 / 60 << " hrs.\n"
+             << "#Total running time: " << 0 << " days.\n";
+      }
+      const std::complex<double>* r = hist.data() + (size_t)k * NE;
+      WriteRho0Line(r, times[i]);
+      FullPrecisionLine(times[i], r);
+      ++k;
+    }
+  }
+  int rc = engine_->Synchronize();
+  if (rc) {
+    Fail(rc, engine_->last_error());
+    return rc;
+  }
+  info_.seconds = Now() - t0;
+  info_.steps = steps;
+  info_.final_time = steps ? times.back() : p.start_time;
+  info_.final_dt = dt;
+  std::vector<std::complex<double>> r0(NE);
+  rc = engine_->GetNode0(r0.data());
+  if (rc) {
+    Fail(rc, engine_->last_error());
+    return rc;
+  }
+  if (p.verbose > -1) {
+    std::ostringstream ss;
+    ss << "#Total number of integration steps = " << steps << "\n"
+       << "#Total integration time = " << info_.seconds / 60 << " mins\n";
+    std::cout << ss.str();
+    if (out_.is_open()) out_ << ss.str();
+  }
+  WriteFinal(r0.data(), info_.final_time);
+  FullPrecisionLine(info_.final_time, r0.data());
+  if (p.restart_output_time > 0) return WriteRestart(info_.final_time, dt);
+  return kOk;
+}
+
+int HierarchyIntegrator::RunRkf45() {
+  const PhiParameters& p = *p_;
+  const int NE = p.num_states * p.num_states;
+  const double end_time = p.integration_time;
+  const double tol = p.rkf45_tolerance, mindt = p.rkf45_minimum_timestep;
+  double mydt = p.timestep, dt_f = p.timestep;
+  double dtup = 1.1;
+  const double dtdn = 0.5;
+  // Fehlberg weights (hierarchy_integrator.cc:1926-1975), each pre-multiplied
+  // by the initial dt and then rescaled in place exactly as PHI does.
+  double b[5][5] = {{2. / 9., 0, 0, 0, 0},
+                    {1. / 12., 1. / 4., 0, 0, 0},
+                    {69. / 128., -243. / 128., 135. / 64., 0, 0},
+                    {-17. / 12., 27. / 4., -27. / 5, 16. / 15, 0},
+                    {65. / 432., -5. / 16, 13. / 16., 4. / 27., 5. / 144.}};
+  double c4[4] = {1. / 9, 9. / 20, 16. / 45, 1. / 12};
+  double c5[5] = {47. / 450, 12. / 25, 32. / 225, 1. / 30, 6. / 25};
+  auto scale_all = [&](double f) {
+    for (int s = 0; s < 5; ++s)
+      for (int i = 0; i <= s; ++i) b[s][i] *= f;
+    for (double& v : c4) v *= f;
+    for (double& v : c5) v *= f;
+  };
+  scale_all(dt_f);
+  double time_now = p.start_time;
+  double next_write_time = time_now;
+  long long i = -1, accepted = 0, inext = 0;
+  std::vector<std::complex<double>> r0(NE);
+  const double t0 = Now();
+  if (p.verbose > -1) std::cout << "\tIntegrating using adaptive timestepping RKF45.\n";
+  while (time_now < end_time) {
+    i += 1;
+    Rkf45Coefs cf{};
+    for (int s = 0; s < 5; ++s)
+      for (int k = 0; k <= s; ++k) cf.b[s][k] = b[s][k];
+    for (int k = 0; k < 4; ++k) cf.c4[k] = c4[k];
+    for (int k = 0; k < 5; ++k) cf.c5[k] = c5[k];
+    double max_diff = 0;
+    int rc = engine_->Rkf45Attempt(cf, &max_diff);
+    if (rc) {
+      Fail(rc, engine_->last_error());
+      return rc;
+    }
+    if (max_diff < tol || dt_f * dtdn < mindt) {
+      engine_->Rkf45Accept();
+      time_now += mydt;
+      accepted += 1;
+      bool update_dt = max_diff < tol / 10;
+      if (time_now + mydt > end_time) {
+        update_dt = true;
+        dtup = (end_time - time_now) / mydt;
+      }
+      if (update_dt) {
+        mydt *= dtup;
+        scale_all(dtup);
+      }
+      dt_f = mydt;
+      // PHI's writer cadence is undefined (it tests an uninitialised wall
+      // clock); here a line is written after each accepted step at or past
+      // next_write_time, which then advances by OutputMinimumTimestep.
+      if (out_.is_open() && i >= inext) {
+        const double el = Now() - t0;
+        out_ << "#Info: Time for single timestep: " << el / (i + 1) << " s.\n"
+             << "#Info: Estimated time for 1 time unit: "
+             << (time_now > p.start_time ? el / (time_now - p.start_time) / 3600 : 0) << " hrs.\n"
+             << "#Info: Total running time: "
+             << (time_now > p.start_time ? el * end_time / (time_now - p.start_time) / 86400 : 0)
+             << " days.\n";
+        inext += 100;
+      }
+      if (time_now >= next_write_time) {
+        rc = engine_->GetNode0(r0.data());
+        if (rc) {
+          Fail(rc, engine_->last_error());
+          return rc;
+        }
+        if (out_.is_open()) {
+          out_ << std::setw(20) << std::setprecision(6) << time_now << std::setw(6)
+               << std::setprecision(4);
+          for (int j = 0; j < NE; ++j) out_ << " " << r0[j];
+          out_ << std::endl;
+        }
+        FullPrecisionLine(time_now, r0.data());
+        next_write_time = time_now + p.output_minimum_timestep;
+      }
+    } else {
+      mydt *= dtdn;
+      scale_all(dtdn);
+      dt_f = mydt;
+      if (dt_f < 1e-10) std::cout << "WARNING!!! dt < 1e-10\n";
+    }
+  }
+  info_.seconds = Now() - t0;
+  info_.steps = i + 1;
+  info_.accepted = accepted;
+  info_.final_time = time_now;
+  info_.final_dt = dt_f;
+  int rc = engine_->GetNode0(r0.data());
+  if (rc) {
+    Fail(rc, engine_->last_error());
+    return rc;
+  }
+  if (p.verbose > -1) {
+    std::ostringstream ss;
+    ss << "#Total number of integration steps = " << i + 1 << "\n"
+       << "#Total number of accepted integration steps = " << accepted << "\n"
+       << "#Total integration time = " << info_.seconds / 60 << " mins.\n";
+    std::cout << ss.str();
+    if (out_.is_open()) out_ << ss.str();
+  }
+  WriteFinal(r0.data(), time_now);
+  FullPrecisionLine(time_now, r0.data());
+  if (p.restart_output_time > 0) return WriteRestart(time_now, dt_f);
+  return kOk;
+}
+
+int HierarchyIntegrator::RunSteadyState() {
+  const PhiParameters& p = *p_;
+  const int N = p.num_states, NE = N * N;
+  int iters = 0;
+  double resid = 0;
+  std::vector<double> hist;
+  const double t0 = Now();
+  int rc = engine_->SteadyState(p.bicgstab_tolerance, p.bicgstab_max_iterations, &iters, &resid,
+                                &hist);
+  info_.seconds = Now() - t0;
+  info_.bicgstab_iterations = iters;
+  info_.bicgstab_residual = resid;
+  if (rc && rc != kErrNotConverged) {
+    Fail(rc, engine_->last_error());
+    return rc;
+  }
+  std::vector<std::complex<double>> r0(NE);
+  int rc2 = engine_->GetNode0(r0.data());
+  if (rc2) {
+    Fail(rc2, engine_->last_error());
+    return rc2;
+  }
+  std::ostringstream ss;
+  if (rc == kErrNotConverged)
+    ss << "#Convergence failed! (steps = " << iters << ", diff = " << resid << ")\n";
+  else
+    ss << "Converged to diff = " << resid << " in " << iters << " steps.\n";
+  std::cout << ss.str();
+  if (out_.is_open()) {
+    for (size_t k = 0; k < hist.size(); ++k) out_ << "#BiCGSTAB " << k << " " << hist[k] << "\n";
+    out_ << ss.str();
+    out_ << "#Info: Total running time: " << info_.seconds / 86400 << " days.\n";
+    out_ << resid << " ";
+    for (int j = 0; j < NE; ++j) out_ << r0[j] << " ";
+    out_ << std::endl;
+  }
+  final_rho0_ = r0;
+  have_final_ = true;
+  FullPrecisionLine((double)iters, r0.data());
+  if (p.restart_output_time > 0) {
+    int w = WriteRestart((double)iters, 0);
+    if (w) return w;
+  }
+  if (rc == kErrNotConverged) {
+    info_.status = kErrNotConverged;
+    error_ = "BiCGSTAB did not converge";
+  }
+  return rc;
+}
+
+// Restart files: PHI's text layout (restart_helper.cc:30-55) - time, dt, then
+// one line per node in id order holding its N*N column-major elements as
+// "(re,im)" - written at 17 significant digits so that a resumed run is
+// lossless; PHI's own reader (operator>>) reads it unchanged.
+int HierarchyIntegrator::WriteRestart(double t, double dt) {
+  std::vector<std::complex<double>> s;
+  int rc = GetState(&s);
+  if (rc) {
+    Fail(rc, engine_->last_error());
+    return rc;
+  }
+  std::ofstream o(p_->restart_output_filename.c_str());
+  if (!o.is_open()) {
+    Fail(kErrIo, "cannot write restart file " + p_->restart_output_filename);
+    return kErrIo;
+  }
+  o << std::setprecision(17) << t << "\n" << dt << "\n";
+  const size_t ne = (size_t)p_->num_states * p_->num_states;
+  for (int n = 0; n < engine_->num_nodes(); ++n) {
+    for (size_t i = 0; i < ne; ++i) o << (i ? " " : "") << s[n * ne + i];
+    o << "\n";
+  }
+  return o.good() ? kOk : kErrIo;
+}
+
+int HierarchyIntegrator::ReadRestart(double* t, double* dt) {
+  std::ifstream in(p_->restart_input_filename.c_str());
+  if (!in.is_open()) {
+    Fail(kErrIo, "cannot read restart file " + p_->restart_input_filename);
+    return kErrIo;
+  }
+  in >> *t >> *dt;
+  std::vector<std::complex<double>> s(engine_->state_elems());
+  for (auto& v : s) in >> v;
+  if (!in) {
+    Fail(kErrIo, "restart file " + p_->restart_input_filename + " is truncated or malformed");
+    return kErrIo;
+  }
+  int rc = engine_->SetState(s.data());
+  if (rc) {
+    Fail(rc, engine_->last_error());
+    return rc;
+  }
+  if (out_.is_open()) {
+    out_ << *t << " ";
+    for (int e = 0; e < p_->num_states * p_->num_states; ++e) out_ << s[e] << " ";
+    out_ << "\n" << std::flush;
+  }
+  FullPrecisionLine(*t, s.data());
+  return kOk;
+}
+
+}  // namespace phi

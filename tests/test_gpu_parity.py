@@ -1,0 +1,211 @@
+"""GPU parity of the HIP hot path against the CPU oracle and PHI's goldens.
+
+Every call goes through the C ABI (libphi_hip.so).  Bars:
+  * FP64 path vs oracle: max |delta rho| <= 1e-10 over the run (north star);
+  * PHI's own integration goldens (hierarchy_integrator_test.cc:316-345):
+    RK4 within 1e-6, RKF45 within 1e-3, plus Hermiticity / trace checks
+    (hierarchy_integrator_test.cc:253-282);
+  * FP32-bulk RK4: populations within 1e-6 of FP64.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import phi_amd as P  # noqa: E402
+from oracle import heom_oracle as HO  # noqa: E402
+from oracle import prm_oracle as PO  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+CFG = os.path.join(ROOT, "phi_amd", "configs")
+
+H2 = np.array([[0.5, 0.5], [0.5, -0.5]])
+RHO0_2 = np.array([[1, 0], [0, 0]], dtype=complex)
+
+
+def colmajor_golden(vals):
+    return np.array(vals, dtype=complex).reshape(2, 2).T
+
+
+# hierarchy_integrator_test.cc:347-468 (IndependentBath goldens).
+KATS = {
+    "vacuum": (dict(gamma=[1.0], lambda_=[0.01], temperature=2.0, lh=1),
+               colmajor_golden([0.788986, 0.211014 - 0.349228j, 0.211014 + 0.349228j, 0.211014])),
+    "weak": (dict(gamma=[1.0, 1.0], lambda_=[0.01, 0.01], temperature=2.0, lh=2),
+             colmajor_golden([0.790012, 0.206755 - 0.346119j, 0.206755 + 0.346119j, 0.209988])),
+    "matsubara": (dict(gamma=[1.0, 1.0], lambda_=[0.01, 0.01], temperature=0.5, lh=3, num_terms=3),
+                  colmajor_golden([0.789331, 0.208929 - 0.348365j, 0.208929 + 0.348365j,
+                                   0.210669])),
+}
+
+
+def assert_valid_density_matrix(r):
+    """hierarchy_integrator_test.cc:253-282."""
+    n = r.shape[0]
+    assert np.all(np.abs(np.imag(np.diag(r))) < 1e-9)
+    assert np.all(np.real(np.diag(r)) >= 0)
+    assert np.all(np.real(np.diag(r)) < 1.00001)
+    assert np.allclose(r, r.conj().T, atol=1e-9)
+    tr = np.real(np.trace(r))
+    assert 0 <= tr < 1.00001
+
+
+@pytest.mark.parametrize("name", sorted(KATS))
+@pytest.mark.parametrize("method,tol", [("rk4", 1e-6), ("rkf45", 1e-3)])
+def test_reference_goldens(name, method, tol):
+    kw, gold = KATS[name]
+    print("case", name, method, flush=True)
+    p = P.params_from_problem(H2, RHO0_2, hbar=1.0, kb=1.0, t_end=1.0, method=method, **kw)
+    it = P.HierarchyIntegrator(p)
+    it.Launch()
+    r = it.density_matrix()
+    assert_valid_density_matrix(r)
+    assert np.abs(r - gold).max() < tol
+
+
+def _oracle_from_prm(path, threads=4):
+    op = PO.read_command_line(path, "rk4")
+    return op, HO.Oracle(HO.Problem.from_prm(op), threads=threads)
+
+
+def test_rhs_matches_oracle_c1():
+    """Batched Get on random Hermitian and non-Hermitian input."""
+    path = os.path.join(CFG, "c1.prm")
+    op, orc = _oracle_from_prm(path)
+    p = P.PhiParameters.read(path, "rk4")
+    ctx = P.Context(p)
+    assert ctx.num_nodes == orc.num_nodes == 495
+    assert np.array_equal(ctx.index(), orc.index())
+    rng = np.random.default_rng(11)
+    for herm in (True, False):
+        x = rng.normal(size=(495, 2, 2)) + 1j * rng.normal(size=(495, 2, 2))
+        if herm:
+            x = x + x.conj().transpose(0, 2, 1)
+        y_gpu = ctx.rhs(x)
+        y_ref = orc.rhs(x)
+        scale = np.abs(y_ref).max()
+        assert np.abs(y_gpu - y_ref).max() <= 1e-12 * scale
+
+
+def test_rhs_matches_oracle_c2_subset():
+    """7-site system at depth 4 (330 nodes), general input."""
+    path = os.path.join(CFG, "c2.prm")
+    op = PO.read_command_line(path, "rk4")
+    op.hierarchy_truncation_level = 5
+    orc = HO.Oracle(HO.Problem.from_prm(op), threads=4)
+    p = P.PhiParameters.read(path, "rk4")
+    p.set_int("hierarchy_truncation_level", 5)
+    ctx = P.Context(p)
+    assert ctx.num_nodes == orc.num_nodes == 330
+    assert np.array_equal(ctx.index(), orc.index())
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(330, 7, 7)) + 1j * rng.normal(size=(330, 7, 7))
+    y_gpu = ctx.rhs(x)
+    y_ref = orc.rhs(x)
+    assert np.abs(y_gpu - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+
+
+def test_rk4_c1_full_run_fp64(tmp_path):
+    """c1 (495 nodes, 1000 RK4 steps): every output time within 1e-10."""
+    path = os.path.join(CFG, "c1.prm")
+    op, orc = _oracle_from_prm(path, threads=1)
+    steps, ts, hist = orc.rk4(op.integration_time, op.timestep, op.output_minimum_timestep)
+    full = tmp_path / "c1.full"
+    out = tmp_path / "c1.out"
+    p = P.PhiParameters.read(path, "rk4")
+    p.set_string("output_filename", str(out))
+    p.set_string("full_precision_output", str(full))
+    p.set_int("verbose", -2)
+    it = P.HierarchyIntegrator(p)
+    it.Launch()
+    info = it.info()
+    assert info["steps"] == steps == 1000
+    rows = np.loadtxt(full)
+    # initial line, 100 in-loop lines, final line
+    assert rows.shape[0] == len(ts) + 2
+    t_gpu = rows[1:-1, 0]
+    assert np.array_equal(t_gpu, ts)
+    r_gpu = rows[1:-1, 1:].view(complex).reshape(-1, 2, 2).transpose(0, 2, 1)
+    assert np.abs(r_gpu - hist).max() <= 1e-10
+    # Whole hierarchy at the end.
+    assert np.abs(it.state() - orc.state()).max() <= 1e-10
+    # PHI-format text output: header equals the oracle's, one line per output.
+    text = out.read_text()
+    hdr = PO.full_header(op, orc.num_nodes)
+    assert text.startswith(hdr.replace("#Threads=1", "#Threads=%d" % op.num_threads))
+
+
+def test_rk4_c2_short_fp64_and_fp32():
+    """7-site exciton, depth 8 (6435 nodes): 40 RK4 steps, FP64 within 1e-10;
+    FP32-bulk populations within 1e-6 of FP64."""
+    path = os.path.join(CFG, "c2.prm")
+    op, orc = _oracle_from_prm(path, threads=8)
+    op_steps = 40
+    orc.rk4(op_steps * op.timestep * (1 - 1e-9), op.timestep, 1.0)
+    ref = orc.state()
+    p = P.PhiParameters.read(path, "rk4")
+    for prec, tol in (("fp64", 1e-10), ("fp32", None)):
+        ctx = P.Context(p, precision=prec)
+        ctx.rk4_steps(op_steps, op.timestep)
+        s = ctx.state()
+        if tol is not None:
+            assert np.abs(s - ref).max() <= tol
+        else:
+            pop_err = np.abs(np.real(np.diag(s[0])) - np.real(np.diag(ref[0]))).max()
+            print("fp32 population error", pop_err)
+            assert pop_err <= 1e-6
+        ctx.close()
+
+
+def test_rkf45_matches_oracle_controller():
+    """RKF45 on the weak-bath golden problem and on c1 for 0.2 ps: same
+    attempted/accepted counts and final state within 1e-10."""
+    cases = []
+    kw, _ = KATS["matsubara"]
+    cases.append((dict(H=H2, rho0=RHO0_2, hbar=1.0, kb=1.0, **kw), 1.0, 1e-6, 1e-6))
+    op = PO.read_command_line(os.path.join(CFG, "c1.prm"), "rkf45")
+    cases.append((dict(H=op.time_indepenent_hamiltonian, rho0=op.initial_density_matrix,
+                       gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature,
+                       lh=op.hierarchy_truncation_level, num_terms=op.num_matsubara_terms),
+                  0.2, 1e-8, 1e-6))
+    for kw, t_end, tol, mindt in cases:
+        prob = HO.Problem(**{k: v for k, v in kw.items()})
+        orc = HO.Oracle(prob, threads=1)
+        res = orc.rkf45(t_end, 0.001, tol, mindt, 0.01)
+        pkw = dict(kw)
+        H = pkw.pop("H")
+        rho0 = pkw.pop("rho0")
+        p = P.params_from_problem(H, rho0, t_end=t_end, method="rkf45", **pkw)
+        p.set_double("rkf45_tolerance", tol)
+        p.set_double("rkf45_minimum_timestep", mindt)
+        it = P.HierarchyIntegrator(p)
+        it.Launch()
+        info = it.info()
+        assert info["steps"] == res["attempted"]
+        assert info["accepted"] == res["accepted"]
+        assert info["final_time"] == res["time"]
+        assert np.abs(it.state() - orc.state()).max() <= 1e-10
+
+
+def test_steady_state_c1_small():
+    """Steady state of a dimer at depth 3: BiCGSTAB residual below tol, the
+    solution is a fixed point of the oracle's operator and has unit trace."""
+    op = PO.read_command_line(os.path.join(CFG, "c1.prm"), "bicgstab")
+    kw = dict(gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature, lh=4,
+              num_terms=op.num_matsubara_terms)
+    p = P.params_from_problem(op.time_indepenent_hamiltonian, op.initial_density_matrix,
+                              method="bicgstab", **kw)
+    ctx = P.Context(p)
+    rc, iters, res = ctx.steady_state(1e-11, 5000)
+    assert rc == 0, (iters, res)
+    x = ctx.state()
+    orc = HO.Oracle(HO.Problem(op.time_indepenent_hamiltonian, op.initial_density_matrix, **kw))
+    Lx = orc.rhs(x)
+    assert abs(np.trace(x[0]) - 1) < 1e-10
+    assert np.abs(Lx).max() < 1e-8
