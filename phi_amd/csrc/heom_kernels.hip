@@ -55,92 +55,165 @@ __device__ __forceinline__ void st(T* __restrict__ p, size_t e, double re, doubl
   reinterpret_cast<V*>(p)[e] = v;
 }
 
-// Block-level Hermitian RHS for the tile of nodes [base, base + nt).
-// On return At holds Y' of each node (tile-local, column-major) and the
-// caller forms k(a,b) = At(a,b) + conj(At(b,a)).
-template <typename TB, typename TI>
-__device__ void rhs_tile(const RhsArgs& A, const TI* __restrict__ in, int base, int nt,
-                         TB* __restrict__ Hs, TB* __restrict__ Yt, TB* __restrict__ At) {
+// --- Hermitian RHS on a tile of nodes -----------------------------------------
+// A block owns the nodes [base, base + nt).  Its LDS holds, per launch:
+//   H (N*N), the tile's stage input Y and the work buffer W (nt*N*N each),
+//   the edge-prefactor tables (D*L), Delta (N), mu/2 (nt), the site->mode
+//   CSR and the tile's neighbour ids and occupations (nt*D each).
+// Every one of these loads, and the epilogue operands each thread keeps in
+// registers, is issued before the first barrier; the only dependent global
+// round trip after it is the neighbour-column gather.
+template <typename TB>
+struct Lds {
   typedef typename C2<TB>::type V;
+  V* H;
+  V* Y;
+  V* W;
+  V* pcol;
+  TB* ns;
+  TB* delta;
+  TB* mu2;
+  int* sp;
+  int* sm;
+  int* nx;
+  int* pv;
+  uint8_t* oc;
+};
+
+__host__ __device__ inline size_t Align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// Byte size of the LDS layout above (host and device agree on it).
+__host__ __device__ inline size_t LdsBytes(int N, int D, int L, int tile, size_t real) {
+  const size_t NE = (size_t)N * N, V = 2 * real;
+  size_t b = V * (NE + 2 * NE * tile + (size_t)D * L);
+  b = Align16(b + real * ((size_t)D * L + N + tile));
+  b = Align16(b + sizeof(int) * ((size_t)N + 1 + D + 2 * (size_t)tile * D));
+  b = Align16(b + (size_t)tile * D);
+  return b;
+}
+
+template <typename TB>
+__device__ __forceinline__ Lds<TB> carve(unsigned char* smem, int N, int D, int L, int tile) {
+  typedef typename C2<TB>::type V;
+  const int NE = N * N;
+  Lds<TB> s;
+  s.H = reinterpret_cast<V*>(smem);
+  s.Y = s.H + NE;
+  s.W = s.Y + NE * tile;
+  s.pcol = s.W + NE * tile;
+  s.ns = reinterpret_cast<TB*>(s.pcol + D * L);
+  s.delta = s.ns + D * L;
+  s.mu2 = s.delta + N;
+  unsigned char* p = reinterpret_cast<unsigned char*>(s.mu2 + tile);
+  p = smem + Align16(p - smem);
+  s.sp = reinterpret_cast<int*>(p);
+  s.sm = s.sp + N + 1;
+  s.nx = s.sm + D;
+  s.pv = s.nx + tile * D;
+  p = reinterpret_cast<unsigned char*>(s.pv + tile * D);
+  p = smem + Align16(p - smem);
+  s.oc = p;
+  return s;
+}
+
+// Phase 0: cooperative loads of everything the tile needs into LDS.
+template <typename TB, typename TI>
+__device__ __forceinline__ void load_tile(const RhsArgs& A, const TI* __restrict__ in, int base,
+                                          int nt, const Lds<TB>& s) {
   const int N = A.N, NE = A.NE, D = A.D, L = A.L;
   const int tid = threadIdx.x, nth = blockDim.x;
-  V* Hv = reinterpret_cast<V*>(Hs);
-  V* Yv = reinterpret_cast<V*>(Yt);
-  V* Av = reinterpret_cast<V*>(At);
-  for (int e = tid; e < NE; e += nth) Hv[e] = ld<TB, double>(reinterpret_cast<const double*>(A.H), e);
-  const int tot = nt * NE;
+  for (int e = tid; e < NE; e += nth)
+    s.H[e] = ld<TB, double>(reinterpret_cast<const double*>(A.H), e);
   const size_t off = (size_t)base * NE;
-  for (int e = tid; e < tot; e += nth) Yv[e] = ld<TB, TI>(in, off + e);
-  // Column gather A(:, m) for every node of the tile.
-  for (int e = tid; e < tot; e += nth) {
-    const int t = e / NE;
-    const int r = e - t * NE;
-    const int m = r / N;
-    const int a = r - m * N;
-    const int n = base + t;
-    TB ax = 0, ay = 0;
-    if (!A.vacuum) {
-      for (int q = A.site_ptr[m]; q < A.site_ptr[m + 1]; ++q) {
-        const int j = A.site_modes[q];
-        const int k = A.occ[(size_t)n * D + j];
-        const int nx = A.next[(size_t)n * D + j];
-        if (nx >= 0) {
-          const TB s = (TB)A.next_s[j * L + k];
-          const V y = ld<TB, TI>(in, (size_t)nx * NE + m * N + a);
-          // (i s) * y
-          ax -= s * y.y;
-          ay += s * y.x;
-        }
-        const int pv = A.prev[(size_t)n * D + j];
-        if (pv >= 0) {
-          const double2 cc = A.prev_col[j * L + k];
-          const TB cr = (TB)cc.x, ci = (TB)cc.y;
-          const V y = ld<TB, TI>(in, (size_t)pv * NE + m * N + a);
-          ax -= cr * y.x - ci * y.y;
-          ay -= cr * y.y + ci * y.x;
-        }
-      }
-    }
-    Av[e] = mk<TB>(ax, ay);
+  for (int e = tid; e < nt * NE; e += nth) s.Y[e] = ld<TB, TI>(in, off + e);
+  for (int e = tid; e < D * L; e += nth) {
+    s.pcol[e] = ld<TB, double>(reinterpret_cast<const double*>(A.prev_col), e);
+    s.ns[e] = (TB)A.next_s[e];
   }
-  __syncthreads();
+  for (int e = tid; e < N; e += nth) s.delta[e] = (TB)A.delta[e];
+  for (int e = tid; e < nt; e += nth) s.mu2[e] = (TB)(0.5 * A.mu[base + e]);
+  for (int e = tid; e <= N; e += nth) s.sp[e] = A.site_ptr[e];
+  for (int e = tid; e < D; e += nth) s.sm[e] = A.site_modes[e];
+  const size_t o = (size_t)base * D;
+  for (int e = tid; e < nt * D; e += nth) {
+    s.nx[e] = A.next[o + e];
+    s.pv[e] = A.prev[o + e];
+    s.oc[e] = A.occ[o + e];
+  }
+}
+
+// Phase 1: W(a,b) = Y'(a,b) = -(i/hbar) (H Y)(a,b) - (mu/2 + (Delta_a +
+// Delta_b)/2 [a != b]) Y(a,b) + A(a,b), where column b of A gathers, over the
+// modes j coupled to site b, i s_j Y_{n+e_j}(a,b) - col_j Y_{n-e_j}(a,b).
+template <typename TB, typename TI>
+__device__ __forceinline__ void rhs_phase(const RhsArgs& A, const TI* __restrict__ in, int nt,
+                                          const Lds<TB>& s) {
+  typedef typename C2<TB>::type V;
+  const int N = A.N, NE = A.NE, D = A.D, L = A.L;
   const TB ih = (TB)A.inv_hbar;
-  for (int e = tid; e < tot; e += nth) {
+  for (int e = threadIdx.x; e < nt * NE; e += blockDim.x) {
     const int t = e / NE;
     const int r = e - t * NE;
     const int b = r / N;
     const int a = r - b * N;
-    const V* yc = Yv + t * NE + b * N;
+    TB ax = 0, ay = 0;
+    if (!A.vacuum) {
+      for (int q = s.sp[b]; q < s.sp[b + 1]; ++q) {
+        const int j = s.sm[q];
+        const int k = s.oc[t * D + j];
+        const int nx = s.nx[t * D + j];
+        if (nx >= 0) {
+          const TB sc = s.ns[j * L + k];
+          const V y = ld<TB, TI>(in, (size_t)nx * NE + r);
+          ax -= sc * y.y;
+          ay += sc * y.x;
+        }
+        const int pv = s.pv[t * D + j];
+        if (pv >= 0) {
+          const V c = s.pcol[j * L + k];
+          const V y = ld<TB, TI>(in, (size_t)pv * NE + r);
+          ax -= c.x * y.x - c.y * y.y;
+          ay -= c.x * y.y + c.y * y.x;
+        }
+      }
+    }
+    const V* yc = s.Y + t * NE + b * N;
     TB xr = 0, xi = 0;
     for (int c = 0; c < N; ++c) {
-      const V h = Hv[a + c * N];
+      const V h = s.H[a + c * N];
       const V y = yc[c];
       xr += h.x * y.x - h.y * y.y;
       xi += h.x * y.y + h.y * y.x;
     }
-    const V y = Yv[e];
-    TB f = (TB)(0.5 * A.mu[base + t]);
-    if (a != b) f += (TB)0.5 * ((TB)A.delta[a] + (TB)A.delta[b]);
-    V v = Av[e];
-    // -(i/hbar) x = (xi, -xr) / hbar
-    v.x += ih * xi - f * y.x;
-    v.y += -ih * xr - f * y.y;
-    Av[e] = v;
+    const V y = s.Y[e];
+    TB f = s.mu2[t];
+    if (a != b) f += (TB)0.5 * (s.delta[a] + s.delta[b]);
+    s.W[e] = mk<TB>(ax + ih * xi - f * y.x, ay - ih * xr - f * y.y);
   }
-  __syncthreads();
 }
 
 // k(a, b) = Y'(a, b) + conj(Y'(b, a)), in double.
 template <typename TB>
-__device__ __forceinline__ double2 kval(const TB* At, int t, int NE, int N, int a, int b) {
+__device__ __forceinline__ double2 kval(const Lds<TB>& s, int e, int NE, int N) {
   typedef typename C2<TB>::type V;
-  const V* Av = reinterpret_cast<const V*>(At);
-  const V p = Av[t * NE + b * N + a];
-  const V q = Av[t * NE + a * N + b];
+  const int t = e / NE;
+  const int r = e - t * NE;
+  const int b = r / N;
+  const int a = r - b * N;
+  const V p = s.W[e];
+  const V q = s.W[t * NE + a * N + b];
   double2 k;
   k.x = (double)(p.x + q.x);
   k.y = (double)(p.y - q.y);
   return k;
+}
+
+// Elements each thread prefetches for the epilogue (tiles hold at most
+// 256 * kPre elements; larger tiles load the rest in the epilogue).
+constexpr int kPre = 4;
+
+__device__ __forceinline__ double2 ld2(const double* p, size_t e) {
+  return reinterpret_cast<const double2*>(p)[e];
 }
 
 // --- RK4 -------------------------------------------------------------------
@@ -158,36 +231,47 @@ __global__ void __launch_bounds__(256) rk4_stage(RhsArgs A, const TI* __restrict
                                                  double* __restrict__ acc,
                                                  TB* __restrict__ yout, double dt) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = A.N, NE = A.NE;
-  TB* Hs = reinterpret_cast<TB*>(smem);
-  TB* Yt = Hs + 2 * NE;
-  TB* At = Yt + 2 * NE * A.tile;
+  const int NE = A.NE;
   const int base = blockIdx.x * A.tile;
   const int nt = min(A.tile, A.num_nodes - base);
   if (nt <= 0) return;
-  rhs_tile<TB, TI>(A, in, base, nt, Hs, Yt, At);
+  const Lds<TB> s = carve<TB>(smem, A.N, A.D, A.L, A.tile);
   const int tot = nt * NE;
   const size_t off = (size_t)base * NE;
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int t = e / NE;
-    const int r = e - t * NE;
-    const int b = r / N;
-    const int a = r - b * N;
-    const double2 k = kval<TB>(At, t, NE, N, a, b);
+  // Epilogue operands, issued with the tile loads.
+  double2 pr[kPre], pa[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    if (e < tot) {
+      if (STAGE <= 2) pr[i] = ld2(rho, off + e);
+      if (STAGE >= 1) pa[i] = ld2(acc, off + e);
+    }
+  }
+  load_tile<TB, TI>(A, in, base, nt, s);
+  __syncthreads();
+  rhs_phase<TB, TI>(A, in, nt, s);
+  __syncthreads();
+  for (int i = 0, e = threadIdx.x; e < tot; ++i, e += blockDim.x) {
+    const double2 k = kval<TB>(s, e, NE, A.N);
     const size_t g = off + e;
+    double2 p, q;
+    if (i < kPre) {
+      p = pr[i];
+      q = pa[i];
+    } else {
+      if (STAGE <= 2) p = ld2(rho, g);
+      if (STAGE >= 1) q = ld2(acc, g);
+    }
     if (STAGE == 0) {
-      const double2 p = reinterpret_cast<const double2*>(rho)[g];
       st<double>(acc, g, p.x + (dt / 6) * k.x, p.y + (dt / 6) * k.y);
       st<TB>(yout, g, p.x + (dt / 2) * k.x, p.y + (dt / 2) * k.y);
     } else if (STAGE == 1 || STAGE == 2) {
-      const double2 p = reinterpret_cast<const double2*>(rho)[g];
-      const double2 s = reinterpret_cast<const double2*>(acc)[g];
-      st<double>(acc, g, s.x + (dt / 3) * k.x, s.y + (dt / 3) * k.y);
+      st<double>(acc, g, q.x + (dt / 3) * k.x, q.y + (dt / 3) * k.y);
       const double c = STAGE == 1 ? dt / 2 : dt;
       st<TB>(yout, g, p.x + c * k.x, p.y + c * k.y);
     } else {
-      const double2 s = reinterpret_cast<const double2*>(acc)[g];
-      st<double>(rho, g, s.x + (dt / 6) * k.x, s.y + (dt / 6) * k.y);
+      st<double>(rho, g, q.x + (dt / 6) * k.x, q.y + (dt / 6) * k.y);
     }
   }
 }
@@ -209,45 +293,49 @@ template <typename TB, typename TI, int STAGE>
 __global__ void __launch_bounds__(256) rkf45_stage(RhsArgs A, const TI* __restrict__ in,
                                                    const double* __restrict__ rho,
                                                    Rkf45Bufs<TB> B, Rkf45Coefs cf) {
+  typedef typename C2<TB>::type V;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[256 / 64];
-  const int N = A.N, NE = A.NE;
-  TB* Hs = reinterpret_cast<TB*>(smem);
-  TB* Yt = Hs + 2 * NE;
-  TB* At = Yt + 2 * NE * A.tile;
+  const int NE = A.NE;
   const int base = blockIdx.x * A.tile;
   const int nt = min(A.tile, A.num_nodes - base);
   if (nt <= 0) return;
-  rhs_tile<TB, TI>(A, in, base, nt, Hs, Yt, At);
+  const Lds<TB> s = carve<TB>(smem, A.N, A.D, A.L, A.tile);
   const int tot = nt * NE;
   const size_t off = (size_t)base * NE;
+  double2 pr[kPre];
+#pragma unroll
+  for (int i = 0; i < kPre; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    if (e < tot) pr[i] = ld2(rho, off + e);
+  }
+  load_tile<TB, TI>(A, in, base, nt, s);
+  __syncthreads();
+  rhs_phase<TB, TI>(A, in, nt, s);
+  __syncthreads();
   double emax = 0;
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int t = e / NE;
-    const int r = e - t * NE;
-    const int b = r / N;
-    const int a = r - b * N;
-    const double2 k = kval<TB>(At, t, NE, N, a, b);
+  for (int i = 0, e = threadIdx.x; e < tot; ++i, e += blockDim.x) {
+    const double2 k = kval<TB>(s, e, NE, A.N);
     const size_t g = off + e;
-    const double2 p = reinterpret_cast<const double2*>(rho)[g];
+    const double2 p = i < kPre ? pr[i] : ld2(rho, g);
     if (STAGE < 5) {
       st<TB>(B.k[STAGE], g, k.x, k.y);
-      // Exact TB value of k_STAGE as stored, so every stage sees the same k.
-      const typename C2<TB>::type ks = ld<TB, TB>(B.k[STAGE], g);
+      // The TB value of k_STAGE as stored, so later stages see the same k.
+      const double ksx = (double)(TB)k.x, ksy = (double)(TB)k.y;
       double sx = p.x, sy = p.y;
-      for (int i = 0; i < STAGE; ++i) {
-        const typename C2<TB>::type ki = ld<TB, TB>(B.k[i], g);
-        sx += (double)ki.x * cf.b[STAGE][i];
-        sy += (double)ki.y * cf.b[STAGE][i];
+      for (int j = 0; j < STAGE; ++j) {
+        const V kj = ld<TB, TB>(B.k[j], g);
+        sx += (double)kj.x * cf.b[STAGE][j];
+        sy += (double)kj.y * cf.b[STAGE][j];
       }
-      sx += (double)ks.x * cf.b[STAGE][STAGE];
-      sy += (double)ks.y * cf.b[STAGE][STAGE];
+      sx += ksx * cf.b[STAGE][STAGE];
+      sy += ksy * cf.b[STAGE][STAGE];
       st<TB>(B.tmp_out, g, sx, sy);
     } else {
-      const typename C2<TB>::type k0 = ld<TB, TB>(B.k[0], g);
-      const typename C2<TB>::type k2 = ld<TB, TB>(B.k[2], g);
-      const typename C2<TB>::type k3 = ld<TB, TB>(B.k[3], g);
-      const typename C2<TB>::type k4 = ld<TB, TB>(B.k[4], g);
+      const V k0 = ld<TB, TB>(B.k[0], g);
+      const V k2 = ld<TB, TB>(B.k[2], g);
+      const V k3 = ld<TB, TB>(B.k[3], g);
+      const V k4 = ld<TB, TB>(B.k[4], g);
       const double k5x = (double)(TB)k.x, k5y = (double)(TB)k.y;
       const double y4x = p.x + (double)k0.x * cf.c4[0] + (double)k2.x * cf.c4[1] +
                          (double)k3.x * cf.c4[2] + (double)k4.x * cf.c4[3];
@@ -467,8 +555,7 @@ __global__ void __launch_bounds__(256) zlincomb(double* __restrict__ y, size_t n
 // --- Host-side launchers -------------------------------------------------------
 
 size_t RhsSharedBytes(const RhsArgs& a, bool fp32) {
-  const size_t real = fp32 ? sizeof(float) : sizeof(double);
-  return real * 2 * (size_t)a.NE * (1 + 2 * (size_t)a.tile);
+  return dev::LdsBytes(a.N, a.D, a.L, a.tile, fp32 ? sizeof(float) : sizeof(double));
 }
 
 static int Blocks(const RhsArgs& a) { return (a.num_nodes + a.tile - 1) / a.tile; }
