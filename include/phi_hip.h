@@ -62,6 +62,8 @@ typedef struct phi_ctx phi_ctx;
 const char* phi_hip_last_error(void);
 /* ABI version (major * 100 + minor). */
 int phi_hip_version(void);
+/* Number of visible HIP devices (0 when none or the runtime fails). */
+int phi_hip_device_count(void);
 
 /* The `phi <paramfile> <method> [threads]` program. */
 int phi_hip_main(int argc, const char* const* argv);
@@ -147,6 +149,12 @@ int phi_hip_rkf45_accept(phi_ctx* c);
  * (0,0) row of node 0 replaced by trace(rho_0) = 1. */
 int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, double* resid);
 int phi_hip_synchronize(phi_ctx* c);
+/* Shape of the stage kernels: gemm = 0 element path, 1 matrix cores
+ * (v_mfma 16x16x4, N <= 32), 8/16/24/32 H rows in registers (N <= gemm),
+ * -1 default; tile = nodes per block (0 = default); threads = threads per
+ * block (64..256 in steps of 64, 0 = 256).  Results do not depend on the
+ * shape beyond rounding. */
+int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads);
 
 #ifdef __cplusplus
 }

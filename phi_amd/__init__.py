@@ -47,6 +47,7 @@ def lib():
     sigs = {
         "phi_hip_last_error": (cs, []),
         "phi_hip_version": (ctypes.c_int, []),
+        "phi_hip_device_count": (ctypes.c_int, []),
         "phi_hip_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(cs)]),
         "phi_hip_params_new": (ctypes.c_int, [ctypes.POINTER(vp)]),
         "phi_hip_params_read": (ctypes.c_int, [cs, cs, ctypes.c_int, ctypes.POINTER(vp)]),
@@ -85,6 +86,7 @@ def lib():
         "phi_hip_rkf45_accept": (ctypes.c_int, [vp]),
         "phi_hip_steady_state": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ip, dp]),
         "phi_hip_synchronize": (ctypes.c_int, [vp]),
+        "phi_hip_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -210,6 +212,11 @@ class PhiParameters:
         buf = ctypes.create_string_buffer(n.value + 1)
         _check(lib().phi_hip_params_header(self._h, num_nodes, buf, n.value + 1, None))
         return buf.value.decode()
+
+
+def device_count():
+    """Visible HIP devices (0 without a GPU)."""
+    return lib().phi_hip_device_count()
 
 
 def validate(kind, text):
@@ -350,6 +357,12 @@ class Context:
 
     def synchronize(self):
         _check(lib().phi_hip_synchronize(self._h))
+
+    def set_launch(self, gemm=-1, tile=0, threads=0):
+        """Stage-kernel shape: gemm 0 = element path, 1 = matrix cores,
+        8/16/24/32 = H rows in registers, -1 = default; tile = nodes per
+        block (0 = default); threads = threads per block (0 = 256)."""
+        _check(lib().phi_hip_set_launch(self._h, int(gemm), int(tile), int(threads)))
 
 
 def params_from_problem(H, rho0, gamma, lambda_, temperature, lh, num_terms=1,

@@ -24,7 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["phi_parameters.cc", "hierarchy.cc", "engine.cc", "integrator.cc", "capi.cc",
-               "heom_kernels.hip"]
+               "heom_kernels.hip", "heom_rk4_f64.hip", "heom_rk4_f32.hip",
+               "heom_rkf45_f64.hip", "heom_rkf45_f32.hip"]
 
 
 def _compile(src):
@@ -43,7 +44,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OUT, exist_ok=True)
-    with concurrent.futures.ThreadPoolExecutor(max_workers=6) as ex:
+    with concurrent.futures.ThreadPoolExecutor(max_workers=8) as ex:
         objs = list(ex.map(_compile, LIB_SOURCES))
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp]

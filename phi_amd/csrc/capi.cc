@@ -131,6 +131,12 @@ extern "C" {
 const char* phi_hip_last_error(void) { return g_error.c_str(); }
 int phi_hip_version(void) { return 100; }
 
+int phi_hip_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
 int phi_hip_main(int argc, const char* const* argv) {
   phi::PhiParameters p;
   p.ReadCommandLine(argc, argv);
@@ -490,6 +496,12 @@ int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, doubl
   if (!c) return Err(PHI_HIP_EINVAL, "null argument");
   int rc = c->e->SteadyState(tol, max_iter, iters, resid, nullptr);
   if (rc == phi::kErrNotConverged) return Err(rc, "BiCGSTAB did not converge");
+  return rc ? Err(rc, c->e->last_error()) : Ok();
+}
+
+int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads) {
+  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+  int rc = c->e->SetLaunch(gemm, tile, threads);
   return rc ? Err(rc, c->e->last_error()) : Ok();
 }
 

@@ -110,6 +110,11 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   // Nodes per 256-thread block: enough to give every thread an element.
   a.tile = std::max(1, std::min(64, 256 / NE));
   a.vacuum = h.vacuum ? 1 : 0;
+  a.real_h = 1;
+  for (const auto& v : p.time_indepenent_hamiltonian)
+    if (v.imag() != 0) a.real_h = 0;
+  a.mN = ((1ull << 32) + N - 1) / N;
+  a.mNE = ((1ull << 32) + NE - 1) / NE;
   a.inv_hbar = 1.0 / p.hbar;
   a.next = e->d_next_;
   a.prev = e->d_prev_;
@@ -124,10 +129,11 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   a.H = e->d_H_;
   a.Hraw = e->d_Hraw_;
   a.Hadj = e->d_Hadj_;
-  if (RhsSharedBytes(a, false) > 64 * 1024) {
-    *why = "NumStates too large for the LDS tile of this build";
+  rc = e->SetLaunch(-1, 0, 0);
+  if (rc) {
+    *why = e->err_;
     delete e;
-    return kErrUnsupported;
+    return rc;
   }
   e->fp32_ = p.bulk_precision == kBulkFp32;
   *out = e;
@@ -145,6 +151,58 @@ Engine::~Engine() {
 }
 
 void Engine::set_fp32(bool v) { fp32_ = v; }
+
+int Engine::SetLaunch(int col_nm, int tile, int threads) {
+  RhsArgs& a = args_;
+  const int N = h_.N, D = h_.D, L = h_.L, NE = N * N;
+  const size_t kLdsBudget = 64 * 1024;
+  if (col_nm < 0) {
+    // Default: matrix cores whenever H fits two 16-row tiles.
+    col_nm = N <= 32 ? 1 : 0;
+  }
+  if (col_nm != 0 && col_nm != 1 && col_nm != 8 && col_nm != 16 && col_nm != 24 &&
+      col_nm != 32) {
+    err_ = "GEMM form must be 0, 1, 8, 16, 24 or 32";
+    return kErrInvalidArgument;
+  }
+  if (col_nm == 1 && N > 32) {
+    err_ = "the matrix-core form needs NumStates <= 32";
+    return kErrInvalidArgument;
+  }
+  if (col_nm > 1 && N > col_nm) {
+    err_ = "register width smaller than NumStates";
+    return kErrInvalidArgument;
+  }
+  if (threads != 0 && (threads < 64 || threads > 256 || threads % 64 != 0)) {
+    err_ = "threads per block must be 64, 128, 192 or 256";
+    return kErrInvalidArgument;
+  }
+  int th = threads > 0 ? threads : 256;
+  if (col_nm > 1) {
+    // A whole number of rows per block keeps each thread on one row of H.
+    if (N > th) {
+      err_ = "the row-register form needs NumStates <= threads per block";
+      return kErrInvalidArgument;
+    }
+    th = (th / N) * N;
+  }
+  const bool with_h = col_nm != 1;
+  a.col_nm = col_nm;
+  for (int f = 0; f < 2; ++f) {
+    // Nodes per block: enough elements for every thread, fewer when the
+    // tile's LDS exceeds the budget.
+    int t = tile > 0 ? tile : std::max(1, std::min(64, th / NE));
+    while (t > 1 && TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) --t;
+    if (TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) {
+      err_ = "NumStates too large for the LDS tile of this build";
+      return kErrUnsupported;
+    }
+    a.col_tile[f] = t;
+    a.col_threads[f] = th;
+  }
+  a.tile = a.col_tile[0];
+  return kOk;
+}
 
 int Engine::AllocBulk() {
   if (bulk_alloc_ && bulk_is_fp32_ == fp32_) return kOk;

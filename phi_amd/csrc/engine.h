@@ -41,6 +41,11 @@ class Engine {
   size_t state_elems() const { return (size_t)h_.num_nodes * h_.N * h_.N; }
   bool fp32() const { return fp32_; }
   void set_fp32(bool v);
+  // Stage-kernel shape: col_nm = GEMM form (0 element, 1 matrix cores,
+  // 8/16/24/32 row registers, -1 default); tile = nodes per block (0 =
+  // default for the LDS budget); threads = threads per block (0 = 256).
+  int SetLaunch(int col_nm, int tile, int threads);
+  const RhsArgs& args() const { return args_; }
   hipStream_t stream() const { return stream_; }
 
   // Full state in PHI id order, column-major complex blocks.

@@ -12,6 +12,15 @@ namespace phi {
 // Everything the RHS of one hierarchy needs, as device pointers.
 struct RhsArgs {
   int N, NE, D, L, num_nodes, tile, vacuum;
+  // GEMM form of the stage kernels: col_nm = 0 element path (H from LDS),
+  // 1 matrix-core path (v_mfma 16x16x4, N <= 32), 8/16/24/32 row-register
+  // path (row a of H in registers sized for N <= col_nm).  Nodes and
+  // threads per block per precision ([0] FP64, [1] FP32).
+  int col_nm;
+  int col_tile[2];
+  int col_threads[2];
+  int real_h;                 // imag(H) == 0 everywhere
+  unsigned long long mN, mNE; // ceil(2^32 / N), ceil(2^32 / N^2)
   double inv_hbar;
   const int* next;          // [num_nodes * D]
   const int* prev;          // [num_nodes * D]
@@ -45,7 +54,11 @@ struct Rkf45Coefs {
   double c5[5];
 };
 
+// Dynamic LDS of one block; `a.tile` must already be the launch's tile.
 size_t RhsSharedBytes(const RhsArgs& a, bool fp32);
+// LDS bytes of a tile of `tile` nodes (with_h: H staged in LDS, i.e. every
+// GEMM form except the matrix-core one).
+size_t TileLdsBytes(int N, int D, int L, int tile, bool fp32, bool with_h);
 
 hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, const void* in, double* rho,
                           double* acc, void* yout, double dt, hipStream_t s);
