@@ -150,11 +150,19 @@ int phi_hip_rkf45_accept(phi_ctx* c);
 int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, double* resid);
 int phi_hip_synchronize(phi_ctx* c);
 /* Shape of the stage kernels: gemm = 0 element path, 1 matrix cores
- * (v_mfma 16x16x4, N <= 32), 8/16/24/32 H rows in registers (N <= gemm),
- * -1 default; tile = nodes per block (0 = default); threads = threads per
- * block (64..256 in steps of 64, 0 = 256).  Results do not depend on the
- * shape beyond rounding. */
-int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads);
+ * (v_mfma 16x16x4, N <= 32), 2 persistent pipelined matrix-core kernels
+ * (N <= 32, at most two bath modes per site; the default where they
+ * apply), -1 default; tile = nodes per block (0 = default); threads =
+ * threads per block (64..256 in steps of 64, 0 = 256); blocks = grid of the
+ * persistent kernels (0 = as many as are resident at once).  Results do
+ * not depend on the shape beyond rounding. */
+int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks);
+/* Stage order of phi_hip_rk4_steps (the RK4 loop body of Rk4Integrate,
+ * hierarchy_integrator.cc:2709-2825): 0 = PHI's k0..k3 form with an
+ * accumulator, 1 = nested form rho + dt L (rho + dt/2 L (rho + dt/3 L (rho +
+ * dt/4 L rho))) (default), the same polynomial for this time-independent
+ * linear RHS, with half the FP64 traffic.  Equal to rounding. */
+int phi_hip_set_rk4_form(phi_ctx* c, int form);
 
 #ifdef __cplusplus
 }

@@ -86,7 +86,9 @@ def lib():
         "phi_hip_rkf45_accept": (ctypes.c_int, [vp]),
         "phi_hip_steady_state": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ip, dp]),
         "phi_hip_synchronize": (ctypes.c_int, [vp]),
-        "phi_hip_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "phi_hip_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int]),
+        "phi_hip_set_rk4_form": (ctypes.c_int, [vp, ctypes.c_int]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -358,11 +360,19 @@ class Context:
     def synchronize(self):
         _check(lib().phi_hip_synchronize(self._h))
 
-    def set_launch(self, gemm=-1, tile=0, threads=0):
+    def set_launch(self, gemm=-1, tile=0, threads=0, blocks=0):
         """Stage-kernel shape: gemm 0 = element path, 1 = matrix cores,
-        8/16/24/32 = H rows in registers, -1 = default; tile = nodes per
-        block (0 = default); threads = threads per block (0 = 256)."""
-        _check(lib().phi_hip_set_launch(self._h, int(gemm), int(tile), int(threads)))
+        2 = persistent matrix-core kernels, -1 = default; tile = nodes per
+        block (0 = default); threads = threads per block (0 = 256); blocks =
+        grid of the persistent kernels (0 = all resident)."""
+        _check(lib().phi_hip_set_launch(self._h, int(gemm), int(tile), int(threads),
+                                        int(blocks)))
+
+    def set_rk4_form(self, form):
+        """RK4 stage order: "classic" (PHI's k0..k3 with an accumulator) or
+        "nested" (Taylor/Horner form, the default)."""
+        f = {"classic": 0, "nested": 1}[form]
+        _check(lib().phi_hip_set_rk4_form(self._h, f))
 
 
 def params_from_problem(H, rho0, gamma, lambda_, temperature, lh, num_terms=1,

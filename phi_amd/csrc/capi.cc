@@ -499,10 +499,17 @@ int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, doubl
   return rc ? Err(rc, c->e->last_error()) : Ok();
 }
 
-int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads) {
+int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks) {
   if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = c->e->SetLaunch(gemm, tile, threads);
+  int rc = c->e->SetLaunch(gemm, tile, threads, blocks);
   return rc ? Err(rc, c->e->last_error()) : Ok();
+}
+
+int phi_hip_set_rk4_form(phi_ctx* c, int form) {
+  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+  if (form != 0 && form != 1) return Err(PHI_HIP_EINVAL, "RK4 form must be 0 or 1");
+  c->e->set_rk4_nested(form == 1);
+  return Ok();
 }
 
 int phi_hip_synchronize(phi_ctx* c) {

@@ -85,6 +85,27 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   if (!rc) rc = e->Check(Upload(&e->d_H_, Hs.data(), Hs.size()), "upload H");
   if (!rc) rc = e->Check(Upload(&e->d_Hraw_, Hraw.data(), Hraw.size()), "upload Hraw");
   if (!rc) rc = e->Check(Upload(&e->d_Hadj_, Hadj.data(), Hadj.size()), "upload Hadj");
+  // Edge records of the persistent kernels (one per node, site and mode slot
+  // of that site), when every site has at most two modes.
+  int mps = 0;
+  for (int b = 0; b < N; ++b) mps = std::max(mps, h.site_ptr[b + 1] - h.site_ptr[b]);
+  if (mps == 0) mps = 1;
+  if (mps <= 2) {
+    std::vector<int4> edge((size_t)h.num_nodes * N * mps);
+    for (int n = 0; n < h.num_nodes; ++n)
+      for (int b = 0; b < N; ++b)
+        for (int q = 0; q < mps; ++q) {
+          int4 r = make_int4(-1, -1, -1, 0);
+          if (!h.vacuum && q < h.site_ptr[b + 1] - h.site_ptr[b]) {
+            const int j = h.site_modes[h.site_ptr[b] + q];
+            const size_t o = (size_t)n * h.D + j;
+            r = make_int4(h.next[o], h.prev[o], j * h.L + h.occ[o], 0);
+          }
+          edge[((size_t)n * N + b) * mps + q] = r;
+        }
+    if (!rc) rc = e->Check(Upload(&e->d_edge_, edge.data(), edge.size()), "upload edges");
+  }
+  e->mps_ = mps;
   const size_t bytes = e->state_elems() * 2 * sizeof(double);
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_rho_, bytes), "alloc rho");
   if (!rc) rc = e->Check(hipMemset(e->d_rho_, 0, bytes), "zero rho");
@@ -129,7 +150,9 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   a.H = e->d_H_;
   a.Hraw = e->d_Hraw_;
   a.Hadj = e->d_Hadj_;
-  rc = e->SetLaunch(-1, 0, 0);
+  a.edge = e->d_edge_;
+  a.mps = e->mps_;
+  rc = e->SetLaunch(-1, 0, 0, 0);
   if (rc) {
     *why = e->err_;
     delete e;
@@ -143,7 +166,7 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   void* ptrs[] = {d_next_, d_prev_, d_occ_, d_mu_, d_site_ptr_, d_site_modes_, d_next_s_,
-                  d_prev_col_, d_prev_row_, d_delta_, d_H_, d_Hraw_, d_Hadj_, d_rho_, d_aux_,
+                  d_prev_col_, d_prev_row_, d_delta_, d_H_, d_Hraw_, d_Hadj_, d_edge_, d_rho_, d_aux_,
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_};
   for (void* p : ptrs) Free(p);
@@ -152,54 +175,77 @@ Engine::~Engine() {
 
 void Engine::set_fp32(bool v) { fp32_ = v; }
 
-int Engine::SetLaunch(int col_nm, int tile, int threads) {
+int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
   RhsArgs& a = args_;
   const int N = h_.N, D = h_.D, L = h_.L, NE = N * N;
   const size_t kLdsBudget = 64 * 1024;
+  const bool stream_ok = N <= 32 && mps_ <= 2 && d_edge_ != nullptr;
   if (col_nm < 0) {
-    // Default: matrix cores whenever H fits two 16-row tiles.
-    col_nm = N <= 32 ? 1 : 0;
+    // Default: the persistent matrix-core kernels where they apply, else
+    // matrix cores whenever H fits two 16-row tiles.
+    col_nm = stream_ok ? 2 : (N <= 32 ? 1 : 0);
   }
-  if (col_nm != 0 && col_nm != 1 && col_nm != 8 && col_nm != 16 && col_nm != 24 &&
-      col_nm != 32) {
-    err_ = "GEMM form must be 0, 1, 8, 16, 24 or 32";
+  if (col_nm < 0 || col_nm > 2) {
+    err_ = "kernel form must be 0 (element), 1 (matrix cores) or 2 (persistent)";
     return kErrInvalidArgument;
   }
-  if (col_nm == 1 && N > 32) {
-    err_ = "the matrix-core form needs NumStates <= 32";
+  if (col_nm >= 1 && N > 32) {
+    err_ = "the matrix-core forms need NumStates <= 32";
     return kErrInvalidArgument;
   }
-  if (col_nm > 1 && N > col_nm) {
-    err_ = "register width smaller than NumStates";
+  if (col_nm == 2 && !stream_ok) {
+    err_ = "the persistent form needs at most two bath modes per site";
     return kErrInvalidArgument;
   }
-  if (threads != 0 && (threads < 64 || threads > 256 || threads % 64 != 0)) {
-    err_ = "threads per block must be 64, 128, 192 or 256";
+  const int max_threads = col_nm == 2 ? 1024 : 256;
+  if (threads != 0 && (threads < 64 || threads > max_threads || threads % 64 != 0)) {
+    err_ = col_nm == 2 ? "threads per block must be a multiple of 64 up to 1024"
+                       : "threads per block must be 64, 128, 192 or 256";
     return kErrInvalidArgument;
   }
-  int th = threads > 0 ? threads : 256;
-  if (col_nm > 1) {
-    // A whole number of rows per block keeps each thread on one row of H.
-    if (N > th) {
-      err_ = "the row-register form needs NumStates <= threads per block";
-      return kErrInvalidArgument;
-    }
-    th = (th / N) * N;
+  if (blocks < 0) {
+    err_ = "blocks must be >= 0";
+    return kErrInvalidArgument;
   }
-  const bool with_h = col_nm != 1;
-  a.col_nm = col_nm;
+  const bool with_h = col_nm == 0;
   for (int f = 0; f < 2; ++f) {
-    // Nodes per block: enough elements for every thread, fewer when the
-    // tile's LDS exceeds the budget.
-    int t = tile > 0 ? tile : std::max(1, std::min(64, th / NE));
-    while (t > 1 && TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) --t;
-    if (TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) {
-      err_ = "NumStates too large for the LDS tile of this build";
-      return kErrUnsupported;
+    int t, th = threads > 0 ? threads : 256;
+    if (col_nm == 2) {
+      // Default: about 512 elements per block, one per thread.  With fewer
+      // threads than elements a thread takes up to 4 (blocks <= 256 then).
+      t = tile > 0 ? tile : std::max(1, 512 / NE);
+      while (t > 1 && ((size_t)t * NE > 1024 ||
+                       StreamTileLdsBytes(N, D, L, t, f == 1) > kLdsBudget))
+        --t;
+      if (threads == 0) th = std::min(1024, (t * NE + 63) / 64 * 64);
+      const int ept = (t * NE + th - 1) / th;
+      if (ept > 4 || (ept > 1 && th > 256) ||
+          StreamTileLdsBytes(N, D, L, t, f == 1) > kLdsBudget) {
+        err_ = "tile does not fit the persistent kernels (at most 4 elements per thread, "
+               "256 threads when more than one)";
+        return kErrInvalidArgument;
+      }
+      // Kernels without a persistent form (RKF45) run the one-shot
+      // matrix-core kernels with 256 threads and their default tile.
+      int t1 = std::max(1, std::min(64, 256 / NE));
+      while (t1 > 1 && TileLdsBytes(N, D, L, t1, f == 1, false) > kLdsBudget) --t1;
+      a.v1_tile[f] = t1;
+    } else {
+      // Nodes per block: enough elements for every thread, fewer when the
+      // tile's LDS exceeds the budget.
+      t = tile > 0 ? tile : std::max(1, std::min(64, th / NE));
+      while (t > 1 && TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) --t;
+      if (TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) {
+        err_ = "NumStates too large for the LDS tile of this build";
+        return kErrUnsupported;
+      }
+      a.v1_tile[f] = t;
     }
     a.col_tile[f] = t;
     a.col_threads[f] = th;
+    a.col_grid[f] = blocks;
   }
+  a.col_nm = col_nm;
   a.tile = a.col_tile[0];
   return kOk;
 }
@@ -264,13 +310,14 @@ int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
 int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int snap_every) {
   int rc = AllocBulk();
   if (rc) return rc;
+  const bool hn = rk4_nested_;
   for (long long s = 0; s < nsteps; ++s) {
     // Stage inputs alternate between the two bulk buffers so that a stage
     // never overwrites what its own neighbour gathers read.
-    rc = Check(LaunchRk4Stage(args_, fp32_, 0, d_rho_, d_rho_, d_aux_, d_y_[0], dt, stream_), "rk4 s0");
-    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 1, d_y_[0], d_rho_, d_aux_, d_y_[1], dt, stream_), "rk4 s1");
-    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 2, d_y_[1], d_rho_, d_aux_, d_y_[0], dt, stream_), "rk4 s2");
-    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 3, d_y_[0], d_rho_, d_aux_, nullptr, dt, stream_), "rk4 s3");
+    rc = Check(LaunchRk4Stage(args_, fp32_, 0, hn, d_rho_, d_rho_, d_aux_, d_y_[0], dt, stream_), "rk4 s0");
+    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 1, hn, d_y_[0], d_rho_, d_aux_, d_y_[1], dt, stream_), "rk4 s1");
+    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 2, hn, d_y_[1], d_rho_, d_aux_, d_y_[0], dt, stream_), "rk4 s2");
+    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 3, hn, d_y_[0], d_rho_, d_aux_, nullptr, dt, stream_), "rk4 s3");
     if (rc) return rc;
     const long long i = step_offset + s;
     if (snap_every > 0 && i % snap_every == 0) {
@@ -294,7 +341,8 @@ int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   for (long long i = 0; i < nsteps && !rc; ++i) {
     rc = Check(hipEventRecord(ev[0], stream_), "record");
     for (int s = 0; s < 4 && !rc; ++s) {
-      rc = Check(LaunchRk4Stage(args_, fp32_, s, in[s], d_rho_, d_aux_, out[s], dt, stream_), "rk4 stage");
+      rc = Check(LaunchRk4Stage(args_, fp32_, s, rk4_nested_, in[s], d_rho_, d_aux_, out[s], dt,
+                                stream_), "rk4 stage");
       if (!rc) rc = Check(hipEventRecord(ev[s + 1], stream_), "record");
     }
     if (!rc) rc = Check(hipEventSynchronize(ev[4]), "event sync");

@@ -41,10 +41,15 @@ class Engine {
   size_t state_elems() const { return (size_t)h_.num_nodes * h_.N * h_.N; }
   bool fp32() const { return fp32_; }
   void set_fp32(bool v);
-  // Stage-kernel shape: col_nm = GEMM form (0 element, 1 matrix cores,
-  // 8/16/24/32 row registers, -1 default); tile = nodes per block (0 =
-  // default for the LDS budget); threads = threads per block (0 = 256).
-  int SetLaunch(int col_nm, int tile, int threads);
+  // Stage-kernel shape: col_nm = kernel form (0 element, 1 matrix cores,
+  // 2 persistent matrix-core kernels, -1 default); tile = nodes per block
+  // (0 = default for the LDS budget); threads = threads per block (0 =
+  // 256); blocks = grid of the persistent kernels (0 = all resident).
+  int SetLaunch(int col_nm, int tile, int threads, int blocks);
+  // RK4 stage order: true = nested (Taylor/Horner) form, false = PHI's
+  // classic k0..k3 form.  Both give PHI's step to rounding.
+  void set_rk4_nested(bool v) { rk4_nested_ = v; }
+  bool rk4_nested() const { return rk4_nested_; }
   const RhsArgs& args() const { return args_; }
   hipStream_t stream() const { return stream_; }
 
@@ -58,7 +63,7 @@ class Engine {
   // Same on device pointers (interleaved doubles).
   int RhsDevice(const double* x, double* y, int trace_row);
 
-  // `nsteps` classic RK4 steps of size dt.  When snap_every > 0, node 0 is
+  // `nsteps` RK4 steps of size dt.  When snap_every > 0, node 0 is
   // appended to the device history after every step i (counted from
   // step_offset) with i % snap_every == 0.
   int Rk4Steps(long long nsteps, double dt, long long step_offset, int snap_every);
@@ -89,6 +94,7 @@ class Engine {
   HierarchyTables h_;
   RhsArgs args_{};
   bool fp32_ = false;
+  bool rk4_nested_ = true;
   hipStream_t stream_ = nullptr;
   std::string err_;
 
@@ -106,6 +112,8 @@ class Engine {
   double2* d_H_ = nullptr;
   double2* d_Hraw_ = nullptr;
   double2* d_Hadj_ = nullptr;
+  int4* d_edge_ = nullptr;
+  int mps_ = 0;
 
   // State buffers (interleaved complex).
   double* d_rho_ = nullptr;

@@ -164,24 +164,13 @@ __device__ __forceinline__ void load_tile(const RhsArgs& A, const TI* __restrict
 // Delta_b)/2 [a != b]) Y(a,b) + A(a,b), where column b of A gathers, over the
 // modes j coupled to site b, i s_j Y_{n+e_j}(a,b) - col_j Y_{n-e_j}(a,b).
 // Consecutive threads take consecutive elements, so the gathers coalesce.
-// With NM > 0 the block size is a multiple of N, every element of a thread
-// lies in the same row a, and row a of H sits in registers: the GEMM then
-// reads only Y(c, b) from LDS, at an address shared by the threads of a
-// column (a broadcast).
-template <typename TB, typename TI, int NM>
+template <typename TB, typename TI>
 __device__ __forceinline__ void rhs_phase(const RhsArgs& A, const TI* __restrict__ in, int nt,
                                           const Lds<TB>& s) {
   typedef typename C2<TB>::type V;
   const int N = A.N, NE = A.NE, D = A.D, L = A.L;
   const int NP = N * (N + 1);
   const TB ih = (TB)A.inv_hbar;
-  V hrow[NM > 0 ? NM : 1];
-  if (NM > 0) {
-    const int a0 = threadIdx.x % N;
-#pragma unroll
-    for (int c = 0; c < (NM > 0 ? NM : 1); ++c)
-      if (c < N) hrow[c] = s.H[a0 + c * N];
-  }
   for (int e = threadIdx.x; e < nt * NE; e += blockDim.x) {
     const int t = fdiv(e, A.mNE);
     const int r = e - t * NE;
@@ -210,23 +199,11 @@ __device__ __forceinline__ void rhs_phase(const RhsArgs& A, const TI* __restrict
     }
     const V* yc = s.Y + t * NE + b * N;
     TB xr = 0, xi = 0;
-    if (NM > 0) {
-#pragma unroll
-      for (int c = 0; c < (NM > 0 ? NM : 1); ++c) {
-        if (c < N) {
-          const V h = hrow[c];
-          const V y = yc[c];
-          xr += h.x * y.x - h.y * y.y;
-          xi += h.x * y.y + h.y * y.x;
-        }
-      }
-    } else {
-      for (int c = 0; c < N; ++c) {
-        const V h = s.H[a + c * N];
-        const V y = yc[c];
-        xr += h.x * y.x - h.y * y.y;
-        xi += h.x * y.y + h.y * y.x;
-      }
+    for (int c = 0; c < N; ++c) {
+      const V h = s.H[a + c * N];
+      const V y = yc[c];
+      xr += h.x * y.x - h.y * y.y;
+      xi += h.x * y.y + h.y * y.x;
     }
     const V y = s.Y[e];
     TB f = s.mu2[t];
@@ -411,15 +388,20 @@ __device__ __forceinline__ double2 ld2(const double* p, size_t e) {
 }
 
 // --- RK4 -------------------------------------------------------------------
-// Buffers: rho (FP64), acc (FP64), Yin/Yout (TB).  Stage 0 reads rho as the
-// RHS input.  Update order:
-//   s0: acc = rho + dt/6 k, Yout = rho + dt/2 k
+// Two stage orders of the same step.  Classic (PHI's form,
+// hierarchy_integrator.cc:2714-2825), buffers rho, acc (FP64), Yin/Yout (TB):
+//   s0: acc = rho + dt/6 k, Yout = rho + dt/2 k      (input rho)
 //   s1: acc += dt/3 k,      Yout = rho + dt/2 k
 //   s2: acc += dt/3 k,      Yout = rho + dt k
 //   s3: rho = acc + dt/6 k
-// which is PHI's rho += dt/6 (k0 + 2k1 + 2k2 + k3) (hierarchy_integrator.cc:
-// 2714-2825) up to rounding.
-template <typename TB, typename TI, int NM, int STAGE>
+// Nested (HORNER): for the time-independent linear RHS the RK4 step is the
+// degree-4 Taylor polynomial of dt L, evaluated as
+//   v1 = rho + dt/4 L rho, v2 = rho + dt/3 L v1, v3 = rho + dt/2 L v2,
+//   rho = rho + dt L v3,
+// which needs no accumulator: a stage reads rho and v and writes v (rho in
+// the last one), N^2 (80 + 6 s) bytes per node-step instead of
+// N^2 (160 + 6 s).  The two orders agree to rounding.
+template <typename TB, typename TI, int NM, int STAGE, bool HORNER>
 __global__ void __launch_bounds__(256) rk4_stage(RhsArgs A, const TI* __restrict__ in,
                                                  double* __restrict__ rho,
                                                  double* __restrict__ acc,
@@ -436,6 +418,8 @@ __global__ void __launch_bounds__(256) rk4_stage(RhsArgs A, const TI* __restrict
   // Epilogue operands stream in chunks of kPre elements per thread, the loads
   // of chunk c+1 in flight while chunk c is combined; chunk 0 is issued
   // with the tile loads.
+  constexpr bool kRho = HORNER || STAGE <= 2;
+  constexpr bool kAcc = !HORNER && STAGE >= 1;
   const int chunk = kPre * blockDim.x;
   double2 pr[kPre], pa[kPre];
   auto fetch = [&](int e0, double2* r, double2* q) {
@@ -443,8 +427,8 @@ __global__ void __launch_bounds__(256) rk4_stage(RhsArgs A, const TI* __restrict
     for (int i = 0; i < kPre; ++i) {
       const int e = e0 + threadIdx.x + i * blockDim.x;
       if (e < tot) {
-        if (STAGE <= 2) r[i] = ld2(rho, off + e);
-        if (STAGE >= 1) q[i] = ld2(acc, off + e);
+        if (kRho) r[i] = ld2(rho, off + e);
+        if (kAcc) q[i] = ld2(acc, off + e);
       }
     }
   };
@@ -453,7 +437,7 @@ __global__ void __launch_bounds__(256) rk4_stage(RhsArgs A, const TI* __restrict
   if constexpr (NM == 1) {
     rhs_phase_mfma<TB, TI>(A, in, nt, s);
   } else {
-    rhs_phase<TB, TI, (NM == 1 ? 0 : NM)>(A, in, nt, s);
+    rhs_phase<TB, TI>(A, in, nt, s);
   }
   __syncthreads();
   for (int e0 = 0; e0 < tot; e0 += chunk) {
@@ -466,7 +450,14 @@ __global__ void __launch_bounds__(256) rk4_stage(RhsArgs A, const TI* __restrict
       const double2 k = kval<TB>(A, s, e);
       const size_t g = off + e;
       const double2 p = pr[i], q = pa[i];
-      if (STAGE == 0) {
+      if (HORNER) {
+        if (STAGE < 3) {
+          const double c = dt / (4 - STAGE);
+          st<TB>(yout, g, p.x + c * k.x, p.y + c * k.y);
+        } else {
+          st<double>(rho, g, p.x + dt * k.x, p.y + dt * k.y);
+        }
+      } else if (STAGE == 0) {
         st<double>(acc, g, p.x + (dt / 6) * k.x, p.y + (dt / 6) * k.y);
         st<TB>(yout, g, p.x + (dt / 2) * k.x, p.y + (dt / 2) * k.y);
       } else if (STAGE == 1 || STAGE == 2) {
@@ -479,8 +470,8 @@ __global__ void __launch_bounds__(256) rk4_stage(RhsArgs A, const TI* __restrict
     }
 #pragma unroll
     for (int i = 0; i < kPre; ++i) {
-      pr[i] = nr[i];
-      pa[i] = na[i];
+      if (kRho) pr[i] = nr[i];
+      if (kAcc) pa[i] = na[i];
     }
   }
 }
@@ -526,7 +517,7 @@ __global__ void __launch_bounds__(256) rkf45_stage(RhsArgs A, const TI* __restri
   if constexpr (NM == 1) {
     rhs_phase_mfma<TB, TI>(A, in, nt, s);
   } else {
-    rhs_phase<TB, TI, (NM == 1 ? 0 : NM)>(A, in, nt, s);
+    rhs_phase<TB, TI>(A, in, nt, s);
   }
   __syncthreads();
   double emax = 0;

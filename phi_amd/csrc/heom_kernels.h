@@ -12,13 +12,23 @@ namespace phi {
 // Everything the RHS of one hierarchy needs, as device pointers.
 struct RhsArgs {
   int N, NE, D, L, num_nodes, tile, vacuum;
-  // GEMM form of the stage kernels: col_nm = 0 element path (H from LDS),
-  // 1 matrix-core path (v_mfma 16x16x4, N <= 32), 8/16/24/32 row-register
-  // path (row a of H in registers sized for N <= col_nm).  Nodes and
-  // threads per block per precision ([0] FP64, [1] FP32).
+  // Form of the stage kernels: col_nm = 0 element path (H from LDS),
+  // 1 matrix-core path (v_mfma 16x16x4, N <= 32), 2 persistent pipelined
+  // matrix-core kernels (heom_stream.h; N <= 32, at most 2 modes per
+  // site).  Nodes, threads and blocks (0 = as many as fit) per launch, per
+  // precision ([0] FP64, [1] FP32).
   int col_nm;
   int col_tile[2];
   int col_threads[2];
+  int col_grid[2];
+  // Tile of the one-shot kernels when col_nm == 2 (their block is 256
+  // threads): RKF45 stages, which have no persistent form.
+  int v1_tile[2];
+  // Edge records of the persistent kernels: edge[(n*N + b)*mps + q] =
+  // {id of n + e_j, id of n - e_j, j*L + n_j, 0} for the q-th mode j of
+  // site b ({-1, -1, -1, 0} when absent).
+  const int4* edge;
+  int mps;
   int real_h;                 // imag(H) == 0 everywhere
   unsigned long long mN, mNE; // ceil(2^32 / N), ceil(2^32 / N^2)
   double inv_hbar;
@@ -59,9 +69,12 @@ size_t RhsSharedBytes(const RhsArgs& a, bool fp32);
 // LDS bytes of a tile of `tile` nodes (with_h: H staged in LDS, i.e. every
 // GEMM form except the matrix-core one).
 size_t TileLdsBytes(int N, int D, int L, int tile, bool fp32, bool with_h);
+// LDS bytes of a block of the persistent kernels (two tiles in flight).
+size_t StreamTileLdsBytes(int N, int D, int L, int tile, bool fp32);
 
-hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, const void* in, double* rho,
-                          double* acc, void* yout, double dt, hipStream_t s);
+// One RK4 stage; horner selects the nested stage order (no accumulator).
+hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, bool horner, const void* in,
+                          double* rho, double* acc, void* yout, double dt, hipStream_t s);
 hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* in,
                             const double* rho, const Rkf45Bufs<float>& bf,
                             const Rkf45Bufs<double>& bd, const Rkf45Coefs& c, hipStream_t s);

@@ -3,7 +3,11 @@
 // stage-kernel dispatch.  The stage kernels themselves are templates in
 // heom_device.h, instantiated per method and precision in heom_rk4_f64.hip,
 // heom_rk4_f32.hip, heom_rkf45_f64.hip and heom_rkf45_f32.hip.
-#include "heom_device.h"
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "heom_stream.h"
 
 namespace phi {
 namespace dev {
@@ -196,28 +200,62 @@ __global__ void __launch_bounds__(256) zlincomb(double* __restrict__ y, size_t n
 
 // --- Host-side launchers -------------------------------------------------------
 
+int ResidentBlocks(const void* kernel, int threads, size_t lds) {
+  struct Key {
+    const void* k;
+    int threads;
+    size_t lds;
+    int dev;
+    bool operator<(const Key& o) const {
+      return std::tie(k, threads, lds, dev) < std::tie(o.k, o.threads, o.lds, o.dev);
+    }
+  };
+  static std::mutex mu;
+  static std::map<Key, int> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const Key key{kernel, threads, lds, dev};
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 1;
+  const int blocks = per_cu * cus;
+  cache[key] = blocks;
+  return blocks;
+}
+
 size_t TileLdsBytes(int N, int D, int L, int tile, bool fp32, bool with_h) {
   return dev::LdsBytes(N, D, L, tile, fp32 ? sizeof(float) : sizeof(double), with_h);
 }
 
 size_t RhsSharedBytes(const RhsArgs& a, bool fp32) {
   return dev::LdsBytes(a.N, a.D, a.L, a.tile, fp32 ? sizeof(float) : sizeof(double),
-                       a.col_nm != 1);
+                       a.col_nm == 0);
 }
 
-hipError_t LaunchRk4F64(const RhsArgs& a, int stage, const void* in, double* rho, double* acc,
-                       void* yout, double dt, hipStream_t s);
-hipError_t LaunchRk4F32(const RhsArgs& a, int stage, const void* in, double* rho, double* acc,
-                       void* yout, double dt, hipStream_t s);
+size_t StreamTileLdsBytes(int N, int D, int L, int tile, bool fp32) {
+  return dev::StreamLdsBytes(N, D, L, tile, fp32 ? sizeof(float) : sizeof(double));
+}
+
+hipError_t LaunchRk4F64(const RhsArgs& a, int stage, bool horner, const void* in, double* rho,
+                        double* acc, void* yout, double dt, hipStream_t s);
+hipError_t LaunchRk4F32(const RhsArgs& a, int stage, bool horner, const void* in, double* rho,
+                        double* acc, void* yout, double dt, hipStream_t s);
 hipError_t LaunchRkf45F64(const RhsArgs& a, int stage, const void* in, const double* rho,
                          const Rkf45Bufs<double>& b, const Rkf45Coefs& c, hipStream_t s);
 hipError_t LaunchRkf45F32(const RhsArgs& a, int stage, const void* in, const double* rho,
                          const Rkf45Bufs<float>& b, const Rkf45Coefs& c, hipStream_t s);
 
-hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, const void* in, double* rho,
-                          double* acc, void* yout, double dt, hipStream_t s) {
-  return fp32 ? LaunchRk4F32(a, stage, in, rho, acc, yout, dt, s)
-              : LaunchRk4F64(a, stage, in, rho, acc, yout, dt, s);
+hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, bool horner, const void* in,
+                          double* rho, double* acc, void* yout, double dt, hipStream_t s) {
+  return fp32 ? LaunchRk4F32(a, stage, horner, in, rho, acc, yout, dt, s)
+              : LaunchRk4F64(a, stage, horner, in, rho, acc, yout, dt, s);
 }
 
 hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* in,

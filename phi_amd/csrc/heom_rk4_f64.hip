@@ -3,11 +3,10 @@
 
 namespace phi {
 
-hipError_t LaunchRk4F64(const RhsArgs& a, int stage, const void* in, double* rho, double* acc,
-                       void* yout, double dt, hipStream_t s) {
-  if (stage == 0)
-    return LaunchRk4T<double, double>(a, 0, (const double*)in, rho, acc, (double*)yout, dt, s);
-  return LaunchRk4T<double, double>(a, stage, (const double*)in, rho, acc, (double*)yout, dt, s);
+hipError_t LaunchRk4F64(const RhsArgs& a, int stage, bool horner, const void* in, double* rho,
+                        double* acc, void* yout, double dt, hipStream_t s) {
+  return LaunchRk4T<double, double>(a, stage, horner, (const double*)in, rho, acc,
+                                    (double*)yout, dt, s);
 }
 
 }  // namespace phi

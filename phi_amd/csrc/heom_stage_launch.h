@@ -4,45 +4,117 @@
 #define PHI_AMD_HEOM_STAGE_LAUNCH_H_
 
 #include "heom_device.h"
+#include "heom_stream.h"
 
 namespace phi {
 
 inline int Blocks(const RhsArgs& a) { return (a.num_nodes + a.tile - 1) / a.tile; }
 
+// Blocks of a persistent kernel that are resident at once on the device
+// (occupancy per CU times the CU count), cached per kernel and shape.
+int ResidentBlocks(const void* kernel, int threads, size_t lds);
+
+// One launch of the persistent stage kernel: `elems` elements per thread
+// (1..4) and `mps` modes per site (1..2) select the instantiation.
+template <typename TB, typename TI, int EPT, int MPS, class EPI>
+inline void LaunchStreamK(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
+                          hipStream_t s) {
+  const size_t sh = dev::StreamLdsBytes(a.N, a.D, a.L, a.tile, sizeof(TB));
+  const void* k = reinterpret_cast<const void*>(&dev::stream_stage<TB, TI, EPT, MPS, EPI>);
+  const int ntiles = (a.num_nodes + a.tile - 1) / a.tile;
+  int g = grid > 0 ? grid : ResidentBlocks(k, threads, sh);
+  if (g > ntiles) g = ntiles;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL((dev::stream_stage<TB, TI, EPT, MPS, EPI>), dim3(g), dim3(threads), sh, s, a,
+                     in, epi);
+}
+
+template <typename TB, typename TI, int MPS, class EPI>
+inline void LaunchStreamE(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
+                          hipStream_t s) {
+  const int ept = (a.tile * a.NE + threads - 1) / threads;
+  switch (ept) {
+    case 1: LaunchStreamK<TB, TI, 1, MPS, EPI>(a, in, epi, threads, grid, s); break;
+    case 2: LaunchStreamK<TB, TI, 2, MPS, EPI>(a, in, epi, threads, grid, s); break;
+    case 3: LaunchStreamK<TB, TI, 3, MPS, EPI>(a, in, epi, threads, grid, s); break;
+    default: LaunchStreamK<TB, TI, 4, MPS, EPI>(a, in, epi, threads, grid, s); break;
+  }
+}
+
+template <typename TB, typename TI, class EPI>
+inline hipError_t LaunchStream(const RhsArgs& a0, const TI* in, const EPI& epi, hipStream_t s) {
+  const bool fp32 = sizeof(TB) == 4;
+  RhsArgs a = a0;
+  a.tile = a0.col_tile[fp32 ? 1 : 0];
+  const int threads = a0.col_threads[fp32 ? 1 : 0];
+  const int grid = a0.col_grid[fp32 ? 1 : 0];
+  if (a.mps <= 1) LaunchStreamE<TB, TI, 1, EPI>(a, in, epi, threads, grid, s);
+  else LaunchStreamE<TB, TI, 2, EPI>(a, in, epi, threads, grid, s);
+  return hipGetLastError();
+}
+
+// RK4 stage through the persistent kernels (stage 0 reads the FP64 state).
+template <typename TB>
+inline hipError_t LaunchRk4Stream(const RhsArgs& a, int stage, bool horner, const void* in,
+                                  double* rho, double* acc, TB* yout, double dt, hipStream_t s) {
+  const double* x = (const double*)in;
+  const TB* y = (const TB*)in;
+  if (horner) {
+    switch (stage) {
+      case 0: return LaunchStream<TB, double>(a, x, dev::EpiNested<TB, 0, true>{rho, yout, dt / 4}, s);
+      case 1: return LaunchStream<TB, TB>(a, y, dev::EpiNested<TB, 1, false>{rho, yout, dt / 3}, s);
+      case 2: return LaunchStream<TB, TB>(a, y, dev::EpiNested<TB, 2, false>{rho, yout, dt / 2}, s);
+      default: return LaunchStream<TB, TB>(a, y, dev::EpiNested<TB, 3, false>{rho, yout, dt}, s);
+    }
+  }
+  switch (stage) {
+    case 0: return LaunchStream<TB, double>(a, x, dev::EpiClassic<TB, 0, true>{rho, acc, yout, dt}, s);
+    case 1: return LaunchStream<TB, TB>(a, y, dev::EpiClassic<TB, 1, false>{rho, acc, yout, dt}, s);
+    case 2: return LaunchStream<TB, TB>(a, y, dev::EpiClassic<TB, 2, false>{rho, acc, yout, dt}, s);
+    default: return LaunchStream<TB, TB>(a, y, dev::EpiClassic<TB, 3, false>{rho, acc, yout, dt}, s);
+  }
+}
+
 // The launch shape of a precision: nodes and threads per block chosen on the
 // host for that precision's LDS budget.
 inline RhsArgs Shape(const RhsArgs& a, bool fp32, int* threads) {
   RhsArgs b = a;
-  b.tile = a.col_tile[fp32 ? 1 : 0];
-  *threads = a.col_threads[fp32 ? 1 : 0];
+  if (a.col_nm == 2) {
+    b.tile = a.v1_tile[fp32 ? 1 : 0];
+    *threads = 256;
+  } else {
+    b.tile = a.col_tile[fp32 ? 1 : 0];
+    *threads = a.col_threads[fp32 ? 1 : 0];
+  }
   return b;
 }
 
-template <typename TB, typename TI, int NM>
+template <typename TB, typename TI, int NM, bool HORNER>
 inline void LaunchRk4NM(const RhsArgs& a, int stage, const TI* in, double* rho, double* acc,
                         TB* yout, double dt, int threads, hipStream_t s) {
   const size_t sh = RhsSharedBytes(a, sizeof(TB) == 4);
   const dim3 grid(Blocks(a)), block(threads);
   switch (stage) {
-    case 0: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 0>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
-    case 1: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 1>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
-    case 2: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 2>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
-    default: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 3>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
+    case 0: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 0, HORNER>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
+    case 1: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 1, HORNER>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
+    case 2: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 2, HORNER>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
+    default: hipLaunchKernelGGL((dev::rk4_stage<TB, TI, NM, 3, HORNER>), grid, block, sh, s, a, in, rho, acc, yout, dt); break;
   }
 }
 
+// `horner` selects the nested stage order (see rk4_stage).
 template <typename TB, typename TI>
-inline hipError_t LaunchRk4T(const RhsArgs& a0, int stage, const TI* in, double* rho, double* acc,
-                             TB* yout, double dt, hipStream_t s) {
+inline hipError_t LaunchRk4T(const RhsArgs& a0, int stage, bool horner, const TI* in, double* rho,
+                             double* acc, TB* yout, double dt, hipStream_t s) {
+  if (a0.col_nm == 2) return LaunchRk4Stream<TB>(a0, stage, horner, in, rho, acc, yout, dt, s);
   int threads;
   const RhsArgs a = Shape(a0, sizeof(TB) == 4, &threads);
-  switch (a.col_nm) {
-    case 1: LaunchRk4NM<TB, TI, 1>(a, stage, in, rho, acc, yout, dt, threads, s); break;
-    case 8: LaunchRk4NM<TB, TI, 8>(a, stage, in, rho, acc, yout, dt, threads, s); break;
-    case 16: LaunchRk4NM<TB, TI, 16>(a, stage, in, rho, acc, yout, dt, threads, s); break;
-    case 24: LaunchRk4NM<TB, TI, 24>(a, stage, in, rho, acc, yout, dt, threads, s); break;
-    case 32: LaunchRk4NM<TB, TI, 32>(a, stage, in, rho, acc, yout, dt, threads, s); break;
-    default: LaunchRk4NM<TB, TI, 0>(a, stage, in, rho, acc, yout, dt, threads, s); break;
+  if (a.col_nm >= 1) {
+    if (horner) LaunchRk4NM<TB, TI, 1, true>(a, stage, in, rho, acc, yout, dt, threads, s);
+    else LaunchRk4NM<TB, TI, 1, false>(a, stage, in, rho, acc, yout, dt, threads, s);
+  } else {
+    if (horner) LaunchRk4NM<TB, TI, 0, true>(a, stage, in, rho, acc, yout, dt, threads, s);
+    else LaunchRk4NM<TB, TI, 0, false>(a, stage, in, rho, acc, yout, dt, threads, s);
   }
   return hipGetLastError();
 }
@@ -68,17 +140,10 @@ inline hipError_t LaunchRkf45T(const RhsArgs& a0, int stage, const TI* in, const
                                const Rkf45Bufs<TB>& b, const Rkf45Coefs& c, hipStream_t s) {
   int threads;
   const RhsArgs a = Shape(a0, sizeof(TB) == 4, &threads);
-  switch (a.col_nm) {
-    case 1: LaunchRkf45NM<TB, TI, 1>(a, stage, in, rho, b, c, threads, s); break;
-    case 8: LaunchRkf45NM<TB, TI, 8>(a, stage, in, rho, b, c, threads, s); break;
-    case 16: LaunchRkf45NM<TB, TI, 16>(a, stage, in, rho, b, c, threads, s); break;
-    case 24: LaunchRkf45NM<TB, TI, 24>(a, stage, in, rho, b, c, threads, s); break;
-    case 32: LaunchRkf45NM<TB, TI, 32>(a, stage, in, rho, b, c, threads, s); break;
-    default: LaunchRkf45NM<TB, TI, 0>(a, stage, in, rho, b, c, threads, s); break;
-  }
+  if (a.col_nm >= 1) LaunchRkf45NM<TB, TI, 1>(a, stage, in, rho, b, c, threads, s);
+  else LaunchRkf45NM<TB, TI, 0>(a, stage, in, rho, b, c, threads, s);
   return hipGetLastError();
 }
-
 
 }  // namespace phi
 

@@ -1,0 +1,379 @@
+// Persistent, software-pipelined stage kernels of the HEOM integrator for
+// systems with N <= 32 (the matrix-core GEMM form).  Included by the
+// per-method translation units.
+//
+// A block owns a sequence of tiles (T consecutive nodes each): tile
+// blockIdx.x, then + gridDim.x, ...  The per-block constants (H as MFMA
+// A-fragments, Delta and the edge-prefactor tables) are staged in LDS once.
+// While tile i is combined, the global loads of tile i+1 are in flight:
+//
+//   top of iteration i:  elementwise part of tile i from registers -> LDS
+//                        issue the epilogue loads of tile i and the input
+//                        and edge-record loads of tile i+1
+//   barrier              -(i/hbar) H Y of tile i on the matrix cores (LDS)
+//   barrier              issue the neighbour gathers of tile i+1 (their edge
+//                        records arrived during the GEMM)
+//                        epilogue of tile i: k = W + W^dagger, stage update
+//
+// LDS holds Y and W of two tiles (alternating), so no barrier is needed
+// between the epilogue of tile i and the elementwise part of tile i+1.
+//
+// Elements are numbered e = t*N*N + b*N + a (node t of the tile, column b,
+// row a), thread tid owns e = tid + i*blockDim.x (i < EPT), so every global
+// access of a wavefront is contiguous within a column.  In LDS, Y and W use
+// column stride N + 1.
+//
+// Edge records (built on the host): edge[(n*N + b)*MPS + q] = {id of
+// n + e_j, id of n - e_j, j*L + n_j, 0} for the q-th mode j coupled to site
+// b, ids -1 where the neighbour does not exist, and {-1, -1, -1, 0} for
+// unused slots.  The RHS it evaluates is the same as rhs_phase_mfma's:
+//   W(a,b) = -(i/hbar) (H Y)(a,b) - (mu/2 + (Delta_a + Delta_b)/2 [a != b]) Y(a,b)
+//            + sum_q i s_q Y_{n+e_j}(a,b) - col_q Y_{n-e_j}(a,b),
+//   k = W + W^dagger.
+#ifndef PHI_AMD_HEOM_STREAM_H_
+#define PHI_AMD_HEOM_STREAM_H_
+
+#include "heom_device.h"
+
+namespace phi {
+namespace dev {
+
+// H fragments in LDS: hf[(mt*KS + st)*64 + lane] = H(16 mt + lane%16,
+// (lane/16)*KS + st) (zero outside N), real parts then imaginary parts.
+__host__ __device__ inline int StreamFragCount(int N) {
+  return ((N + 15) >> 4) * ((N + 3) >> 2) * 64;
+}
+
+__host__ __device__ inline size_t StreamLdsBytes(int N, int D, int L, int T, size_t real) {
+  const size_t V = 2 * real, NP = (size_t)N * (N + 1);
+  size_t b = V * (4 * NP * T + (size_t)D * L);        // Y[2], W[2], pcol
+  b += real * ((size_t)D * L + N + 2 * (size_t)StreamFragCount(N));  // ns, delta, hf
+  return Align16(b);
+}
+
+template <typename TB>
+struct StreamLds {
+  typedef typename C2<TB>::type V;
+  V* Y;       // two tiles, the second at Y + bstride
+  V* W;       // two tiles, the second at W + bstride
+  int bstride;
+  V* pcol;
+  TB* ns;
+  TB* delta;
+  TB* hfr;
+  TB* hfi;
+};
+
+template <typename TB>
+__device__ __forceinline__ StreamLds<TB> stream_carve(unsigned char* smem, int N, int D, int L,
+                                                     int T) {
+  typedef typename C2<TB>::type V;
+  const int NP = N * (N + 1);
+  StreamLds<TB> s;
+  V* p = reinterpret_cast<V*>(smem);
+  s.bstride = NP * T;
+  s.Y = p;
+  s.W = p + 2 * NP * T;
+  s.pcol = p + 4 * NP * T;
+  s.ns = reinterpret_cast<TB*>(s.pcol + D * L);
+  s.delta = s.ns + D * L;
+  s.hfr = s.delta + N;
+  s.hfi = s.hfr + StreamFragCount(N);
+  return s;
+}
+
+// -(i/hbar) H Y of one tile on the matrix cores, added into W.  Lane l of a
+// 16x16x4 MFMA holds A[l%16][l/16] and B[l/16][l%16]; the K dimension is
+// split so that lane group g = l/16 covers rows g*KS .. g*KS + KS - 1.
+template <typename TB>
+__device__ __forceinline__ void stream_gemm(const RhsArgs& A, const StreamLds<TB>& s,
+                                            const typename C2<TB>::type* Y,
+                                            typename C2<TB>::type* W, int nt) {
+  typedef typename C2<TB>::type V;
+  typedef Mfma<TB> M;
+  const int N = A.N;
+  const int NP = N * (N + 1);
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4;
+  const int c16 = lane & 15;
+  const int KS = (N + 3) >> 2;
+  const int MT = (N + 15) >> 4;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const TB ih = (TB)A.inv_hbar;
+  const bool realh = A.real_h != 0;
+  const int ncols = nt * N;
+  const int nslices = (ncols + 15) >> 4;
+  // Work items: (16-column slice, 16-row tile of H) pairs over the waves.
+  for (int item = wave; item < nslices * MT; item += nwaves) {
+    const int slice = item / MT;
+    const int mt = item - slice * MT;
+    const int j = slice * 16 + c16;
+    const bool jok = j < ncols;
+    const int tj = jok ? fdiv(j, A.mN) : 0;
+    const int bj = j - tj * N;
+    const V* ycol = Y + tj * NP + bj * (N + 1);
+    typename M::acc xr = M::zero(), xi = M::zero();
+    for (int st = 0; st < KS; ++st) {
+      const int k = g * KS + st;
+      V y = mk<TB>(0, 0);
+      if (jok && k < N) y = ycol[k];
+      const TB hr = s.hfr[(mt * KS + st) * 64 + lane];
+      xr = M::mac(hr, y.x, xr);
+      xi = M::mac(hr, y.y, xi);
+      if (!realh) {
+        const TB hi = s.hfi[(mt * KS + st) * 64 + lane];
+        xr = M::mac(-hi, y.y, xr);
+        xi = M::mac(hi, y.x, xi);
+      }
+    }
+    if (jok) {
+      V* wcol = W + tj * NP + bj * (N + 1);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * mt + M::row(g, r);
+        if (row < N) {
+          V w = wcol[row];
+          w.x += ih * xi[r];
+          w.y -= ih * xr[r];
+          wcol[row] = w;
+        }
+      }
+    }
+  }
+}
+
+// Registers of the next tile in flight (input element, 0.5 mu, edge records
+// and neighbour gathers of each owned element).
+template <typename TB, typename TI, int EPT, int MPS>
+struct StreamRegs {
+  typedef typename C2<TI>::type VI;
+  VI y[EPT];
+  TB mu2[EPT];
+  int4 rec[EPT][MPS];
+  VI gn[EPT][MPS];
+  VI gp[EPT][MPS];
+};
+
+// Element e of this thread: node te, in-node offset r = b*N + a.
+__device__ __forceinline__ void stream_coords(const RhsArgs& A, int i, int* te, int* r, int* b,
+                                              int* a) {
+  const int e = threadIdx.x + i * blockDim.x;
+  *te = fdiv(e, A.mNE);
+  *r = e - *te * A.NE;
+  *b = fdiv(*r, A.mN);
+  *a = *r - *b * A.N;
+}
+
+template <typename TB, typename TI, int EPT, int MPS>
+__device__ __forceinline__ void stream_load1(const RhsArgs& A, const TI* __restrict__ in, int base,
+                                             int tot, StreamRegs<TB, TI, EPT, MPS>& R) {
+  typedef typename C2<TI>::type VI;
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    if (e < tot) {
+      int te, r, b, a;
+      stream_coords(A, i, &te, &r, &b, &a);
+      const int n = base + te;
+      R.y[i] = reinterpret_cast<const VI*>(in)[(size_t)n * A.NE + r];
+      R.mu2[i] = (TB)(0.5 * A.mu[n]);
+      const int4* ed = A.edge + ((size_t)n * A.N + b) * MPS;
+#pragma unroll
+      for (int q = 0; q < MPS; ++q) R.rec[i][q] = ed[q];
+    }
+  }
+}
+
+template <typename TB, typename TI, int EPT, int MPS>
+__device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restrict__ in, int tot,
+                                             StreamRegs<TB, TI, EPT, MPS>& R) {
+  typedef typename C2<TI>::type VI;
+  const VI* v = reinterpret_cast<const VI*>(in);
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    if (e < tot) {
+      const int r = e - fdiv(e, A.mNE) * A.NE;
+#pragma unroll
+      for (int q = 0; q < MPS; ++q) {
+        const int4 c = R.rec[i][q];
+        if (c.x >= 0) R.gn[i][q] = v[(size_t)c.x * A.NE + r];
+        if (c.y >= 0) R.gp[i][q] = v[(size_t)c.y * A.NE + r];
+      }
+    }
+  }
+}
+
+// One element per thread allows blocks of up to 1024 threads (a whole
+// 24 x 24 node per block); more elements per thread keep blocks at <= 256.
+template <typename TB, typename TI, int EPT, int MPS, class EPI>
+__global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
+    stream_stage(RhsArgs A, const TI* __restrict__ in, EPI epi) {
+  typedef typename C2<TB>::type V;
+  typedef StreamRegs<TB, TI, EPT, MPS> Regs;
+  typedef typename EPI::Pre Pre;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N, NE = A.NE, D = A.D, L = A.L, T = A.tile;
+  const int NP = N * (N + 1);
+  const StreamLds<TB> s = stream_carve<TB>(smem, N, D, L, T);
+  const int ntiles = (A.num_nodes + T - 1) / T;
+  int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+
+  // Per-block constants.
+  for (int e = threadIdx.x; e < D * L; e += blockDim.x) {
+    s.pcol[e] = ld<TB, double>(reinterpret_cast<const double*>(A.prev_col), e);
+    s.ns[e] = (TB)A.next_s[e];
+  }
+  for (int e = threadIdx.x; e < N; e += blockDim.x) s.delta[e] = (TB)A.delta[e];
+  {
+    const int KS = (N + 3) >> 2, nf = StreamFragCount(N);
+    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+      const int lane = f & 63, ms = f >> 6;
+      const int mt = ms / KS, st = ms - mt * KS;
+      const int row = 16 * mt + (lane & 15), k = (lane >> 4) * KS + st;
+      double2 h = make_double2(0., 0.);
+      if (row < N && k < N) h = A.H[row + k * N];
+      s.hfr[f] = (TB)h.x;
+      s.hfi[f] = (TB)h.y;
+    }
+  }
+
+  Regs cur;
+  int base = tile * T;
+  int nt = min(T, A.num_nodes - base);
+  stream_load1<TB, TI, EPT, MPS>(A, in, base, nt * NE, cur);
+  stream_load2<TB, TI, EPT, MPS>(A, in, nt * NE, cur);
+  __syncthreads();  // per-block tables visible
+
+  for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1) {
+    const int tot = nt * NE;
+    V* Yb = s.Y + buf * s.bstride;
+    V* Wb = s.W + buf * s.bstride;
+    // (1) elementwise part of this tile -> LDS; epilogue operands of this tile
+    Pre pre[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < tot) {
+        int te, r, b, a;
+        stream_coords(A, i, &te, &r, &b, &a);
+        const size_t g = (size_t)(base + te) * NE + r;
+        if (EPI::kInIsState) pre[i] = epi.from_state(make_double2((double)cur.y[i].x,
+                                                                  (double)cur.y[i].y));
+        else pre[i] = epi.pre(g);
+        const V y = mk<TB>((TB)cur.y[i].x, (TB)cur.y[i].y);
+        TB f = cur.mu2[i];
+        if (a != b) f += (TB)0.5 * (s.delta[a] + s.delta[b]);
+        TB ax = -f * y.x, ay = -f * y.y;
+#pragma unroll
+        for (int q = 0; q < MPS; ++q) {
+          const int4 c = cur.rec[i][q];
+          if (c.x >= 0) {
+            const TB sc = s.ns[c.z];
+            const TB gx = (TB)cur.gn[i][q].x, gy = (TB)cur.gn[i][q].y;
+            ax -= sc * gy;
+            ay += sc * gx;
+          }
+          if (c.y >= 0) {
+            const V pc = s.pcol[c.z];
+            const TB gx = (TB)cur.gp[i][q].x, gy = (TB)cur.gp[i][q].y;
+            ax -= pc.x * gx - pc.y * gy;
+            ay -= pc.x * gy + pc.y * gx;
+          }
+        }
+        const int o = te * NP + b * (N + 1) + a;
+        Yb[o] = y;
+        Wb[o] = mk<TB>(ax, ay);
+      }
+    }
+    // (2) first loads of the next tile
+    const int ntile = tile + gridDim.x;
+    const int nbase = ntile * T;
+    const int nnt = ntile < ntiles ? min(T, A.num_nodes - nbase) : 0;
+    if (nnt > 0) stream_load1<TB, TI, EPT, MPS>(A, in, nbase, nnt * NE, cur);
+    __syncthreads();
+    // (3) matrix-core product
+    stream_gemm<TB>(A, s, Yb, Wb, nt);
+    __syncthreads();
+    // (4) neighbour gathers of the next tile
+    if (nnt > 0) stream_load2<TB, TI, EPT, MPS>(A, in, nnt * NE, cur);
+    // (5) epilogue of this tile
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < tot) {
+        int te, r, b, a;
+        stream_coords(A, i, &te, &r, &b, &a);
+        const V p = Wb[te * NP + b * (N + 1) + a];
+        const V q = Wb[te * NP + a * (N + 1) + b];
+        double2 k;
+        k.x = (double)(p.x + q.x);
+        k.y = (double)(p.y - q.y);
+        epi.apply((size_t)(base + te) * NE + r, k, pre[i]);
+      }
+    }
+    base = nbase;
+    nt = nnt;
+  }
+}
+
+// --- RK4 epilogues ---------------------------------------------------------------
+// Nested form: stage s < 3 writes v = rho + c k (c = dt/(4-s)), stage 3
+// writes rho += dt k.  Stage 0 reads rho as its input (kInIsState).
+template <typename TB, int STAGE, bool IN_STATE>
+struct EpiNested {
+  struct Pre { double2 x; };
+  static constexpr bool kInIsState = IN_STATE;
+  double* rho;
+  TB* yout;
+  double c;
+  __device__ __forceinline__ Pre pre(size_t g) const { return Pre{ld2(rho, g)}; }
+  __device__ __forceinline__ Pre from_state(double2 x) const { return Pre{x}; }
+  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p) const {
+    if (STAGE < 3) st<TB>(yout, g, p.x.x + c * k.x, p.x.y + c * k.y);
+    else st<double>(rho, g, p.x.x + c * k.x, p.x.y + c * k.y);
+  }
+};
+
+// Classic form (PHI's order):
+//   s0: acc = rho + dt/6 k, v = rho + dt/2 k     s1: acc += dt/3 k, v = rho + dt/2 k
+//   s2: acc += dt/3 k, v = rho + dt k            s3: rho = acc + dt/6 k
+template <typename TB, int STAGE, bool IN_STATE>
+struct EpiClassic {
+  struct Pre { double2 x, q; };
+  static constexpr bool kInIsState = IN_STATE;
+  double* rho;
+  double* acc;
+  TB* yout;
+  double dt;
+  __device__ __forceinline__ Pre pre(size_t g) const {
+    Pre p;
+    if (STAGE <= 2) p.x = ld2(rho, g);
+    if (STAGE >= 1) p.q = ld2(acc, g);
+    return p;
+  }
+  __device__ __forceinline__ Pre from_state(double2 x) const {
+    Pre p;
+    p.x = x;
+    return p;
+  }
+  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p) const {
+    if (STAGE == 0) {
+      st<double>(acc, g, p.x.x + (dt / 6) * k.x, p.x.y + (dt / 6) * k.y);
+      st<TB>(yout, g, p.x.x + (dt / 2) * k.x, p.x.y + (dt / 2) * k.y);
+    } else if (STAGE <= 2) {
+      st<double>(acc, g, p.q.x + (dt / 3) * k.x, p.q.y + (dt / 3) * k.y);
+      const double c = STAGE == 1 ? dt / 2 : dt;
+      st<TB>(yout, g, p.x.x + c * k.x, p.x.y + c * k.y);
+    } else {
+      st<double>(rho, g, p.q.x + (dt / 6) * k.x, p.q.y + (dt / 6) * k.y);
+    }
+  }
+};
+
+}  // namespace dev
+}  // namespace phi
+
+#endif  // PHI_AMD_HEOM_STREAM_H_

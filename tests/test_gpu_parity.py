@@ -209,3 +209,62 @@ def test_steady_state_c1_small():
     Lx = orc.rhs(x)
     assert abs(np.trace(x[0]) - 1) < 1e-10
     assert np.abs(Lx).max() < 1e-8
+
+
+def _subset_params(cfg, lh):
+    path = os.path.join(CFG, cfg + ".prm")
+    op = PO.read_command_line(path, "rk4")
+    op.hierarchy_truncation_level = lh
+    p = P.PhiParameters.read(path, "rk4")
+    p.set_int("hierarchy_truncation_level", lh)
+    return op, p
+
+
+@pytest.mark.parametrize("cfg,lh,steps", [("c2", 5, 30), ("c3", 3, 20)])
+def test_rk4_forms_and_launch_shapes_match_oracle(cfg, lh, steps):
+    """Both RK4 stage orders (PHI's classic k0..k3 and the nested Taylor
+    form), both GEMM forms (matrix cores / element) and several block
+    shapes give the oracle's state within 1e-10 (FP64)."""
+    op, p = _subset_params(cfg, lh)
+    orc = HO.Oracle(HO.Problem.from_prm(op), threads=8)
+    dt = op.timestep
+    orc.rk4(steps * dt * (1 - 1e-9), dt, 1.0)
+    ref = orc.state()
+    shapes = ((2, 0, 0, 0), (2, 1, 0, 0), (2, 3, 0, 7), (2, 1, 64, 0), (2, 2, 256, 5),
+              (1, 0, 0, 0), (0, 0, 0, 0), (1, 1, 64, 0), (1, 2, 128, 0), (0, 3, 192, 0))
+    for form in ("nested", "classic"):
+        for gemm, tile, threads, blocks in shapes:
+            print("case", cfg, form, gemm, tile, threads, blocks, flush=True)
+            ctx = P.Context(p)
+            ctx.set_rk4_form(form)
+            try:
+                ctx.set_launch(gemm, tile, threads, blocks)
+            except P.PhiError as e:
+                # a shape this system cannot take (e.g. 4+ elements per thread)
+                print("  skipped:", e)
+                ctx.close()
+                continue
+            ctx.rk4_steps(steps, dt)
+            err = np.abs(ctx.state() - ref).max()
+            ctx.close()
+            assert err <= 1e-10, (form, gemm, tile, threads, blocks, err)
+
+
+@pytest.mark.parametrize("cfg,steps", [("c2", 1000), ("c3", 1000)])
+def test_fp32_bulk_population_error_vs_fp64(cfg, steps):
+    """FP32-bulk RK4 (FP32 RHS, FP64 state) against the FP64 path over 1 ps:
+    population error <= 1e-6; coherences reported."""
+    p = P.PhiParameters.read(os.path.join(CFG, cfg + ".prm"), "rk4")
+    dt = p.get_double("timestep")
+    out = {}
+    for prec in ("fp64", "fp32"):
+        ctx = P.Context(p, precision=prec)
+        ctx.rk4_steps(steps, dt)
+        out[prec] = ctx.rho0()
+        ctx.close()
+    d = out["fp32"] - out["fp64"]
+    pop = np.abs(np.real(np.diag(d))).max()
+    coh = np.abs(d - np.diag(np.diag(d))).max()
+    print("%s fp32 population error %.3e coherence error %.3e" % (cfg, pop, coh))
+    assert abs(np.trace(out["fp64"]) - 1) < 1e-10
+    assert pop <= 1e-6
