@@ -1,26 +1,34 @@
-"""Benchmark: HEOM RK4 on the 7-site exciton (configs c2), one MI355X.
+"""Benchmark: HEOM RK4 on the 7-site exciton, one MI355X.
 
-Workload: 7-site exciton, depth 8 (PHI HierarchyTruncation=9), 6435 nodes of
-7x7 complex ADOs, RK4 with dt = 1 fs; one step = one RK4 step over every node
-(4 fused RHS+stage kernels).  K timed steps (default 1000 = 1 ps) after W
-warmup steps, with the state resident in HBM.  Prints one JSON line.
+Workload (the reference's headline configuration): 7-site exciton with 7
+projector baths, depth 8 (PHI HierarchyTruncation=9), 6435 nodes of 7x7
+complex ADOs, RK4 with dt = 1 fs; one step = one RK4 step over every node
+(4 fused RHS + stage kernels, nested form).  W warm-up steps, then K timed
+steps (default 1000 = 1 ps) with the state resident in HBM, bracketed by
+device synchronisation.  Prints one JSON line on stdout.
 
-value = node-steps/s of the plain FP64 path; the FP32-bulk path (FP32 RHS,
-FP64 state) is reported beside it with its population error against FP64.
-
-roofline: the dominant kernel is the RK4 stage-1 kernel (RHS + acc += dt/3 k,
-Y = rho + dt/2 k).  Algorithmic bytes per node for it are N^2 (48 + 2 s)
-(reads Y, rho, acc; writes acc, Y'; s = 16 B for FP64) - stage 1 of the
-per-step total N^2 (160 + 6 s); neighbour reads count as cache hits.  The
-kernel's mean duration comes from HIP events on the engine's stream.
-
-cpu_baseline: the oracle's pthread restatement of PHI's RK4 loop on the
-host cores, timed on a bounded sample of the same workload.
+value    node-steps/s of the plain FP64 path (max |drho| <= 1e-10 against the
+         oracle in the GPU tests); the FP32-bulk path (FP32 RHS, FP64 state)
+         is reported beside it with its population error against FP64 after
+         the same W + K steps.
+roofline the dominant kernel (the stage kernel with the largest mean
+         duration in the FP64 run): algorithmic bytes per launch over its
+         mean duration, both from HIP events on the engine's stream.
+         Algorithmic bytes per node (nested RK4 form, s = bytes per bulk
+         complex element): stage 0 N^2 (16 + s), stages 1-2 N^2 (16 + 2 s),
+         stage 3 N^2 (32 + s); neighbour columns count as cache hits.
+         traffic = HBM bytes per launch of that kernel from the committed
+         rocprofv3 counter summary (profiles/), when present.
+cpu_baseline  the oracle's pthread restatement of PHI's RK4 loop (FP64) on
+         the host cores, timed on a bounded sample of the same workload.
+c3_fp32  the 24-site ring at depth 4 (20475 nodes of 24x24), FP32-bulk RK4:
+         node-steps/s and the HBM fraction of its stage kernels.
 """
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -29,14 +37,21 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 CFG = os.path.join(ROOT, "phi_amd", "configs", "c2.prm")
+CFG3 = os.path.join(ROOT, "phi_amd", "configs", "c3.prm")
+PROFILE = os.path.join(ROOT, "profiles", "rk4_stage_7site_depth8_traffic.txt")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def stage_bytes(N, s):
+    """Algorithmic bytes per node of the four nested-form stage kernels."""
+    ne = N * N
+    return [ne * (16 + s), ne * (16 + 2 * s), ne * (16 + 2 * s), ne * (32 + s)]
+
+
 def cpu_baseline(seconds=12.0):
-    import numpy as np  # noqa: F401
     from oracle import heom_oracle as HO
     from oracle import prm_oracle as PO
     op = PO.read_command_line(CFG, "rk4")
@@ -56,12 +71,50 @@ def cpu_baseline(seconds=12.0):
                       "%d pthreads, %.1f s" % (steps, threads, el)}
 
 
+def profile_traffic(kernel_stage, prec):
+    """HBM bytes per launch of a stage kernel from the committed counter
+    summary (lines 'stage=<s> prec=<p> fetch_bytes=<F> write_bytes=<W>')."""
+    if not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        m = re.match(r"\s*stage=(\d) prec=(\w+) fetch_bytes=([\d.eE+]+) write_bytes=([\d.eE+]+)",
+                     line)
+        if m and int(m.group(1)) == kernel_stage and m.group(2) == prec:
+            return float(m.group(3)) + float(m.group(4))
+    return None
+
+
+def run_rk4(P, p, prec, steps, warmup, tune=True):
+    dt = p.get_double("timestep")
+    ctx = P.Context(p, precision=prec)
+    shape = None
+    if tune:
+        _, shape = ctx.autotune(dt, steps=5)
+    ctx.rk4_steps(warmup, dt)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.rk4_steps(steps, dt)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    rho0 = ctx.rho0()
+    ms = ctx.rk4_stage_times(min(200, max(steps, 20)), dt) / min(200, max(steps, 20))
+    out = {"seconds": el, "node_steps_per_s": ctx.num_nodes * steps / el,
+           "ms_per_step": 1e3 * el / steps, "stage_ms": ms.tolist(), "rho0": rho0,
+           "nodes": ctx.num_nodes, "shape": shape}
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="first copy 1 GiB device to device (a plain copy of known size, "
+                         "the reference point of the HBM byte counters under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this integrator runs on a single MI355X "
@@ -71,41 +124,36 @@ def main():
     import numpy as np
     import phi_amd as P
 
+    if args.calibration_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("bench: calibration copy of %d bytes done" % (a.numel() * 4))
+        del a, b
+
     p = P.PhiParameters.read(CFG, "rk4")
     N = p.get_int("num_states")
     dt = p.get_double("timestep")
     res = {}
     for prec in ("fp64", "fp32"):
-        log("bench: %s context" % prec)
-        ctx = P.Context(p, precision=prec)
-        nodes = ctx.num_nodes
-        ctx.rk4_steps(args.warmup, dt)
-        ctx.synchronize()
-        ctx2 = P.Context(p, precision=prec)
-        ctx2.synchronize()
-        t0 = time.perf_counter()
-        ctx2.rk4_steps(args.steps, dt)
-        ctx2.synchronize()
-        el = time.perf_counter() - t0
-        rho0 = ctx2.rho0()
-        st_steps = min(args.steps, 200)
-        ms = ctx.rk4_stage_times(st_steps, dt) / st_steps
-        res[prec] = {"seconds": el, "node_steps_per_s": nodes * args.steps / el,
-                     "ms_per_step": 1e3 * el / args.steps, "stage_ms": ms.tolist(),
-                     "rho0": rho0}
-        log("bench: %s %.3f ms/step, stage ms %s" % (prec, 1e3 * el / args.steps, ms))
-        ctx.close()
-        ctx2.close()
+        log("bench: 7-site depth 8, %s" % prec)
+        res[prec] = run_rk4(P, p, prec, args.steps, args.warmup)
+        log("bench: %s %.4f ms/step, shape %s, stage ms %s"
+            % (prec, res[prec]["ms_per_step"], res[prec]["shape"], res[prec]["stage_ms"]))
     pop64 = np.real(np.diag(res["fp64"]["rho0"]))
     pop32 = np.real(np.diag(res["fp32"]["rho0"]))
     pop_err = float(np.abs(pop64 - pop32).max())
 
-    s = 16
-    ne = N * N
-    nodes = 6435
-    stage1_bytes = ne * (48 + 2 * s) * nodes
-    stage1_s = res["fp64"]["stage_ms"][1] * 1e-3
-    achieved = stage1_bytes / stage1_s / 1e9
+    nodes = res["fp64"]["nodes"]
+    sb = stage_bytes(N, 16)
+    st_ms = res["fp64"]["stage_ms"]
+    dom = int(np.argmax(st_ms))
+    per_launch = sb[dom] * nodes
+    achieved = per_launch / (st_ms[dom] * 1e-3) / 1e9
+    traffic = profile_traffic(dom, "fp64")
+    steps_per_ps = round(1.0 / dt)
     line = {
         "metric": "node-steps/s (RK4, 7-site exciton depth 8, FP64) + time-to-1ps",
         "value": res["fp64"]["node_steps_per_s"],
@@ -118,20 +166,34 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded 7-site exciton Hamiltonian, configs c2)",
+        "data": "synthetic (seeded 7-site exciton Hamiltonian, phi_amd/configs/c2.prm)",
         "config": {"workload": "7-site exciton HEOM, depth 8 (6435 nodes of 7x7), RK4 dt=1 fs",
-                   "nodes": nodes, "num_states": N, "dt_ps": dt,
-                   "time_to_1ps_s_fp64": res["fp64"]["ms_per_step"] * 1000 * 1e-3,
-                   "time_to_1ps_s_fp32_bulk": res["fp32"]["ms_per_step"] * 1000 * 1e-3},
+                   "nodes": nodes, "num_states": N, "dt_ps": dt, "rk4_form": "nested",
+                   "kernel_shape_fp64": res["fp64"]["shape"],
+                   "kernel_shape_fp32": res["fp32"]["shape"],
+                   "time_to_1ps_s_fp64": res["fp64"]["ms_per_step"] * steps_per_ps * 1e-3,
+                   "time_to_1ps_s_fp32_bulk": res["fp32"]["ms_per_step"] * steps_per_ps * 1e-3},
         "fp32_bulk": {"value": res["fp32"]["node_steps_per_s"], "unit": "node-steps/s",
                       "ms_per_step": res["fp32"]["ms_per_step"],
-                      "population_error_vs_fp64": pop_err},
+                      "population_error_vs_fp64": pop_err,
+                      "stage_ms": res["fp32"]["stage_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "rk4_stage<double,double,1>",
-                     "stage_ms_fp64": res["fp64"]["stage_ms"],
-                     "stage_ms_fp32": res["fp32"]["stage_ms"]},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stream_stage (RK4 nested stage %d, FP64)" % dom,
+                     "bytes_per_launch": per_launch, "stage_ms_fp64": st_ms},
     }
+    if not args.no_c3:
+        log("bench: 24-site depth 4, fp32")
+        p3 = P.PhiParameters.read(CFG3, "rk4")
+        r3 = run_rk4(P, p3, "fp32", 50, 5)
+        N3 = p3.get_int("num_states")
+        b3 = stage_bytes(N3, 8)
+        step_bytes = sum(b3) * r3["nodes"]
+        line["c3_fp32"] = {
+            "workload": "24-site ring, depth 4 (20475 nodes of 24x24), FP32-bulk RK4 dt=1 fs",
+            "node_steps_per_s": r3["node_steps_per_s"], "ms_per_step": r3["ms_per_step"],
+            "hbm_frac_step": step_bytes / (r3["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "stage_ms": r3["stage_ms"], "kernel_shape": r3["shape"]}
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline()

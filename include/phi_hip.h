@@ -57,6 +57,7 @@ extern "C" {
 typedef struct phi_params phi_params;
 typedef struct phi_integrator phi_integrator;
 typedef struct phi_ctx phi_ctx;
+typedef struct phi_mlp phi_mlp;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* phi_hip_last_error(void);
@@ -126,6 +127,64 @@ void phi_hip_integrator_destroy(phi_integrator* it);
 /* precision: 0 = FP64, 1 = FP32 bulk with FP64 state. */
 int phi_hip_create(const phi_params* p, int precision, phi_ctx** out);
 void phi_hip_destroy(phi_ctx* c);
+
+/* ---- Ensembles of hierarchies (static disorder) ------------------------------- */
+/* No counterpart in PHI, which integrates one hierarchy per process
+ * (phi.cc:4-10); replaces running PHI once per realisation.  `members`
+ * hierarchies of p's bath model, member m with Hamiltonian H[m*N*N ..]
+ * (complex, column-major) and depth depths[m] (HierarchyTruncation >= 2),
+ * all starting from p's InitialDensityMatrix, in one device slab advanced by
+ * one launch per RK4 stage (persistent kernels; N <= 32).  The context
+ * takes the RK4 calls below (phi_hip_rk4_steps, phi_hip_rk4_history,
+ * phi_hip_autotune, phi_hip_set_launch, ...); num_nodes and the state
+ * calls refer to the whole slab. */
+int phi_hip_ensemble_create(const phi_params* p, int members, const double* H, const int* depths,
+                            int precision, phi_ctx** out);
+/* Number of members (0 for a single hierarchy); counts[m] = nodes of member m. */
+int phi_hip_ensemble_members(const phi_ctx* c, int* counts, int cap);
+/* rho_0 of every member (members * N * N complex; N * N for a single hierarchy). */
+int phi_hip_ensemble_rho0(phi_ctx* c, double* out, size_t n_complex);
+/* nsteps RK4 steps recording, after every step i with i % every == 0, rho_0
+ * (a single hierarchy) or the mean of the members' rho_0 (an ensemble):
+ * *count snapshots of N * N complex into out (the output lines of
+ * Rk4Integrate, hierarchy_integrator.cc:2859-2880). */
+int phi_hip_rk4_history(phi_ctx* c, long long nsteps, double dt, int every, double* out,
+                        size_t n_complex, int* count);
+
+/* ---- Learned truncation closure ------------------------------------------------ */
+/* No counterpart in PHI, which truncates the hierarchy by dropping the tier
+ * beyond HierarchyTruncation (LinkHierarchyNodes, hierarchy_integrator.cc:
+ * 1204-1338, links no neighbour there).  A 3-layer perceptron
+ * (n_in -> n_hidden -> n_hidden -> n_out, GELU) with bfloat16 GEMM operands
+ * and float32 accumulation; parameters are one float32 array W1 (n_hidden x
+ * n_in, row-major), b1, W2, b2, W3 (n_out x n_hidden), b3.  For the closure
+ * of N states and D bath modes: n_in = 2 N^2 + D, n_out = 2 N^2 D. */
+int phi_hip_mlp_create(int n_in, int n_hidden, int n_out, int max_batch, phi_mlp** out);
+void phi_hip_mlp_destroy(phi_mlp* m);
+long long phi_hip_mlp_param_count(const phi_mlp* m);
+int phi_hip_mlp_set_params(phi_mlp* m, const float* flat, size_t n);
+int phi_hip_mlp_get_params(phi_mlp* m, float* flat, size_t n);
+/* y (batch x n_out) = network(x (batch x n_in)), row-major host arrays. */
+int phi_hip_mlp_forward(phi_mlp* m, const float* x, int batch, float* y);
+/* *loss = mean((network(x) - t)^2) and its gradient (parameter layout). */
+int phi_hip_mlp_gradients(phi_mlp* m, const float* x, const float* t, int batch, double* loss,
+                          float* grads, size_t n);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on `samples` rows of X / T, minibatches
+ * of `batch` from a seeded shuffle each epoch; epoch_loss[e] = mean loss. */
+int phi_hip_mlp_train(phi_mlp* m, const float* X, const float* T, long long samples, int batch,
+                      int epochs, float lr, unsigned seed, double* epoch_loss);
+/* Training rows from a context's current state for hierarchies of depth
+ * shallow_depth (PHI's HierarchyTruncation; the context is deeper): input
+ * = [Re rho_n / s, Im rho_n / s, n / (shallow_depth - 1)], target = the D
+ * next-tier ADOs of n / s (each [Re, Im], column-major), for every node n of
+ * tier shallow_depth - 1 of every member; s = max |rho_n|. */
+int phi_hip_closure_samples(phi_ctx* c, int shallow_depth, float* X, float* T,
+                            long long cap_rows, long long* rows);
+/* Couple a network to an ensemble context of uniform depth (its last tier
+ * gains predicted next-tier neighbours before every RK4 stage); resets the
+ * state to the initial density matrix.  The network must outlive the
+ * context's RK4 calls. */
+int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m);
 int phi_hip_num_nodes(const phi_ctx* c);
 int phi_hip_num_modes(const phi_ctx* c);
 /* Multi-indices (num_nodes * D ints), PHI id order. */
@@ -163,6 +222,17 @@ int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks);
  * dt/4 L rho))) (default), the same polynomial for this time-independent
  * linear RHS, with half the FP64 traffic.  Equal to rounding. */
 int phi_hip_set_rk4_form(phi_ctx* c, int form);
+/* RK4 step blocks replayed from a captured hipGraph (on = 1, the default)
+ * or launched kernel by kernel (0).  Same results either way. */
+int phi_hip_set_graphs(phi_ctx* c, int on);
+/* Autotuner of the stage-kernel shape (no counterpart in PHI, whose
+ * partitioning is PartitionHierarchy, hierarchy_integrator.cc:636-808):
+ * times `steps` RK4 steps of size dt for every candidate shape in the
+ * context's precision and RK4 form, keeps the fastest and leaves the state
+ * unchanged.  Cached per device and problem shape for the process.
+ * *best_ms = milliseconds per step; shape[4] = {gemm, tile, threads,
+ * blocks} as in phi_hip_set_launch. */
+int phi_hip_autotune(phi_ctx* c, int steps, double dt, double* best_ms, int* shape);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,14 @@ def lib():
         "phi_hip_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int]),
         "phi_hip_set_rk4_form": (ctypes.c_int, [vp, ctypes.c_int]),
+        "phi_hip_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
+        "phi_hip_ensemble_create": (ctypes.c_int, [vp, ctypes.c_int, dp, ip, ctypes.c_int,
+                                                   ctypes.POINTER(vp)]),
+        "phi_hip_ensemble_members": (ctypes.c_int, [vp, ip, ctypes.c_int]),
+        "phi_hip_ensemble_rho0": (ctypes.c_int, [vp, dp, sz]),
+        "phi_hip_rk4_history": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double,
+                                               ctypes.c_int, dp, sz, ip]),
+        "phi_hip_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, dp, ip]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -275,14 +283,30 @@ class HierarchyIntegrator:
 class Context:
     """Device-resident hierarchy (phi_hip_create): the hot path."""
 
-    def __init__(self, params, precision="fp64"):
+    def __init__(self, params, precision="fp64", _handle=None):
         self.params = params
-        h = ctypes.c_void_p()
-        _check(lib().phi_hip_create(params._h, 1 if precision == "fp32" else 0, ctypes.byref(h)))
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _check(lib().phi_hip_create(params._h, 1 if precision == "fp32" else 0,
+                                        ctypes.byref(h)))
+        else:
+            h = _handle
         self._h = h
         self.num_nodes = lib().phi_hip_num_nodes(h)
         self.D = lib().phi_hip_num_modes(h)
         self.N = params.get_int("num_states")
+
+    def rk4_history(self, nsteps, dt, every):
+        """nsteps RK4 steps; rho_0 (an ensemble: the mean of the members'
+        rho_0) after every step i with i % every == 0.  Returns (k, N, N)."""
+        k_max = (int(nsteps) + int(every) - 1) // int(every)
+        out = np.zeros(2 * max(k_max, 1) * self.N * self.N)
+        count = ctypes.c_int()
+        _check(lib().phi_hip_rk4_history(self._h, int(nsteps), float(dt), int(every), _dp(out),
+                                         max(k_max, 1) * self.N * self.N, ctypes.byref(count)))
+        k = count.value
+        return out[:2 * k * self.N * self.N].view(complex).reshape(k, self.N, self.N).transpose(
+            0, 2, 1).copy()
 
     def close(self):
         if getattr(self, "_h", None):
@@ -368,11 +392,53 @@ class Context:
         _check(lib().phi_hip_set_launch(self._h, int(gemm), int(tile), int(threads),
                                         int(blocks)))
 
+    def set_graphs(self, on):
+        """RK4 step blocks replayed from a captured hipGraph (True) or
+        launched kernel by kernel (False)."""
+        _check(lib().phi_hip_set_graphs(self._h, 1 if on else 0))
+
+    def autotune(self, dt, steps=5):
+        """Pick the fastest stage-kernel shape for this problem, precision
+        and RK4 form (state unchanged).  Returns (ms_per_step, shape)."""
+        ms = ctypes.c_double()
+        shape = (ctypes.c_int * 4)()
+        _check(lib().phi_hip_autotune(self._h, int(steps), float(dt), ctypes.byref(ms), shape))
+        return ms.value, tuple(shape)
+
     def set_rk4_form(self, form):
         """RK4 stage order: "classic" (PHI's k0..k3 with an accumulator) or
         "nested" (Taylor/Horner form, the default)."""
         f = {"classic": 0, "nested": 1}[form]
         _check(lib().phi_hip_set_rk4_form(self._h, f))
+
+
+class Ensemble(Context):
+    """Many hierarchies of one bath model in one device slab
+    (phi_hip_ensemble_create): member m has Hamiltonian Hs[m] and PHI depth
+    depths[m] (HierarchyTruncation); all start from params' initial density
+    matrix.  RK4 calls of Context apply to every member at once."""
+
+    def __init__(self, params, Hs, depths, precision="fp64"):
+        Hs = np.asarray(Hs, dtype=complex)
+        m = Hs.shape[0]
+        flat = np.ascontiguousarray(Hs.transpose(0, 2, 1)).view(np.float64).reshape(-1)
+        d = np.ascontiguousarray(np.asarray(depths, dtype=np.int32))
+        assert d.size == m
+        h = ctypes.c_void_p()
+        _check(lib().phi_hip_ensemble_create(params._h, m, _dp(flat),
+                                             d.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                             1 if precision == "fp32" else 0, ctypes.byref(h)))
+        super().__init__(params, precision, _handle=h)
+        self.members = m
+        counts = np.zeros(m, dtype=np.int32)
+        lib().phi_hip_ensemble_members(h, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), m)
+        self.counts = counts
+
+    def member_rho0(self):
+        """rho_0 of every member, (members, N, N)."""
+        out = np.zeros(2 * self.members * self.N * self.N)
+        _check(lib().phi_hip_ensemble_rho0(self._h, _dp(out), self.members * self.N * self.N))
+        return out.view(complex).reshape(self.members, self.N, self.N).transpose(0, 2, 1).copy()
 
 
 def params_from_problem(H, rho0, gamma, lambda_, temperature, lh, num_terms=1,

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "engine.h"
+#include "mlp.h"
 #include "integrator.h"
 #include "phi_parameters.h"
 
@@ -20,6 +21,10 @@ struct phi_integrator {
 };
 struct phi_ctx {
   phi::Engine* e;
+};
+
+struct phi_mlp {
+  phi::Mlp* m;
 };
 
 namespace {
@@ -415,10 +420,131 @@ int phi_hip_create(const phi_params* h, int precision, phi_ctx** out) {
   return Ok();
 }
 
+int phi_hip_ensemble_create(const phi_params* h, int members, const double* H, const int* depths,
+                            int precision, phi_ctx** out) {
+  if (!h || !H || !depths || !out || members < 1) return Err(PHI_HIP_EINVAL, "bad argument");
+  phi::PhiParameters p = h->p;
+  p.bulk_precision = precision ? phi::kBulkFp32 : phi::kBulkFp64;
+  phi::Engine* e = nullptr;
+  std::string why;
+  int rc = phi::Engine::CreateEnsemble(p, members, reinterpret_cast<const std::complex<double>*>(H),
+                                       depths, &e, &why);
+  if (rc) return Err(rc, why);
+  *out = new phi_ctx{e};
+  return Ok();
+}
+
+int phi_hip_ensemble_members(const phi_ctx* c, int* counts, int cap) {
+  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+  const int m = c->e->members();
+  if (counts) {
+    const auto& n = c->e->member_counts();
+    for (int i = 0; i < m && i < cap; ++i) counts[i] = n[i];
+  }
+  return m;
+}
+
+int phi_hip_ensemble_rho0(phi_ctx* c, double* out, size_t n) {
+  if (!c || !out) return Err(PHI_HIP_EINVAL, "null argument");
+  const size_t m = c->e->members() > 0 ? c->e->members() : 1;
+  if (n < m * c->e->N() * c->e->N()) return Err(PHI_HIP_EINVAL, "buffer too small");
+  int rc = c->e->GetMemberRho0(reinterpret_cast<std::complex<double>*>(out));
+  return rc ? Err(rc, c->e->last_error()) : Ok();
+}
+
+int phi_hip_rk4_history(phi_ctx* c, long long nsteps, double dt, int every, double* out,
+                        size_t n, int* count) {
+  if (!c || !out || !count || nsteps < 0 || every < 1) return Err(PHI_HIP_EINVAL, "bad argument");
+  phi::Engine* e = c->e;
+  const size_t ne = (size_t)e->N() * e->N();
+  if ((nsteps + every - 1) / every > e->history_capacity())
+    return Err(PHI_HIP_EINVAL, "more snapshots than the device history holds");
+  std::vector<std::complex<double>> h;
+  int rc = e->DrainHistory(&h);  // discard anything recorded before
+  if (rc >= 0) rc = e->Rk4Steps(nsteps, dt, 0, every);
+  if (rc >= 0) rc = e->DrainHistory(&h);
+  if (rc < 0) return Err(rc, e->last_error());
+  const size_t k = h.size() / ne;
+  if (k * ne > n) return Err(PHI_HIP_EINVAL, "history buffer too small");
+  std::memcpy(out, h.data(), h.size() * sizeof(std::complex<double>));
+  *count = (int)k;
+  return Ok();
+}
+
 void phi_hip_destroy(phi_ctx* c) {
   if (!c) return;
   delete c->e;
   delete c;
+}
+
+// ---- Closure network ----------------------------------------------------------
+
+int phi_hip_mlp_create(int n_in, int n_hidden, int n_out, int max_batch, phi_mlp** out) {
+  if (!out) return Err(PHI_HIP_EINVAL, "null argument");
+  phi::Mlp* m = nullptr;
+  std::string why;
+  int rc = phi::Mlp::Create(n_in, n_hidden, n_out, max_batch, &m, &why);
+  if (rc) return Err(rc, why);
+  *out = new phi_mlp{m};
+  return Ok();
+}
+
+void phi_hip_mlp_destroy(phi_mlp* m) {
+  if (!m) return;
+  delete m->m;
+  delete m;
+}
+
+long long phi_hip_mlp_param_count(const phi_mlp* m) {
+  return m ? m->m->param_count() : PHI_HIP_EINVAL;
+}
+
+int phi_hip_mlp_set_params(phi_mlp* m, const float* flat, size_t n) {
+  if (!m || !flat) return Err(PHI_HIP_EINVAL, "null argument");
+  if ((long long)n != m->m->param_count()) return Err(PHI_HIP_EINVAL, "parameter count mismatch");
+  int rc = m->m->SetParams(flat);
+  return rc ? Err(rc, m->m->last_error()) : Ok();
+}
+
+int phi_hip_mlp_get_params(phi_mlp* m, float* flat, size_t n) {
+  if (!m || !flat) return Err(PHI_HIP_EINVAL, "null argument");
+  if ((long long)n < m->m->param_count()) return Err(PHI_HIP_EINVAL, "buffer too small");
+  int rc = m->m->GetParams(flat);
+  return rc ? Err(rc, m->m->last_error()) : Ok();
+}
+
+int phi_hip_mlp_forward(phi_mlp* m, const float* x, int batch, float* y) {
+  if (!m || !x || !y) return Err(PHI_HIP_EINVAL, "null argument");
+  int rc = m->m->Forward(x, batch, y);
+  return rc ? Err(rc, m->m->last_error()) : Ok();
+}
+
+int phi_hip_mlp_gradients(phi_mlp* m, const float* x, const float* t, int batch, double* loss,
+                          float* grads, size_t n) {
+  if (!m || !x || !t || !loss || !grads) return Err(PHI_HIP_EINVAL, "null argument");
+  if ((long long)n < m->m->param_count()) return Err(PHI_HIP_EINVAL, "buffer too small");
+  int rc = m->m->Gradients(x, t, batch, loss, grads);
+  return rc ? Err(rc, m->m->last_error()) : Ok();
+}
+
+int phi_hip_mlp_train(phi_mlp* m, const float* X, const float* T, long long samples, int batch,
+                      int epochs, float lr, unsigned seed, double* epoch_loss) {
+  if (!m || !X || !T) return Err(PHI_HIP_EINVAL, "null argument");
+  int rc = m->m->Train(X, T, samples, batch, epochs, lr, seed, epoch_loss);
+  return rc ? Err(rc, m->m->last_error()) : Ok();
+}
+
+int phi_hip_closure_samples(phi_ctx* c, int shallow_depth, float* X, float* T,
+                            long long cap_rows, long long* rows) {
+  if (!c || !X || !T || !rows) return Err(PHI_HIP_EINVAL, "null argument");
+  int rc = c->e->ClosureSamples(shallow_depth, X, T, cap_rows, rows);
+  return rc ? Err(rc, c->e->last_error()) : Ok();
+}
+
+int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m) {
+  if (!c || !m) return Err(PHI_HIP_EINVAL, "null argument");
+  int rc = c->e->AttachClosure(m->m);
+  return rc ? Err(rc, c->e->last_error()) : Ok();
 }
 
 int phi_hip_num_nodes(const phi_ctx* c) { return c ? c->e->num_nodes() : PHI_HIP_EINVAL; }
@@ -502,6 +628,18 @@ int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, doubl
 int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks) {
   if (!c) return Err(PHI_HIP_EINVAL, "null argument");
   int rc = c->e->SetLaunch(gemm, tile, threads, blocks);
+  return rc ? Err(rc, c->e->last_error()) : Ok();
+}
+
+int phi_hip_set_graphs(phi_ctx* c, int on) {
+  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+  c->e->set_graphs(on != 0);
+  return Ok();
+}
+
+int phi_hip_autotune(phi_ctx* c, int steps, double dt, double* best_ms, int* shape) {
+  if (!c || !best_ms || !shape || steps < 1 || !(dt > 0)) return Err(PHI_HIP_EINVAL, "bad argument");
+  int rc = c->e->Autotune(steps, dt, best_ms, shape);
   return rc ? Err(rc, c->e->last_error()) : Ok();
 }
 

@@ -1,0 +1,26 @@
+// Launchers of the closure data-movement kernels (closure_kernels.hip).
+#ifndef PHI_AMD_CLOSURE_KERNELS_H_
+#define PHI_AMD_CLOSURE_KERNELS_H_
+
+#include <hip/hip_runtime.h>
+
+namespace phi {
+
+// Network inputs X (rows x (2 N^2 + D)) and scales S of the boundary nodes
+// first .. first + nb - 1 of every member, from a stage input in FP32 or
+// FP64; bmi = the boundary multi-indices / L (nb x D).
+hipError_t LaunchClosureInputs(const void* in, bool in_fp32, const int* moff, int members,
+                               int first, int nb, int ne, int D, const float* bmi, float* X,
+                               float* S, hipStream_t s);
+// Training targets T (rows x 2 N^2 D) from a deeper hierarchy's state.
+hipError_t LaunchClosureTargets(const double* rho, const int* moff, int members, int nb, int ne,
+                                int D, const int* children, const float* S, float* T,
+                                hipStream_t s);
+// Network outputs Y -> virtual neighbour slots vbase + b*D + j of each member.
+hipError_t LaunchClosureScatter(const float* Y, const float* S, const int* moff, int members,
+                                int nb, int ne, int D, int vbase, void* out, bool out_fp32,
+                                hipStream_t s);
+
+}  // namespace phi
+
+#endif  // PHI_AMD_CLOSURE_KERNELS_H_

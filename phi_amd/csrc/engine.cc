@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace phi {
 
@@ -90,22 +93,10 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   int mps = 0;
   for (int b = 0; b < N; ++b) mps = std::max(mps, h.site_ptr[b + 1] - h.site_ptr[b]);
   if (mps == 0) mps = 1;
-  if (mps <= 2) {
-    std::vector<int4> edge((size_t)h.num_nodes * N * mps);
-    for (int n = 0; n < h.num_nodes; ++n)
-      for (int b = 0; b < N; ++b)
-        for (int q = 0; q < mps; ++q) {
-          int4 r = make_int4(-1, -1, -1, 0);
-          if (!h.vacuum && q < h.site_ptr[b + 1] - h.site_ptr[b]) {
-            const int j = h.site_modes[h.site_ptr[b] + q];
-            const size_t o = (size_t)n * h.D + j;
-            r = make_int4(h.next[o], h.prev[o], j * h.L + h.occ[o], 0);
-          }
-          edge[((size_t)n * N + b) * mps + q] = r;
-        }
-    if (!rc) rc = e->Check(Upload(&e->d_edge_, edge.data(), edge.size()), "upload edges");
-  }
   e->mps_ = mps;
+  if (!rc && mps <= 2) rc = e->UploadEdges(0, 0, 0);
+  e->slab_nodes_ = h.num_nodes;
+  e->rho0_init_.assign(p.initial_density_matrix.begin(), p.initial_density_matrix.end());
   const size_t bytes = e->state_elems() * 2 * sizeof(double);
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_rho_, bytes), "alloc rho");
   if (!rc) rc = e->Check(hipMemset(e->d_rho_, 0, bytes), "zero rho");
@@ -163,12 +154,185 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   return kOk;
 }
 
+// Edge records: node n's q-th mode j of site b -> {n + e_j, n - e_j, j*L + n_j}.
+// With a closure, the nodes first .. first + nb - 1 of the last tier point
+// their (missing) n + e_j at virtual slot vbase + (n - first)*D + j.
+int Engine::UploadEdges(int first, int nb, int vbase) {
+  const HierarchyTables& h = h_;
+  const int N = h.N, mps = mps_;
+  std::vector<int4> edge((size_t)h.num_nodes * N * mps);
+  for (int n = 0; n < h.num_nodes; ++n)
+    for (int b = 0; b < N; ++b)
+      for (int q = 0; q < mps; ++q) {
+        int4 r = make_int4(-1, -1, -1, 0);
+        if (!h.vacuum && q < h.site_ptr[b + 1] - h.site_ptr[b]) {
+          const int j = h.site_modes[h.site_ptr[b] + q];
+          const size_t o = (size_t)n * h.D + j;
+          int nx = h.next[o];
+          if (nb > 0 && n >= first && n < first + nb && nx < 0)
+            nx = vbase + (n - first) * h.D + j;
+          r = make_int4(nx, h.prev[o], j * h.L + h.occ[o], 0);
+        }
+        edge[((size_t)n * N + b) * mps + q] = r;
+      }
+  Free(d_edge_);
+  d_edge_ = nullptr;
+  const int rc = Check(Upload(&d_edge_, edge.data(), edge.size()), "upload edges");
+  args_.edge = d_edge_;
+  return rc;
+}
+
+int Engine::CreateEnsemble(const PhiParameters& base, int members,
+                           const std::complex<double>* H, const int* depths, Engine** out,
+                           std::string* why) {
+  *out = nullptr;
+  const int N = base.num_states, NE = N * N;
+  if (members < 1 || !H || !depths || N < 1) {
+    *why = "an ensemble needs at least one member, its Hamiltonians and depths";
+    return kErrInvalidArgument;
+  }
+  int lh_max = 0;
+  for (int m = 0; m < members; ++m) {
+    if (depths[m] < 1) {
+      *why = "ensemble depths (HierarchyTruncation) must be >= 1";
+      return kErrInvalidArgument;
+    }
+    lh_max = std::max(lh_max, depths[m]);
+  }
+  PhiParameters p = base;
+  p.hierarchy_truncation_level = lh_max;
+  p.time_indepenent_hamiltonian.assign(H, H + NE);
+  Engine* e = nullptr;
+  int rc = Create(p, &e, why);
+  if (rc) return rc;
+  if (e->h_.vacuum) {
+    *why = "an ensemble needs a hierarchy (HierarchyTruncation >= 2 for some member)";
+    delete e;
+    return kErrUnsupported;
+  }
+  if (N > 32 || e->mps_ > 2 || !e->d_edge_) {
+    *why = "ensembles run on the persistent kernels (NumStates <= 32, at most two bath "
+           "modes per site)";
+    delete e;
+    return kErrUnsupported;
+  }
+  // A member of depth 1 would need the vacuum equation (no dephasing).
+  for (int m = 0; m < members; ++m)
+    if (depths[m] < 2) {
+      *why = "ensemble members need HierarchyTruncation >= 2";
+      delete e;
+      return kErrUnsupported;
+    }
+  e->members_ = members;
+  e->mcount_.resize(members);
+  e->moff_.resize(members);
+  long long total = 0;
+  for (int m = 0; m < members; ++m) {
+    e->mcount_[m] = e->h_.tier_start[depths[m]];
+    e->moff_[m] = (int)total;
+    total += e->mcount_[m];
+  }
+  e->mreal_ = e->mcount_;
+  if (total * NE > (1LL << 31) - 1) {
+    *why = "ensemble slab too large (more than 2^31 complex elements)";
+    delete e;
+    return kErrUnsupported;
+  }
+  e->slab_nodes_ = (int)total;
+  // Every member's H, shifted by its own mean diagonal; all real or not.
+  std::vector<double2> Hm((size_t)members * NE);
+  int real_h = 1;
+  for (int m = 0; m < members; ++m) {
+    const std::complex<double>* Hr = H + (size_t)m * NE;
+    double mean = 0;
+    for (int a = 0; a < N; ++a) mean += Hr[(size_t)a * (N + 1)].real();
+    mean /= N;
+    for (int i = 0; i < NE; ++i) {
+      const bool diag = i % (N + 1) == 0;
+      Hm[(size_t)m * NE + i] = make_double2(Hr[i].real() - (diag ? mean : 0.0), Hr[i].imag());
+      if (Hr[i].imag() != 0) real_h = 0;
+    }
+  }
+  Free(e->d_rho_);
+  e->d_rho_ = nullptr;
+  const size_t bytes = e->state_elems() * 2 * sizeof(double);
+  rc = e->Check(Upload(&e->d_Hm_, Hm.data(), Hm.size()), "upload member H");
+  if (!rc) rc = e->Check(Upload(&e->d_mcount_, e->mcount_.data(), e->mcount_.size()), "upload counts");
+  if (!rc) rc = e->Check(Upload(&e->d_moff_, e->moff_.data(), e->moff_.size()), "upload offsets");
+  if (!rc) rc = e->Check(hipMalloc((void**)&e->d_rho_, bytes), "alloc ensemble rho");
+  if (!rc) rc = e->Check(hipMemset(e->d_rho_, 0, bytes), "zero ensemble rho");
+  // rho_0 of every member from the base initial density matrix.
+  if (!rc && (int)base.initial_density_matrix.size() == NE) {
+    for (int m = 0; m < members && !rc; ++m)
+      rc = e->Check(hipMemcpy(e->d_rho_ + (size_t)e->moff_[m] * NE * 2,
+                              base.initial_density_matrix.data(), NE * sizeof(double2),
+                              hipMemcpyHostToDevice), "initial state");
+  }
+  if (rc) {
+    *why = e->err_;
+    delete e;
+    return rc;
+  }
+  RhsArgs& a = e->args_;
+  a.num_nodes = e->slab_nodes_;
+  a.H = e->d_Hm_;
+  a.real_h = real_h;
+  a.mcount = e->d_mcount_;
+  rc = e->SetLaunch(2, 0, 0, 0);
+  if (rc) {
+    *why = e->err_;
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return kOk;
+}
+
+// Tile descriptors of an ensemble for the current tile size: each member's
+// nodes cut into tiles of at most `tile` nodes, members in order.
+int Engine::BuildTiles() {
+  if (members_ == 0) return kOk;
+  const int T = args_.col_tile[fp32_ ? 1 : 0];
+  std::vector<int4> t;
+  for (int m = 0; m < members_; ++m)
+    for (int lb = 0; lb < mreal_[m]; lb += T)
+      t.push_back(make_int4(moff_[m] + lb, m, lb, std::min(T, mreal_[m] - lb)));
+  int rc = kOk;
+  if ((int)t.size() > tiles_cap_) {
+    Free(d_tiles_);
+    d_tiles_ = nullptr;
+    tiles_cap_ = 0;
+    rc = Check(hipMalloc((void**)&d_tiles_, t.size() * sizeof(int4)), "alloc tiles");
+    if (rc) return rc;
+    tiles_cap_ = (int)t.size();
+  }
+  rc = Check(hipMemcpy(d_tiles_, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice),
+             "upload tiles");
+  if (rc) return rc;
+  args_.tiles = d_tiles_;
+  args_.ntiles = (int)t.size();
+  tiles_for_ = T;
+  return kOk;
+}
+
+int Engine::GetMemberRho0(std::complex<double>* host) {
+  const int NE = N() * N();
+  int rc = kOk;
+  if (members_ == 0) return GetNode0(host);
+  for (int m = 0; m < members_ && !rc; ++m)
+    rc = Check(hipMemcpyAsync(host + (size_t)m * NE, d_rho_ + (size_t)moff_[m] * NE * 2,
+                              NE * sizeof(double2), hipMemcpyDeviceToHost, stream_), "member rho0");
+  return rc ? rc : Synchronize();
+}
+
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  DropGraph();
   void* ptrs[] = {d_next_, d_prev_, d_occ_, d_mu_, d_site_ptr_, d_site_modes_, d_next_s_,
                   d_prev_col_, d_prev_row_, d_delta_, d_H_, d_Hraw_, d_Hadj_, d_edge_, d_rho_, d_aux_,
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
-                  d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_};
+                  d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_mcount_, d_moff_, d_Hm_,
+                  d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_};
   for (void* p : ptrs) Free(p);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -176,7 +340,7 @@ Engine::~Engine() {
 void Engine::set_fp32(bool v) { fp32_ = v; }
 
 int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
-  RhsArgs& a = args_;
+  RhsArgs a = args_;
   const int N = h_.N, D = h_.D, L = h_.L, NE = N * N;
   const size_t kLdsBudget = 64 * 1024;
   const bool stream_ok = N <= 32 && mps_ <= 2 && d_edge_ != nullptr;
@@ -187,6 +351,10 @@ int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
   }
   if (col_nm < 0 || col_nm > 2) {
     err_ = "kernel form must be 0 (element), 1 (matrix cores) or 2 (persistent)";
+    return kErrInvalidArgument;
+  }
+  if (members_ && col_nm != 2) {
+    err_ = "ensembles run on the persistent kernels (form 2) only";
     return kErrInvalidArgument;
   }
   if (col_nm >= 1 && N > 32) {
@@ -247,10 +415,131 @@ int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
   }
   a.col_nm = col_nm;
   a.tile = a.col_tile[0];
+  args_ = a;
+  return EnsureTiles();
+}
+
+int Engine::Unsupported(const char* what) {
+  err_ = std::string(what) + " is not available for an ensemble (RK4 only)";
+  return kErrUnsupported;
+}
+
+namespace {
+
+struct TuneKey {
+  int dev, N, D, L, nodes, mps, members;
+  bool fp32, nested;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dev, N, D, L, nodes, mps, members, fp32, nested) <
+           std::tie(o.dev, o.N, o.D, o.L, o.nodes, o.mps, o.members, o.fp32, o.nested);
+  }
+};
+struct TuneVal {
+  int col_nm, tile, threads, blocks;
+  double ms;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+}  // namespace
+
+int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
+  if (steps < 1) steps = 1;
+  int rc = AllocBulk();
+  if (rc) return rc;
+  const int N = h_.N, NE = N * N;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const TuneKey key{dev, N, h_.D, h_.L, slab_nodes_, mps_, members_, fp32_, rk4_nested_};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      const TuneVal& v = it->second;
+      rc = SetLaunch(v.col_nm, v.tile, v.threads, v.blocks);
+      if (!rc) {
+        *best_ms = v.ms;
+        shape_out[0] = v.col_nm;
+        shape_out[1] = v.tile;
+        shape_out[2] = v.threads;
+        shape_out[3] = v.blocks;
+      }
+      return rc;
+    }
+  }
+  int cus = 1;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  // Candidates: the persistent kernels over tiles of up to 1024 elements and
+  // grids of 0 (all resident) or 1..4 blocks per CU; the one-shot forms as
+  // a reference point.
+  struct Cand { int col_nm, tile, threads, blocks; };
+  std::vector<Cand> cands;
+  const bool stream_ok = N <= 32 && mps_ <= 2 && d_edge_ != nullptr;
+  if (stream_ok) {
+    const int tiles[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 48, 64, 96, 128, 192, 256};
+    for (int t : tiles) {
+      if ((long long)t * NE > 1024) break;
+      for (int b : {0, 1, 2, 3, 4}) cands.push_back({2, t, 0, b * cus});
+    }
+  }
+  if (N <= 32) cands.push_back({1, 0, 0, 0});
+  cands.push_back({0, 0, 0, 0});
+  // Save the state; tuning steps advance it.
+  const size_t bytes = state_elems() * 2 * sizeof(double);
+  double* saved = nullptr;
+  rc = Check(hipMalloc((void**)&saved, bytes), "alloc tune state");
+  if (rc) return rc;
+  rc = Check(hipMemcpyAsync(saved, d_rho_, bytes, hipMemcpyDeviceToDevice, stream_), "save state");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) rc = Check(hipEventCreate(&e0), "event");
+  if (!rc) rc = Check(hipEventCreate(&e1), "event");
+  const RhsArgs keep = args_;
+  TuneVal best{keep.col_nm, keep.col_tile[fp32_ ? 1 : 0], keep.col_threads[fp32_ ? 1 : 0],
+               keep.col_grid[fp32_ ? 1 : 0], 1e300};
+  for (const Cand& c : cands) {
+    if (rc) break;
+    if (SetLaunch(c.col_nm, c.tile, c.threads, c.blocks) != kOk) continue;
+    rc = Rk4Step(dt, false);
+    if (!rc) rc = Check(hipEventRecord(e0, stream_), "record");
+    for (int s = 0; s < steps && !rc; ++s) rc = Rk4Step(dt, false);
+    if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");
+    if (!rc) rc = Check(hipEventSynchronize(e1), "event sync");
+    float ms = 0;
+    if (!rc) rc = Check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (!rc && ms / steps < best.ms) {
+      const int f = fp32_ ? 1 : 0;
+      best = TuneVal{c.col_nm, args_.col_tile[f], args_.col_threads[f], args_.col_grid[f],
+                     ms / steps};
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  const int rc_restore = Check(hipMemcpyAsync(d_rho_, saved, bytes, hipMemcpyDeviceToDevice,
+                                              stream_), "restore state");
+  const int rc_sync = Synchronize();
+  Free(saved);
+  if (!rc) rc = rc_restore ? rc_restore : rc_sync;
+  if (rc) {
+    args_ = keep;
+    return rc;
+  }
+  rc = SetLaunch(best.col_nm, best.tile, best.threads, best.blocks);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[key] = best;
+  }
+  *best_ms = best.ms;
+  shape_out[0] = best.col_nm;
+  shape_out[1] = best.tile;
+  shape_out[2] = best.threads;
+  shape_out[3] = best.blocks;
   return kOk;
 }
 
 int Engine::AllocBulk() {
+  const int rt = EnsureTiles();
+  if (rt) return rt;
   if (bulk_alloc_ && bulk_is_fp32_ == fp32_) return kOk;
   Free(d_y_[0]);
   Free(d_y_[1]);
@@ -290,10 +579,12 @@ int Engine::GetNode0(std::complex<double>* host) {
 int Engine::Synchronize() { return Check(hipStreamSynchronize(stream_), "synchronize"); }
 
 int Engine::RhsDevice(const double* x, double* y, int trace_row) {
+  if (members_) return Unsupported("the general RHS");
   return Check(LaunchRhsGeneral(args_, x, y, trace_row, stream_), "rhs_general");
 }
 
 int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
+  if (members_) return Unsupported("the general RHS");
   const size_t bytes = state_elems() * sizeof(std::complex<double>);
   double *dx = nullptr, *dy = nullptr;
   int rc = Check(hipMalloc((void**)&dx, bytes), "alloc rhs in");
@@ -307,23 +598,95 @@ int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
   return rc;
 }
 
+int Engine::Rk4Step(double dt, bool snapshot) {
+  // The closed RHS is not linear: PHI's classic stage order then.
+  const bool hn = rk4_nested_ && !closure_;
+  // Stage inputs alternate between the two bulk buffers so that a stage
+  // never overwrites what its own neighbour gathers read.
+  const void* in[4] = {d_rho_, d_y_[0], d_y_[1], d_y_[0]};
+  void* out[4] = {d_y_[0], d_y_[1], d_y_[0], nullptr};
+  int rc = kOk;
+  for (int s = 0; s < 4 && !rc; ++s) {
+    if (closure_) rc = ClosurePredict(in[s], s > 0 && fp32_);
+    if (!rc)
+      rc = Check(LaunchRk4Stage(args_, fp32_, s, hn, in[s], d_rho_, d_aux_, out[s], dt, stream_),
+                 "rk4 stage");
+  }
+  if (!rc && snapshot) {
+    // A single hierarchy records rho_0; an ensemble the member average.
+    if (members_)
+      rc = Check(LaunchSnapshotMean(d_rho_, d_moff_, members_, d_hist_, d_hist_count_, N() * N(),
+                                    hist_capacity_, stream_), "snapshot mean");
+    else
+      rc = Check(LaunchSnapshot(d_rho_, d_hist_, d_hist_count_, N() * N(), hist_capacity_, stream_),
+                 "snapshot");
+  }
+  return rc;
+}
+
+void Engine::DropGraph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  graph_exec_ = nullptr;
+}
+
+// A block of `steps` RK4 steps (with node-0 snapshots after steps j with
+// j % snap_every == 0) captured once and replayed; re-captured when any
+// launch argument changes.  Only kernels are captured.
+int Engine::GraphBlock(int steps, int snap_every, double dt) {
+  GraphKey key{steps, snap_every, dt, fp32_, rk4_nested_, args_.col_nm,
+               args_.col_tile[0], args_.col_tile[1], args_.col_threads[0], args_.col_threads[1],
+               args_.col_grid[0], args_.col_grid[1], d_y_[0], d_rho_, d_tiles_};
+  if (graph_exec_ && key == graph_key_) return kOk;
+  DropGraph();
+  // The persistent kernels query their occupancy on first launch; callers
+  // run at least one eager step with the same shape before capturing
+  // (Rk4Steps does), so the capture holds only kernel launches.
+  hipGraph_t g = nullptr;
+  int rc = Check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+  if (rc) return rc;
+  for (int j = 0; j < steps && !rc; ++j) rc = Rk4Step(dt, snap_every > 0 && j % snap_every == 0);
+  const int rc_end = Check(hipStreamEndCapture(stream_, &g), "end capture");
+  if (!rc) rc = rc_end;
+  if (!rc) rc = Check(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), "instantiate");
+  if (g) (void)hipGraphDestroy(g);
+  if (rc) {
+    graph_exec_ = nullptr;
+    return rc;
+  }
+  graph_key_ = key;
+  return kOk;
+}
+
 int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int snap_every) {
   int rc = AllocBulk();
   if (rc) return rc;
-  const bool hn = rk4_nested_;
-  for (long long s = 0; s < nsteps; ++s) {
-    // Stage inputs alternate between the two bulk buffers so that a stage
-    // never overwrites what its own neighbour gathers read.
-    rc = Check(LaunchRk4Stage(args_, fp32_, 0, hn, d_rho_, d_rho_, d_aux_, d_y_[0], dt, stream_), "rk4 s0");
-    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 1, hn, d_y_[0], d_rho_, d_aux_, d_y_[1], dt, stream_), "rk4 s1");
-    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 2, hn, d_y_[1], d_rho_, d_aux_, d_y_[0], dt, stream_), "rk4 s2");
-    if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, 3, hn, d_y_[0], d_rho_, d_aux_, nullptr, dt, stream_), "rk4 s3");
+  auto snap = [&](long long s) { return snap_every > 0 && (step_offset + s) % snap_every == 0; };
+  // Graph blocks of G steps, G a multiple of the snapshot period, aligned so
+  // that every block starts on a snapshot step.
+  int G = 0;
+  if (use_graphs_) {
+    G = kGraphSteps;
+    if (snap_every > 0) G = std::max(1, kGraphSteps / snap_every) * snap_every;
+  }
+  long long s = 0;
+  // Eager head: at least one step (settles launch parameters), then up to
+  // the first aligned step.
+  while (s < nsteps && (s == 0 || (G > 0 && snap_every > 0 && (step_offset + s) % G != 0))) {
+    rc = Rk4Step(dt, snap(s));
     if (rc) return rc;
-    const long long i = step_offset + s;
-    if (snap_every > 0 && i % snap_every == 0) {
-      rc = Check(LaunchSnapshot(d_rho_, d_hist_, d_hist_count_, N() * N(), hist_capacity_, stream_), "snapshot");
+    ++s;
+  }
+  if (G > 0 && nsteps - s >= 2LL * G) {
+    rc = GraphBlock(G, snap_every, dt);
+    if (rc) return rc;
+    for (; s + G <= nsteps; s += G) {
+      rc = Check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
       if (rc) return rc;
     }
+  }
+  for (; s < nsteps; ++s) {
+    rc = Rk4Step(dt, snap(s));
+    if (rc) return rc;
   }
   return kOk;
 }
@@ -331,28 +694,36 @@ int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int sna
 int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   int rc = AllocBulk();
   if (rc) return rc;
-  hipEvent_t ev[5];
+  for (int s = 0; s < 4; ++s) ms[s] = 0;
+  // Every launch and event of a batch is queued before the first wait, so
+  // the device runs the kernels back to back and an event pair brackets
+  // one kernel.
+  const long long kBatch = 64;
+  std::vector<hipEvent_t> ev(4 * kBatch + 1, nullptr);
   for (auto& e : ev)
     if (!rc) rc = Check(hipEventCreate(&e), "event");
-  if (rc) return rc;
-  for (int s = 0; s < 4; ++s) ms[s] = 0;
   const void* in[4] = {d_rho_, d_y_[0], d_y_[1], d_y_[0]};
   void* out[4] = {d_y_[0], d_y_[1], d_y_[0], nullptr};
-  for (long long i = 0; i < nsteps && !rc; ++i) {
+  for (long long done = 0; done < nsteps && !rc;) {
+    const long long nb = std::min(kBatch, nsteps - done);
     rc = Check(hipEventRecord(ev[0], stream_), "record");
-    for (int s = 0; s < 4 && !rc; ++s) {
-      rc = Check(LaunchRk4Stage(args_, fp32_, s, rk4_nested_, in[s], d_rho_, d_aux_, out[s], dt,
-                                stream_), "rk4 stage");
-      if (!rc) rc = Check(hipEventRecord(ev[s + 1], stream_), "record");
-    }
-    if (!rc) rc = Check(hipEventSynchronize(ev[4]), "event sync");
-    for (int s = 0; s < 4 && !rc; ++s) {
-      float t = 0;
-      rc = Check(hipEventElapsedTime(&t, ev[s], ev[s + 1]), "elapsed");
-      ms[s] += t;
-    }
+    for (long long i = 0; i < nb && !rc; ++i)
+      for (int s = 0; s < 4 && !rc; ++s) {
+        rc = Check(LaunchRk4Stage(args_, fp32_, s, rk4_nested_, in[s], d_rho_, d_aux_, out[s], dt,
+                                  stream_), "rk4 stage");
+        if (!rc) rc = Check(hipEventRecord(ev[4 * i + s + 1], stream_), "record");
+      }
+    if (!rc) rc = Check(hipEventSynchronize(ev[4 * nb]), "event sync");
+    for (long long i = 0; i < nb && !rc; ++i)
+      for (int s = 0; s < 4 && !rc; ++s) {
+        float t = 0;
+        rc = Check(hipEventElapsedTime(&t, ev[4 * i + s], ev[4 * i + s + 1]), "elapsed");
+        ms[s] += t;
+      }
+    done += nb;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   return rc;
 }
 
@@ -379,6 +750,7 @@ int Engine::DrainHistory(std::vector<std::complex<double>>* out) {
 }
 
 int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
+  if (members_) return Unsupported("RKF45");
   int rc = AllocBulk();
   if (rc) return rc;
   if (!rkf45_alloc_) {
@@ -418,6 +790,7 @@ void Engine::Rkf45Accept() { std::swap(d_rho_, d_aux_); }
 
 int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
                         std::vector<double>* history) {
+  if (members_) return Unsupported("the steady state");
   // Unpreconditioned complex BiCGSTAB (van der Vorst 1992) on A x = b with
   // b = e_(node 0, element (0,0)); inner products conjugate the first
   // argument; all scalars and reductions in FP64.
@@ -463,7 +836,7 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   if (!rc) rc = Check(hipMemcpyAsync(rh, r, bytes, hipMemcpyDeviceToDevice, stream_), "rhat");
   if (!rc) rc = Check(hipMemsetAsync(pp, 0, bytes, stream_), "p0");
   if (!rc) rc = Check(hipMemsetAsync(vv, 0, bytes, stream_), "v0");
-  Z rho_old(1, 0), alpha(1, 0), omega(1, 0), tmp;
+  Z rho_old(1, 0), alpha(1, 0), omega(1, 0);
   Z rn2;
   if (!rc) rc = dot(r, r, &rn2);
   double res = std::sqrt(rn2.real());

@@ -35,10 +35,44 @@ class Engine {
   static int Create(const PhiParameters& p, Engine** out, std::string* why);
   ~Engine();
 
+  // An ensemble of hierarchies of one bath model: member m has Hamiltonian
+  // H[m*N*N .. ] (column-major) and depth depths[m] (PHI's
+  // HierarchyTruncation); the members' nodes are the first nodes of the
+  // deepest member's enumeration, stored one member after the other in one
+  // slab.  Every member starts from base.initial_density_matrix.  RK4 only.
+  static int CreateEnsemble(const PhiParameters& base, int members,
+                            const std::complex<double>* H, const int* depths, Engine** out,
+                            std::string* why);
+
   const HierarchyTables& tables() const { return h_; }
-  int num_nodes() const { return h_.num_nodes; }
+  // Nodes of the slab (all members of an ensemble).
+  int num_nodes() const { return slab_nodes_; }
   int N() const { return h_.N; }
-  size_t state_elems() const { return (size_t)h_.num_nodes * h_.N * h_.N; }
+  size_t state_elems() const { return (size_t)slab_nodes_ * h_.N * h_.N; }
+  int members() const { return members_; }
+  // Nodes of every member (integrated nodes; without closure slots).
+  const std::vector<int>& member_counts() const { return mreal_; }
+  const std::vector<int>& member_offsets() const { return moff_; }
+  // Node 0 of every member (members * N * N complex).
+  int GetMemberRho0(std::complex<double>* host);
+
+  // --- Learned truncation closure (engine_closure.cc) ---------------------------
+  // Training samples of the closure network for hierarchies of PHI depth
+  // shallow_lh, read from this ensemble's current state (every member at
+  // least one tier deeper): for every member and every node n of tier
+  // shallow_lh - 1, the network input (ADO of n scaled by its largest
+  // magnitude s, and n / (shallow_lh - 1)) and the target (the ADOs of its
+  // D next-tier neighbours, scaled by the same s).  Host arrays of
+  // *rows = members * (nodes of that tier) rows.
+  int ClosureSamples(int shallow_lh, float* X, float* T, long long cap_rows, long long* rows);
+  // Couples a trained network to this ensemble (every member of the same
+  // depth): each node of the last tier gains D virtual next-tier neighbours
+  // whose ADOs the network predicts from the node's stage input before every
+  // RK4 stage.  Resets the state to the initial density matrix; RK4 runs in
+  // the classic stage order (the closed RHS is not linear).  The network is
+  // borrowed and must outlive the engine's use of it.
+  int AttachClosure(class Mlp* mlp);
+  bool has_closure() const { return closure_ != nullptr; }
   bool fp32() const { return fp32_; }
   void set_fp32(bool v);
   // Stage-kernel shape: col_nm = kernel form (0 element, 1 matrix cores,
@@ -71,6 +105,7 @@ class Engine {
   // on the engine stream; ms[s] += duration of stage s summed over steps.
   int Rk4StageTimes(long long nsteps, double dt, double ms[4]);
   int HistoryCount();
+  int history_capacity() const { return hist_capacity_; }
   int DrainHistory(std::vector<std::complex<double>>* out);  // and resets it
 
   // One RKF45 attempt with the given (dt-scaled) weights; *err = max|y4-y5|.
@@ -86,10 +121,46 @@ class Engine {
   int Synchronize();
   std::string last_error() const { return err_; }
 
+  // RK4 blocks replayed from a captured hipGraph (default on).
+  void set_graphs(bool v) {
+    use_graphs_ = v;
+    if (!v) DropGraph();
+  }
+  // Autotuner: times `steps` RK4 steps (after one warm-up step) for each
+  // candidate shape of the persistent kernels (tile x grid) in the current
+  // precision and RK4 form, keeps the fastest, and restores the state.
+  // Results are cached per (device, problem shape, precision, form) for the
+  // process.  *best_ms = milliseconds per step of the winner; shape_out =
+  // {form, tile, threads, blocks}.
+  int Autotune(int steps, double dt, double* best_ms, int shape_out[4]);
+
  private:
   Engine() = default;
   int Check(hipError_t e, const char* where);
   int AllocBulk();
+  int Rk4Step(double dt, bool snapshot);
+  int GraphBlock(int steps, int snap_every, double dt);
+  void DropGraph();
+
+  struct GraphKey {
+    int steps, snap_every;
+    double dt;
+    bool fp32, nested;
+    int col_nm, tile0, tile1, th0, th1, grid0, grid1;
+    const void* y0;
+    const void* rho;
+    const void* tiles;
+    bool operator==(const GraphKey& o) const {
+      return steps == o.steps && snap_every == o.snap_every && dt == o.dt && fp32 == o.fp32 &&
+             nested == o.nested && col_nm == o.col_nm && tile0 == o.tile0 && tile1 == o.tile1 &&
+             th0 == o.th0 && th1 == o.th1 && grid0 == o.grid0 && grid1 == o.grid1 &&
+             y0 == o.y0 && rho == o.rho && tiles == o.tiles;
+    }
+  };
+  static constexpr int kGraphSteps = 32;
+  bool use_graphs_ = true;
+  hipGraphExec_t graph_exec_ = nullptr;
+  GraphKey graph_key_{};
 
   HierarchyTables h_;
   RhsArgs args_{};
@@ -114,6 +185,37 @@ class Engine {
   double2* d_Hadj_ = nullptr;
   int4* d_edge_ = nullptr;
   int mps_ = 0;
+
+  // Slab and ensemble members (members_ == 0: a single hierarchy).  mcount_
+  // is each member's slab span (integrated nodes + closure slots), mreal_
+  // its integrated nodes.
+  int slab_nodes_ = 0;
+  int members_ = 0;
+  std::vector<int> mcount_, mreal_, moff_;
+  std::vector<std::complex<double>> rho0_init_;  // InitialDensityMatrix (N*N or empty)
+
+  // Closure.
+  class Mlp* closure_ = nullptr;
+  int closure_first_ = 0, closure_nb_ = 0, closure_vbase_ = 0;
+  float* d_cbmi_ = nullptr;
+  float* d_cx_ = nullptr;
+  float* d_cs_ = nullptr;
+  float* d_cy_ = nullptr;
+  int ClosurePredict(const void* in, bool in_fp32);
+  int UploadEdges(int closure_first, int closure_nb, int closure_vbase);
+  int* d_mcount_ = nullptr;
+  int* d_moff_ = nullptr;
+  double2* d_Hm_ = nullptr;
+  int4* d_tiles_ = nullptr;
+  int tiles_cap_ = 0;
+  int tiles_for_ = 0;  // tile size the descriptors were built for
+  int BuildTiles();
+  // Rebuilds the ensemble tile descriptors when the active tile size
+  // changed (precision or launch shape); a no-op for a single hierarchy.
+  int EnsureTiles() {
+    return members_ && tiles_for_ != args_.col_tile[fp32_ ? 1 : 0] ? BuildTiles() : 0;
+  }
+  int Unsupported(const char* what);
 
   // State buffers (interleaved complex).
   double* d_rho_ = nullptr;

@@ -29,6 +29,14 @@ struct RhsArgs {
   // site b ({-1, -1, -1, 0} when absent).
   const int4* edge;
   int mps;
+  // Ensembles (persistent kernels only): slab of several members of one
+  // bath model sharing the tables of the deepest member.  tiles[t] =
+  // {slab node base, member, member-local node, nodes}, mcount[m] = nodes of
+  // member m, and H holds every member's shifted H (stride N*N).  nullptr
+  // for a single hierarchy.
+  const int4* tiles;
+  int ntiles;
+  const int* mcount;
   int real_h;                 // imag(H) == 0 everywhere
   unsigned long long mN, mNE; // ceil(2^32 / N), ceil(2^32 / N^2)
   double inv_hbar;
@@ -82,6 +90,9 @@ hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int tr
                             hipStream_t s);
 hipError_t LaunchSnapshot(const double* src, double* hist, int* counter, int ne, int capacity,
                           hipStream_t s);
+// hist[slot] = mean over the members of their node 0 (members in order).
+hipError_t LaunchSnapshotMean(const double* src, const int* member_offsets, int members,
+                              double* hist, int* counter, int ne, int capacity, hipStream_t s);
 hipError_t LaunchFillU64(unsigned long long* p, unsigned long long v, int n, hipStream_t s);
 hipError_t LaunchCastToFloat(const double* src, float* dst, size_t n, hipStream_t s);
 hipError_t LaunchDot(const double* u, const double* v, size_t n, int mode, double* partial,

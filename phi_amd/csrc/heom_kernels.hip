@@ -104,6 +104,29 @@ __global__ void snapshot_node0(const double* __restrict__ src, double* __restric
   if (threadIdx.x == 0) *counter = slot + 1;
 }
 
+// As snapshot_node0 for an ensemble: the mean of every member's node 0,
+// summed in member order (deterministic).  One block.
+__global__ void snapshot_mean(const double* __restrict__ src, const int* __restrict__ moff,
+                              int members, double* __restrict__ hist, int* __restrict__ counter,
+                              int ne, int capacity) {
+  const int slot = *counter;
+  if (slot < capacity) {
+    const double2* s = reinterpret_cast<const double2*>(src);
+    const double w = 1.0 / members;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      double x = 0, y = 0;
+      for (int m = 0; m < members; ++m) {
+        const double2 v = s[(size_t)moff[m] * ne + e];
+        x += v.x;
+        y += v.y;
+      }
+      reinterpret_cast<double2*>(hist)[(size_t)slot * ne + e] = make_double2(x * w, y * w);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *counter = slot + 1;
+}
+
 __global__ void fill_u64(unsigned long long* p, unsigned long long v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -277,6 +300,13 @@ hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int tr
 hipError_t LaunchSnapshot(const double* src, double* hist, int* counter, int ne, int capacity,
                           hipStream_t s) {
   hipLaunchKernelGGL(dev::snapshot_node0, dim3(1), dim3(256), 0, s, src, hist, counter, ne, capacity);
+  return hipGetLastError();
+}
+
+hipError_t LaunchSnapshotMean(const double* src, const int* member_offsets, int members,
+                              double* hist, int* counter, int ne, int capacity, hipStream_t s) {
+  hipLaunchKernelGGL(dev::snapshot_mean, dim3(1), dim3(256), 0, s, src, member_offsets, members,
+                     hist, counter, ne, capacity);
   return hipGetLastError();
 }
 

@@ -21,7 +21,7 @@ inline void LaunchStreamK(const RhsArgs& a, const TI* in, const EPI& epi, int th
                           hipStream_t s) {
   const size_t sh = dev::StreamLdsBytes(a.N, a.D, a.L, a.tile, sizeof(TB));
   const void* k = reinterpret_cast<const void*>(&dev::stream_stage<TB, TI, EPT, MPS, EPI>);
-  const int ntiles = (a.num_nodes + a.tile - 1) / a.tile;
+  const int ntiles = a.tiles ? a.ntiles : (a.num_nodes + a.tile - 1) / a.tile;
   int g = grid > 0 ? grid : ResidentBlocks(k, threads, sh);
   if (g > ntiles) g = ntiles;
   if (g < 1) g = 1;
@@ -119,6 +119,22 @@ inline hipError_t LaunchRk4T(const RhsArgs& a0, int stage, bool horner, const TI
   return hipGetLastError();
 }
 
+// RKF45 stage through the persistent kernels (stage 0 reads the FP64 state).
+template <typename TB>
+inline hipError_t LaunchRkf45Stream(const RhsArgs& a, int stage, const void* in, const double* rho,
+                                    const Rkf45Bufs<TB>& b, const Rkf45Coefs& c, hipStream_t s) {
+  const double* x = (const double*)in;
+  const TB* y = (const TB*)in;
+  switch (stage) {
+    case 0: return LaunchStream<TB, double>(a, x, dev::EpiRkf45<TB, 0, true>{b, rho, c}, s);
+    case 1: return LaunchStream<TB, TB>(a, y, dev::EpiRkf45<TB, 1, false>{b, rho, c}, s);
+    case 2: return LaunchStream<TB, TB>(a, y, dev::EpiRkf45<TB, 2, false>{b, rho, c}, s);
+    case 3: return LaunchStream<TB, TB>(a, y, dev::EpiRkf45<TB, 3, false>{b, rho, c}, s);
+    case 4: return LaunchStream<TB, TB>(a, y, dev::EpiRkf45<TB, 4, false>{b, rho, c}, s);
+    default: return LaunchStream<TB, TB>(a, y, dev::EpiRkf45<TB, 5, false>{b, rho, c}, s);
+  }
+}
+
 template <typename TB, typename TI, int NM>
 inline void LaunchRkf45NM(const RhsArgs& a, int stage, const TI* in, const double* rho,
                           const Rkf45Bufs<TB>& b, const Rkf45Coefs& c, int threads,
@@ -138,6 +154,7 @@ inline void LaunchRkf45NM(const RhsArgs& a, int stage, const TI* in, const doubl
 template <typename TB, typename TI>
 inline hipError_t LaunchRkf45T(const RhsArgs& a0, int stage, const TI* in, const double* rho,
                                const Rkf45Bufs<TB>& b, const Rkf45Coefs& c, hipStream_t s) {
+  if (a0.col_nm == 2) return LaunchRkf45Stream<TB>(a0, stage, in, rho, b, c, s);
   int threads;
   const RhsArgs a = Shape(a0, sizeof(TB) == 4, &threads);
   if (a.col_nm >= 1) LaunchRkf45NM<TB, TI, 1>(a, stage, in, rho, b, c, threads, s);

@@ -165,31 +165,66 @@ __device__ __forceinline__ void stream_coords(const RhsArgs& A, int i, int* te, 
   *a = *r - *b * A.N;
 }
 
+// A tile: slab node `base` holds local node `local` of its member (the
+// member's nodes are slab nodes base - local .. base - local + count - 1),
+// nt nodes.  A single hierarchy is one member with count = num_nodes.
+struct TileInfo {
+  int base, local, nt, member, count;
+};
+
+__device__ __forceinline__ TileInfo stream_tile(const RhsArgs& A, int tile) {
+  TileInfo t;
+  if (A.tiles) {
+    const int4 d = A.tiles[tile];
+    t.base = d.x;
+    t.member = d.y;
+    t.local = d.z;
+    t.nt = d.w;
+    t.count = A.mcount[d.y];
+  } else {
+    t.base = tile * A.tile;
+    t.local = t.base;
+    t.nt = min(A.tile, A.num_nodes - t.base);
+    t.member = 0;
+    t.count = A.num_nodes;
+  }
+  return t;
+}
+
+// Loads of a tile that need no other load first: its input elements, mu
+// and the edge records (tables indexed by the member-local node id).
 template <typename TB, typename TI, int EPT, int MPS>
-__device__ __forceinline__ void stream_load1(const RhsArgs& A, const TI* __restrict__ in, int base,
-                                             int tot, StreamRegs<TB, TI, EPT, MPS>& R) {
+__device__ __forceinline__ void stream_load1(const RhsArgs& A, const TI* __restrict__ in,
+                                             const TileInfo& t,
+                                             StreamRegs<TB, TI, EPT, MPS>& R) {
   typedef typename C2<TI>::type VI;
+  const int tot = t.nt * A.NE;
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
     const int e = threadIdx.x + i * blockDim.x;
     if (e < tot) {
       int te, r, b, a;
       stream_coords(A, i, &te, &r, &b, &a);
-      const int n = base + te;
-      R.y[i] = reinterpret_cast<const VI*>(in)[(size_t)n * A.NE + r];
-      R.mu2[i] = (TB)(0.5 * A.mu[n]);
-      const int4* ed = A.edge + ((size_t)n * A.N + b) * MPS;
+      const int ln = t.local + te;
+      R.y[i] = reinterpret_cast<const VI*>(in)[(size_t)(t.base + te) * A.NE + r];
+      R.mu2[i] = (TB)(0.5 * A.mu[ln]);
+      const int4* ed = A.edge + ((size_t)ln * A.N + b) * MPS;
 #pragma unroll
       for (int q = 0; q < MPS; ++q) R.rec[i][q] = ed[q];
     }
   }
 }
 
+// Neighbour gathers of a tile (after its edge records arrived).  Local ids
+// become slab ids; neighbours beyond the member's depth do not exist.
 template <typename TB, typename TI, int EPT, int MPS>
-__device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restrict__ in, int tot,
+__device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restrict__ in,
+                                             const TileInfo& t,
                                              StreamRegs<TB, TI, EPT, MPS>& R) {
   typedef typename C2<TI>::type VI;
   const VI* v = reinterpret_cast<const VI*>(in);
+  const int tot = t.nt * A.NE;
+  const int off = t.base - t.local;
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
     const int e = threadIdx.x + i * blockDim.x;
@@ -197,11 +232,32 @@ __device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restr
       const int r = e - fdiv(e, A.mNE) * A.NE;
 #pragma unroll
       for (int q = 0; q < MPS; ++q) {
-        const int4 c = R.rec[i][q];
+        int4 c = R.rec[i][q];
+        c.x = (c.x >= 0 && c.x < t.count) ? c.x + off : -1;
+        c.y = (c.y >= 0 && c.y < t.count) ? c.y + off : -1;
+        R.rec[i][q] = c;
         if (c.x >= 0) R.gn[i][q] = v[(size_t)c.x * A.NE + r];
         if (c.y >= 0) R.gp[i][q] = v[(size_t)c.y * A.NE + r];
       }
     }
+  }
+}
+
+// H of `member` (shifted by its mean diagonal) as MFMA A-fragments in LDS.
+template <typename TB>
+__device__ __forceinline__ void stream_load_h(const RhsArgs& A, const StreamLds<TB>& s,
+                                              int member) {
+  const int N = A.N;
+  const int KS = (N + 3) >> 2, nf = StreamFragCount(N);
+  const double2* H = A.H + (size_t)member * A.NE;
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    const int lane = f & 63, ms = f >> 6;
+    const int mt = ms / KS, st = ms - mt * KS;
+    const int row = 16 * mt + (lane & 15), k = (lane >> 4) * KS + st;
+    double2 h = make_double2(0., 0.);
+    if (row < N && k < N) h = H[row + k * N];
+    s.hfr[f] = (TB)h.x;
+    s.hfi[f] = (TB)h.y;
   }
 }
 
@@ -217,9 +273,21 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
   const int N = A.N, NE = A.NE, D = A.D, L = A.L, T = A.tile;
   const int NP = N * (N + 1);
   const StreamLds<TB> s = stream_carve<TB>(smem, N, D, L, T);
-  const int ntiles = (A.num_nodes + T - 1) / T;
-  int tile = blockIdx.x;
-  if (tile >= ntiles) return;
+  // Tiles of this block: a contiguous range of the descriptor list for an
+  // ensemble (members change rarely, so does H), else every gridDim.x-th
+  // tile of the hierarchy (neighbouring tiles in flight together).
+  const int ntiles = A.tiles ? A.ntiles : (A.num_nodes + T - 1) / T;
+  int tile, tile_end, step;
+  if (A.tiles) {
+    tile = (int)(((long long)blockIdx.x * ntiles) / gridDim.x);
+    tile_end = (int)(((long long)(blockIdx.x + 1) * ntiles) / gridDim.x);
+    step = 1;
+  } else {
+    tile = blockIdx.x;
+    tile_end = ntiles;
+    step = gridDim.x;
+  }
+  if (tile >= tile_end) return;
 
   // Per-block constants.
   for (int e = threadIdx.x; e < D * L; e += blockDim.x) {
@@ -227,30 +295,27 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     s.ns[e] = (TB)A.next_s[e];
   }
   for (int e = threadIdx.x; e < N; e += blockDim.x) s.delta[e] = (TB)A.delta[e];
-  {
-    const int KS = (N + 3) >> 2, nf = StreamFragCount(N);
-    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-      const int lane = f & 63, ms = f >> 6;
-      const int mt = ms / KS, st = ms - mt * KS;
-      const int row = 16 * mt + (lane & 15), k = (lane >> 4) * KS + st;
-      double2 h = make_double2(0., 0.);
-      if (row < N && k < N) h = A.H[row + k * N];
-      s.hfr[f] = (TB)h.x;
-      s.hfi[f] = (TB)h.y;
-    }
-  }
 
+  double emax = 0;
   Regs cur;
-  int base = tile * T;
-  int nt = min(T, A.num_nodes - base);
-  stream_load1<TB, TI, EPT, MPS>(A, in, base, nt * NE, cur);
-  stream_load2<TB, TI, EPT, MPS>(A, in, nt * NE, cur);
+  TileInfo ti = stream_tile(A, tile);
+  int hmember = ti.member;
+  stream_load_h<TB>(A, s, hmember);
+  stream_load1<TB, TI, EPT, MPS>(A, in, ti, cur);
+  stream_load2<TB, TI, EPT, MPS>(A, in, ti, cur);
   __syncthreads();  // per-block tables visible
 
-  for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1) {
+  for (int buf = 0; tile < tile_end; tile += step, buf ^= 1) {
+    const int nt = ti.nt, base = ti.base;
     const int tot = nt * NE;
     V* Yb = s.Y + buf * s.bstride;
     V* Wb = s.W + buf * s.bstride;
+    // H of a new member (the previous tile's product finished before the
+    // last barrier; the next barrier publishes the new fragments).
+    if (ti.member != hmember) {
+      hmember = ti.member;
+      stream_load_h<TB>(A, s, hmember);
+    }
     // (1) elementwise part of this tile -> LDS; epilogue operands of this tile
     Pre pre[EPT];
 #pragma unroll
@@ -289,16 +354,19 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
       }
     }
     // (2) first loads of the next tile
-    const int ntile = tile + gridDim.x;
-    const int nbase = ntile * T;
-    const int nnt = ntile < ntiles ? min(T, A.num_nodes - nbase) : 0;
-    if (nnt > 0) stream_load1<TB, TI, EPT, MPS>(A, in, nbase, nnt * NE, cur);
+    const int ntile = tile + step;
+    const bool more = ntile < tile_end;
+    TileInfo nti = ti;
+    if (more) {
+      nti = stream_tile(A, ntile);
+      stream_load1<TB, TI, EPT, MPS>(A, in, nti, cur);
+    }
     __syncthreads();
     // (3) matrix-core product
     stream_gemm<TB>(A, s, Yb, Wb, nt);
     __syncthreads();
     // (4) neighbour gathers of the next tile
-    if (nnt > 0) stream_load2<TB, TI, EPT, MPS>(A, in, nnt * NE, cur);
+    if (more) stream_load2<TB, TI, EPT, MPS>(A, in, nti, cur);
     // (5) epilogue of this tile
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
@@ -311,11 +379,26 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
         double2 k;
         k.x = (double)(p.x + q.x);
         k.y = (double)(p.y - q.y);
-        epi.apply((size_t)(base + te) * NE + r, k, pre[i]);
+        epi.apply((size_t)(base + te) * NE + r, k, pre[i], &emax);
       }
     }
-    base = nbase;
-    nt = nnt;
+    ti = nti;
+  }
+  // Epilogues that reduce (the RKF45 error norm): max over the block, one
+  // atomic per block.
+  if constexpr (EPI::kReduce) {
+    __shared__ double red[16];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v = __shfl_xor(emax, o, 64);
+      emax = v > emax ? v : emax;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = emax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] > m ? red[w] : m;
+      epi.finalize(m);
+    }
   }
 }
 
@@ -326,15 +409,17 @@ template <typename TB, int STAGE, bool IN_STATE>
 struct EpiNested {
   struct Pre { double2 x; };
   static constexpr bool kInIsState = IN_STATE;
+  static constexpr bool kReduce = false;
   double* rho;
   TB* yout;
   double c;
   __device__ __forceinline__ Pre pre(size_t g) const { return Pre{ld2(rho, g)}; }
   __device__ __forceinline__ Pre from_state(double2 x) const { return Pre{x}; }
-  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p) const {
+  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p, double*) const {
     if (STAGE < 3) st<TB>(yout, g, p.x.x + c * k.x, p.x.y + c * k.y);
     else st<double>(rho, g, p.x.x + c * k.x, p.x.y + c * k.y);
   }
+  __device__ __forceinline__ void finalize(double) const {}
 };
 
 // Classic form (PHI's order):
@@ -344,10 +429,12 @@ template <typename TB, int STAGE, bool IN_STATE>
 struct EpiClassic {
   struct Pre { double2 x, q; };
   static constexpr bool kInIsState = IN_STATE;
+  static constexpr bool kReduce = false;
   double* rho;
   double* acc;
   TB* yout;
   double dt;
+  __device__ __forceinline__ void finalize(double) const {}
   __device__ __forceinline__ Pre pre(size_t g) const {
     Pre p;
     if (STAGE <= 2) p.x = ld2(rho, g);
@@ -359,7 +446,7 @@ struct EpiClassic {
     p.x = x;
     return p;
   }
-  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p) const {
+  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p, double*) const {
     if (STAGE == 0) {
       st<double>(acc, g, p.x.x + (dt / 6) * k.x, p.x.y + (dt / 6) * k.y);
       st<TB>(yout, g, p.x.x + (dt / 2) * k.x, p.x.y + (dt / 2) * k.y);
@@ -371,6 +458,72 @@ struct EpiClassic {
       st<double>(rho, g, p.q.x + (dt / 6) * k.x, p.q.y + (dt / 6) * k.y);
     }
   }
+};
+
+// --- RKF45 epilogue ------------------------------------------------------------
+// Stage s < 5 stores k_s (bulk precision) and the next stage input
+//   tmp = rho + sum_{j <= s} b[s][j] k_j
+// (k_s as stored, so later stages see the same value; left-to-right sums as
+// PHI, hierarchy_integrator.cc:2019-2253).  Stage 5 forms
+//   y4 = rho + c4 . (k0, k2, k3, k4),  y5 = rho + c5 . (k0, k2, k3, k4, k5),
+// stores y5 (FP64) and reduces max |y4 - y5| (hierarchy_integrator.cc:
+// 2317-2328) into *err (one atomic max per block on the bits of the
+// non-negative double).
+template <typename TB, int STAGE, bool IN_STATE>
+struct EpiRkf45 {
+  typedef typename C2<TB>::type V;
+  struct Pre {
+    double2 x;
+    V k[5];
+  };
+  static constexpr bool kInIsState = IN_STATE;
+  static constexpr bool kReduce = STAGE == 5;
+  Rkf45Bufs<TB> B;
+  const double* rho;
+  Rkf45Coefs cf;
+  __device__ __forceinline__ Pre pre(size_t g) const {
+    Pre p;
+    p.x = ld2(rho, g);
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      if (j < STAGE && !(STAGE == 5 && j == 1)) p.k[j] = ld<TB, TB>(B.k[j], g);
+    return p;
+  }
+  __device__ __forceinline__ Pre from_state(double2 x) const {
+    Pre p;
+    p.x = x;
+    return p;
+  }
+  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p, double* emax) const {
+    const double ksx = (double)(TB)k.x, ksy = (double)(TB)k.y;
+    if (STAGE < 5) {
+      st<TB>(B.k[STAGE], g, k.x, k.y);
+      double sx = p.x.x, sy = p.x.y;
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        if (j < STAGE) {
+          sx += (double)p.k[j].x * cf.b[STAGE][j];
+          sy += (double)p.k[j].y * cf.b[STAGE][j];
+        }
+      sx += ksx * cf.b[STAGE][STAGE];
+      sy += ksy * cf.b[STAGE][STAGE];
+      st<TB>(B.tmp_out, g, sx, sy);
+    } else {
+      const V k0 = p.k[0], k2 = p.k[2], k3 = p.k[3], k4 = p.k[4];
+      const double y4x = p.x.x + (double)k0.x * cf.c4[0] + (double)k2.x * cf.c4[1] +
+                         (double)k3.x * cf.c4[2] + (double)k4.x * cf.c4[3];
+      const double y4y = p.x.y + (double)k0.y * cf.c4[0] + (double)k2.y * cf.c4[1] +
+                         (double)k3.y * cf.c4[2] + (double)k4.y * cf.c4[3];
+      const double y5x = p.x.x + (double)k0.x * cf.c5[0] + (double)k2.x * cf.c5[1] +
+                         (double)k3.x * cf.c5[2] + (double)k4.x * cf.c5[3] + ksx * cf.c5[4];
+      const double y5y = p.x.y + (double)k0.y * cf.c5[0] + (double)k2.y * cf.c5[1] +
+                         (double)k3.y * cf.c5[2] + (double)k4.y * cf.c5[3] + ksy * cf.c5[4];
+      st<double>(B.rho_new, g, y5x, y5y);
+      const double d = hypot(y4x - y5x, y4y - y5y);
+      *emax = d > *emax ? d : *emax;
+    }
+  }
+  __device__ __forceinline__ void finalize(double m) const { atomic_max_nonneg(B.err, m); }
 };
 
 }  // namespace dev

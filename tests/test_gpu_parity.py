@@ -268,3 +268,111 @@ def test_fp32_bulk_population_error_vs_fp64(cfg, steps):
     print("%s fp32 population error %.3e coherence error %.3e" % (cfg, pop, coh))
     assert abs(np.trace(out["fp64"]) - 1) < 1e-10
     assert pop <= 1e-6
+
+
+@pytest.mark.parametrize("cfg,lh,prec", [("c2", 5, "fp64"), ("c2", 5, "fp32"), ("c1", 9, "fp64")])
+def test_graph_replay_matches_eager(cfg, lh, prec):
+    """RK4 blocks replayed from a captured hipGraph give the kernel-by-kernel
+    result bit for bit, over several replays with host reads in between."""
+    _, p = _subset_params(cfg, lh)
+    dt = p.get_double("timestep")
+    states = {}
+    for graphs in (False, True):
+        ctx = P.Context(p, precision=prec)
+        ctx.set_graphs(graphs)
+        got = []
+        for chunk in (70, 130, 3, 97):
+            ctx.rk4_steps(chunk, dt)
+            got.append(ctx.state())
+        states[graphs] = got
+        ctx.close()
+    for a, b in zip(states[False], states[True]):
+        assert np.array_equal(a, b)
+
+
+def test_autotune_keeps_state_and_parity():
+    """The autotuner leaves the state untouched and its chosen shape still
+    gives the oracle's result (7-site system, depth 4)."""
+    op, p = _subset_params("c2", 5)
+    dt = op.timestep
+    orc = HO.Oracle(HO.Problem.from_prm(op), threads=8)
+    orc.rk4(20 * dt * (1 - 1e-9), dt, 1.0)
+    ctx = P.Context(p)
+    before = ctx.state()
+    ms, shape = ctx.autotune(dt, steps=3)
+    print("autotune: %.4f ms/step, shape %s" % (ms, shape))
+    assert ms > 0
+    assert np.array_equal(ctx.state(), before)
+    ctx.rk4_steps(20, dt)
+    assert np.abs(ctx.state() - orc.state()).max() <= 1e-10
+    # cached: a second context of the same problem gets the same shape
+    ctx2 = P.Context(p)
+    ms2, shape2 = ctx2.autotune(dt, steps=3)
+    assert shape2 == shape
+    ctx.close()
+    ctx2.close()
+
+
+def _ensemble_members(count, max_lh):
+    from phi_amd.configs.make_configs import c4_member
+    Hs, depths = [], []
+    for r in range(count):
+        H, lh = c4_member(r)
+        Hs.append(H)
+        depths.append(min(lh, max_lh))
+    return np.array(Hs), depths
+
+
+def _member_params(p, H, lh):
+    q = P.PhiParameters.read(os.path.join(CFG, "c2.prm"), "rk4")
+    q.set_matrix("time_indepenent_hamiltonian", H)
+    q.set_int("hierarchy_truncation_level", lh)
+    return q
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_ensemble_matches_member_runs(prec):
+    """An ensemble of 7-site hierarchies of uneven depth (static disorder)
+    advances every member exactly as integrating it alone; the recorded
+    history is the member mean; one member checked against the oracle."""
+    p = P.PhiParameters.read(os.path.join(CFG, "c2.prm"), "rk4")
+    dt = p.get_double("timestep")
+    Hs, depths = _ensemble_members(6, 6)
+    print("depths", depths, flush=True)
+    ens = P.Ensemble(p, Hs, depths, precision=prec)
+    assert ens.members == 6
+    steps = 30
+    ens.rk4_steps(steps, dt)
+    slab = ens.state()
+    r0 = ens.member_rho0()
+    off = 0
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    for m in range(6):
+        ctx = P.Context(_member_params(p, Hs[m], depths[m]), precision=prec)
+        assert ctx.num_nodes == ens.counts[m]
+        ctx.rk4_steps(steps, dt)
+        single = ctx.state()
+        d = np.abs(slab[off:off + ctx.num_nodes] - single).max()
+        print("member %d nodes %d max|d| %.3e" % (m, ctx.num_nodes, d), flush=True)
+        assert d <= tol
+        assert np.abs(r0[m] - ctx.rho0()).max() <= tol
+        off += ctx.num_nodes
+        ctx.close()
+    assert off == ens.num_nodes
+    if prec == "fp64":
+        op = PO.read_command_line(os.path.join(CFG, "c2.prm"), "rk4")
+        op.time_indepenent_hamiltonian = Hs[2]
+        op.hierarchy_truncation_level = depths[2]
+        orc = HO.Oracle(HO.Problem.from_prm(op), threads=8)
+        orc.rk4(steps * dt * (1 - 1e-9), dt, 1.0)
+        assert np.abs(r0[2] - orc.state()[0]).max() <= 1e-10
+    ens.close()
+    # the recorded history is the mean of the members' rho_0
+    ens2 = P.Ensemble(p, Hs, depths, precision=prec)
+    hist = ens2.rk4_history(21, dt, 10)
+    assert hist.shape == (3, 7, 7)
+    ens3 = P.Ensemble(p, Hs, depths, precision=prec)
+    ens3.rk4_steps(21, dt)
+    assert np.abs(hist[-1] - ens3.member_rho0().mean(axis=0)).max() <= 1e-13
+    ens2.close()
+    ens3.close()
