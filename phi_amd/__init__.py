@@ -44,6 +44,7 @@ def lib():
     vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     sz, cs = ctypes.c_size_t, ctypes.c_char_p
     psz = ctypes.POINTER(ctypes.c_size_t)
+    fp = ctypes.POINTER(ctypes.c_float)
     sigs = {
         "phi_hip_last_error": (cs, []),
         "phi_hip_version": (ctypes.c_int, []),
@@ -96,6 +97,19 @@ def lib():
         "phi_hip_ensemble_rho0": (ctypes.c_int, [vp, dp, sz]),
         "phi_hip_rk4_history": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double,
                                                ctypes.c_int, dp, sz, ip]),
+        "phi_hip_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(vp)]),
+        "phi_hip_mlp_destroy": (None, [vp]),
+        "phi_hip_mlp_param_count": (ctypes.c_longlong, [vp]),
+        "phi_hip_mlp_set_params": (ctypes.c_int, [vp, fp, sz]),
+        "phi_hip_mlp_get_params": (ctypes.c_int, [vp, fp, sz]),
+        "phi_hip_mlp_forward": (ctypes.c_int, [vp, fp, ctypes.c_int, fp]),
+        "phi_hip_mlp_gradients": (ctypes.c_int, [vp, fp, fp, ctypes.c_int, dp, fp, sz]),
+        "phi_hip_mlp_train": (ctypes.c_int, [vp, fp, fp, ctypes.c_longlong, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_float, ctypes.c_uint, dp]),
+        "phi_hip_closure_samples": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, ctypes.c_longlong,
+                                                   ctypes.POINTER(ctypes.c_longlong)]),
+        "phi_hip_closure_attach": (ctypes.c_int, [vp, vp]),
         "phi_hip_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, dp, ip]),
     }
     for name, (res, args) in sigs.items():
@@ -439,6 +453,98 @@ class Ensemble(Context):
         out = np.zeros(2 * self.members * self.N * self.N)
         _check(lib().phi_hip_ensemble_rho0(self._h, _dp(out), self.members * self.N * self.N))
         return out.view(complex).reshape(self.members, self.N, self.N).transpose(0, 2, 1).copy()
+
+    def index(self):
+        """Multi-indices of the deepest member (every member's nodes are the
+        first counts[m] of them)."""
+        n = int(self.counts.max())
+        out = np.zeros(n * self.D, dtype=np.int32)
+        _check(lib().phi_hip_get_index(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       out.size))
+        return out.reshape(n, self.D)
+
+    def closure_samples(self, shallow_depth):
+        """Training rows for hierarchies of depth `shallow_depth` (PHI's
+        HierarchyTruncation) from the current state: (X, T) float32."""
+        N, D = self.N, self.D
+        nb = int(np.sum(self.index().sum(axis=1) == shallow_depth - 1))
+        cap = max(1, self.members * nb)
+        X = np.zeros((cap, 2 * N * N + D), dtype=np.float32)
+        T = np.zeros((cap, 2 * N * N * D), dtype=np.float32)
+        rows = ctypes.c_longlong()
+        _check(lib().phi_hip_closure_samples(self._h, int(shallow_depth), _fp(X), _fp(T), cap,
+                                             ctypes.byref(rows)))
+        return X[:rows.value], T[:rows.value]
+
+    def attach_closure(self, net):
+        """Couple a trained closure network (Mlp); resets the state."""
+        _check(lib().phi_hip_closure_attach(self._h, net._h))
+        self._net = net
+        self.num_nodes = lib().phi_hip_num_nodes(self._h)
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """The truncation-closure network on the device (phi_hip_mlp_*): 3-layer
+    perceptron n_in -> n_hidden -> n_hidden -> n_out with GELU, bfloat16 GEMM
+    operands and float32 accumulation; flat float32 parameters W1, b1, W2, b2,
+    W3, b3 (weights (out, in) row-major)."""
+
+    def __init__(self, n_in, n_hidden, n_out, max_batch=4096):
+        h = ctypes.c_void_p()
+        _check(lib().phi_hip_mlp_create(int(n_in), int(n_hidden), int(n_out), int(max_batch),
+                                        ctypes.byref(h)))
+        self._h = h
+        self.n_in, self.n_hidden, self.n_out = n_in, n_hidden, n_out
+        self.max_batch = max_batch
+        self.param_count = lib().phi_hip_mlp_param_count(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().phi_hip_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        f = np.ascontiguousarray(np.asarray(flat, dtype=np.float32))
+        _check(lib().phi_hip_mlp_set_params(self._h, _fp(f), f.size))
+
+    def get_params(self):
+        f = np.zeros(self.param_count, dtype=np.float32)
+        _check(lib().phi_hip_mlp_get_params(self._h, _fp(f), f.size))
+        return f
+
+    def forward(self, x):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+        y = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        _check(lib().phi_hip_mlp_forward(self._h, _fp(x), x.shape[0], _fp(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+        t = np.ascontiguousarray(np.asarray(t, dtype=np.float32))
+        g = np.zeros(self.param_count, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().phi_hip_mlp_gradients(self._h, _fp(x), _fp(t), x.shape[0],
+                                           ctypes.byref(loss), _fp(g), g.size))
+        return loss.value, g
+
+    def train(self, X, T, batch=256, epochs=1, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(np.asarray(X, dtype=np.float32))
+        T = np.ascontiguousarray(np.asarray(T, dtype=np.float32))
+        losses = np.zeros(epochs)
+        _check(lib().phi_hip_mlp_train(self._h, _fp(X), _fp(T), X.shape[0], int(batch),
+                                       int(epochs), float(lr), int(seed), _dp(losses)))
+        return losses
 
 
 def params_from_problem(H, rho0, gamma, lambda_, temperature, lh, num_terms=1,

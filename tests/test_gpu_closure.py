@@ -77,9 +77,10 @@ def test_mlp_training_reduces_loss():
     T = np.tanh(X @ A.T).astype(np.float32)
     net = P.Mlp(n_in, n_hid, n_out, max_batch=256)
     net.set_params(_flat(M.init_params(n_in, n_hid, n_out, seed=2)))
-    losses = net.train(X, T, batch=128, epochs=6, lr=3e-3, seed=5)
+    losses = net.train(X, T, batch=128, epochs=10, lr=3e-3, seed=5)
     print("epoch losses", losses)
-    assert losses[-1] < 0.2 * losses[0]
+    assert np.all(np.diff(losses) < 0)
+    assert losses[-1] < 0.3 * losses[0]
     # the trained network's forward pass agrees with the oracle on its parameters
     p = _unflat(net.get_params(), [n_in, n_hid, n_hid, n_out])
     y = net.forward(X[:128])
