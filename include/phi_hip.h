@@ -185,6 +185,10 @@ int phi_hip_closure_samples(phi_ctx* c, int shallow_depth, float* X, float* T,
  * state to the initial density matrix.  The network must outlive the
  * context's RK4 calls. */
 int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m);
+/* Closure predictions by one fused launch per stage (on = 1, default) or by
+ * the unfused kernels (inputs, the network's GEMMs, scatter); the network's
+ * parameters are packed when the closure is attached. */
+int phi_hip_closure_fused(phi_ctx* c, int on);
 int phi_hip_num_nodes(const phi_ctx* c);
 int phi_hip_num_modes(const phi_ctx* c);
 /* Multi-indices (num_nodes * D ints), PHI id order. */

@@ -110,6 +110,7 @@ def lib():
         "phi_hip_closure_samples": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, ctypes.c_longlong,
                                                    ctypes.POINTER(ctypes.c_longlong)]),
         "phi_hip_closure_attach": (ctypes.c_int, [vp, vp]),
+        "phi_hip_closure_fused": (ctypes.c_int, [vp, ctypes.c_int]),
         "phi_hip_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, dp, ip]),
     }
     for name, (res, args) in sigs.items():
@@ -481,6 +482,10 @@ class Ensemble(Context):
         _check(lib().phi_hip_closure_attach(self._h, net._h))
         self._net = net
         self.num_nodes = lib().phi_hip_num_nodes(self._h)
+
+    def set_closure_fused(self, on):
+        """Closure predictions by the fused pass (True) or unfused kernels."""
+        _check(lib().phi_hip_closure_fused(self._h, 1 if on else 0))
 
 
 def _fp(a):

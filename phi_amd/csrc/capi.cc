@@ -547,6 +547,12 @@ int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m) {
   return rc ? Err(rc, c->e->last_error()) : Ok();
 }
 
+int phi_hip_closure_fused(phi_ctx* c, int on) {
+  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+  c->e->set_closure_fused(on != 0);
+  return Ok();
+}
+
 int phi_hip_num_nodes(const phi_ctx* c) { return c ? c->e->num_nodes() : PHI_HIP_EINVAL; }
 int phi_hip_num_modes(const phi_ctx* c) { return c ? c->e->tables().D : PHI_HIP_EINVAL; }
 

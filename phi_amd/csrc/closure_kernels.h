@@ -6,6 +6,28 @@
 
 namespace phi {
 
+// The closure network packed for the fused inference pass: weights as
+// bfloat16 rows (out x K, row-major; K = k1 for layer 1 and hid for layers
+// 2-3, rows padded to multiples of 64 with zeros), biases float32.
+struct FusedNet {
+  const unsigned short* w1;
+  const unsigned short* w2;
+  const unsigned short* w3;
+  const float* b1;
+  const float* b2;
+  const float* b3;
+  int k1;     // inputs rounded up to a multiple of 32
+  int hid;    // hidden width, a multiple of 32
+  int n_out;
+  int width;  // max(k1, hid): row width of the LDS activations
+};
+
+size_t FusedClosureLds(const FusedNet& net);
+// Inputs, the three layers and the scatter in one launch (32 rows per block).
+hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
+                              int members, int first, int nb, int ne, int D, const float* bmi,
+                              int vbase, hipStream_t s);
+
 // Network inputs X (rows x (2 N^2 + D)) and scales S of the boundary nodes
 // first .. first + nb - 1 of every member, from a stage input in FP32 or
 // FP64; bmi = the boundary multi-indices / L (nb x D).

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "closure_kernels.h"
 #include "heom_kernels.h"
 #include "hierarchy.h"
 #include "phi_parameters.h"
@@ -73,6 +74,12 @@ class Engine {
   // borrowed and must outlive the engine's use of it.
   int AttachClosure(class Mlp* mlp);
   bool has_closure() const { return closure_ != nullptr; }
+  // Closure predictions by the fused pass (default) or by the unfused
+  // kernels (inputs, the network's GEMMs, scatter); same arithmetic.
+  void set_closure_fused(bool v) {
+    closure_fused_ = v;
+    DropGraph();
+  }
   bool fp32() const { return fp32_; }
   void set_fp32(bool v);
   // Stage-kernel shape: col_nm = kernel form (0 element, 1 matrix cores,
@@ -196,6 +203,8 @@ class Engine {
 
   // Closure.
   class Mlp* closure_ = nullptr;
+  bool closure_fused_ = true;
+  FusedNet fused_{};
   int closure_first_ = 0, closure_nb_ = 0, closure_vbase_ = 0;
   float* d_cbmi_ = nullptr;
   float* d_cx_ = nullptr;

@@ -169,6 +169,10 @@ int Engine::AttachClosure(Mlp* mlp) {
   if (!rc) rc = Check(hipMalloc((void**)&d_cx_, (size_t)nrows * mlp->n_in() * 4), "alloc X");
   if (!rc) rc = Check(hipMalloc((void**)&d_cs_, (size_t)nrows * 4), "alloc S");
   if (!rc) rc = Check(hipMalloc((void**)&d_cy_, (size_t)nrows * mlp->n_out() * 4), "alloc Y");
+  if (!rc) {
+    rc = mlp->Pack(&fused_);
+    if (rc) err_ = "closure network: " + mlp->last_error();
+  }
   if (rc) return rc;
   args_.num_nodes = slab_nodes_;
   closure_ = mlp;
@@ -179,8 +183,15 @@ int Engine::AttachClosure(Mlp* mlp) {
   return EnsureTiles();
 }
 
+// Predictions for the virtual slots of the stage input `in` (one fused
+// launch; the unfused path, kept for reference, runs the inputs kernel,
+// the network's GEMMs and the scatter kernel).
 int Engine::ClosurePredict(const void* in, bool in_fp32) {
   const int NE = h_.N * h_.N, D = h_.D;
+  if (closure_fused_)
+    return Check(LaunchClosureFused(fused_, const_cast<void*>(in), in_fp32, d_moff_, members_,
+                                    closure_first_, closure_nb_, NE, D, d_cbmi_, closure_vbase_,
+                                    stream_), "closure fused");
   const long long nrows = (long long)members_ * closure_nb_;
   int rc = Check(LaunchClosureInputs(in, in_fp32, d_moff_, members_, closure_first_, closure_nb_,
                                      NE, D, d_cbmi_, d_cx_, d_cs_, stream_), "closure inputs");

@@ -3,8 +3,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <random>
+
+#include "closure_kernels.h"
 
 namespace phi {
 
@@ -78,9 +81,62 @@ int Mlp::Create(int n_in, int n_hidden, int n_out, int max_batch, Mlp** out, std
 Mlp::~Mlp() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   void* ptrs[] = {d_p_, d_g_, d_m_, d_v_, d_x_, d_t_, d_z1_, d_h1_, d_z2_, d_h2_, d_y_,
-                  d_dy_, d_dh2_, d_dh1_, d_partial_};
+                  d_dy_, d_dh2_, d_dh1_, d_partial_, d_pack_};
   for (void* p : ptrs) FreeDev(p);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+namespace {
+
+// float32 -> bfloat16 bits, round to nearest even (as the device kernels).
+unsigned short Bf16(float f) {
+  unsigned u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+}  // namespace
+
+int Mlp::Pack(FusedNet* out) {
+  if (n_hid_ % 32 != 0) {
+    err_ = "the fused closure pass needs a hidden width that is a multiple of 32";
+    return -3;
+  }
+  std::vector<float> p(nparam_);
+  int rc = GetParams(p.data());
+  if (rc) return rc;
+  const int k1 = (n_in_ + 31) / 32 * 32;
+  const int hid = n_hid_;
+  auto rows64 = [](int n) { return (n + 63) / 64 * 64; };
+  const size_t n1 = (size_t)rows64(hid) * k1, n2 = (size_t)rows64(hid) * hid,
+               n3 = (size_t)rows64(n_out_) * hid;
+  std::vector<unsigned short> w(n1 + n2 + n3, 0);
+  for (int o = 0; o < hid; ++o)
+    for (int i = 0; i < n_in_; ++i) w[(size_t)o * k1 + i] = Bf16(p[off_[0] + (size_t)o * n_in_ + i]);
+  for (int o = 0; o < hid; ++o)
+    for (int i = 0; i < hid; ++i) w[n1 + (size_t)o * hid + i] = Bf16(p[off_[2] + (size_t)o * hid + i]);
+  for (int o = 0; o < n_out_; ++o)
+    for (int i = 0; i < hid; ++i)
+      w[n1 + n2 + (size_t)o * hid + i] = Bf16(p[off_[4] + (size_t)o * hid + i]);
+  FreeDev(d_pack_);
+  d_pack_ = nullptr;
+  rc = Check(hipMalloc((void**)&d_pack_, w.size() * sizeof(unsigned short)), "alloc pack");
+  if (!rc) rc = Check(hipMemcpy(d_pack_, w.data(), w.size() * sizeof(unsigned short),
+                                hipMemcpyHostToDevice), "upload pack");
+  if (rc) return rc;
+  out->w1 = d_pack_;
+  out->w2 = d_pack_ + n1;
+  out->w3 = d_pack_ + n1 + n2;
+  out->b1 = d_p_ + off_[1];
+  out->b2 = d_p_ + off_[3];
+  out->b3 = d_p_ + off_[5];
+  out->k1 = k1;
+  out->hid = hid;
+  out->n_out = n_out_;
+  out->width = std::max(k1, hid);
+  return 0;
 }
 
 int Mlp::SetParams(const float* flat) {

@@ -201,3 +201,54 @@ def test_closure_injects_predicted_neighbours():
     print("closure coupling: max |expect| %.3e, max err %.3e" % (scale, err))
     assert scale > 0
     assert err <= 1e-3 * scale
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_closure_pass_equals_unfused_and_network(prec):
+    """The fused closure launch (inputs + 3 layers + scatter) gives the same
+    states as the unfused kernels, and the virtual slots written at stage 0
+    hold the network's prediction for the boundary ADOs."""
+    from phi_amd.configs.make_configs import c4_member
+    p = _c2_params()
+    Hs = np.array([c4_member(r)[0] for r in range(3)])
+    rng = np.random.default_rng(11)
+    params = M.init_params(105, 256, 686, seed=4)
+    params = [(W, (rng.standard_normal(b.shape) * 0.05).astype(np.float32)) for W, b in params]
+    net = P.Mlp(105, 256, 686, max_batch=4096)
+    net.set_params(_flat(params))
+    lh = 4
+    states = {}
+    ens0 = None
+    for fused in (True, False):
+        ens = P.Ensemble(p, Hs, [lh] * 3, precision=prec)
+        ens.attach_closure(net)
+        ens.set_closure_fused(fused)
+        n_real = ens.counts[0]
+        span = ens.num_nodes // 3
+        if ens0 is None:
+            x0 = 0.05 * (rng.standard_normal((ens.num_nodes, 7, 7))
+                         + 1j * rng.standard_normal((ens.num_nodes, 7, 7)))
+            ens0 = x0 + x0.conj().transpose(0, 2, 1)
+        ens.set_state(ens0)
+        ens.rk4_steps(1, 1e-4)
+        states[fused] = ens.state()
+        ens.close()
+    d = np.abs(states[True] - states[False]).max()
+    print("fused vs unfused max |d| %.3e" % d)
+    assert d <= 1e-12 * np.abs(states[False]).max()
+    # virtual slots of member 1 after the step = network(boundary ADOs of x0)
+    idx = P.Context(_member_param(p, Hs[0], lh)).index()
+    bnodes = np.nonzero(idx.sum(axis=1) == lh - 1)[0]
+    m = 1
+    xs, ss = [], []
+    for n in bnodes:
+        x, s = M.encode(ens0[m * span + n], idx[n], lh - 1)
+        xs.append(x)
+        ss.append(s)
+    y = net.forward(np.array(xs))
+    for bi in (0, len(bnodes) // 2, len(bnodes) - 1):
+        kids = M.decode_children(y[bi], ss[bi], 7, 7)
+        for j in range(7):
+            slot = states[True][m * span + n_real + bi * 7 + j]
+            tol = (1e-5 if prec == "fp64" else 1e-5) * max(1e-30, np.abs(kids[j]).max())
+            assert np.abs(slot - kids[j]).max() <= tol
