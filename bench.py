@@ -23,6 +23,12 @@ cpu_baseline  the oracle's pthread restatement of PHI's RK4 loop (FP64) on
          the host cores, timed on a bounded sample of the same workload.
 c3_fp32  the 24-site ring at depth 4 (20475 nodes of 24x24), FP32-bulk RK4:
          node-steps/s and the HBM fraction of its stage kernels.
+c4_ensemble  512 static-disorder realisations of the 7-site system with
+         depths U{4..8} in one slab (persistent kernels, autotuned shape,
+         hipGraph step blocks): time to 1 ps, FP32-bulk and FP64.
+c5_closure   the learned truncation closure: trained on depth-10 runs,
+         depth 4 + closure vs plain depth 4 vs plain depth 10 on held-out
+         Hamiltonians (populations and time to 1 ps).
 """
 
 import argparse
@@ -105,6 +111,72 @@ def run_rk4(P, p, prec, steps, warmup, tune=True):
     return out
 
 
+def run_ensemble(P, np, steps, dt, members, log):
+    """Static-disorder ensemble: `members` realisations of the 7-site system
+    (site energies + N(0, 100 cm^-1), depth U{4..8}) in one slab, FP32-bulk
+    and FP64 RK4 to `steps` steps with the member-averaged rho_0 recorded
+    every 10 steps."""
+    from phi_amd.configs.make_configs import c4_member
+    p = P.PhiParameters.read(CFG, "rk4")
+    Hs, depths = [], []
+    for r in range(members):
+        H, lh = c4_member(r)
+        Hs.append(H)
+        depths.append(lh)
+    out = {}
+    for prec in ("fp32", "fp64"):
+        ens = P.Ensemble(p, np.array(Hs), depths, precision=prec)
+        _, shape = ens.autotune(dt, steps=3)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        hist = ens.rk4_history(steps, dt, 10)
+        ens.synchronize()
+        el = time.perf_counter() - t0
+        out[prec] = {"seconds": el, "nodes": ens.num_nodes, "hist": hist, "shape": shape}
+        log("bench: ensemble %s %d members %d nodes: %.3f s" % (prec, members, ens.num_nodes, el))
+        ens.close()
+    pops = {k: np.real(np.einsum("tii->ti", v["hist"])) for k, v in out.items()}
+    return {
+        "workload": "%d realisations of the 7-site exciton (site-energy disorder N(0,100), "
+                    "depth U{4..8}), one slab, RK4 dt=1 fs, member-averaged populations every 10 fs"
+                    % members,
+        "members": members, "nodes": out["fp32"]["nodes"],
+        "time_to_1ps_s_fp32_bulk": out["fp32"]["seconds"] * (1000.0 / steps),
+        "time_to_1ps_s_fp64": out["fp64"]["seconds"] * (1000.0 / steps),
+        "node_steps_per_s_fp32_bulk": out["fp32"]["nodes"] * steps / out["fp32"]["seconds"],
+        "node_steps_per_s_fp64": out["fp64"]["nodes"] * steps / out["fp64"]["seconds"],
+        "averaged_population_error_fp32_vs_fp64": float(np.abs(pops["fp32"] - pops["fp64"]).max()),
+        "kernel_shape_fp32": out["fp32"]["shape"]}
+
+
+def run_closure(P, np, steps, dt, train_members, test_members, log):
+    """Learned truncation closure: train on deep (depth 10) trajectories of
+    random 7-site Hamiltonians, then integrate held-out ones at depth 4 with
+    and without the closure and at depth 10 (FP64), to 1 ps."""
+    from phi_amd import closure as C
+    t0 = time.perf_counter()
+    X, T = C.generate_samples(C.hamiltonians(0, train_members), deep=11, shallow=5, steps=1000,
+                              dt=dt, every=100, keep=0.25, seed=1, members=256, log=log)
+    t_data = time.perf_counter() - t0
+    net, losses = C.train(X, T, epochs=20, batch=512, lr=1e-3, seed=2, log=log)
+    res = C.evaluate(C.hamiltonians(100000, test_members), net, deep=11, shallow=5, steps=steps,
+                     dt=dt, every=10, log=log)
+    scale = 1000.0 / steps
+    return {
+        "workload": "7-site exciton, random Hamiltonians; network trained on %d depth-10 "
+                    "trajectories (%d rows), evaluated on %d held-out ones, FP64 RK4 dt=1 fs"
+                    % (train_members, len(X), test_members),
+        "train_seconds_data": t_data, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+        "time_to_1ps_s_depth10": res["deep"]["seconds"] * scale,
+        "time_to_1ps_s_depth4": res["shallow"]["seconds"] * scale,
+        "time_to_1ps_s_depth4_closure": res["closure"]["seconds"] * scale,
+        "speedup_closure_vs_depth10": res["deep"]["seconds"] / res["closure"]["seconds"],
+        "max_population_error_vs_depth10": {"depth4": res["shallow"]["max_pop_err"],
+                                            "depth4_closure": res["closure"]["max_pop_err"]},
+        "mean_population_error_vs_depth10": {"depth4": res["shallow"]["mean_pop_err"],
+                                             "depth4_closure": res["closure"]["mean_pop_err"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +184,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=512)
+    ap.add_argument("--closure-train", type=int, default=512)
+    ap.add_argument("--closure-test", type=int, default=256)
     ap.add_argument("--calibration-copy", action="store_true",
                     help="first copy 1 GiB device to device (a plain copy of known size, "
                          "the reference point of the HBM byte counters under rocprofv3)")
@@ -194,6 +271,13 @@ def main():
             "node_steps_per_s": r3["node_steps_per_s"], "ms_per_step": r3["ms_per_step"],
             "hbm_frac_step": step_bytes / (r3["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "stage_ms": r3["stage_ms"], "kernel_shape": r3["shape"]}
+    if not args.no_ensemble:
+        log("bench: static-disorder ensemble")
+        line["c4_ensemble"] = run_ensemble(P, np, 1000, dt, args.ensemble_members, log)
+    if not args.no_closure:
+        log("bench: learned closure")
+        line["c5_closure"] = run_closure(P, np, 1000, dt, args.closure_train, args.closure_test,
+                                         log)
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline()
