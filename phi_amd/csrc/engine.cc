@@ -257,7 +257,6 @@ int Engine::CreateEnsemble(const PhiParameters& base, int members,
   e->d_rho_ = nullptr;
   const size_t bytes = e->state_elems() * 2 * sizeof(double);
   rc = e->Check(Upload(&e->d_Hm_, Hm.data(), Hm.size()), "upload member H");
-  if (!rc) rc = e->Check(Upload(&e->d_mcount_, e->mcount_.data(), e->mcount_.size()), "upload counts");
   if (!rc) rc = e->Check(Upload(&e->d_moff_, e->moff_.data(), e->moff_.size()), "upload offsets");
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_rho_, bytes), "alloc ensemble rho");
   if (!rc) rc = e->Check(hipMemset(e->d_rho_, 0, bytes), "zero ensemble rho");
@@ -277,7 +276,6 @@ int Engine::CreateEnsemble(const PhiParameters& base, int members,
   a.num_nodes = e->slab_nodes_;
   a.H = e->d_Hm_;
   a.real_h = real_h;
-  a.mcount = e->d_mcount_;
   rc = e->SetLaunch(2, 0, 0, 0);
   if (rc) {
     *why = e->err_;
@@ -293,20 +291,20 @@ int Engine::CreateEnsemble(const PhiParameters& base, int members,
 int Engine::BuildTiles() {
   if (members_ == 0) return kOk;
   const int T = args_.col_tile[fp32_ ? 1 : 0];
-  std::vector<int4> t;
+  std::vector<TileDesc> t;
   for (int m = 0; m < members_; ++m)
     for (int lb = 0; lb < mreal_[m]; lb += T)
-      t.push_back(make_int4(moff_[m] + lb, m, lb, std::min(T, mreal_[m] - lb)));
+      t.push_back(TileDesc{moff_[m] + lb, lb, std::min(T, mreal_[m] - lb), m, mcount_[m], 0, 0, 0});
   int rc = kOk;
   if ((int)t.size() > tiles_cap_) {
     Free(d_tiles_);
     d_tiles_ = nullptr;
     tiles_cap_ = 0;
-    rc = Check(hipMalloc((void**)&d_tiles_, t.size() * sizeof(int4)), "alloc tiles");
+    rc = Check(hipMalloc((void**)&d_tiles_, t.size() * sizeof(TileDesc)), "alloc tiles");
     if (rc) return rc;
     tiles_cap_ = (int)t.size();
   }
-  rc = Check(hipMemcpy(d_tiles_, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice),
+  rc = Check(hipMemcpy(d_tiles_, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice),
              "upload tiles");
   if (rc) return rc;
   args_.tiles = d_tiles_;
@@ -331,7 +329,7 @@ Engine::~Engine() {
   void* ptrs[] = {d_next_, d_prev_, d_occ_, d_mu_, d_site_ptr_, d_site_modes_, d_next_s_,
                   d_prev_col_, d_prev_row_, d_delta_, d_H_, d_Hraw_, d_Hadj_, d_edge_, d_rho_, d_aux_,
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
-                  d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_mcount_, d_moff_, d_Hm_,
+                  d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
                   d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_};
   for (void* p : ptrs) Free(p);
   if (stream_) (void)hipStreamDestroy(stream_);

@@ -212,10 +212,9 @@ class Engine {
   float* d_cy_ = nullptr;
   int ClosurePredict(const void* in, bool in_fp32);
   int UploadEdges(int closure_first, int closure_nb, int closure_vbase);
-  int* d_mcount_ = nullptr;
   int* d_moff_ = nullptr;
   double2* d_Hm_ = nullptr;
-  int4* d_tiles_ = nullptr;
+  TileDesc* d_tiles_ = nullptr;
   int tiles_cap_ = 0;
   int tiles_for_ = 0;  // tile size the descriptors were built for
   int BuildTiles();

@@ -160,8 +160,6 @@ int Engine::AttachClosure(Mlp* mlp) {
                            NE * sizeof(double2), hipMemcpyHostToDevice), "initial state");
   if (!rc) rc = Check(hipMemcpy(d_moff_, moff_.data(), members_ * sizeof(int),
                                 hipMemcpyHostToDevice), "upload offsets");
-  if (!rc) rc = Check(hipMemcpy(d_mcount_, mcount_.data(), members_ * sizeof(int),
-                                hipMemcpyHostToDevice), "upload spans");
   if (!rc) rc = UploadEdges(first, nb, vbase);
   const std::vector<float> bmi = BoundaryIndex(h, first, nb, B);
   const long long nrows = (long long)members_ * nb;

@@ -9,6 +9,13 @@
 
 namespace phi {
 
+// Tile of an ensemble slab: slab nodes base .. base + nt - 1 are the local
+// nodes local .. local + nt - 1 of `member`, which has `count` nodes.
+// 32 bytes, so a block reads one with a single scalar load.
+struct __attribute__((aligned(32))) TileDesc {
+  int base, local, nt, member, count, pad0, pad1, pad2;
+};
+
 // Everything the RHS of one hierarchy needs, as device pointers.
 struct RhsArgs {
   int N, NE, D, L, num_nodes, tile, vacuum;
@@ -30,13 +37,11 @@ struct RhsArgs {
   const int4* edge;
   int mps;
   // Ensembles (persistent kernels only): slab of several members of one
-  // bath model sharing the tables of the deepest member.  tiles[t] =
-  // {slab node base, member, member-local node, nodes}, mcount[m] = nodes of
-  // member m, and H holds every member's shifted H (stride N*N).  nullptr
-  // for a single hierarchy.
-  const int4* tiles;
+  // bath model sharing the tables of the deepest member.  tiles[t] describes
+  // tile t (TileDesc) and H holds every member's shifted H (stride N*N).
+  // tiles is nullptr for a single hierarchy.
+  const TileDesc* tiles;
   int ntiles;
-  const int* mcount;
   int real_h;                 // imag(H) == 0 everywhere
   unsigned long long mN, mNE; // ceil(2^32 / N), ceil(2^32 / N^2)
   double inv_hbar;

@@ -16,17 +16,26 @@ int ResidentBlocks(const void* kernel, int threads, size_t lds);
 
 // One launch of the persistent stage kernel: `elems` elements per thread
 // (1..4) and `mps` modes per site (1..2) select the instantiation.
-template <typename TB, typename TI, int EPT, int MPS, class EPI>
-inline void LaunchStreamK(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
-                          hipStream_t s) {
+template <typename TB, typename TI, int EPT, int MPS, bool ENS, class EPI>
+inline void LaunchStreamKE(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
+                           hipStream_t s) {
   const size_t sh = dev::StreamLdsBytes(a.N, a.D, a.L, a.tile, sizeof(TB));
-  const void* k = reinterpret_cast<const void*>(&dev::stream_stage<TB, TI, EPT, MPS, EPI>);
-  const int ntiles = a.tiles ? a.ntiles : (a.num_nodes + a.tile - 1) / a.tile;
+  const void* k = reinterpret_cast<const void*>(&dev::stream_stage<TB, TI, EPT, MPS, ENS, EPI>);
+  const int ntiles = ENS ? a.ntiles : (a.num_nodes + a.tile - 1) / a.tile;
   int g = grid > 0 ? grid : ResidentBlocks(k, threads, sh);
   if (g > ntiles) g = ntiles;
   if (g < 1) g = 1;
-  hipLaunchKernelGGL((dev::stream_stage<TB, TI, EPT, MPS, EPI>), dim3(g), dim3(threads), sh, s, a,
-                     in, epi);
+  hipLaunchKernelGGL((dev::stream_stage<TB, TI, EPT, MPS, ENS, EPI>), dim3(g), dim3(threads), sh,
+                     s, a, in, epi);
+}
+
+// Ensembles (a descriptor list) and single hierarchies are separate
+// instantiations, so the single-hierarchy loop carries no descriptor code.
+template <typename TB, typename TI, int EPT, int MPS, class EPI>
+inline void LaunchStreamK(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
+                          hipStream_t s) {
+  if (a.tiles) LaunchStreamKE<TB, TI, EPT, MPS, true, EPI>(a, in, epi, threads, grid, s);
+  else LaunchStreamKE<TB, TI, EPT, MPS, false, EPI>(a, in, epi, threads, grid, s);
 }
 
 template <typename TB, typename TI, int MPS, class EPI>

@@ -143,13 +143,15 @@ __device__ __forceinline__ void stream_gemm(const RhsArgs& A, const StreamLds<TB
   }
 }
 
-// Registers of the next tile in flight (input element, 0.5 mu, edge records
-// and neighbour gathers of each owned element).
+// Registers of the next tile in flight (input element, mu, edge records and
+// neighbour gathers of each owned element).  mu stays as loaded until the
+// tile is combined: converting it at load time would make the wavefront wait
+// for the load before the GEMM, which is what the prefetch is meant to hide.
 template <typename TB, typename TI, int EPT, int MPS>
 struct StreamRegs {
   typedef typename C2<TI>::type VI;
   VI y[EPT];
-  TB mu2[EPT];
+  double mu[EPT];
   int4 rec[EPT][MPS];
   VI gn[EPT][MPS];
   VI gp[EPT][MPS];
@@ -172,15 +174,22 @@ struct TileInfo {
   int base, local, nt, member, count;
 };
 
+// ENS: the tile comes from the descriptor list (an ensemble).  The tile index
+// is uniform over the block, so the descriptor is read through the scalar
+// cache (a constant-address-space load): it does not enter the in-order
+// vector-memory counter, so waiting for it never waits for the vector loads
+// in flight.
+template <bool ENS>
 __device__ __forceinline__ TileInfo stream_tile(const RhsArgs& A, int tile) {
   TileInfo t;
-  if (A.tiles) {
-    const int4 d = A.tiles[tile];
-    t.base = d.x;
-    t.member = d.y;
-    t.local = d.z;
-    t.nt = d.w;
-    t.count = A.mcount[d.y];
+  if constexpr (ENS) {
+    typedef const __attribute__((address_space(4))) int* ConstInt;
+    const ConstInt d = (ConstInt)(A.tiles + tile);
+    t.base = d[0];
+    t.local = d[1];
+    t.nt = d[2];
+    t.member = d[3];
+    t.count = d[4];
   } else {
     t.base = tile * A.tile;
     t.local = t.base;
@@ -207,7 +216,7 @@ __device__ __forceinline__ void stream_load1(const RhsArgs& A, const TI* __restr
       stream_coords(A, i, &te, &r, &b, &a);
       const int ln = t.local + te;
       R.y[i] = reinterpret_cast<const VI*>(in)[(size_t)(t.base + te) * A.NE + r];
-      R.mu2[i] = (TB)(0.5 * A.mu[ln]);
+      R.mu[i] = A.mu[ln];
       const int4* ed = A.edge + ((size_t)ln * A.N + b) * MPS;
 #pragma unroll
       for (int q = 0; q < MPS; ++q) R.rec[i][q] = ed[q];
@@ -215,9 +224,10 @@ __device__ __forceinline__ void stream_load1(const RhsArgs& A, const TI* __restr
   }
 }
 
-// Neighbour gathers of a tile (after its edge records arrived).  Local ids
-// become slab ids; neighbours beyond the member's depth do not exist.
-template <typename TB, typename TI, int EPT, int MPS>
+// Neighbour gathers of a tile (after its edge records arrived).  In an
+// ensemble, local ids become slab ids and neighbours beyond the member's
+// depth do not exist; a single hierarchy's ids are slab ids already.
+template <typename TB, typename TI, int EPT, int MPS, bool ENS>
 __device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restrict__ in,
                                              const TileInfo& t,
                                              StreamRegs<TB, TI, EPT, MPS>& R) {
@@ -233,9 +243,11 @@ __device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restr
 #pragma unroll
       for (int q = 0; q < MPS; ++q) {
         int4 c = R.rec[i][q];
-        c.x = (c.x >= 0 && c.x < t.count) ? c.x + off : -1;
-        c.y = (c.y >= 0 && c.y < t.count) ? c.y + off : -1;
-        R.rec[i][q] = c;
+        if constexpr (ENS) {
+          c.x = (c.x >= 0 && c.x < t.count) ? c.x + off : -1;
+          c.y = (c.y >= 0 && c.y < t.count) ? c.y + off : -1;
+          R.rec[i][q] = c;
+        }
         if (c.x >= 0) R.gn[i][q] = v[(size_t)c.x * A.NE + r];
         if (c.y >= 0) R.gp[i][q] = v[(size_t)c.y * A.NE + r];
       }
@@ -263,7 +275,7 @@ __device__ __forceinline__ void stream_load_h(const RhsArgs& A, const StreamLds<
 
 // One element per thread allows blocks of up to 1024 threads (a whole
 // 24 x 24 node per block); more elements per thread keep blocks at <= 256.
-template <typename TB, typename TI, int EPT, int MPS, class EPI>
+template <typename TB, typename TI, int EPT, int MPS, bool ENS, class EPI>
 __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     stream_stage(RhsArgs A, const TI* __restrict__ in, EPI epi) {
   typedef typename C2<TB>::type V;
@@ -276,9 +288,9 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
   // Tiles of this block: a contiguous range of the descriptor list for an
   // ensemble (members change rarely, so does H), else every gridDim.x-th
   // tile of the hierarchy (neighbouring tiles in flight together).
-  const int ntiles = A.tiles ? A.ntiles : (A.num_nodes + T - 1) / T;
+  const int ntiles = ENS ? A.ntiles : (A.num_nodes + T - 1) / T;
   int tile, tile_end, step;
-  if (A.tiles) {
+  if constexpr (ENS) {
     tile = (int)(((long long)blockIdx.x * ntiles) / gridDim.x);
     tile_end = (int)(((long long)(blockIdx.x + 1) * ntiles) / gridDim.x);
     step = 1;
@@ -298,11 +310,11 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
 
   double emax = 0;
   Regs cur;
-  TileInfo ti = stream_tile(A, tile);
+  TileInfo ti = stream_tile<ENS>(A, tile);
   int hmember = ti.member;
   stream_load_h<TB>(A, s, hmember);
   stream_load1<TB, TI, EPT, MPS>(A, in, ti, cur);
-  stream_load2<TB, TI, EPT, MPS>(A, in, ti, cur);
+  stream_load2<TB, TI, EPT, MPS, ENS>(A, in, ti, cur);
   __syncthreads();  // per-block tables visible
 
   for (int buf = 0; tile < tile_end; tile += step, buf ^= 1) {
@@ -312,7 +324,7 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     V* Wb = s.W + buf * s.bstride;
     // H of a new member (the previous tile's product finished before the
     // last barrier; the next barrier publishes the new fragments).
-    if (ti.member != hmember) {
+    if (ENS && ti.member != hmember) {
       hmember = ti.member;
       stream_load_h<TB>(A, s, hmember);
     }
@@ -329,7 +341,7 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
                                                                   (double)cur.y[i].y));
         else pre[i] = epi.pre(g);
         const V y = mk<TB>((TB)cur.y[i].x, (TB)cur.y[i].y);
-        TB f = cur.mu2[i];
+        TB f = (TB)(0.5 * cur.mu[i]);
         if (a != b) f += (TB)0.5 * (s.delta[a] + s.delta[b]);
         TB ax = -f * y.x, ay = -f * y.y;
 #pragma unroll
@@ -358,7 +370,7 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     const bool more = ntile < tile_end;
     TileInfo nti = ti;
     if (more) {
-      nti = stream_tile(A, ntile);
+      nti = stream_tile<ENS>(A, ntile);
       stream_load1<TB, TI, EPT, MPS>(A, in, nti, cur);
     }
     __syncthreads();
@@ -366,7 +378,7 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     stream_gemm<TB>(A, s, Yb, Wb, nt);
     __syncthreads();
     // (4) neighbour gathers of the next tile
-    if (more) stream_load2<TB, TI, EPT, MPS>(A, in, nti, cur);
+    if (more) stream_load2<TB, TI, EPT, MPS, ENS>(A, in, nti, cur);
     // (5) epilogue of this tile
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
