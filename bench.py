@@ -17,8 +17,12 @@ roofline the dominant kernel (the stage kernel with the largest mean
          Algorithmic bytes per node (nested RK4 form, s = bytes per bulk
          complex element): stage 0 N^2 (16 + s), stages 1-2 N^2 (16 + 2 s),
          stage 3 N^2 (32 + s); neighbour columns count as cache hits.
+         classic_accounting repeats the figures with the bytes of classic
+         RK4 (N^2 (160 + 6 s) per node-step), which the nested form beats.
          traffic = HBM bytes per launch of that kernel from the committed
          rocprofv3 counter summary (profiles/), when present.
+--shape-fp64/--shape-fp32/--shape-c3 pin a leg's launch shape instead of
+autotuning it (profiles of the shape an earlier run chose).
 cpu_baseline  the oracle's pthread restatement of PHI's RK4 loop (FP64) on
          the host cores, timed on a bounded sample of the same workload.
 c3_fp32  the 24-site ring at depth 4 (20475 nodes of 24x24), FP32-bulk RK4:
@@ -57,6 +61,17 @@ def stage_bytes(N, s):
     return [ne * (16 + s), ne * (16 + 2 * s), ne * (16 + 2 * s), ne * (32 + s)]
 
 
+def classic_stage_bytes(N, s):
+    """Bytes per node of the four stages of classic RK4 with an FP64 state,
+    an FP64 accumulator and bulk stage vectors streamed once per stage
+    (stage 1 reads rho, writes acc and Y; stages 2-3 read Y, rho, acc and
+    write acc, Y; stage 4 reads Y, acc and writes rho): N^2 (160 + 6 s) per
+    node-step in total.  The nested form moves fewer bytes; this accounting
+    is reported beside it."""
+    ne = N * N
+    return [ne * (32 + s), ne * (48 + 2 * s), ne * (48 + 2 * s), ne * (32 + s)]
+
+
 def cpu_baseline(seconds=12.0):
     from oracle import heom_oracle as HO
     from oracle import prm_oracle as PO
@@ -90,12 +105,16 @@ def profile_traffic(kernel_stage, prec):
     return None
 
 
-def run_rk4(P, p, prec, steps, warmup, tune=True):
+def run_rk4(P, p, prec, steps, warmup, shape=None):
+    """RK4 timing of one configuration; `shape` = (form, tile, threads,
+    blocks) pins the launch shape, None autotunes it."""
     dt = p.get_double("timestep")
     ctx = P.Context(p, precision=prec)
-    shape = None
-    if tune:
+    if shape is None:
         _, shape = ctx.autotune(dt, steps=5)
+    else:
+        ctx.set_launch(*shape)
+        shape = tuple(shape)
     ctx.rk4_steps(warmup, dt)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -192,6 +211,11 @@ def main():
     ap.add_argument("--calibration-copy", action="store_true",
                     help="first copy 1 GiB device to device (a plain copy of known size, "
                          "the reference point of the HBM byte counters under rocprofv3)")
+    shape = lambda s: tuple(int(v) for v in s.split(","))
+    for leg in ("fp64", "fp32", "c3"):
+        ap.add_argument("--shape-" + leg, type=shape, default=None,
+                        help="launch shape form,tile,threads,blocks of the %s leg instead of "
+                             "autotuning it (e.g. to profile the shape a previous run chose)" % leg)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this integrator runs on a single MI355X "
@@ -216,7 +240,8 @@ def main():
     res = {}
     for prec in ("fp64", "fp32"):
         log("bench: 7-site depth 8, %s" % prec)
-        res[prec] = run_rk4(P, p, prec, args.steps, args.warmup)
+        res[prec] = run_rk4(P, p, prec, args.steps, args.warmup,
+                            getattr(args, "shape_" + prec))
         log("bench: %s %.4f ms/step, shape %s, stage ms %s"
             % (prec, res[prec]["ms_per_step"], res[prec]["shape"], res[prec]["stage_ms"]))
     pop64 = np.real(np.diag(res["fp64"]["rho0"]))
@@ -257,12 +282,18 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stream_stage (RK4 nested stage %d, FP64)" % dom,
-                     "bytes_per_launch": per_launch, "stage_ms_fp64": st_ms},
+                     "bytes_per_launch": per_launch, "stage_ms_fp64": st_ms,
+                     "classic_accounting": {
+                         "bytes_per_launch": classic_stage_bytes(N, 16)[dom] * nodes,
+                         "achieved": classic_stage_bytes(N, 16)[dom] * nodes
+                                     / (st_ms[dom] * 1e-3) / 1e9,
+                         "frac_step": nodes * N * N * (160 + 6 * 16)
+                                      / (res["fp64"]["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9)}},
     }
     if not args.no_c3:
         log("bench: 24-site depth 4, fp32")
         p3 = P.PhiParameters.read(CFG3, "rk4")
-        r3 = run_rk4(P, p3, "fp32", 50, 5)
+        r3 = run_rk4(P, p3, "fp32", 50, 5, args.shape_c3)
         N3 = p3.get_int("num_states")
         b3 = stage_bytes(N3, 8)
         step_bytes = sum(b3) * r3["nodes"]
@@ -270,6 +301,9 @@ def main():
             "workload": "24-site ring, depth 4 (20475 nodes of 24x24), FP32-bulk RK4 dt=1 fs",
             "node_steps_per_s": r3["node_steps_per_s"], "ms_per_step": r3["ms_per_step"],
             "hbm_frac_step": step_bytes / (r3["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "hbm_frac_step_classic_accounting":
+                r3["nodes"] * N3 * N3 * (160 + 6 * 8) / (r3["ms_per_step"] * 1e-3)
+                / (HBM_PEAK_GBS * 1e9),
             "stage_ms": r3["stage_ms"], "kernel_shape": r3["shape"]}
     if not args.no_ensemble:
         log("bench: static-disorder ensemble")

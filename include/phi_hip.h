@@ -200,8 +200,10 @@ int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n_complex);
 int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
 /* nsteps RK4 steps of size dt on the device state. */
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
-/* nsteps RK4 steps with a HIP event around every stage kernel (on the
- * context's stream); ms[s] = total milliseconds of stage s. */
+/* Stage-kernel timing: stage s (RK4 order, RK4's buffers) launched nsteps
+ * times back to back between two HIP events on the context's stream;
+ * ms[s] = total milliseconds of those nsteps launches.  The state is
+ * restored afterwards. */
 int phi_hip_rk4_stage_times(phi_ctx* c, long long nsteps, double dt, double* ms);
 /* One RKF45 attempt.  coefs: 15 b's (b10, b20, b21, ..., b54), 4 c's of y4
  * (c50, c52, c53, c54) and 5 of y5 (c60, c62, c63, c64, c65), all already

@@ -375,7 +375,8 @@ class Context:
         _check(lib().phi_hip_rk4_steps(self._h, int(nsteps), float(dt)))
 
     def rk4_stage_times(self, nsteps, dt):
-        """Milliseconds per stage kernel (4 values), summed over nsteps."""
+        """Milliseconds of each stage kernel (4 values) summed over nsteps
+        back-to-back launches of that stage; the state is left unchanged."""
         ms = np.zeros(4)
         _check(lib().phi_hip_rk4_stage_times(self._h, int(nsteps), float(dt), _dp(ms)))
         return ms

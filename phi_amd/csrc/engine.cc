@@ -693,36 +693,41 @@ int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   int rc = AllocBulk();
   if (rc) return rc;
   for (int s = 0; s < 4; ++s) ms[s] = 0;
-  // Every launch and event of a batch is queued before the first wait, so
-  // the device runs the kernels back to back and an event pair brackets
-  // one kernel.
-  const long long kBatch = 64;
-  std::vector<hipEvent_t> ev(4 * kBatch + 1, nullptr);
-  for (auto& e : ev)
-    if (!rc) rc = Check(hipEventCreate(&e), "event");
+  if (nsteps < 1) return kOk;
+  // Stage s is launched nsteps times back to back between one event pair
+  // (an event between every kernel would add its own gap to each), after
+  // one untimed launch.  The stages run in RK4 order on RK4's buffers; the
+  // repeated last stage moves rho, so the state is saved and restored.
+  const size_t bytes = state_elems() * 2 * sizeof(double);
+  double* saved = nullptr;
+  rc = Check(hipMalloc((void**)&saved, bytes), "alloc timing state");
+  if (rc) return rc;
+  rc = Check(hipMemcpyAsync(saved, d_rho_, bytes, hipMemcpyDeviceToDevice, stream_), "save state");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) rc = Check(hipEventCreate(&e0), "event");
+  if (!rc) rc = Check(hipEventCreate(&e1), "event");
+  const bool hn = rk4_nested_ && !closure_;
   const void* in[4] = {d_rho_, d_y_[0], d_y_[1], d_y_[0]};
   void* out[4] = {d_y_[0], d_y_[1], d_y_[0], nullptr};
-  for (long long done = 0; done < nsteps && !rc;) {
-    const long long nb = std::min(kBatch, nsteps - done);
-    rc = Check(hipEventRecord(ev[0], stream_), "record");
-    for (long long i = 0; i < nb && !rc; ++i)
-      for (int s = 0; s < 4 && !rc; ++s) {
-        rc = Check(LaunchRk4Stage(args_, fp32_, s, rk4_nested_, in[s], d_rho_, d_aux_, out[s], dt,
-                                  stream_), "rk4 stage");
-        if (!rc) rc = Check(hipEventRecord(ev[4 * i + s + 1], stream_), "record");
-      }
-    if (!rc) rc = Check(hipEventSynchronize(ev[4 * nb]), "event sync");
-    for (long long i = 0; i < nb && !rc; ++i)
-      for (int s = 0; s < 4 && !rc; ++s) {
-        float t = 0;
-        rc = Check(hipEventElapsedTime(&t, ev[4 * i + s], ev[4 * i + s + 1]), "elapsed");
-        ms[s] += t;
-      }
-    done += nb;
+  for (int s = 0; s < 4 && !rc; ++s) {
+    for (long long i = 0; i <= nsteps && !rc; ++i) {
+      if (i == 1) rc = Check(hipEventRecord(e0, stream_), "record");
+      if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, s, hn, in[s], d_rho_, d_aux_, out[s], dt,
+                                         stream_), "rk4 stage");
+    }
+    if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");
+    if (!rc) rc = Check(hipEventSynchronize(e1), "event sync");
+    float t = 0;
+    if (!rc) rc = Check(hipEventElapsedTime(&t, e0, e1), "elapsed");
+    ms[s] = t;
   }
-  for (auto& e : ev)
-    if (e) (void)hipEventDestroy(e);
-  return rc;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  const int rc_restore = Check(hipMemcpyAsync(d_rho_, saved, bytes, hipMemcpyDeviceToDevice,
+                                              stream_), "restore state");
+  const int rc_sync = Synchronize();
+  Free(saved);
+  return rc ? rc : (rc_restore ? rc_restore : rc_sync);
 }
 
 int Engine::HistoryCount() {

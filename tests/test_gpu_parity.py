@@ -313,6 +313,27 @@ def test_autotune_keeps_state_and_parity():
     ctx2.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_stage_times_keep_state(prec):
+    """Stage-kernel timing leaves the state bit-identical, so a run that is
+    timed and one that is not give the same result."""
+    op, p = _subset_params("c2", 5)
+    dt = op.timestep
+    ctx = P.Context(p, precision=prec)
+    ref = P.Context(p, precision=prec)
+    ctx.rk4_steps(5, dt)
+    ref.rk4_steps(5, dt)
+    before = ctx.state()
+    ms = ctx.rk4_stage_times(7, dt)
+    assert ms.shape == (4,) and np.all(ms > 0)
+    assert np.array_equal(ctx.state(), before)
+    ctx.rk4_steps(5, dt)
+    ref.rk4_steps(5, dt)
+    assert np.array_equal(ctx.state(), ref.state())
+    ctx.close()
+    ref.close()
+
+
 def _ensemble_members(count, max_lh):
     from phi_amd.configs.make_configs import c4_member
     Hs, depths = [], []
