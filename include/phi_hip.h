@@ -196,7 +196,10 @@ int phi_hip_get_index(const phi_ctx* c, int* out, size_t cap);
 int phi_hip_set_state(phi_ctx* c, const double* state, size_t n_complex);
 int phi_hip_get_state(phi_ctx* c, double* state, size_t n_complex);
 int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n_complex);
-/* out = L(in) for every node (PHI's Get, any complex input); host arrays. */
+/* out = L(in) for every node (PHI's Get, any complex input); host arrays.
+ * With FilterTolerance > 0 the current node flags apply: inactive nodes and
+ * neighbours contribute nothing, and an inactive node's out holds only its
+ * neighbour terms (PHI's Get adds them onto drho, here onto zero). */
 int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
 /* nsteps RK4 steps of size dt on the device state. */
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
@@ -210,6 +213,12 @@ int phi_hip_rk4_stage_times(phi_ctx* c, long long nsteps, double dt, double* ms)
  * multiplied by dt.  *err = max |y4 - y5|.  Accept keeps y5. */
 int phi_hip_rkf45_attempt(phi_ctx* c, const double* coefs, double* err);
 int phi_hip_rkf45_accept(phi_ctx* c);
+/* Adaptive hierarchy truncation (FilterTolerance > 0): the activity flag of
+ * every node (HierarchyNode::is_active, hierarchy_node.h; updated after each
+ * accepted step as hierarchy_integrator.cc:2358-2385, 2828-2851) into
+ * flags[0 .. num_nodes-1] (1 = active), *count = active nodes.  Without the
+ * filter every node is active.  n = capacity of flags. */
+int phi_hip_active_flags(phi_ctx* c, int* flags, int n, int* count);
 /* Steady state from the current state: BiCGSTAB on L x = 0 with the
  * (0,0) row of node 0 replaced by trace(rho_0) = 1. */
 int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, double* resid);

@@ -85,6 +85,7 @@ def lib():
         "phi_hip_rk4_stage_times": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
         "phi_hip_rkf45_attempt": (ctypes.c_int, [vp, dp, dp]),
         "phi_hip_rkf45_accept": (ctypes.c_int, [vp]),
+        "phi_hip_active_flags": (ctypes.c_int, [vp, ip, ctypes.c_int, ip]),
         "phi_hip_steady_state": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ip, dp]),
         "phi_hip_synchronize": (ctypes.c_int, [vp]),
         "phi_hip_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -390,6 +391,15 @@ class Context:
 
     def rkf45_accept(self):
         _check(lib().phi_hip_rkf45_accept(self._h))
+
+    def active_flags(self):
+        """Adaptive truncation flags (1 = active) of every node, PHI id order."""
+        out = np.zeros(self.num_nodes, dtype=np.int32)
+        cnt = ctypes.c_int()
+        _check(lib().phi_hip_active_flags(self._h,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          self.num_nodes, ctypes.byref(cnt)))
+        return out
 
     def steady_state(self, tol, max_iter):
         it = ctypes.c_int()

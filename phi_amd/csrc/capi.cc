@@ -620,7 +620,16 @@ int phi_hip_rkf45_attempt(phi_ctx* c, const double* coefs, double* err) {
 
 int phi_hip_rkf45_accept(phi_ctx* c) {
   if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  c->e->Rkf45Accept();
+  int rc = c->e->Rkf45Accept();
+  return rc ? Err(rc, c->e->last_error()) : Ok();
+}
+
+int phi_hip_active_flags(phi_ctx* c, int* flags, int n, int* count) {
+  if (!c || !flags || !count) return Err(PHI_HIP_EINVAL, "null argument");
+  if (n < c->e->num_nodes()) return Err(PHI_HIP_EINVAL, "flags buffer too small");
+  const int rc = c->e->ActiveFlags(flags);
+  if (rc < 0) return Err(rc, c->e->last_error());
+  *count = rc;
   return Ok();
 }
 

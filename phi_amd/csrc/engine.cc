@@ -8,6 +8,8 @@
 #include <mutex>
 #include <tuple>
 
+#include "filter_kernels.h"
+
 namespace phi {
 
 namespace {
@@ -108,6 +110,13 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_partial_, e->red_blocks_ * 2 * sizeof(double)), "alloc partial");
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_dot_, 2 * sizeof(double)), "alloc dot");
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_err_, sizeof(unsigned long long)), "alloc err");
+  // Adaptive truncation: every node starts active (hierarchy_node.cc:16).
+  // The vacuum updater ignores the flags.
+  if (!rc && p.filter_tolerance > 0 && !h.vacuum) {
+    e->filter_ = p.filter_tolerance;
+    rc = e->Check(hipMalloc((void**)&e->d_active_, h.num_nodes), "alloc flags");
+    if (!rc) rc = e->Check(hipMemset(e->d_active_, 1, h.num_nodes), "set flags");
+  }
   if (rc) {
     *why = e->err_;
     delete e;
@@ -213,6 +222,11 @@ int Engine::CreateEnsemble(const PhiParameters& base, int members,
   if (N > 32 || e->mps_ > 2 || !e->d_edge_) {
     *why = "ensembles run on the persistent kernels (NumStates <= 32, at most two bath "
            "modes per site)";
+    delete e;
+    return kErrUnsupported;
+  }
+  if (e->filter_on()) {
+    *why = "FilterTolerance (adaptive truncation) is not available for an ensemble";
     delete e;
     return kErrUnsupported;
   }
@@ -330,7 +344,8 @@ Engine::~Engine() {
                   d_prev_col_, d_prev_row_, d_delta_, d_H_, d_Hraw_, d_Hadj_, d_edge_, d_rho_, d_aux_,
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
-                  d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_};
+                  d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_active_, d_kf_[0], d_kf_[1],
+                  d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_kraw_, d_ym_};
   for (void* p : ptrs) Free(p);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -588,7 +603,15 @@ int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
   int rc = Check(hipMalloc((void**)&dx, bytes), "alloc rhs in");
   if (!rc) rc = Check(hipMalloc((void**)&dy, bytes), "alloc rhs out");
   if (!rc) rc = Check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, stream_), "rhs upload");
-  if (!rc) rc = RhsDevice(dx, dy, 0);
+  if (filter_on()) {
+    // Current flags: the plain operator on the masked input.
+    if (!rc) rc = FilterAlloc(0);
+    if (!rc) rc = Check(LaunchMaskNodes(dx, d_active_, d_ym_, N() * N(), num_nodes(), stream_),
+                        "mask nodes");
+    if (!rc) rc = RhsDevice(d_ym_, dy, 0);
+  } else if (!rc) {
+    rc = RhsDevice(dx, dy, 0);
+  }
   if (!rc) rc = Check(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, stream_), "rhs download");
   if (!rc) rc = Synchronize();
   Free(dx);
@@ -659,6 +682,10 @@ int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int sna
   int rc = AllocBulk();
   if (rc) return rc;
   auto snap = [&](long long s) { return snap_every > 0 && (step_offset + s) % snap_every == 0; };
+  if (filter_on()) {
+    for (long long s = 0; s < nsteps && !rc; ++s) rc = FilterRk4Step(dt, snap(s));
+    return rc;
+  }
   // Graph blocks of G steps, G a multiple of the snapshot period, aligned so
   // that every block starts on a snapshot step.
   int G = 0;
@@ -764,6 +791,7 @@ int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
     if (rc) return rc;
     rkf45_alloc_ = true;
   }
+  if (filter_on()) return FilterRkf45Attempt(c, err);
   Rkf45Bufs<float> bf{};
   Rkf45Bufs<double> bd{};
   for (int i = 0; i < 5; ++i) {
@@ -789,7 +817,12 @@ int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
   return kOk;
 }
 
-void Engine::Rkf45Accept() { std::swap(d_rho_, d_aux_); }
+int Engine::Rkf45Accept() {
+  std::swap(d_rho_, d_aux_);
+  if (!filter_on()) return kOk;
+  return Check(LaunchUpdateActive(d_rho_, d_active_, N() * N(), num_nodes(), filter_, stream_),
+               "flag update");
+}
 
 int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
                         std::vector<double>* history) {

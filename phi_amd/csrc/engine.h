@@ -108,8 +108,8 @@ class Engine {
   // appended to the device history after every step i (counted from
   // step_offset) with i % snap_every == 0.
   int Rk4Steps(long long nsteps, double dt, long long step_offset, int snap_every);
-  // As Rk4Steps without snapshots, timing every stage kernel with HIP events
-  // on the engine stream; ms[s] += duration of stage s summed over steps.
+  // Stage-kernel timing: stage s launched nsteps times back to back between
+  // two HIP events; ms[s] = their total.  The state is restored.
   int Rk4StageTimes(long long nsteps, double dt, double ms[4]);
   int HistoryCount();
   int history_capacity() const { return hist_capacity_; }
@@ -118,7 +118,17 @@ class Engine {
   // One RKF45 attempt with the given (dt-scaled) weights; *err = max|y4-y5|.
   // The 5th-order solution is held until Rkf45Accept().
   int Rkf45Attempt(const Rkf45Coefs& c, double* err);
-  void Rkf45Accept();
+  int Rkf45Accept();
+
+  // Adaptive hierarchy truncation (PHI's FilterTolerance > 0 with a
+  // hierarchy): RK4 and RKF45 run the filtered operator (filter_kernels.h)
+  // in FP64 with PHI's classic stage order and per-stage k buffers, and the
+  // node flags are updated after every accepted step.  The per-call RHS
+  // applies the current flags (an inactive node's result holds its
+  // neighbour terms only); the steady state uses the plain operator.
+  bool filter_on() const { return filter_ > 0; }
+  // Current flags (num_nodes entries, 1 = active); returns the active count.
+  int ActiveFlags(int* out);
 
   // Steady state: BiCGSTAB on A x = b (A = L with the trace row), from the
   // current state.  Returns kOk or kErrNotConverged; *iters, *resid report.
@@ -224,6 +234,17 @@ class Engine {
     return members_ && tiles_for_ != args_.col_tile[fp32_ ? 1 : 0] ? BuildTiles() : 0;
   }
   int Unsupported(const char* what);
+
+  // Adaptive truncation.
+  double filter_ = 0;
+  unsigned char* d_active_ = nullptr;
+  double* d_kf_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* d_kraw_ = nullptr;
+  double* d_ym_ = nullptr;
+  int FilterAlloc(int nk);
+  int FilterRhs(const double* in, int s);
+  int FilterRk4Step(double dt, bool snapshot);
+  int FilterRkf45Attempt(const Rkf45Coefs& c, double* err);
 
   // State buffers (interleaved complex).
   double* d_rho_ = nullptr;

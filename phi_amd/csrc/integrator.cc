@@ -214,10 +214,6 @@ int HierarchyIntegrator::Initialize() {
     Fail(kErrUnsupported, "HilbertSpaceSize > NumStates is not supported");
     return info_.status;
   }
-  if (p.filter_tolerance > 0) {
-    Fail(kErrUnsupported, "FilterTolerance (adaptive truncation) is not supported");
-    return info_.status;
-  }
   std::string why;
   int rc = Engine::Create(p, &engine_, &why);
   if (rc) {
@@ -230,6 +226,14 @@ int HierarchyIntegrator::Initialize() {
   WriteHeader();
   if (!p.full_precision_output.empty()) full_.open(p.full_precision_output.c_str());
   return kOk;
+}
+
+void HierarchyIntegrator::ActiveLine(std::ostream* os) {
+  if (!engine_ || !engine_->filter_on() || p_->verbose <= 0) return;
+  std::vector<int> flags(engine_->num_nodes());
+  const int active = engine_->ActiveFlags(flags.data());
+  if (active >= 0)
+    *os << "#Total number of active matrices = " << active << "/" << engine_->num_nodes() << "\n";
 }
 
 void HierarchyIntegrator::WriteHeader() {
@@ -383,6 +387,7 @@ int HierarchyIntegrator::RunRk4() {
   }
   if (p.verbose > -1) {
     std::ostringstream ss;
+    ActiveLine(&ss);
     ss << "#Total number of integration steps = " << steps << "\n"
        << "#Total integration time = " << info_.seconds / 60 << " mins\n";
     std::cout << ss.str();
@@ -438,7 +443,11 @@ int HierarchyIntegrator::RunRkf45() {
       return rc;
     }
     if (max_diff < tol || dt_f * dtdn < mindt) {
-      engine_->Rkf45Accept();
+      rc = engine_->Rkf45Accept();
+      if (rc) {
+        Fail(rc, engine_->last_error());
+        return rc;
+      }
       time_now += mydt;
       accepted += 1;
       bool update_dt = max_diff < tol / 10;
@@ -498,6 +507,7 @@ int HierarchyIntegrator::RunRkf45() {
   }
   if (p.verbose > -1) {
     std::ostringstream ss;
+    ActiveLine(&ss);
     ss << "#Total number of integration steps = " << i + 1 << "\n"
        << "#Total number of accepted integration steps = " << accepted << "\n"
        << "#Total integration time = " << info_.seconds / 60 << " mins.\n";

@@ -57,6 +57,9 @@ class HierarchyIntegrator {
   void WriteRho0Line(const std::complex<double>* rho0, double t);
   void WriteFinal(const std::complex<double>* rho0, double t);
   void FullPrecisionLine(double t, const std::complex<double>* rho0);
+  // "#Total number of active matrices = a/n" with the filter on and
+  // verbose > 0 (hierarchy_integrator.cc:2540-2543, 2908-2911).
+  void ActiveLine(std::ostream* os);
   int WriteRestart(double t, double dt);
   int ReadRestart(double* t, double* dt);
   void Fail(int code, const std::string& msg);

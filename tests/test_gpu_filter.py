@@ -1,0 +1,107 @@
+"""GPU parity of the adaptive hierarchy truncation (FilterTolerance > 0)
+against the oracle (pinned in test_oracle_filter.py): RK4 and RKF45 states
+within 1e-10 and the node flags identical, through the C ABI."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import phi_amd as P  # noqa: E402
+from oracle import heom_oracle as HO  # noqa: E402
+from oracle import prm_oracle as PO  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+C1 = os.path.join(ROOT, "phi_amd", "configs", "c1.prm")
+C2 = os.path.join(ROOT, "phi_amd", "configs", "c2.prm")
+
+
+def _pair(path, tol, lh=None, method="rk4"):
+    op = PO.read_command_line(path, method)
+    prob = HO.Problem.from_prm(op)
+    prob.filter_tolerance = tol
+    p = P.PhiParameters.read(path, method)
+    p.set_double("filter_tolerance", tol)
+    if lh is not None:
+        prob.lh = lh
+        op.hierarchy_truncation_level = lh
+        p.set_int("hierarchy_truncation_level", lh)
+    return op, HO.Oracle(prob, threads=4), p
+
+
+@pytest.mark.parametrize("path,lh,tol,steps", [(C1, None, 1e-4, 60), (C1, None, 1e-2, 60),
+                                              (C2, 5, 1e-4, 30)])
+def test_rk4_filter_matches_oracle(path, lh, tol, steps):
+    print("case", os.path.basename(path), lh, tol, steps, flush=True)
+    op, orc, p = _pair(path, tol, lh)
+    dt = op.timestep
+    ctx = P.Context(p)
+    for chunk in (1, steps - 1):
+        orc.rk4(chunk * dt * (1 - 1e-9), dt, 1.0)
+        ctx.rk4_steps(chunk, dt)
+        flags = ctx.active_flags()
+        assert np.array_equal(flags, orc.active())
+        assert np.abs(ctx.state() - orc.state()).max() <= 1e-10
+    assert 0 < flags.sum() < ctx.num_nodes
+    # per-call RHS under the current flags
+    rng = np.random.default_rng(9)
+    N = orc.N
+    x = rng.standard_normal((orc.num_nodes, N, N)) + 1j * rng.standard_normal((orc.num_nodes, N, N))
+    want = orc.rhs(x, out=np.zeros_like(x))
+    got = ctx.rhs(x)
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    ctx.close()
+
+
+def test_rk4_filter_all_inactive_straight_line():
+    op, orc, p = _pair(C1, 10.0)
+    dt = op.timestep
+    ctx = P.Context(p)
+    rho0 = ctx.state()
+    ctx.rk4_steps(1, dt)
+    rho1 = ctx.state()
+    assert not ctx.active_flags().any()
+    ctx.rk4_steps(29, dt)
+    want = rho0 + 30 * (rho1 - rho0)
+    assert np.abs(ctx.state() - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
+    orc.rk4(30 * dt * (1 - 1e-9), dt, 1.0)
+    assert np.abs(ctx.state() - orc.state()).max() <= 1e-10
+    ctx.close()
+
+
+def test_rkf45_filter_matches_oracle():
+    """RKF45 with the filter on c1 for 0.1 ps: same attempted/accepted
+    counts, final time, flags and state."""
+    op = PO.read_command_line(C1, "rkf45")
+    tol_f = 1e-4
+    prob = HO.Problem.from_prm(op)
+    prob.filter_tolerance = tol_f
+    orc = HO.Oracle(prob, threads=1)
+    res = orc.rkf45(0.1, 0.001, 1e-8, 1e-6, 0.01)
+    p = P.PhiParameters.read(C1, "rkf45")
+    p.set_double("filter_tolerance", tol_f)
+    p.set_double("integration_time", 0.1)
+    p.set_double("rkf45_tolerance", 1e-8)
+    p.set_double("rkf45_minimum_timestep", 1e-6)
+    p.set_int("verbose", -2)
+    p.set_string("output_filename", "")
+    it = P.HierarchyIntegrator(p)
+    it.Launch()
+    info = it.info()
+    assert info["steps"] == res["attempted"]
+    assert info["accepted"] == res["accepted"]
+    assert info["final_time"] == res["time"]
+    assert np.abs(it.state() - orc.state()).max() <= 1e-10
+
+
+def test_filter_rejects_fp32_bulk():
+    op, orc, p = _pair(C1, 1e-4)
+    ctx = P.Context(p, precision="fp32")
+    with pytest.raises(RuntimeError, match="FP64 only"):
+        ctx.rk4_steps(1, op.timestep)
+    ctx.close()
