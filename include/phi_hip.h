@@ -113,7 +113,9 @@ int phi_hip_integrator_create(phi_params* p, phi_integrator** out);
 /* Launch(): runs params' run_method, writes the output file.  Returns the
  * run status (0, or PHI_HIP_E*). */
 int phi_hip_integrator_launch(phi_integrator* it);
-/* density_matrix(): rho_0 after Launch (N*N complex). */
+/* density_matrix(): rho_0 after Launch (N*N complex; for a truncated
+ * system, HilbertSpaceSize > NumStates, rho_0 mapped back to the full
+ * space: HilbertSpaceSize^2 complex, hierarchy_integrator.cc:177-191). */
 int phi_hip_integrator_density_matrix(phi_integrator* it, double* out, size_t cap);
 /* Whole ADO state after Launch (num_nodes*N*N complex). */
 int phi_hip_integrator_state(phi_integrator* it, double* out, size_t cap);

@@ -526,6 +526,17 @@ void oracle_set_state(void* h, const double* in) {
   std::memcpy((void*)o->rho.data(), in, o->rho.size() * sizeof(cplx));
 }
 
+// Replaces the system Hamiltonian (N x N, column-major interleaved complex)
+// and its adjoint, as UpdateTimeDependentHamiltonian does between stages
+// (hierarchy_integrator.cc:1617-1667).
+void oracle_set_hamiltonian(void* h, const double* H) {
+  Oracle* o = (Oracle*)h;
+  const int N = o->N;
+  std::memcpy((void*)o->H.data(), H, (size_t)N * N * sizeof(cplx));
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) o->Hadj[i * N + j] = std::conj(o->H[j * N + i]);
+}
+
 void oracle_get_active(void* h, int* out) {
   Oracle* o = (Oracle*)h;
   for (size_t i = 0; i < o->nodes.size(); ++i) out[i] = o->nodes[i].active ? 1 : 0;

@@ -70,6 +70,7 @@ def lib():
         L.oracle_get_state.argtypes = [vp, dp]
         L.oracle_set_state.argtypes = [vp, dp]
         L.oracle_get_active.argtypes = [vp, ip]
+        L.oracle_set_hamiltonian.argtypes = [vp, dp]
         L.oracle_rhs.argtypes = [vp, dp, dp]
         L.oracle_steady_matvec.argtypes = [vp, dp, dp]
         L.oracle_rk4.restype = ctypes.c_longlong
@@ -190,6 +191,11 @@ class Oracle:
     def set_state(self, rho):
         flat = np.ascontiguousarray(np.asarray(rho, dtype=complex).transpose(0, 2, 1))
         lib().oracle_set_state(self.h, _dp(flat.view(np.float64).reshape(-1)))
+
+    def set_hamiltonian(self, H):
+        """Replace the N x N system Hamiltonian (H[i, j] = element (i, j))."""
+        self._hnew = np.ascontiguousarray(np.asarray(H, dtype=complex).T).view(np.float64)
+        lib().oracle_set_hamiltonian(self.h, _dp(self._hnew.reshape(-1)))
 
     def active(self):
         out = np.zeros(self.num_nodes, dtype=np.int32)

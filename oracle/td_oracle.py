@@ -1,0 +1,175 @@
+"""Time-dependent (annealing) and Hilbert-space-truncated systems in the CPU
+oracle: PHI's CubicSpline, HilbertSpaceTruncator and the RK4 loop with
+UpdateTimeDependentHamiltonian, around the C++ oracle's RHS.
+
+TEST INFRASTRUCTURE ONLY: used by tests/ as the checker; the product
+(phi_amd/) never imports this module.
+
+Follows
+  cubic_spline.cc:1-119                 spline on x_i = i/(m-1), knot
+                                        derivatives from the (2,4,..,4,2; 1)
+                                        system with PHI's right-hand side
+  hilbert_space_truncator.cc:34-129     Truncate, Rotate, UnTruncate,
+                                        UpdateWith (lowest NumStates
+                                        eigenvectors, zheevr range 'I')
+  hierarchy_integrator.cc:1617-1667     UpdateTimeDependentHamiltonian
+                                        (MinimumFieldChange gate; last
+                                        multipliers start at the largest
+                                        float, :518-523)
+  hierarchy_integrator.cc:2684-2826     RK4 order: update at t, t + dt/2,
+                                        t + dt; rotations of the live
+                                        buffers after each update
+Eigenvector phases are not fixed by PHI (zheevr's output); here each
+eigenvector's first largest-magnitude entry is made real and positive.
+Quantities in the eigenbasis are therefore compared up to that convention;
+full-space results do not depend on it.
+"""
+
+import numpy as np
+
+
+class CubicSpline:
+    def __init__(self, y):
+        y = np.asarray(y, dtype=float)
+        m = len(y)
+        self.x = np.arange(m) * (1.0 / (m - 1))
+        self.a = y.copy()
+        b = np.zeros(m)
+        b[0] = 3.0 * (y[1] - y[0])
+        b[m - 1] = 3.0 * (y[m - 1] - y[m - 2])
+        for i in range(1, m - 1):
+            b[i] = 3.0 * (y[i + 1] - y[i])
+        T = np.diag(np.full(m, 4.0)) + np.diag(np.ones(m - 1), 1) + np.diag(np.ones(m - 1), -1)
+        T[0, 0] = T[m - 1, m - 1] = 2.0
+        self.b = np.linalg.solve(T, b)
+        self.c = np.zeros(m)
+        self.d = np.zeros(m)
+        for i in range(m - 1):
+            self.c[i] = 3.0 * (self.a[i + 1] - self.a[i]) - 2.0 * self.b[i] - self.b[i + 1]
+            self.d[i] = 2.0 * (self.a[i] - self.a[i + 1]) + self.b[i] + self.b[i + 1]
+
+    def __call__(self, x):
+        m = len(self.x)
+        if x > self.x[m - 1]:
+            return self.a[m - 1]
+        if x < self.x[0]:
+            return self.a[0]
+        i = min(int(np.searchsorted(self.x, x, side="right")) - 1, m - 2)
+        t = (x - self.x[i]) / (self.x[i + 1] - self.x[i])
+        return self.a[i] + self.b[i] * t + self.c[i] * t * t + self.d[i] * t * t * t
+
+
+def lowest_eigenvectors(H, n):
+    """n lowest eigenpairs of Hermitian H (ascending), phase-normalised."""
+    w, V = np.linalg.eigh(np.asarray(H, dtype=complex), UPLO="L")
+    V = V[:, :n].copy()
+    for j in range(n):
+        # first entry of largest magnitude, ties within a relative 1e-12
+        k, mag = 0, -1.0
+        for r, m in enumerate(np.abs(V[:, j])):
+            if m > mag * (1 + 1e-12):
+                k, mag = r, m
+        V[:, j] *= np.conj(V[k, j]) / abs(V[k, j])
+    return w[:n], V
+
+
+class Hamiltonian:
+    """Static or annealing H, optionally truncated to n states."""
+
+    def __init__(self, n, H=None, H_initial=None, H_final=None, f_initial=(1.0, 0.0),
+                 f_final=(0.0, 1.0), t_total=1.0, min_change=0.01):
+        self.n = n
+        self.td = H is None
+        self.T = t_total
+        self.min_change = min_change
+        if self.td:
+            self.Hi = np.asarray(H_initial, dtype=complex)
+            self.Hf = np.asarray(H_final, dtype=complex)
+            self.si = CubicSpline(f_initial)
+            self.sf = CubicSpline(f_final)
+            full = self.Hi.shape[0]
+        else:
+            self.Hs = np.asarray(H, dtype=complex)
+            full = self.Hs.shape[0]
+        self.full = full
+        self.old_i = self.old_f = np.finfo(np.float64).max
+        self.V = None
+        self.R = None
+        self._set(self._full_at(0.0), rotate=False)
+
+    @property
+    def truncated(self):
+        return self.full > self.n
+
+    def _full_at(self, alpha):
+        if not self.td:
+            return self.Hs
+        return self.si(alpha) * self.Hi + self.sf(alpha) * self.Hf
+
+    def _set(self, Hfull, rotate):
+        if not self.truncated:
+            self.H = Hfull.copy()
+            return
+        old = self.V
+        _, self.V = lowest_eigenvectors(Hfull, self.n)
+        self.H = self.truncate(Hfull)
+        self.R = self.V.conj().T @ old if (rotate and old is not None) else np.eye(self.n)
+
+    def update(self, t):
+        if not self.td:
+            return False
+        alpha = t / self.T
+        vi, vf = self.si(alpha), self.sf(alpha)
+        # a multiplier that was 0 gives inf (IEEE division, as in PHI): update
+        with np.errstate(divide="ignore", invalid="ignore"):
+            di, df = abs(vi / self.old_i - 1.0), abs(vf / self.old_f - 1.0)
+        if di < self.min_change and df < self.min_change:
+            return False
+        self.old_i, self.old_f = vi, vf
+        self._set(self._full_at(alpha), rotate=True)
+        return True
+
+    def truncate(self, A):
+        if not self.truncated:
+            return np.asarray(A, dtype=complex).copy()
+        return self.V.conj().T @ A @ self.V
+
+    def untruncate(self, a):
+        if not self.truncated:
+            return np.asarray(a, dtype=complex).copy()
+        return self.V @ a @ self.V.conj().T
+
+
+def rk4(orc, ham, t_start, t_end, dt):
+    """PHI's RK4 loop with H updates (classic stage order); the oracle's
+    state is advanced in place.  Returns the number of steps."""
+    rot = lambda x: ham.R @ x @ ham.R.conj().T if ham.truncated else x  # noqa: E731
+    orc.set_hamiltonian(ham.H)
+    t = t_start
+    steps = 0
+    rho = orc.state()
+    while t < t_end:
+        if ham.update(t):
+            orc.set_hamiltonian(ham.H)
+            if ham.truncated:
+                rho = rot(rho)
+        k0 = orc.rhs(rho)
+        y = rho + k0 * (dt / 2)
+        if ham.update(t + dt / 2):
+            orc.set_hamiltonian(ham.H)
+            if ham.truncated:
+                rho, y, k0 = rot(rho), rot(y), rot(k0)
+        k1 = orc.rhs(y)
+        y = rho + k1 * (dt / 2)
+        k2 = orc.rhs(y)
+        y = rho + k2 * dt
+        if ham.update(t + dt):
+            orc.set_hamiltonian(ham.H)
+            if ham.truncated:
+                rho, y, k0, k1, k2 = rot(rho), rot(y), rot(k0), rot(k1), rot(k2)
+        k3 = orc.rhs(y)
+        rho = rho + (dt / 6) * (k0 + 2 * k1 + 2 * k2 + k3)
+        t += dt
+        steps += 1
+    orc.set_state(rho)
+    return steps

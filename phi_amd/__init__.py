@@ -275,7 +275,8 @@ class HierarchyIntegrator:
         return rc
 
     def density_matrix(self):
-        n = self.params.get_int("num_states")
+        """rho_0 after Launch (full space for a truncated system)."""
+        n = max(self.params.get_int("num_states"), self.params.get_int("hilbert_space_size"))
         out = np.zeros(2 * n * n)
         _check(lib().phi_hip_integrator_density_matrix(self._h, _dp(out), n * n))
         return _from_colmajor(out, n)

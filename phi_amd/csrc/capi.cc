@@ -358,7 +358,8 @@ int phi_hip_integrator_launch(phi_integrator* it) {
 
 int phi_hip_integrator_density_matrix(phi_integrator* it, double* out, size_t cap) {
   if (!it || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  const size_t ne = (size_t)it->p->num_states * it->p->num_states;
+  const size_t dim = (size_t)it->it->density_matrix_dim();
+  const size_t ne = dim * dim;
   if (cap < ne) return Err(PHI_HIP_EINVAL, "buffer too small");
   std::complex<double>* d = it->it->density_matrix();
   std::memcpy(out, (const void*)d, ne * sizeof(std::complex<double>));

@@ -345,8 +345,9 @@ Engine::~Engine() {
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
                   d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_active_, d_kf_[0], d_kf_[1],
-                  d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_kraw_, d_ym_};
+                  d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_kraw_, d_ym_, d_rot_};
   for (void* p : ptrs) Free(p);
+  if (h_stage_) (void)hipHostFree(h_stage_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 

@@ -29,6 +29,19 @@ enum Status {
   kErrNoDevice = -7,
 };
 
+// A time-dependent system Hamiltonian seen by the RK4 step: Update(t) is
+// true when H changed at time t (PHI's UpdateTimeDependentHamiltonian);
+// H() is then the new N x N Hamiltonian (column-major) and Rotation(), for a
+// truncated system, the basis change R of the ADOs (a -> R a R^dagger), else
+// null.
+class HamiltonianSource {
+ public:
+  virtual ~HamiltonianSource() {}
+  virtual bool Update(double t) = 0;
+  virtual const std::complex<double>* H() const = 0;
+  virtual const std::complex<double>* Rotation() const = 0;
+};
+
 class Engine {
  public:
   // Builds the hierarchy and uploads every table.  Returns kOk or an error
@@ -108,6 +121,17 @@ class Engine {
   // appended to the device history after every step i (counted from
   // step_offset) with i % snap_every == 0.
   int Rk4Steps(long long nsteps, double dt, long long step_offset, int snap_every);
+  // RK4 with a time-dependent Hamiltonian (PHI's classic stage order,
+  // hierarchy_integrator.cc:2684-2826): H is updated at t before stage 1,
+  // at t + dt/2 before stage 2 and at t + dt before stage 4, each time
+  // rotating the live ADO buffers for a truncated system.  t_before[s] is
+  // the time at the start of step s (PHI's accumulated time).  Snapshots as
+  // Rk4Steps.
+  int Rk4StepsTD(long long nsteps, const double* t_before, double dt, long long step_offset,
+                 int snap_every, HamiltonianSource* src);
+  // Replaces the system Hamiltonian (N x N, column-major) on the device, in
+  // stream order.
+  int SetHamiltonian(const std::complex<double>* H);
   // Stage-kernel timing: stage s launched nsteps times back to back between
   // two HIP events; ms[s] = their total.  The state is restored.
   int Rk4StageTimes(long long nsteps, double dt, double ms[4]);
@@ -245,6 +269,13 @@ class Engine {
   int FilterRhs(const double* in, int s);
   int FilterRk4Step(double dt, bool snapshot);
   int FilterRkf45Attempt(const Rkf45Coefs& c, double* err);
+
+  // Time-dependent Hamiltonian: pinned staging of H (three slots per step,
+  // reused after the step's synchronisation) and the device rotation.
+  std::complex<double>* h_stage_ = nullptr;
+  double2* d_rot_ = nullptr;
+  int stage_slot_ = 0;
+  int Rotate(const std::complex<double>* R, int nbuf, void* const* bufs, const bool* fp32);
 
   // State buffers (interleaved complex).
   double* d_rho_ = nullptr;

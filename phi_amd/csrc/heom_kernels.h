@@ -100,6 +100,11 @@ hipError_t LaunchSnapshotMean(const double* src, const int* member_offsets, int 
                               double* hist, int* counter, int ne, int capacity, hipStream_t s);
 hipError_t LaunchFillU64(unsigned long long* p, unsigned long long v, int n, hipStream_t s);
 hipError_t LaunchCastToFloat(const double* src, float* dst, size_t n, hipStream_t s);
+// Basis change of every ADO of a slab: a_n <- R a_n R^dagger (R N x N,
+// column-major, on the device), in FP64 arithmetic; buf holds FP64 or FP32
+// interleaved complex ADOs.
+hipError_t LaunchRotateNodes(void* buf, bool fp32, const double2* R, int N, int nodes,
+                             hipStream_t s);
 hipError_t LaunchDot(const double* u, const double* v, size_t n, int mode, double* partial,
                      int nblocks, double* out, hipStream_t s);
 hipError_t LaunchLincomb(double* y, size_t n, int nv, const double* const* x, const double2* c,

@@ -127,6 +127,49 @@ __global__ void snapshot_mean(const double* __restrict__ src, const int* __restr
   if (threadIdx.x == 0) *counter = slot + 1;
 }
 
+// One block per ADO (grid-stride over nodes): T = a R^dagger, then
+// a <- R T, through LDS (three N x N complex matrices).
+template <typename T>
+__global__ void __launch_bounds__(256) rotate_nodes(T* __restrict__ buf,
+                                                    const double2* __restrict__ R, int N,
+                                                    int nodes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* r = reinterpret_cast<double2*>(smem);
+  double2* a = r + N * N;
+  double2* t = a + N * N;
+  const int NE = N * N;
+  for (int e = threadIdx.x; e < NE; e += blockDim.x) r[e] = R[e];
+  for (int n = blockIdx.x; n < nodes; n += gridDim.x) {
+    T* m = buf + (size_t)n * NE * 2;
+    __syncthreads();
+    for (int e = threadIdx.x; e < NE; e += blockDim.x)
+      a[e] = make_double2((double)m[2 * e], (double)m[2 * e + 1]);
+    __syncthreads();
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+      const int i = e % N, j = e / N;  // element (i, j)
+      double x = 0, y = 0;
+      for (int c = 0; c < N; ++c) {
+        const double2 p = a[i + c * N], q = r[j + c * N];  // a(i,c) conj(R(j,c))
+        x += p.x * q.x + p.y * q.y;
+        y += p.y * q.x - p.x * q.y;
+      }
+      t[e] = make_double2(x, y);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+      const int i = e % N, j = e / N;
+      double x = 0, y = 0;
+      for (int c = 0; c < N; ++c) {
+        const double2 p = r[i + c * N], q = t[c + j * N];  // R(i,c) T(c,j)
+        x += p.x * q.x - p.y * q.y;
+        y += p.x * q.y + p.y * q.x;
+      }
+      m[2 * e] = (T)x;
+      m[2 * e + 1] = (T)y;
+    }
+  }
+}
+
 __global__ void fill_u64(unsigned long long* p, unsigned long long v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -312,6 +355,20 @@ hipError_t LaunchSnapshotMean(const double* src, const int* member_offsets, int 
 
 hipError_t LaunchFillU64(unsigned long long* p, unsigned long long v, int n, hipStream_t s) {
   hipLaunchKernelGGL(dev::fill_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t LaunchRotateNodes(void* buf, bool fp32, const double2* R, int N, int nodes,
+                             hipStream_t s) {
+  if (N < 1 || N > 32 || nodes < 1) return hipErrorInvalidValue;
+  const size_t sh = 3 * (size_t)N * N * sizeof(double2);
+  const int grid = nodes < 65535 ? nodes : 65535;
+  if (fp32)
+    hipLaunchKernelGGL(dev::rotate_nodes<float>, dim3(grid), dim3(256), sh, s, (float*)buf, R, N,
+                       nodes);
+  else
+    hipLaunchKernelGGL(dev::rotate_nodes<double>, dim3(grid), dim3(256), sh, s, (double*)buf, R,
+                       N, nodes);
   return hipGetLastError();
 }
 

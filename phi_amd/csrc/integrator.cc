@@ -10,6 +10,8 @@
 // The arithmetic of each step runs on the GPU (engine.cc / heom_kernels.hip).
 #include "integrator.h"
 
+#include "system_hamiltonian.h"
+
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -114,7 +116,14 @@ HierarchyIntegrator::HierarchyIntegrator(PhiParameters* p) : p_(p) {
   }
 }
 
-HierarchyIntegrator::~HierarchyIntegrator() { delete engine_; }
+HierarchyIntegrator::~HierarchyIntegrator() {
+  delete engine_;
+  delete sh_;
+}
+
+int HierarchyIntegrator::density_matrix_dim() const {
+  return sh_ && sh_->truncated() ? sh_->full_size() : p_->num_states;
+}
 
 void HierarchyIntegrator::Fail(int code, const std::string& msg) {
   info_.status = code;
@@ -123,12 +132,14 @@ void HierarchyIntegrator::Fail(int code, const std::string& msg) {
 }
 
 std::complex<double>* HierarchyIntegrator::density_matrix() {
-  const int N = p_->num_states;
-  std::complex<double>* out = new std::complex<double>[(size_t)N * N];
-  if (have_final_) {
+  const int N = p_->num_states, F = density_matrix_dim();
+  std::complex<double>* out = new std::complex<double>[(size_t)F * F];
+  if (have_final_ && F != N) {
+    sh_->UnTruncate(final_rho0_.data(), out);
+  } else if (have_final_) {
     for (size_t i = 0; i < (size_t)N * N; ++i) out[i] = final_rho0_[i];
   } else {
-    for (size_t i = 0; i < (size_t)N * N; ++i) out[i] = 0;
+    for (size_t i = 0; i < (size_t)F * F; ++i) out[i] = 0;
   }
   return out;
 }
@@ -197,11 +208,6 @@ void HierarchyIntegrator::Launch() {
 
 int HierarchyIntegrator::Initialize() {
   const PhiParameters& p = *p_;
-  if (p.time_indepenent_hamiltonian.empty()) {
-    Fail(kErrUnsupported, "time-dependent (annealing) Hamiltonians are not supported; "
-                          "give a Hamiltonian");
-    return info_.status;
-  }
   if (p.is_diagonal_bath_coupling || p.is_full_bath_coupling) {
     Fail(kErrUnsupported, "only independent (site projector) baths are supported");
     return info_.status;
@@ -210,12 +216,35 @@ int HierarchyIntegrator::Initialize() {
     Fail(kErrUnsupported, "time-local truncation is not supported");
     return info_.status;
   }
-  if (p.is_truncated_system) {
-    Fail(kErrUnsupported, "HilbertSpaceSize > NumStates is not supported");
-    return info_.status;
-  }
   std::string why;
-  int rc = Engine::Create(p, &engine_, &why);
+  const PhiParameters* engine_params = &p;
+  if (p.time_indepenent_hamiltonian.empty() || p.is_truncated_system) {
+    // Annealing and/or truncated system: the engine integrates the
+    // NumStates x NumStates Hamiltonian at the start of the run; the rest
+    // is driven from here (hierarchy_integrator.cc:880-946, 1617-1667).
+    sh_ = new SystemHamiltonian;
+    why = sh_->Init(p);
+    if (!why.empty()) {
+      Fail(kErrInvalidParameters, why);
+      return info_.status;
+    }
+    if (sh_->time_dependent() && p.run_method != RK4) {
+      Fail(kErrUnsupported, "a time-dependent (annealing) Hamiltonian runs with RK4 only");
+      return info_.status;
+    }
+    if (sh_->time_dependent() && p.is_restarted) {
+      Fail(kErrUnsupported, "restarting a time-dependent (annealing) run is not supported");
+      return info_.status;
+    }
+    ep_ = p;
+    const int N = p.num_states;
+    ep_.time_indepenent_hamiltonian.assign(sh_->H(), sh_->H() + (size_t)N * N);
+    ep_.hilbert_space_size = N;
+    ep_.hilbert_space_num_elements = N * N;
+    ep_.is_truncated_system = false;
+    engine_params = &ep_;
+  }
+  int rc = Engine::Create(*engine_params, &engine_, &why);
   if (rc) {
     Fail(rc, why);
     return rc;
@@ -271,17 +300,18 @@ int HierarchyIntegrator::InitialState() {
     p_->timestep = dt;
     return kOk;
   }
-  if ((int)p.initial_density_matrix.size() != NE) {
+  const int F = sh_ && sh_->truncated() ? sh_->full_size() : N;
+  if ((int)p.initial_density_matrix.size() != F * F) {
     Fail(kErrUnsupported, "InitialDensityMatrix is required");
     return info_.status;
   }
   // ADOs stay Hermitian only from a Hermitian start; the integrator relies
   // on it (rho_n^dagger obeys the same equations).
   double scale = 0, asym = 0;
-  for (int a = 0; a < N; ++a)
-    for (int b = 0; b < N; ++b) {
-      const std::complex<double> x = p.initial_density_matrix[a + (size_t)b * N];
-      const std::complex<double> y = p.initial_density_matrix[b + (size_t)a * N];
+  for (int a = 0; a < F; ++a)
+    for (int b = 0; b < F; ++b) {
+      const std::complex<double> x = p.initial_density_matrix[a + (size_t)b * F];
+      const std::complex<double> y = p.initial_density_matrix[b + (size_t)a * F];
       scale = std::max(scale, std::abs(x));
       asym = std::max(asym, std::abs(x - std::conj(y)));
     }
@@ -290,8 +320,11 @@ int HierarchyIntegrator::InitialState() {
     return info_.status;
   }
   // InitializeDensityMatrix (hierarchy_integrator.cc:1070-1091): node 0 gets
-  // rho0 / scale with scale = 1 at n = 0; all other ADOs start at zero.
-  for (int e = 0; e < NE; ++e) state[e] = p.initial_density_matrix[e];
+  // rho0 / scale with scale = 1 at n = 0; all other ADOs start at zero.  A
+  // truncated system starts from rho0 in the eigenbasis (:1073-1076).
+  if (F != N) sh_->Truncate(p.initial_density_matrix.data(), state.data());
+  else
+    for (int e = 0; e < NE; ++e) state[e] = p.initial_density_matrix[e];
   int rc = engine_->SetState(state.data());
   if (rc) {
     Fail(rc, engine_->last_error());
@@ -346,9 +379,14 @@ int HierarchyIntegrator::RunRk4() {
   // Chunks sized so the node-0 history never overflows.
   const long long chunk = (long long)output_factor * 2048;
   std::vector<std::complex<double>> hist;
+  // Time at the start of each step (a time-dependent H is evaluated there).
+  std::vector<double> t_before(steps);
+  for (long long i = 0; i < steps; ++i) t_before[i] = i == 0 ? p.start_time : times[i - 1];
   for (long long s = 0; s < steps; s += chunk) {
     const long long n = std::min(chunk, steps - s);
-    int rc = engine_->Rk4Steps(n, dt, s, output_factor);
+    int rc = sh_ && sh_->time_dependent()
+                 ? engine_->Rk4StepsTD(n, t_before.data() + s, dt, s, output_factor, sh_)
+                 : engine_->Rk4Steps(n, dt, s, output_factor);
     if (!rc) rc = engine_->DrainHistory(&hist);
     if (rc < 0) {
       Fail(rc, engine_->last_error());

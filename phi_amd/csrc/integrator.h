@@ -37,7 +37,11 @@ class HierarchyIntegrator {
   ~HierarchyIntegrator();
 
   void Launch();
+  // rho_0 after Launch; for a truncated system (HilbertSpaceSize >
+  // NumStates) in the full space, HilbertSpaceSize^2 entries
+  // (hierarchy_integrator.cc:177-191).
   std::complex<double>* density_matrix();
+  int density_matrix_dim() const;
 
   // Additions of this build.
   int status() const { return info_.status; }
@@ -66,6 +70,11 @@ class HierarchyIntegrator {
 
   PhiParameters* p_;
   Engine* engine_ = nullptr;
+  // Time-dependent or truncated system Hamiltonian (null for a plain static
+  // H); the engine then runs on ep_, the parameters with the NumStates x
+  // NumStates integration Hamiltonian.
+  class SystemHamiltonian* sh_ = nullptr;
+  PhiParameters ep_;
   std::ofstream out_;
   std::ofstream full_;
   std::vector<std::complex<double>> final_rho0_;

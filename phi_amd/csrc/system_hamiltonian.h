@@ -1,0 +1,88 @@
+// The system Hamiltonian of a run on the host: PHI's time-independent H, its
+// annealing form
+//   H(t) = f_i(t / T) H_initial + f_f(t / T) H_final
+// with cubic-spline field multipliers (hierarchy_integrator.cc:1617-1667,
+// cubic_spline.cc), and the Hilbert-space truncation onto the NumStates
+// lowest eigenvectors of the full H when HilbertSpaceSize > NumStates
+// (HilbertSpaceTruncator, hilbert_space_truncator.cc:34-129).
+#ifndef PHI_AMD_SYSTEM_HAMILTONIAN_H_
+#define PHI_AMD_SYSTEM_HAMILTONIAN_H_
+
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "phi_parameters.h"
+
+namespace phi {
+
+typedef std::complex<double> cplx;
+
+// Eigen-decomposition of a Hermitian n x n matrix A (column-major; only the
+// lower triangle is read, as zheevr with UPLO='L'): the k lowest
+// eigenvalues w (ascending) and their eigenvectors V (n x k, column-major)
+// by cyclic complex Jacobi rotations.  Each eigenvector is scaled so that
+// its first entry of largest magnitude is real and positive.
+void HermitianEigen(int n, const cplx* A, int k, double* w, cplx* V);
+
+// PHI's CubicSpline on the uniform grid x_i = i / (m - 1) of [0, 1]: the
+// knot derivatives solve the tridiagonal system (2, 4, ..., 4, 2; 1) with
+// right-hand side 3 (y_1 - y_0), 3 (y_{i+1} - y_i), 3 (y_{m-1} - y_{m-2})
+// (cubic_spline.cc:67-86, 104-119); Get clamps outside [0, 1]
+// (cubic_spline.cc:44-55).
+class CubicSpline {
+ public:
+  explicit CubicSpline(const std::vector<double>& y);
+  double Get(double x);
+
+ private:
+  std::vector<double> x_, a_, b_, c_, d_;
+  size_t last_ = 0;
+};
+
+class SystemHamiltonian : public HamiltonianSource {
+ public:
+  // From the parameters (NumStates, HilbertSpaceSize, the static or the two
+  // annealing Hamiltonians and fields, MinimumFieldChange).  Returns "" or
+  // an error message.
+  std::string Init(const PhiParameters& p);
+
+  bool time_dependent() const { return td_; }
+  bool truncated() const { return full_ > n_; }
+  int n() const { return n_; }
+  int full_size() const { return full_; }
+  // Integration Hamiltonian (n x n, column-major) at the start of the run.
+  const cplx* H() const override { return H_.data(); }
+  // PHI's UpdateTimeDependentHamiltonian(t): false when H is static or when
+  // both field multipliers changed by less than MinimumFieldChange
+  // (relative) since the last update; else recomputes H and, for a
+  // truncated system, the eigenvectors and the rotation of the ADOs.
+  bool Update(double t) override;
+  const cplx* Rotation() const override { return truncated() ? R_.data() : nullptr; }
+  // Full n_full x n_full matrix -> n x n in the current eigenbasis
+  // (V^dagger A V) and back (V a V^dagger).
+  void Truncate(const cplx* full, cplx* out) const;
+  void UnTruncate(const cplx* small, cplx* full) const;
+
+ private:
+  void FullAt(double alpha, std::vector<cplx>* Hf);
+  void SetFrom(const std::vector<cplx>& Hf, bool rotate);
+
+  int n_ = 0, full_ = 0;
+  bool td_ = false;
+  double T_ = 1, min_change_ = 0.01;
+  std::vector<cplx> Hstatic_, Hi_, Hf_;
+  CubicSpline* fi_ = nullptr;
+  CubicSpline* ff_ = nullptr;
+  double old_i_ = 0, old_f_ = 0;
+  std::vector<cplx> H_, V_, R_;
+  std::vector<double> w_;
+
+ public:
+  ~SystemHamiltonian();
+};
+
+}  // namespace phi
+
+#endif  // PHI_AMD_SYSTEM_HAMILTONIAN_H_
