@@ -65,6 +65,11 @@ struct Oracle {
   double filter = 0;
   bool vacuum = false;
   std::vector<cplx> H, Hadj;      // column-major N x N
+  // Coupling model (hierarchy_integrator.cc:1772-1790): 0 IndependentBath
+  // (site projectors), 1 DiagonalBath, 2 FullBath.  bath_ops: M operators,
+  // N entries each (diagonal) or N x N column-major (full).
+  int bath_model = 0;
+  std::vector<cplx> bath_ops;
   std::vector<cplx> corr;         // c_{jk}, D entries
   std::vector<double> abs_corr;
   std::vector<cplx> matsubara;    // Delta_j, M entries
@@ -227,6 +232,87 @@ void InitializeConstants(Oracle* o) {
   }
 }
 
+// c = alpha * A B + beta * c for N x N column-major matrices (NOBLAS Gemm
+// order, blas_wrapper.cc:11-31: c(a,b) = beta c(a,b) + sum_n alpha (A(a,n)
+// B(n,b))).
+void Gemm(int N, cplx alpha, const cplx* A, const cplx* B, cplx beta, cplx* C) {
+  for (int a = 0; a < N; ++a)
+    for (int b = 0; b < N; ++b) {
+      cplx c = C[a + b * N] * beta;
+      for (int n = 0; n < N; ++n) c += alpha * (A[a + n * N] * B[n + b * N]);
+      C[a + b * N] = c;
+    }
+}
+
+// The coupling terms of DiagonalBath and FullBath (hierarchy_updater.cc:
+// 10-127) with the operators of Hierarchy::FillDiagonalBathOperators /
+// FillFullBathOperators (hierarchy.cc:157-225):
+//   diagonal: same_m(r,c) = alpha_m (b_r - b_c)^2, next_m(r,c) = -i (b_r - b_c),
+//             prev_{mk}(r,c) = -i (b_r c_mk - b_c conj(c_mk));
+//             DoNext adds next_m .* rho_{n+e} (no prefactor), DoPrevious adds
+//             n_mk prev_mk .* rho_{n-e} (PHI's DiagonalBath ignores the ADO
+//             normalisation);
+//   full:     same_m = sqrt(alpha_m) F_m, next_m = -i F_m, prev_m = -F_m;
+//             DoMatsubara: drho -= [S, [S, rho]];
+//             DoNext: drho += (-i p) next_m rho' + (i p) rho' next_m;
+//             DoPrevious: drho += row prev_m rho' + col rho' prev_m.
+// Called after DoUnitary and DoDephasing.
+void NodeRhsBath(const Oracle* o, const cplx* x, const Node& nd, const cplx* rho, cplx* drho) {
+  const int N = o->N, NE = N * N, K = o->K;
+  const cplx I(0, 1);
+  if (o->bath_model == 1) {
+    auto b = [&](int m, int r) { return o->bath_ops[(size_t)m * N + r]; };
+    for (int m = 0; m < o->M; ++m)
+      for (int c = 0; c < N; ++c)
+        for (int r = 0; r < N; ++r) {
+          const cplx same = o->matsubara[m] * std::pow(b(m, r) - b(m, c), 2);
+          drho[r + c * N] -= same * rho[r + c * N];
+        }
+    for (size_t j = 0; j < nd.next_node.size(); ++j) {
+      const int m = nd.next_mode[j] / K;
+      const cplx* r1 = x + (size_t)nd.next_node[j] * NE;
+      for (int c = 0; c < N; ++c)
+        for (int r = 0; r < N; ++r)
+          drho[r + c * N] += -I * (b(m, r) - b(m, c)) * r1[r + c * N];
+    }
+    for (size_t j = 0; j < nd.prev_node.size(); ++j) {
+      const int mode = nd.prev_mode[j], m = mode / K;
+      const cplx cc = o->corr[mode];
+      const cplx pf(nd.index[mode], 0);
+      const cplx* r1 = x + (size_t)nd.prev_node[j] * NE;
+      for (int c = 0; c < N; ++c)
+        for (int r = 0; r < N; ++r)
+          drho[r + c * N] +=
+              pf * (-I * (b(m, r) * cc - b(m, c) * std::conj(cc))) * r1[r + c * N];
+    }
+    return;
+  }
+  // Full coupling operators.
+  std::vector<cplx> S(NE), T(NE), F(NE);
+  for (int m = 0; m < o->M; ++m) {
+    const cplx sq = std::sqrt(o->matsubara[m]);
+    for (int e = 0; e < NE; ++e) S[e] = sq * o->bath_ops[(size_t)m * NE + e];
+    Gemm(N, cplx(1, 0), S.data(), rho, cplx(0, 0), T.data());     // T = S rho
+    Gemm(N, cplx(-1, 0), rho, S.data(), cplx(1, 0), T.data());    // T -= rho S
+    Gemm(N, cplx(-1, 0), S.data(), T.data(), cplx(1, 0), drho);   // drho -= S T
+    Gemm(N, cplx(1, 0), T.data(), S.data(), cplx(1, 0), drho);    // drho += T S
+  }
+  for (size_t j = 0; j < nd.next_node.size(); ++j) {
+    const int m = nd.next_mode[j] / K;
+    for (int e = 0; e < NE; ++e) F[e] = -I * o->bath_ops[(size_t)m * NE + e];
+    const cplx* r1 = x + (size_t)nd.next_node[j] * NE;
+    Gemm(N, -I * nd.next_pref[j], F.data(), r1, cplx(1, 0), drho);
+    Gemm(N, I * nd.next_pref[j], r1, F.data(), cplx(1, 0), drho);
+  }
+  for (size_t j = 0; j < nd.prev_node.size(); ++j) {
+    const int m = nd.prev_mode[j] / K;
+    for (int e = 0; e < NE; ++e) F[e] = -o->bath_ops[(size_t)m * NE + e];
+    const cplx* r1 = x + (size_t)nd.prev_node[j] * NE;
+    Gemm(N, nd.prev_row[j], F.data(), r1, cplx(1, 0), drho);
+    Gemm(N, nd.prev_col[j], r1, F.data(), cplx(1, 0), drho);
+  }
+}
+
 // hierarchy_updater.h:36-75 and hierarchy_updater.cc:129-144, 216-304.
 // drho for node i from the ADO array x (PHI id order, column-major blocks).
 // Arithmetic order follows the NOBLAS Gemm (blas_wrapper.cc:11-31).
@@ -262,6 +348,10 @@ void NodeRhs(const Oracle* o, const cplx* x, int i, cplx* drho) {
     // DoDephasing.
     const cplx mu(nd.dephasing, 0);
     for (int n = 0; n < NE; ++n) drho[n] -= mu * rho[n];
+    if (o->bath_model != 0) {
+      NodeRhsBath(o, x, nd, rho, drho);
+      return;
+    }
     // IndependentBath::DoMatsubara (single bath per state).
     for (int m = 0; m < N; ++m) {
       for (int n = 0; n < N; ++n) {
@@ -524,6 +614,19 @@ void oracle_get_state(void* h, double* out) {
 void oracle_set_state(void* h, const double* in) {
   Oracle* o = (Oracle*)h;
   std::memcpy((void*)o->rho.data(), in, o->rho.size() * sizeof(cplx));
+}
+
+// Coupling model and operators (interleaved complex): model 1 (diagonal)
+// M x N entries, model 2 (full) M x N x N column-major.  The adaptive filter
+// does not apply to these models (phi_parameters.cc:847-851).
+void oracle_set_bath(void* h, int model, const double* ops) {
+  Oracle* o = (Oracle*)h;
+  o->bath_model = model;
+  const size_t n = model == 1 ? (size_t)o->M * o->N : (size_t)o->M * o->N * o->N;
+  o->bath_ops.assign(n, cplx(0, 0));
+  if (model != 0)
+    for (size_t e = 0; e < n; ++e) o->bath_ops[e] = cplx(ops[2 * e], ops[2 * e + 1]);
+  if (model != 0) o->filter = 0;
 }
 
 // Replaces the system Hamiltonian (N x N, column-major interleaved complex)

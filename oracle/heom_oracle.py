@@ -53,7 +53,10 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        # (re)built when missing or older than its source; the GPU box gets
+        # the library built here (no compiler step there)
+        if not os.path.exists(LIB) or (os.path.exists(SRC) and
+                                       os.path.getmtime(LIB) < os.path.getmtime(SRC)):
             build()
         L = ctypes.CDLL(LIB)
         vp = ctypes.c_void_p
@@ -71,6 +74,7 @@ def lib():
         L.oracle_set_state.argtypes = [vp, dp]
         L.oracle_get_active.argtypes = [vp, ip]
         L.oracle_set_hamiltonian.argtypes = [vp, dp]
+        L.oracle_set_bath.argtypes = [vp, ctypes.c_int, dp]
         L.oracle_rhs.argtypes = [vp, dp, dp]
         L.oracle_steady_matvec.argtypes = [vp, dp, dp]
         L.oracle_rk4.restype = ctypes.c_longlong
@@ -86,16 +90,20 @@ def _dp(a):
 
 
 class Problem:
-    """An IndependentBath HEOM problem in PHI's units and conventions.
+    """A HEOM problem in PHI's units and conventions.
 
     H and rho0 are N x N complex (H[i, j] = element (i, j)); gamma, lambda_
     have one entry per bath; num_terms is PHI's num_matsubara_terms (the
-    .prm MatsubaraTerms + 1); lh is PHI's HierarchyTruncation.
+    .prm MatsubaraTerms + 1); lh is PHI's HierarchyTruncation.  bath_model
+    "independent" (site projectors), "diagonal" (bath_ops: M x N diagonal
+    entries) or "full" (bath_ops: M x N x N, ops[m][i, j] = element (i, j)).
     """
 
     def __init__(self, H, rho0, gamma, lambda_, temperature, lh, num_terms=1,
                  hbar=5.3088354, kb=0.6950344, bath_site=None, normalized=True,
-                 filter_tolerance=0.0):
+                 filter_tolerance=0.0, bath_model="independent", bath_ops=None):
+        self.bath_model = bath_model
+        self.bath_ops = None if bath_ops is None else np.asarray(bath_ops, dtype=complex)
         self.H = np.asarray(H, dtype=complex)
         self.rho0 = np.asarray(rho0, dtype=complex)
         self.N = self.H.shape[0]
@@ -155,6 +163,8 @@ class Oracle:
         self.num_nodes = L.oracle_num_nodes(self.h)
         self.D = L.oracle_num_modes(self.h)
         self.N = prob.N
+        if prob.bath_model != "independent":
+            self.set_bath(prob.bath_model, prob.bath_ops)
 
     def close(self):
         if self.h:
@@ -191,6 +201,15 @@ class Oracle:
     def set_state(self, rho):
         flat = np.ascontiguousarray(np.asarray(rho, dtype=complex).transpose(0, 2, 1))
         lib().oracle_set_state(self.h, _dp(flat.view(np.float64).reshape(-1)))
+
+    def set_bath(self, model, ops):
+        """Coupling model and operators (see Problem)."""
+        code = {"independent": 0, "diagonal": 1, "full": 2}[model]
+        ops = np.asarray(ops, dtype=complex)
+        if code == 2:
+            ops = ops.transpose(0, 2, 1)  # column-major blocks
+        self._ops = np.ascontiguousarray(ops).view(np.float64).reshape(-1).copy()
+        lib().oracle_set_bath(self.h, code, _dp(self._ops))
 
     def set_hamiltonian(self, H):
         """Replace the N x N system Hamiltonian (H[i, j] = element (i, j))."""

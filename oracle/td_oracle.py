@@ -77,8 +77,13 @@ class Hamiltonian:
     """Static or annealing H, optionally truncated to n states."""
 
     def __init__(self, n, H=None, H_initial=None, H_final=None, f_initial=(1.0, 0.0),
-                 f_final=(0.0, 1.0), t_total=1.0, min_change=0.01):
+                 f_final=(0.0, 1.0), t_total=1.0, min_change=0.01, bath_ops=None):
+        """bath_ops: full-space coupling operators (M x F x F) of a FullBath
+        model; for a truncated system they are truncated once with the
+        initial eigenvectors and then rotated with every basis change
+        (hierarchy_integrator.cc:1170-1180, 1661-1667), as PHI does."""
         self.n = n
+        self.ops = None if bath_ops is None else np.asarray(bath_ops, dtype=complex)
         self.td = H is None
         self.T = t_total
         self.min_change = min_change
@@ -96,6 +101,8 @@ class Hamiltonian:
         self.V = None
         self.R = None
         self._set(self._full_at(0.0), rotate=False)
+        if self.ops is not None and self.truncated:
+            self.ops = np.array([self.truncate(F) for F in self.ops])
 
     @property
     def truncated(self):
@@ -127,6 +134,8 @@ class Hamiltonian:
             return False
         self.old_i, self.old_f = vi, vf
         self._set(self._full_at(alpha), rotate=True)
+        if self.ops is not None and self.truncated:
+            self.ops = self.R @ self.ops @ self.R.conj().T
         return True
 
     def truncate(self, A):
@@ -144,19 +153,25 @@ def rk4(orc, ham, t_start, t_end, dt):
     """PHI's RK4 loop with H updates (classic stage order); the oracle's
     state is advanced in place.  Returns the number of steps."""
     rot = lambda x: ham.R @ x @ ham.R.conj().T if ham.truncated else x  # noqa: E731
-    orc.set_hamiltonian(ham.H)
+
+    def push():
+        orc.set_hamiltonian(ham.H)
+        if ham.ops is not None:
+            orc.set_bath("full", ham.ops)
+
+    push()
     t = t_start
     steps = 0
     rho = orc.state()
     while t < t_end:
         if ham.update(t):
-            orc.set_hamiltonian(ham.H)
+            push()
             if ham.truncated:
                 rho = rot(rho)
         k0 = orc.rhs(rho)
         y = rho + k0 * (dt / 2)
         if ham.update(t + dt / 2):
-            orc.set_hamiltonian(ham.H)
+            push()
             if ham.truncated:
                 rho, y, k0 = rot(rho), rot(y), rot(k0)
         k1 = orc.rhs(y)
@@ -164,7 +179,7 @@ def rk4(orc, ham, t_start, t_end, dt):
         k2 = orc.rhs(y)
         y = rho + k2 * dt
         if ham.update(t + dt):
-            orc.set_hamiltonian(ham.H)
+            push()
             if ham.truncated:
                 rho, y, k0, k1, k2 = rot(rho), rot(y), rot(k0), rot(k1), rot(k2)
         k3 = orc.rhs(y)

@@ -117,6 +117,20 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
     rc = e->Check(hipMalloc((void**)&e->d_active_, h.num_nodes), "alloc flags");
     if (!rc) rc = e->Check(hipMemset(e->d_active_, 1, h.num_nodes), "set flags");
   }
+  // Diagonal / full coupling operators (the vacuum updater ignores them).
+  if (!rc && (p.is_diagonal_bath_coupling || p.is_full_bath_coupling) && !h.vacuum) {
+    e->bath_model_ = p.is_full_bath_coupling ? 2 : 1;
+    std::vector<double2> corr(h.D);
+    for (int j = 0; j < h.D; ++j) corr[j] = make_double2(h.corr[j].real(), h.corr[j].imag());
+    rc = e->Check(Upload(&e->d_alpha_, h.alpha.data(), h.alpha.size()), "upload alpha");
+    if (!rc) rc = e->Check(Upload(&e->d_corr_, corr.data(), corr.size()), "upload corr");
+    const size_t per = e->bath_model_ == 1 ? (size_t)N : (size_t)NE;
+    if (!rc && p.bath_coupling_op.size() == (size_t)h.M * per) {
+      std::vector<std::complex<double>> ops(p.bath_coupling_op.begin(), p.bath_coupling_op.end());
+      rc = e->SetBathOps(ops.data());
+      if (!rc) rc = e->Synchronize();
+    }
+  }
   if (rc) {
     *why = e->err_;
     delete e;
@@ -225,8 +239,9 @@ int Engine::CreateEnsemble(const PhiParameters& base, int members,
     delete e;
     return kErrUnsupported;
   }
-  if (e->filter_on()) {
-    *why = "FilterTolerance (adaptive truncation) is not available for an ensemble";
+  if (e->explicit_path()) {
+    *why = "FilterTolerance and diagonal/full coupling operators are not available for an "
+           "ensemble";
     delete e;
     return kErrUnsupported;
   }
@@ -345,9 +360,11 @@ Engine::~Engine() {
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
                   d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_active_, d_kf_[0], d_kf_[1],
-                  d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_kraw_, d_ym_, d_rot_};
+                  d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_kraw_, d_ym_, d_rot_,
+                  d_ops_, d_alpha_, d_corr_};
   for (void* p : ptrs) Free(p);
   if (h_stage_) (void)hipHostFree(h_stage_);
+  if (h_ops_) (void)hipHostFree(h_ops_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -458,6 +475,11 @@ std::map<TuneKey, TuneVal> g_tune;
 }  // namespace
 
 int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
+  if (explicit_path()) {
+    err_ = "the autotuner covers the fused stage kernels (not adaptive truncation or "
+           "diagonal/full coupling operators)";
+    return kErrUnsupported;
+  }
   if (steps < 1) steps = 1;
   int rc = AllocBulk();
   if (rc) return rc;
@@ -606,12 +628,12 @@ int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
   if (!rc) rc = Check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, stream_), "rhs upload");
   if (filter_on()) {
     // Current flags: the plain operator on the masked input.
-    if (!rc) rc = FilterAlloc(0);
+    if (!rc) rc = ExplicitAlloc(0);
     if (!rc) rc = Check(LaunchMaskNodes(dx, d_active_, d_ym_, N() * N(), num_nodes(), stream_),
                         "mask nodes");
-    if (!rc) rc = RhsDevice(d_ym_, dy, 0);
+    if (!rc) rc = OperatorRhs(d_ym_, dy);
   } else if (!rc) {
-    rc = RhsDevice(dx, dy, 0);
+    rc = OperatorRhs(dx, dy);
   }
   if (!rc) rc = Check(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, stream_), "rhs download");
   if (!rc) rc = Synchronize();
@@ -683,8 +705,8 @@ int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int sna
   int rc = AllocBulk();
   if (rc) return rc;
   auto snap = [&](long long s) { return snap_every > 0 && (step_offset + s) % snap_every == 0; };
-  if (filter_on()) {
-    for (long long s = 0; s < nsteps && !rc; ++s) rc = FilterRk4Step(dt, snap(s));
+  if (explicit_path()) {
+    for (long long s = 0; s < nsteps && !rc; ++s) rc = ExplicitRk4Step(0.0, dt, snap(s), nullptr);
     return rc;
   }
   // Graph blocks of G steps, G a multiple of the snapshot period, aligned so
@@ -718,6 +740,11 @@ int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int sna
 }
 
 int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
+  if (explicit_path()) {
+    err_ = "stage timing covers the fused stage kernels (not adaptive truncation or "
+           "diagonal/full coupling operators)";
+    return kErrUnsupported;
+  }
   int rc = AllocBulk();
   if (rc) return rc;
   for (int s = 0; s < 4; ++s) ms[s] = 0;
@@ -792,7 +819,7 @@ int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
     if (rc) return rc;
     rkf45_alloc_ = true;
   }
-  if (filter_on()) return FilterRkf45Attempt(c, err);
+  if (explicit_path()) return ExplicitRkf45Attempt(c, err);
   Rkf45Bufs<float> bf{};
   Rkf45Bufs<double> bd{};
   for (int i = 0; i < 5; ++i) {
@@ -828,6 +855,10 @@ int Engine::Rkf45Accept() {
 int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
                         std::vector<double>* history) {
   if (members_) return Unsupported("the steady state");
+  if (bath_model_ != 0) {
+    err_ = "the steady state with diagonal/full coupling operators is not supported";
+    return kErrUnsupported;
+  }
   // Unpreconditioned complex BiCGSTAB (van der Vorst 1992) on A x = b with
   // b = e_(node 0, element (0,0)); inner products conjugate the first
   // argument; all scalars and reductions in FP64.

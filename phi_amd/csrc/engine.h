@@ -40,6 +40,9 @@ class HamiltonianSource {
   virtual bool Update(double t) = 0;
   virtual const std::complex<double>* H() const = 0;
   virtual const std::complex<double>* Rotation() const = 0;
+  // Full coupling operators in the current basis (M x N x N), when they
+  // follow the basis of a truncated system; else null.
+  virtual const std::complex<double>* BathOps() const { return nullptr; }
 };
 
 class Engine {
@@ -154,6 +157,16 @@ class Engine {
   // Current flags (num_nodes entries, 1 = active); returns the active count.
   int ActiveFlags(int* out);
 
+  // Diagonal / full coupling operators (PHI's DiagonalBath / FullBath,
+  // bath_kernels.h): 0 = site projectors (IndependentBath), 1 diagonal, 2
+  // full.  RK4 and RKF45 run the explicit-k path in FP64; the operators
+  // come from the parameters (real) or from SetBathOps (complex, e.g. a
+  // truncated system's operators in its eigenbasis).
+  int bath_model() const { return bath_model_; }
+  int SetBathOps(const std::complex<double>* ops);
+  // Runs that take the explicit-k RK4/RKF45 path (engine_filter.cc).
+  bool explicit_path() const { return filter_on() || bath_model_ != 0; }
+
   // Steady state: BiCGSTAB on A x = b (A = L with the trace row), from the
   // current state.  Returns kOk or kErrNotConverged; *iters, *resid report.
   int SteadyState(double tol, int max_iter, int* iters, double* resid,
@@ -265,10 +278,17 @@ class Engine {
   double* d_kf_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double* d_kraw_ = nullptr;
   double* d_ym_ = nullptr;
-  int FilterAlloc(int nk);
-  int FilterRhs(const double* in, int s);
-  int FilterRk4Step(double dt, bool snapshot);
-  int FilterRkf45Attempt(const Rkf45Coefs& c, double* err);
+  int bath_model_ = 0;
+  double2* d_ops_ = nullptr;
+  double* d_alpha_ = nullptr;
+  double2* d_corr_ = nullptr;
+  std::complex<double>* h_ops_ = nullptr;
+  int ExplicitAlloc(int nk);
+  int OperatorRhs(const double* x, double* y);
+  int ExplicitRhs(const double* in, int s);
+  int ExplicitUpdate(HamiltonianSource* src, double t, int nbuf, void* const* bufs);
+  int ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSource* src);
+  int ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err);
 
   // Time-dependent Hamiltonian: pinned staging of H (three slots per step,
   // reused after the step's synchronisation) and the device rotation.

@@ -1,102 +1,157 @@
-// Adaptive hierarchy truncation (PHI's FilterTolerance > 0): the RK4 and
-// RKF45 steps of the filtered operator and the flag update.
+// Explicit-k RK4 and RKF45: the runs whose RHS the fused stage kernels do
+// not cover.  The RHS of a stage is evaluated into its k buffer, then PHI's
+// classic combinations run as separate FP64 kernels.  Two cases:
 //
-// PHI (IndependentBathAdaptiveTruncation, hierarchy_updater.h:217-245,
-// hierarchy_updater.cc:256-304; flag loops hierarchy_integrator.cc:2358-2385
-// for RKF45 and 2828-2851 for RK4).  Stage s of a step, with the flags fixed:
-//   kraw = L(input with the ADOs of inactive nodes zeroed)   (plain operator)
-//   k_s  = kraw on active nodes, k_s(previous contents) + kraw on inactive
-//          ones (PHI's per-node k buffers persist across steps and attempts)
-// then PHI's classic combinations, all in FP64.  Flags change after each
-// accepted step.  PHI's RK4 updates a thread's flags right after its own
-// final update while other threads may still evaluate k3 (no barrier
-// between them, hierarchy_integrator.cc:2810-2851); with one thread the
-// update follows the whole step, which is the order used here.
+// * Adaptive truncation (FilterTolerance > 0; IndependentBathAdaptiveTruncation,
+//   hierarchy_updater.h:217-245, hierarchy_updater.cc:256-304; flag loops
+//   hierarchy_integrator.cc:2358-2385 for RKF45 and 2828-2851 for RK4).
+//   With the flags fixed during a step,
+//     kraw = L(input with the ADOs of inactive nodes zeroed)   (plain operator)
+//     k_s  = kraw on active nodes, k_s(previous contents) + kraw on inactive
+//            ones (PHI's per-node k buffers persist across steps and attempts,
+//            and the adaptive Get skips DoUnitary, the call that overwrites
+//            drho);
+//   flags change after each accepted step.  PHI's RK4 updates a thread's
+//   flags right after its own final update while other threads may still
+//   evaluate k3 (no barrier, hierarchy_integrator.cc:2810-2851); with one
+//   thread the update follows the whole step, the order used here.
+// * Diagonal and full coupling operators (DiagonalBath / FullBath,
+//   bath_kernels.h): k_s = L(input) with the general-coupling RHS kernel.
+//   With a time-dependent Hamiltonian the H updates, the basis rotations of
+//   the live buffers (rho, the stage input and k0..k2, as
+//   hierarchy_integrator.cc:2684-2826) and the rotated coupling operators
+//   of a truncated system are applied between stages.
 #include <cstring>
 #include <vector>
 
+#include "bath_kernels.h"
 #include "engine.h"
 #include "filter_kernels.h"
 
 namespace phi {
 
-int Engine::FilterAlloc(int nk) {
+int Engine::ExplicitAlloc(int nk) {
   const size_t bytes = state_elems() * 2 * sizeof(double);
   int rc = kOk;
-  if (!d_ym_) rc = Check(hipMalloc((void**)&d_ym_, bytes), "alloc filter input");
-  if (!rc && !d_kraw_) rc = Check(hipMalloc((void**)&d_kraw_, bytes), "alloc filter rhs");
+  if (filter_on()) {
+    if (!d_ym_) rc = Check(hipMalloc((void**)&d_ym_, bytes), "alloc filter input");
+    if (!rc && !d_kraw_) rc = Check(hipMalloc((void**)&d_kraw_, bytes), "alloc filter rhs");
+  }
   for (int s = 0; s < nk && !rc; ++s) {
     if (d_kf_[s]) continue;
-    rc = Check(hipMalloc((void**)&d_kf_[s], bytes), "alloc filter k");
-    if (!rc) rc = Check(hipMemsetAsync(d_kf_[s], 0, bytes, stream_), "zero filter k");
+    rc = Check(hipMalloc((void**)&d_kf_[s], bytes), "alloc stage k");
+    if (!rc) rc = Check(hipMemsetAsync(d_kf_[s], 0, bytes, stream_), "zero stage k");
   }
   return rc;
 }
 
-int Engine::FilterRhs(const double* in, int s) {
+int Engine::OperatorRhs(const double* x, double* y) {
+  if (bath_model_ != 0) {
+    BathArgs b{bath_model_, h_.M, h_.K, d_ops_, d_alpha_, d_corr_};
+    return Check(LaunchRhsBath(args_, b, x, y, stream_), "rhs_bath");
+  }
+  return Check(LaunchRhsGeneral(args_, x, y, 0, stream_), "rhs_general");
+}
+
+int Engine::ExplicitRhs(const double* in, int s) {
+  if (!filter_on()) return OperatorRhs(in, d_kf_[s]);
   const int ne = N() * N(), nodes = num_nodes();
   int rc = Check(LaunchMaskNodes(in, d_active_, d_ym_, ne, nodes, stream_), "mask nodes");
-  if (!rc) rc = Check(LaunchRhsGeneral(args_, d_ym_, d_kraw_, 0, stream_), "rhs_general");
-  if (!rc && s >= 0) rc = Check(LaunchFilterK(d_kraw_, d_kf_[s], d_active_, ne, nodes, stream_),
-                                "filter k");
+  if (!rc) rc = OperatorRhs(d_ym_, d_kraw_);
+  if (!rc) rc = Check(LaunchFilterK(d_kraw_, d_kf_[s], d_active_, ne, nodes, stream_), "filter k");
   return rc;
 }
 
-int Engine::FilterRk4Step(double dt, bool snapshot) {
+int Engine::ExplicitUpdate(HamiltonianSource* src, double t, int nbuf, void* const* bufs) {
+  if (!src || !src->Update(t)) return kOk;
+  int rc = SetHamiltonian(src->H());
+  if (!rc && src->BathOps()) rc = SetBathOps(src->BathOps());
+  if (!rc && src->Rotation()) {
+    bool f64[8];
+    for (int i = 0; i < nbuf; ++i) f64[i] = false;
+    rc = Rotate(src->Rotation(), nbuf, bufs, f64);
+  }
+  ++stage_slot_;
+  return rc;
+}
+
+int Engine::ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSource* src) {
   if (fp32_) {
-    err_ = "FilterTolerance (adaptive truncation) runs in FP64 only (BulkPrecision=fp64)";
+    err_ = "adaptive truncation and diagonal/full coupling operators run in FP64 only "
+           "(BulkPrecision=fp64)";
     return kErrUnsupported;
   }
-  int rc = FilterAlloc(4);
+  int rc = ExplicitAlloc(4);
   if (rc) return rc;
   const size_t n = state_elems();
   double* y = (double*)d_y_[0];
   const double half = dt / 2;
-  const double c[3] = {half, half, dt};
-  const double* in = d_rho_;
-  for (int s = 0; s < 3 && !rc; ++s) {
-    rc = FilterRhs(in, s);
-    const double* k[1] = {d_kf_[s]};
-    if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k, &c[s], 1, y, n, stream_), "filter combine");
-    in = y;
+  // H at t (rho live), at t + dt/2 (rho, Y, k0), at t + dt (rho, Y, k0..k2).
+  {
+    void* b[1] = {d_rho_};
+    rc = ExplicitUpdate(src, t, 1, b);
   }
-  if (!rc) rc = FilterRhs(y, 3);
+  if (!rc) rc = ExplicitRhs(d_rho_, 0);
+  const double* k0[1] = {d_kf_[0]};
+  if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k0, &half, 1, y, n, stream_), "combine");
+  if (!rc) {
+    void* b[3] = {d_rho_, y, d_kf_[0]};
+    rc = ExplicitUpdate(src, t + dt / 2, 3, b);
+  }
+  if (!rc) rc = ExplicitRhs(y, 1);
+  const double* k1[1] = {d_kf_[1]};
+  if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k1, &half, 1, y, n, stream_), "combine");
+  if (!rc) rc = ExplicitRhs(y, 2);
+  const double* k2[1] = {d_kf_[2]};
+  if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k2, &dt, 1, y, n, stream_), "combine");
+  if (!rc) {
+    void* b[5] = {d_rho_, y, d_kf_[0], d_kf_[1], d_kf_[2]};
+    rc = ExplicitUpdate(src, t + dt, 5, b);
+  }
+  if (!rc) rc = ExplicitRhs(y, 3);
   const double* k4[4] = {d_kf_[0], d_kf_[1], d_kf_[2], d_kf_[3]};
-  if (!rc) rc = Check(LaunchFilterRk4Final(d_rho_, k4, dt / 6, n, stream_), "filter rk4 update");
-  if (!rc) rc = Check(LaunchUpdateActive(d_rho_, d_active_, N() * N(), num_nodes(), filter_,
-                                         stream_), "flag update");
+  if (!rc) rc = Check(LaunchFilterRk4Final(d_rho_, k4, dt / 6, n, stream_), "rk4 update");
+  if (!rc && filter_on())
+    rc = Check(LaunchUpdateActive(d_rho_, d_active_, N() * N(), num_nodes(), filter_, stream_),
+               "flag update");
   if (!rc && snapshot)
     rc = Check(LaunchSnapshot(d_rho_, d_hist_, d_hist_count_, N() * N(), hist_capacity_, stream_),
                "snapshot");
+  if (!rc && src) {
+    // The step's staging slots are reused by the next step.
+    rc = Synchronize();
+    stage_slot_ = 0;
+  }
   return rc;
 }
 
-int Engine::FilterRkf45Attempt(const Rkf45Coefs& c, double* err) {
+int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err) {
   if (fp32_) {
-    err_ = "FilterTolerance (adaptive truncation) runs in FP64 only (BulkPrecision=fp64)";
+    err_ = "adaptive truncation and diagonal/full coupling operators run in FP64 only "
+           "(BulkPrecision=fp64)";
     return kErrUnsupported;
   }
-  int rc = FilterAlloc(6);
+  int rc = ExplicitAlloc(6);
   if (rc) return rc;
   const size_t n = state_elems();
   double* y = (double*)d_y_[0];
   rc = Check(LaunchFillU64(d_err_, 0ull, 1, stream_), "zero err");
   const double* in = d_rho_;
   for (int s = 0; s < 5 && !rc; ++s) {
-    rc = FilterRhs(in, s);
+    rc = ExplicitRhs(in, s);
     const double* k[5];
     double w[5];
     for (int j = 0; j <= s; ++j) {
       k[j] = d_kf_[j];
       w[j] = c.b[s][j];
     }
-    if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k, w, s + 1, y, n, stream_), "filter combine");
+    if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k, w, s + 1, y, n, stream_), "combine");
     in = y;
   }
-  if (!rc) rc = FilterRhs(y, 5);
+  if (!rc) rc = ExplicitRhs(y, 5);
   const double* k6[6] = {d_kf_[0], d_kf_[1], d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5]};
   if (!rc) rc = Check(LaunchFilterRkf45Final(d_rho_, k6, c.c4, c.c5, d_aux_, d_err_, n, stream_),
-                      "filter rkf45 final");
+                      "rkf45 final");
   unsigned long long bits = 0;
   if (!rc) rc = Check(hipMemcpyAsync(&bits, d_err_, sizeof(bits), hipMemcpyDeviceToHost, stream_),
                       "err");
@@ -104,6 +159,23 @@ int Engine::FilterRkf45Attempt(const Rkf45Coefs& c, double* err) {
   if (rc) return rc;
   std::memcpy(err, &bits, sizeof(double));
   return kOk;
+}
+
+int Engine::SetBathOps(const std::complex<double>* ops) {
+  if (bath_model_ == 0) return kOk;
+  const size_t n = (size_t)h_.M * (bath_model_ == 1 ? N() : N() * N());
+  int rc = kOk;
+  if (!h_ops_) {
+    rc = Check(hipHostMalloc((void**)&h_ops_, 3 * n * sizeof(double2)), "pinned operators");
+    if (rc) return rc;
+  }
+  if (!d_ops_) rc = Check(hipMalloc((void**)&d_ops_, n * sizeof(double2)), "alloc operators");
+  if (rc) return rc;
+  // A slot per update of a step (the step synchronises before reuse).
+  double2* slot = reinterpret_cast<double2*>(h_ops_) + (size_t)(stage_slot_ % 3) * n;
+  for (size_t i = 0; i < n; ++i) slot[i] = make_double2(ops[i].real(), ops[i].imag());
+  return Check(hipMemcpyAsync(d_ops_, slot, n * sizeof(double2), hipMemcpyHostToDevice, stream_),
+               "upload operators");
 }
 
 int Engine::ActiveFlags(int* out) {

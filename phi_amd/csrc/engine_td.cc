@@ -96,6 +96,14 @@ int Engine::Rk4StepsTD(long long nsteps, const double* t_before, double dt, long
   }
   int rc = AllocBulk();
   if (rc) return rc;
+  if (bath_model_ != 0) {
+    // Diagonal / full coupling operators: the explicit-k step applies the
+    // updates itself (engine_filter.cc).
+    for (long long s = 0; s < nsteps && !rc; ++s)
+      rc = ExplicitRk4Step(t_before[s], dt, snap_every > 0 && (step_offset + s) % snap_every == 0,
+                           src);
+    return rc;
+  }
   const void* in[4] = {d_rho_, d_y_[0], d_y_[1], d_y_[0]};
   void* out[4] = {d_y_[0], d_y_[1], d_y_[0], nullptr};
   for (long long s = 0; s < nsteps && !rc; ++s) {

@@ -148,11 +148,15 @@ std::string BuildHierarchy(const PhiParameters& p, HierarchyTables* h) {
   }
   h->delta.assign(N, 0);
   if (!h->vacuum)
-    for (int a = 0; a < N; ++a) h->delta[a] = delta_bath[a];
+    for (int a = 0; a < N && a < M; ++a) h->delta[a] = delta_bath[a];
+  h->alpha = delta_bath;
 
-  // Mode -> site map (LinkHierarchyNodes: bath_indices[j / K]).
+  // Mode -> site map (LinkHierarchyNodes: bath_indices[j / K]).  Diagonal
+  // and full coupling operators act through their matrices instead; their
+  // modes map to site 0 here and the tables are unused.
+  const bool projectors = !p.is_diagonal_bath_coupling && !p.is_full_bath_coupling;
   h->site_of_mode.assign(D, 0);
-  for (int j = 0; j < D; ++j) {
+  for (int j = 0; j < D && projectors; ++j) {
     const int site = p.multiple_independent_bath_indices.empty()
                          ? j / K
                          : p.multiple_independent_bath_indices[j / K];

@@ -40,6 +40,7 @@ struct HierarchyTables {
   std::vector<double> next_s;
   std::vector<std::complex<double>> prev_col, prev_row;
   std::vector<double> delta;     // per state, PHI's matsubara_prefactor
+  std::vector<double> alpha;     // per bath, the same prefactor (general couplings)
   std::vector<int> site_of_mode;
   std::vector<int> site_ptr, site_modes;
   std::vector<std::complex<double>> corr;

@@ -208,10 +208,6 @@ void HierarchyIntegrator::Launch() {
 
 int HierarchyIntegrator::Initialize() {
   const PhiParameters& p = *p_;
-  if (p.is_diagonal_bath_coupling || p.is_full_bath_coupling) {
-    Fail(kErrUnsupported, "only independent (site projector) baths are supported");
-    return info_.status;
-  }
   if (p.use_time_local_truncation) {
     Fail(kErrUnsupported, "time-local truncation is not supported");
     return info_.status;
@@ -248,6 +244,15 @@ int HierarchyIntegrator::Initialize() {
   if (rc) {
     Fail(rc, why);
     return rc;
+  }
+  // A truncated system's full coupling operators, in its eigenbasis.
+  if (sh_ && sh_->BathOps()) {
+    rc = engine_->SetBathOps(sh_->BathOps());
+    if (!rc) rc = engine_->Synchronize();
+    if (rc) {
+      Fail(rc, engine_->last_error());
+      return rc;
+    }
   }
   info_.num_nodes = engine_->num_nodes();
   if (p.verbose > 0)

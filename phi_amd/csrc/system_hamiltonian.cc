@@ -167,6 +167,19 @@ std::string SystemHamiltonian::Init(const PhiParameters& p) {
   if (td_) FullAt(0.0, &Hfull);
   else Hfull = Hstatic_;
   SetFrom(Hfull, false);
+  if (truncated() && p.is_diagonal_bath_coupling)
+    return "diagonal coupling operators of a truncated system (HilbertSpaceSize > NumStates) "
+           "are not supported";
+  if (truncated() && p.is_full_bath_coupling) {
+    M_ = p.num_bath_couplings;
+    if (p.bath_coupling_op.size() != (size_t)M_ * FE) return "BathCouplingOp has the wrong size";
+    ops_.assign((size_t)M_ * n_ * n_, 0.0);
+    std::vector<cplx> F(FE);
+    for (int m = 0; m < M_; ++m) {
+      for (size_t e = 0; e < FE; ++e) F[e] = p.bath_coupling_op[(size_t)m * FE + e];
+      Truncate(F.data(), ops_.data() + (size_t)m * n_ * n_);
+    }
+  }
   return "";
 }
 
@@ -213,6 +226,24 @@ bool SystemHamiltonian::Update(double t) {
   std::vector<cplx> Hfull;
   FullAt(alpha, &Hfull);
   SetFrom(Hfull, true);
+  // Coupling operators follow the basis: F <- R F R^dagger.
+  const size_t NE = (size_t)n_ * n_;
+  std::vector<cplx> T(NE);
+  for (int m = 0; m < M_; ++m) {
+    cplx* F = ops_.data() + (size_t)m * NE;
+    for (int c = 0; c < n_; ++c)
+      for (int r = 0; r < n_; ++r) {
+        cplx s = 0;
+        for (int k = 0; k < n_; ++k) s += F[r + (size_t)k * n_] * std::conj(R_[c + (size_t)k * n_]);
+        T[r + (size_t)c * n_] = s;  // F R^dagger
+      }
+    for (int c = 0; c < n_; ++c)
+      for (int r = 0; r < n_; ++r) {
+        cplx s = 0;
+        for (int k = 0; k < n_; ++k) s += R_[r + (size_t)k * n_] * T[k + (size_t)c * n_];
+        F[r + (size_t)c * n_] = s;
+      }
+  }
   return true;
 }
 

@@ -60,6 +60,10 @@ class SystemHamiltonian : public HamiltonianSource {
   // truncated system, the eigenvectors and the rotation of the ADOs.
   bool Update(double t) override;
   const cplx* Rotation() const override { return truncated() ? R_.data() : nullptr; }
+  // Full coupling operators of a truncated system in the current eigenbasis:
+  // truncated once at the start, then rotated with every basis change
+  // (hierarchy_integrator.cc:1170-1180, 1661-1667).
+  const cplx* BathOps() const override { return ops_.empty() ? nullptr : ops_.data(); }
   // Full n_full x n_full matrix -> n x n in the current eigenbasis
   // (V^dagger A V) and back (V a V^dagger).
   void Truncate(const cplx* full, cplx* out) const;
@@ -76,8 +80,9 @@ class SystemHamiltonian : public HamiltonianSource {
   CubicSpline* fi_ = nullptr;
   CubicSpline* ff_ = nullptr;
   double old_i_ = 0, old_f_ = 0;
-  std::vector<cplx> H_, V_, R_;
+  std::vector<cplx> H_, V_, R_, ops_;
   std::vector<double> w_;
+  int M_ = 0;
 
  public:
   ~SystemHamiltonian();
