@@ -27,6 +27,10 @@ cpu_baseline  the oracle's pthread restatement of PHI's RK4 loop (FP64) on
          the host cores, timed on a bounded sample of the same workload.
 c3_fp32  the 24-site ring at depth 4 (20475 nodes of 24x24), FP32-bulk RK4:
          node-steps/s and the HBM fraction of its stage kernels.
+c3_rkf45 the 24-site ring with RKF45 (abs tol 1e-8) to 1 ps, FP64 and FP32
+         bulk: attempts, time to 1 ps, HBM fraction of N^2 (112 + 29 s) per
+         attempted node-step, FP32-vs-FP64 population error.
+c2_steady_state  BiCGSTAB to residual 1e-10 from the 7-site state at 1 ps.
 c4_ensemble  512 static-disorder realisations of the 7-site system with
          depths U{4..8} in one slab (persistent kernels, autotuned shape,
          hipGraph step blocks): time to 1 ps, FP32-bulk and FP64.
@@ -130,6 +134,61 @@ def run_rk4(P, p, prec, steps, warmup, shape=None):
     return out
 
 
+def run_steady_state(P, np, p, steps, tol=1e-10, max_iter=20000):
+    """BiCGSTAB steady state of the 7-site system from its state after
+    `steps` RK4 steps (the reference configuration's second phase)."""
+    dt = p.get_double("timestep")
+    ctx = P.Context(p)
+    ctx.rk4_steps(steps, dt)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rc, iters, resid = ctx.steady_state(tol, max_iter)
+    el = time.perf_counter() - t0
+    r0 = ctx.rho0()
+    nodes = ctx.num_nodes
+    N = r0.shape[0]
+    ctx.close()
+    # 24 FP64 vector passes per iteration: 384 N^2 B per node
+    per_iter = nodes * N * N * 384
+    return {"tolerance": tol, "status": rc, "iterations": iters, "residual": resid,
+            "seconds": el, "trace": float(np.trace(r0).real),
+            "hbm_frac_iteration": (per_iter * iters / el / (HBM_PEAK_GBS * 1e9)) if el > 0 else None}
+
+
+def run_c3_rkf45(P, np, log):
+    """The 24-site ring with RKF45 (abs tol 1e-8) to 1 ps, FP64 and FP32
+    bulk: attempts, time to 1 ps, attempted node-steps/s and the HBM
+    fraction of the attempt bytes N^2 (112 + 29 s)."""
+    out = {}
+    for prec in ("fp64", "fp32"):
+        p3 = P.PhiParameters.read(CFG3, "rkf45")
+        p3.set_string("output_filename", "")
+        p3.set_int("verbose", -2)
+        p3.set_int("bulk_precision", 1 if prec == "fp32" else 0)
+        it = P.HierarchyIntegrator(p3)
+        it.Launch()
+        info = it.info()
+        r = it.density_matrix()
+        N = r.shape[0]
+        s = 8 if prec == "fp32" else 16
+        nodes = int(info["num_nodes"])
+        bytes_attempt = nodes * N * N * (112 + 29 * s)
+        out[prec] = {"attempts": int(info["steps"]), "accepted": int(info["accepted"]),
+                     "time_to_1ps_s": info["seconds"],
+                     "node_steps_per_s": nodes * info["steps"] / info["seconds"],
+                     "hbm_frac_attempt": bytes_attempt * info["steps"] / info["seconds"]
+                                         / (HBM_PEAK_GBS * 1e9),
+                     "pops": np.real(np.diag(r))}
+        log("bench: c3 rkf45 %s %d attempts %.3f s" % (prec, info["steps"], info["seconds"]))
+    err = float(np.abs(out["fp32"]["pops"] - out["fp64"]["pops"]).max())
+    for v in out.values():
+        del v["pops"]
+    out["population_error_fp32_vs_fp64"] = err
+    out["workload"] = ("24-site ring, depth 4 (20475 nodes of 24x24), RKF45 abs tol 1e-8, "
+                       "initial dt 1 fs, to 1 ps")
+    return out
+
+
 def run_ensemble(P, np, steps, dt, members, log):
     """Static-disorder ensemble: `members` realisations of the 7-site system
     (site energies + N(0, 100 cm^-1), depth U{4..8}) in one slab, FP32-bulk
@@ -205,6 +264,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-steady-state", action="store_true")
+    ap.add_argument("--no-c3-rkf45", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=512)
     ap.add_argument("--closure-train", type=int, default=512)
     ap.add_argument("--closure-test", type=int, default=256)
@@ -305,6 +366,12 @@ def main():
                 r3["nodes"] * N3 * N3 * (160 + 6 * 8) / (r3["ms_per_step"] * 1e-3)
                 / (HBM_PEAK_GBS * 1e9),
             "stage_ms": r3["stage_ms"], "kernel_shape": r3["shape"]}
+    if not args.no_c3 and not args.no_c3_rkf45:
+        log("bench: 24-site depth 4, rkf45")
+        line["c3_rkf45"] = run_c3_rkf45(P, np, log)
+    if not args.no_steady_state:
+        log("bench: 7-site steady state")
+        line["c2_steady_state"] = run_steady_state(P, np, p, 1000)
     if not args.no_ensemble:
         log("bench: static-disorder ensemble")
         line["c4_ensemble"] = run_ensemble(P, np, 1000, dt, args.ensemble_members, log)
