@@ -188,3 +188,85 @@ def rk4(orc, ham, t_start, t_end, dt):
         steps += 1
     orc.set_state(rho)
     return steps
+
+
+# PHI's RKF45 tableau (hierarchy_integrator.cc:1926-1946).
+_A = [2. / 9, 1. / 3, 3. / 4, 1.0, 5. / 6]
+_B = [[2. / 9.], [1. / 12., 1. / 4.], [69. / 128., -243. / 128., 135. / 64.],
+      [-17. / 12., 27. / 4., -27. / 5, 16. / 15], [65. / 432., -5. / 16, 13. / 16., 4. / 27., 5. / 144.]]
+_C4 = [1. / 9, 9. / 20, 16. / 45, 1. / 12]              # k0, k2, k3, k4
+_C5 = [47. / 450, 12. / 25, 32. / 225, 1. / 30, 6. / 25]  # k0, k2, k3, k4, k5
+
+
+def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt):
+    """PHI's RKF45 loop (hierarchy_integrator.cc:1797-2573) with H updates
+    at t and t + a_s dt before each stage, rotating rho, the stage input and
+    the live k's (k1 is not rotated before the last stage, :2257-2263).
+    The weights are multiplied by dt once and then rescaled in place by the
+    same factors as dt, the end clamp replaces the growth factor, a step is
+    accepted when err < tol or dt * 0.5 < mindt.  Returns (attempts,
+    accepted, final time)."""
+    rot = lambda x: ham.R @ x @ ham.R.conj().T if ham.truncated else x  # noqa: E731
+
+    def push():
+        orc.set_hamiltonian(ham.H)
+        if ham.ops is not None:
+            orc.set_bath("full", ham.ops)
+
+    push()
+    mydt = dt0
+    dt_f = dt0
+    dtup, dtdn = 1.1, 0.5
+    a = [x * mydt for x in _A]
+    b = [[x * dt0 for x in row] for row in _B]
+    c4 = [x * dt0 for x in _C4]
+    c5 = [x * dt0 for x in _C5]
+    t = t_start
+    attempts = accepted = 0
+    rho = orc.state()
+    while t < t_end:
+        attempts += 1
+        if ham.update(t):
+            push()
+            if ham.truncated:
+                rho = rot(rho)
+        k = [orc.rhs(rho)]
+        for s in range(5):
+            y = rho.copy()
+            for j in range(s + 1):
+                y = y + k[j] * b[s][j]
+            if ham.update(t + a[s]):
+                push()
+                if ham.truncated:
+                    rho, y = rot(rho), rot(y)
+                    k = [kk if (s == 4 and j == 1) else rot(kk) for j, kk in enumerate(k)]
+            k.append(orc.rhs(y))
+        y4 = rho + k[0] * c4[0] + k[2] * c4[1] + k[3] * c4[2] + k[4] * c4[3]
+        y5 = rho + k[0] * c5[0] + k[2] * c5[1] + k[3] * c5[2] + k[4] * c5[3] + k[5] * c5[4]
+        err = float(np.abs(y4 - y5).max())
+        if err < tol or dt_f * dtdn < mindt:
+            rho = y5
+            t += mydt
+            accepted += 1
+            update = err < tol / 10
+            f = dtup
+            if t + mydt > t_end:
+                update = True
+                dtup = (t_end - t) / mydt
+                f = dtup
+            if update:
+                mydt *= f
+                a = [x * f for x in a]
+                b = [[x * f for x in row] for row in b]
+                c4 = [x * f for x in c4]
+                c5 = [x * f for x in c5]
+            dt_f = mydt
+        else:
+            mydt *= dtdn
+            a = [x * dtdn for x in a]
+            b = [[x * dtdn for x in row] for row in b]
+            c4 = [x * dtdn for x in c4]
+            c5 = [x * dtdn for x in c5]
+            dt_f = mydt
+    orc.set_state(rho)
+    return attempts, accepted, t

@@ -807,8 +807,17 @@ int Engine::DrainHistory(std::vector<std::complex<double>>* out) {
   return rc ? rc : n;
 }
 
-int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
+int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err, double t, const double* a,
+                         HamiltonianSource* src) {
   if (members_) return Unsupported("RKF45");
+  if (src && filter_on()) {
+    err_ = "FilterTolerance with a time-dependent Hamiltonian is not supported";
+    return kErrUnsupported;
+  }
+  if (src && closure_) {
+    err_ = "a closure with a time-dependent Hamiltonian is not supported";
+    return kErrUnsupported;
+  }
   int rc = AllocBulk();
   if (rc) return rc;
   if (!rkf45_alloc_) {
@@ -819,7 +828,7 @@ int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
     if (rc) return rc;
     rkf45_alloc_ = true;
   }
-  if (explicit_path()) return ExplicitRkf45Attempt(c, err);
+  if (explicit_path()) return ExplicitRkf45Attempt(c, err, t, a, src);
   Rkf45Bufs<float> bf{};
   Rkf45Bufs<double> bd{};
   for (int i = 0; i < 5; ++i) {
@@ -832,6 +841,30 @@ int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
   bf.err = bd.err = d_err_;
   rc = Check(LaunchFillU64(d_err_, 0ull, 1, stream_), "zero err");
   for (int s = 0; s < 6 && !rc; ++s) {
+    if (src && src->Update(s == 0 ? t : t + a[s - 1])) {
+      rc = SetHamiltonian(src->H());
+      const std::complex<double>* R = src->Rotation();
+      if (!rc && R) {
+        // rho (FP64); the stage input and k0..k(s-1) (bulk precision).
+        void* bufs[7];
+        bool f32[7];
+        int nb = 0;
+        bufs[nb] = d_rho_;
+        f32[nb++] = false;
+        if (s > 0) {
+          bufs[nb] = tin[(s - 1) & 1];
+          f32[nb++] = fp32_;
+          for (int j = 0; j < s; ++j)
+            if (!(s == 5 && j == 1)) {
+              bufs[nb] = d_k_[j];
+              f32[nb++] = fp32_;
+            }
+        }
+        rc = Rotate(R, nb, bufs, f32);
+      }
+      ++stage_slot_;
+      if (rc) break;
+    }
     bf.tmp_out = (float*)tin[s & 1];
     bd.tmp_out = (double*)tin[s & 1];
     const void* in = s == 0 ? (const void*)d_rho_ : tin[(s - 1) & 1];
@@ -840,6 +873,8 @@ int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err) {
   unsigned long long bits = 0;
   if (!rc) rc = Check(hipMemcpyAsync(&bits, d_err_, sizeof(bits), hipMemcpyDeviceToHost, stream_), "err");
   if (!rc) rc = Synchronize();
+  // The attempt's staging slots are free again.
+  stage_slot_ = 0;
   if (rc) return rc;
   std::memcpy(err, &bits, sizeof(double));
   return kOk;

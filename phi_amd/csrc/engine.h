@@ -143,8 +143,14 @@ class Engine {
   int DrainHistory(std::vector<std::complex<double>>* out);  // and resets it
 
   // One RKF45 attempt with the given (dt-scaled) weights; *err = max|y4-y5|.
-  // The 5th-order solution is held until Rkf45Accept().
-  int Rkf45Attempt(const Rkf45Coefs& c, double* err);
+  // The 5th-order solution is held until Rkf45Accept().  With a
+  // time-dependent Hamiltonian (src non-null) H is updated at t before
+  // stage 0 and at t + a[s-1] before stage s = 1..5 (a: the dt-scaled
+  // stage times), each update rotating the live ADO buffers of a
+  // truncated system: rho; then rho, the stage input and k0..k(s-1), k1
+  // excepted before stage 5 (hierarchy_integrator.cc:1984-2270).
+  int Rkf45Attempt(const Rkf45Coefs& c, double* err, double t = 0.0, const double* a = nullptr,
+                   HamiltonianSource* src = nullptr);
   int Rkf45Accept();
 
   // Adaptive hierarchy truncation (PHI's FilterTolerance > 0 with a
@@ -288,7 +294,8 @@ class Engine {
   int ExplicitRhs(const double* in, int s);
   int ExplicitUpdate(HamiltonianSource* src, double t, int nbuf, void* const* bufs);
   int ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSource* src);
-  int ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err);
+  int ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err, double t, const double* a,
+                           HamiltonianSource* src);
 
   // Time-dependent Hamiltonian: pinned staging of H (three slots per step,
   // reused after the step's synchronisation) and the device rotation.

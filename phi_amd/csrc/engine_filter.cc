@@ -18,9 +18,10 @@
 // * Diagonal and full coupling operators (DiagonalBath / FullBath,
 //   bath_kernels.h): k_s = L(input) with the general-coupling RHS kernel.
 //   With a time-dependent Hamiltonian the H updates, the basis rotations of
-//   the live buffers (rho, the stage input and k0..k2, as
-//   hierarchy_integrator.cc:2684-2826) and the rotated coupling operators
-//   of a truncated system are applied between stages.
+//   the live buffers (rho, the stage input and the earlier k's, as
+//   hierarchy_integrator.cc:2684-2826 for RK4 and 1984-2270 for RKF45) and
+//   the rotated coupling operators of a truncated system are applied
+//   between stages.
 #include <cstring>
 #include <vector>
 
@@ -125,7 +126,8 @@ int Engine::ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSourc
   return rc;
 }
 
-int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err) {
+int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err, double t, const double* a,
+                                 HamiltonianSource* src) {
   if (fp32_) {
     err_ = "adaptive truncation and diagonal/full coupling operators run in FP64 only "
            "(BulkPrecision=fp64)";
@@ -136,9 +138,24 @@ int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err) {
   const size_t n = state_elems();
   double* y = (double*)d_y_[0];
   rc = Check(LaunchFillU64(d_err_, 0ull, 1, stream_), "zero err");
+  // H at t (rho live) and at t + a[s-1] before stage s (rho, Y and
+  // k0..k(s-1) live, k1 excepted before stage 5).
+  auto update = [&](int s) -> int {
+    if (!src) return kOk;
+    void* b[6];
+    int nb = 0;
+    b[nb++] = d_rho_;
+    if (s > 0) {
+      b[nb++] = y;
+      for (int j = 0; j < s; ++j)
+        if (!(s == 5 && j == 1)) b[nb++] = d_kf_[j];
+    }
+    return ExplicitUpdate(src, s == 0 ? t : t + a[s - 1], nb, b);
+  };
   const double* in = d_rho_;
   for (int s = 0; s < 5 && !rc; ++s) {
-    rc = ExplicitRhs(in, s);
+    rc = update(s);
+    if (!rc) rc = ExplicitRhs(in, s);
     const double* k[5];
     double w[5];
     for (int j = 0; j <= s; ++j) {
@@ -148,6 +165,7 @@ int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err) {
     if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k, w, s + 1, y, n, stream_), "combine");
     in = y;
   }
+  if (!rc) rc = update(5);
   if (!rc) rc = ExplicitRhs(y, 5);
   const double* k6[6] = {d_kf_[0], d_kf_[1], d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5]};
   if (!rc) rc = Check(LaunchFilterRkf45Final(d_rho_, k6, c.c4, c.c5, d_aux_, d_err_, n, stream_),
@@ -156,6 +174,8 @@ int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err) {
   if (!rc) rc = Check(hipMemcpyAsync(&bits, d_err_, sizeof(bits), hipMemcpyDeviceToHost, stream_),
                       "err");
   if (!rc) rc = Synchronize();
+  // The attempt's staging slots are free again.
+  stage_slot_ = 0;
   if (rc) return rc;
   std::memcpy(err, &bits, sizeof(double));
   return kOk;

@@ -224,8 +224,8 @@ int HierarchyIntegrator::Initialize() {
       Fail(kErrInvalidParameters, why);
       return info_.status;
     }
-    if (sh_->time_dependent() && p.run_method != RK4) {
-      Fail(kErrUnsupported, "a time-dependent (annealing) Hamiltonian runs with RK4 only");
+    if (sh_->time_dependent() && p.run_method != RK4 && p.run_method != RKF45) {
+      Fail(kErrUnsupported, "a time-dependent (annealing) Hamiltonian runs with RK4 or RKF45 only");
       return info_.status;
     }
     if (sh_->time_dependent() && p.is_restarted) {
@@ -459,13 +459,17 @@ int HierarchyIntegrator::RunRkf45() {
                     {65. / 432., -5. / 16, 13. / 16., 4. / 27., 5. / 144.}};
   double c4[4] = {1. / 9, 9. / 20, 16. / 45, 1. / 12};
   double c5[5] = {47. / 450, 12. / 25, 32. / 225, 1. / 30, 6. / 25};
+  // Stage times of a time-dependent Hamiltonian, scaled with the weights.
+  double a[5] = {2. / 9, 1. / 3, 3. / 4, 1.0, 5. / 6};
   auto scale_all = [&](double f) {
     for (int s = 0; s < 5; ++s)
       for (int i = 0; i <= s; ++i) b[s][i] *= f;
     for (double& v : c4) v *= f;
     for (double& v : c5) v *= f;
+    for (double& v : a) v *= f;
   };
   scale_all(dt_f);
+  HamiltonianSource* src = sh_ && sh_->time_dependent() ? sh_ : nullptr;
   double time_now = p.start_time;
   double next_write_time = time_now;
   long long i = -1, accepted = 0, inext = 0;
@@ -480,7 +484,7 @@ int HierarchyIntegrator::RunRkf45() {
     for (int k = 0; k < 4; ++k) cf.c4[k] = c4[k];
     for (int k = 0; k < 5; ++k) cf.c5[k] = c5[k];
     double max_diff = 0;
-    int rc = engine_->Rkf45Attempt(cf, &max_diff);
+    int rc = engine_->Rkf45Attempt(cf, &max_diff, time_now, a, src);
     if (rc) {
       Fail(rc, engine_->last_error());
       return rc;

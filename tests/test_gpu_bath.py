@@ -112,9 +112,11 @@ def test_truncated_full_bath_rkf45(matsubara):
     assert np.abs(it.state() - orc.state()).max() <= 1e-10
 
 
+@pytest.mark.parametrize("method", ["rk4", "rkf45"])
 @pytest.mark.parametrize("model,truncated", [("diagonal", False), ("full", False),
                                              ("full", True)])
-def test_anneal_bath_models(model, truncated):
+def test_anneal_bath_models(model, truncated, method):
+    print("case", model, truncated, method, flush=True)
     full = 3 if truncated else 2
     M = full
     if model == "diagonal":
@@ -123,9 +125,10 @@ def test_anneal_bath_models(model, truncated):
         ops = full_independent(M)
     Hi = embed(HX, 3, 9.9) if truncated else HX
     Hf = embed(HZ, 3, 9.9) if truncated else HZ
-    it = launch(params(model, ops, 3, [0.5] * M, [0.5] * M, Hi=Hi, Hf=Hf, full=full))
+    it = launch(params(model, ops, 3, [0.5] * M, [0.5] * M, Hi=Hi, Hf=Hf, full=full,
+                       method=method))
     gold = embed(K_MEDIUM_ANNEAL, 3) if truncated else K_MEDIUM_ANNEAL
-    assert np.abs(it.density_matrix() - gold).max() < 1e-6
+    assert np.abs(it.density_matrix() - gold).max() < (1e-6 if method == "rk4" else 1e-3)
     ham = TD.Hamiltonian(2, H_initial=Hi, H_final=Hf, t_total=1.0,
                          bath_ops=ops if (truncated and model == "full") else None)
     rho = np.zeros((full, full), dtype=complex)
@@ -134,7 +137,12 @@ def test_anneal_bath_models(model, truncated):
                       temperature=2.0, lh=3, num_terms=1, hbar=1.0, kb=1.0, bath_model=model,
                       bath_ops=ham.ops if ham.ops is not None else ops)
     orc = HO.Oracle(prob, threads=1)
-    TD.rk4(orc, ham, 0.0, 1.0, 0.001)
+    if method == "rk4":
+        TD.rk4(orc, ham, 0.0, 1.0, 0.001)
+    else:
+        attempts, accepted, _ = TD.rkf45(orc, ham, 0.0, 1.0, 0.001, 1e-6, 1e-6)
+        info = it.info()
+        assert (info["steps"], info["accepted"]) == (attempts, accepted)
     assert np.abs(it.state() - orc.state()).max() <= 1e-10
 
 

@@ -79,7 +79,7 @@ CASES = {
 }
 
 
-def oracle_run(kw):
+def oracle_run(kw, method="rk4"):
     kw = dict(kw)
     n, lh, gamma, lam = kw.pop("n"), kw.pop("lh"), kw.pop("gamma"), kw.pop("lam")
     full = kw.pop("full", None) or n
@@ -90,8 +90,12 @@ def oracle_run(kw):
     prob = HO.Problem(H=ham.H, rho0=ham.truncate(rho_full), gamma=gamma, lambda_=lam,
                       temperature=2.0, lh=lh, num_terms=1, hbar=1.0, kb=1.0)
     orc = HO.Oracle(prob, threads=1)
-    TD.rk4(orc, ham, 0.0, 1.0, 0.001)
-    return orc, ham
+    if method == "rk4":
+        TD.rk4(orc, ham, 0.0, 1.0, 0.001)
+        return orc, ham
+    # PHI's defaults: RKF45Tolerance 1e-6, RKF45MinimumTimestep 1e-6
+    counts = TD.rkf45(orc, ham, 0.0, 1.0, 0.001, 1e-6, 1e-6)
+    return orc, ham, counts
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -120,8 +124,35 @@ def test_anneal_fp32_bulk_populations():
     assert np.abs(np.diag(r).real - np.diag(gold).real).max() < 1e-5
 
 
-def test_anneal_rejects_rkf45():
-    kw, _ = CASES["anneal_vacuum"]
+TD_CASES = sorted(k for k in CASES if "anneal" in k)
+
+
+@pytest.mark.parametrize("name", TD_CASES)
+def test_anneal_rkf45_goldens_and_oracle(name):
+    # PHI's RunRkf45Test (within 1e-3) and the oracle's RKF45 loop with H
+    # updates at the stage times: the same attempts and accepted steps, the
+    # same hierarchy within 1e-10
+    kw, gold = CASES[name]
+    print("case", name, flush=True)
     it = P.HierarchyIntegrator(params(**kw, method="rkf45"))
-    assert it.Launch(check=False) != 0
-    assert "RK4 only" in P.lib().phi_hip_last_error().decode()
+    it.Launch()
+    r = it.density_matrix()
+    if gold is not None:
+        assert np.abs(r - gold).max() < 1e-3
+    orc, ham, (attempts, accepted, _) = oracle_run(kw, "rkf45")
+    info = it.info()
+    assert (info["steps"], info["accepted"]) == (attempts, accepted)
+    assert abs(info["final_time"] - 1.0) < 1e-12
+    assert np.abs(r - ham.untruncate(orc.state()[0])).max() <= 1e-10
+    assert np.abs(it.state() - orc.state()).max() <= 1e-10
+
+
+def test_anneal_rkf45_fp32_bulk():
+    # FP32 bulk with the rotations applied to FP32 stage inputs and k's:
+    # the FP64 run's density matrix within 1e-5
+    kw, _ = CASES["truncated_anneal_medium"]
+    ref = P.HierarchyIntegrator(params(**kw, method="rkf45"))
+    ref.Launch()
+    it = P.HierarchyIntegrator(params(**kw, method="rkf45", precision="fp32"))
+    it.Launch()
+    assert np.abs(it.density_matrix() - ref.density_matrix()).max() < 1e-5

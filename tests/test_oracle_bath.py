@@ -121,8 +121,17 @@ def test_truncated_full_bath_rkf45_goldens(matsubara):
     assert np.abs(r - embed(gold, 3)).max() < 1e-3
 
 
+def td_run(ham, orc, method):
+    if method == "rk4":
+        TD.rk4(orc, ham, 0.0, 1.0, 0.001)
+        return 1e-6
+    TD.rkf45(orc, ham, 0.0, 1.0, 0.001, 1e-6, 1e-6)
+    return 1e-3
+
+
+@pytest.mark.parametrize("method", ["rk4", "rkf45"])
 @pytest.mark.parametrize("model", ["diagonal", "full"])
-def test_anneal_medium_bath_models_golden(model):
+def test_anneal_medium_bath_models_golden(model, method):
     """TestAnnealMediumDiagonalIndependentBath / ...FullIndependentBath."""
     ops = diag_independent(2) if model == "diagonal" else full_independent(2)
     ham = TD.Hamiltonian(2, H_initial=HX, H_final=HZ, t_total=1.0)
@@ -131,12 +140,13 @@ def test_anneal_medium_bath_models_golden(model):
     prob = HO.Problem(H=ham.H, rho0=rho, gamma=[0.5, 0.5], lambda_=[0.5, 0.5], temperature=2.0,
                       lh=3, num_terms=1, hbar=1.0, kb=1.0, bath_model=model, bath_ops=ops)
     orc = HO.Oracle(prob, threads=1)
-    TD.rk4(orc, ham, 0.0, 1.0, 0.001)
-    assert np.abs(orc.state()[0] - K_MEDIUM_ANNEAL).max() < 1e-6
+    tol = td_run(ham, orc, method)
+    assert np.abs(orc.state()[0] - K_MEDIUM_ANNEAL).max() < tol
 
 
-def test_truncated_anneal_full_bath_golden():
-    """TestTruncatedAnnealMediumFullIndependentBath (:613-622), RK4 part."""
+@pytest.mark.parametrize("method", ["rk4", "rkf45"])
+def test_truncated_anneal_full_bath_golden(method):
+    """TestTruncatedAnnealMediumFullIndependentBath (:613-622)."""
     ham = TD.Hamiltonian(2, H_initial=embed(HX, 3, 9.9), H_final=embed(HZ, 3, 9.9), t_total=1.0,
                          bath_ops=full_independent(3))
     rho = np.zeros((3, 3), dtype=complex)
@@ -145,6 +155,6 @@ def test_truncated_anneal_full_bath_golden():
                       temperature=2.0, lh=3, num_terms=1, hbar=1.0, kb=1.0, bath_model="full",
                       bath_ops=ham.ops)
     orc = HO.Oracle(prob, threads=1)
-    TD.rk4(orc, ham, 0.0, 1.0, 0.001)
+    tol = td_run(ham, orc, method)
     r = ham.untruncate(orc.state()[0])
-    assert np.abs(r - embed(K_MEDIUM_ANNEAL, 3)).max() < 1e-6
+    assert np.abs(r - embed(K_MEDIUM_ANNEAL, 3)).max() < tol

@@ -3,8 +3,10 @@ paths against PHI's own known answers (hierarchy_integrator_test.cc):
 TestAnnealVacuum, TestAnnealMediumIndependentBath (kVacuumAnnealingSoln,
 kMediumIndependentCouplingAnnealingSoln, :554-592), TestTruncatedVacuum
 (kVacuumSoln embedded in 3 states, :350-373) and TestTruncatedAnnealVacuum
-(:569-580), RK4 within 1e-6 as PHI's RunRk4Test; plus closed forms of the
-spline and the truncator."""
+(:569-580), RK4 within 1e-6 as PHI's RunRk4Test and RKF45 within 1e-3 as
+its RunRkf45Test; the RKF45 loop equals the C++ oracle's controller for a
+static H and the RK4 loop for an H that follows every stage; plus closed
+forms of the spline and the truncator."""
 
 import os
 import sys
@@ -74,6 +76,66 @@ def test_truncated_anneal_vacuum_golden():
     r, _, ham = run(2, 1, [0.5], [0.5], full=3, Hi=embed(HX, 3, 9.9), Hf=embed(HZ, 3, 9.9))
     assert ham.truncated
     assert np.abs(r - embed(K_VACUUM_ANNEAL, 3)).max() < 1e-6
+
+
+def run_rkf45(n, lh, gamma, lam, full=None, H=None, Hi=None, Hf=None, dt0=0.001, tol=1e-6):
+    full = full or n
+    ham = TD.Hamiltonian(n, H=H, H_initial=Hi, H_final=Hf, t_total=1.0)
+    rho_full = np.zeros((full, full), dtype=complex)
+    rho_full[0, 0] = 1
+    prob = HO.Problem(H=ham.H, rho0=ham.truncate(rho_full), gamma=gamma, lambda_=lam,
+                      temperature=2.0, lh=lh, num_terms=1, hbar=1.0, kb=1.0)
+    orc = HO.Oracle(prob, threads=1)
+    attempts, accepted, t = TD.rkf45(orc, ham, 0.0, 1.0, dt0, tol, 1e-6)
+    assert abs(t - 1.0) < 1e-12
+    return ham.untruncate(orc.state()[0]), (attempts, accepted), prob
+
+
+def test_rkf45_static_matches_controller():
+    # with a static H the loop is the C++ oracle's controller: same attempts,
+    # same accepted steps, same state
+    r, counts, prob = run_rkf45(2, 3, [0.5, 0.5], [0.5, 0.5], H=H2, dt0=0.2, tol=1e-7)
+    orc = HO.Oracle(prob, threads=1)
+    res = orc.rkf45(1.0, 0.2, 1e-7, 1e-6, 0.01)
+    assert counts == (res["attempted"], res["accepted"])
+    assert counts[0] > counts[1]          # some attempts were rejected
+    assert np.abs(r - orc.state()[0]).max() < 1e-13
+
+
+def test_rkf45_anneal_goldens():
+    # PHI's RunRkf45Test of TestAnnealVacuum, TestAnnealMediumIndependentBath
+    # and TestTruncatedAnnealVacuum (within 1e-3)
+    r, _, _ = run_rkf45(2, 1, [0.5], [0.5], Hi=HX, Hf=HZ)
+    assert np.abs(r - K_VACUUM_ANNEAL).max() < 1e-3
+    r, _, _ = run_rkf45(2, 3, [0.5, 0.5], [0.5, 0.5], Hi=HX, Hf=HZ)
+    assert np.abs(r - K_MEDIUM_ANNEAL).max() < 1e-3
+    r, _, _ = run_rkf45(2, 1, [0.5], [0.5], full=3, Hi=embed(HX, 3, 9.9), Hf=embed(HZ, 3, 9.9))
+    assert np.abs(r - embed(K_VACUUM_ANNEAL, 3)).max() < 1e-3
+
+
+@pytest.mark.parametrize("full", [2, 3])
+def test_rkf45_anneal_agrees_with_rk4(full):
+    # With MinimumFieldChange 0 the Hamiltonian follows every stage time, and
+    # the adaptive run at tolerance 1e-10 and the RK4 run at dt 1e-3 must then
+    # reach one state to ~1e-11: a stage evaluated at a wrong time or a live
+    # buffer left in the old eigenbasis would be off by O(dt).
+    def go(method):
+        Hi = embed(HX, 3, 9.9) if full == 3 else HX
+        Hf = embed(HZ, 3, 9.9) if full == 3 else HZ
+        ham = TD.Hamiltonian(2, H_initial=Hi, H_final=Hf, t_total=1.0, min_change=0.0)
+        rho = np.zeros((full, full), dtype=complex)
+        rho[0, 0] = 1
+        prob = HO.Problem(H=ham.H, rho0=ham.truncate(rho), gamma=[0.5, 0.5],
+                          lambda_=[0.5, 0.5], temperature=2.0, lh=3, num_terms=1, hbar=1.0,
+                          kb=1.0)
+        orc = HO.Oracle(prob, threads=1)
+        if method == "rk4":
+            TD.rk4(orc, ham, 0.0, 1.0, 0.001)
+        else:
+            TD.rkf45(orc, ham, 0.0, 1.0, 0.01, 1e-10, 1e-6)
+        return ham.untruncate(orc.state()[0])
+
+    assert np.abs(go("rkf45") - go("rk4")).max() < 1e-10
 
 
 def test_spline_closed_forms():
