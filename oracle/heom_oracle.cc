@@ -70,6 +70,11 @@ struct Oracle {
   // N entries each (diagonal) or N x N column-major (full).
   int bath_model = 0;
   std::vector<cplx> bath_ops;
+  // Time-local truncation: the summed matrices Q_m (M x N x N, column-major),
+  // applied to the nodes of the deepest tier (hierarchy_integrator.cc:
+  // 1318-1333) in place of their (empty) DoNext.
+  bool tl_on = false;
+  std::vector<cplx> tl_Q;
   std::vector<cplx> corr;         // c_{jk}, D entries
   std::vector<double> abs_corr;
   std::vector<cplx> matsubara;    // Delta_j, M entries
@@ -313,9 +318,42 @@ void NodeRhsBath(const Oracle* o, const cplx* x, const Node& nd, const cplx* rho
   }
 }
 
+// DoTimelocal of the three coupling models (hierarchy_updater.cc:28-43,
+// 94-107, 170-213): with X_m = Q_m rho - rho Q_m^dagger,
+//   IndependentBath: column s += X_m(:, s), row s -= X_m(s, :)  (s = site of
+//                    bath m), i.e. drho -= [P_s, X_m];
+//   DiagonalBath:    drho(r, c) -= (b_r - b_c) X_m(r, c);
+//   FullBath:        drho -= F_m X_m - X_m F_m.
+void NodeTimeLocal(const Oracle* o, const cplx* rho, cplx* drho) {
+  const int N = o->N, NE = N * N;
+  std::vector<cplx> X(NE), Qa(NE);
+  for (int m = 0; m < o->M; ++m) {
+    const cplx* Q = o->tl_Q.data() + (size_t)m * NE;
+    for (int r = 0; r < N; ++r)
+      for (int c = 0; c < N; ++c) Qa[r + c * N] = std::conj(Q[c + r * N]);
+    Gemm(N, cplx(1, 0), Q, rho, cplx(0, 0), X.data());       // X = Q rho
+    Gemm(N, cplx(-1, 0), rho, Qa.data(), cplx(1, 0), X.data());  // X -= rho Q^dagger
+    if (o->bath_model == 0) {
+      const int s = o->bath_site[m];
+      for (int r = 0; r < N; ++r) drho[r + s * N] += X[r + s * N];
+      for (int c = 0; c < N; ++c) drho[s + c * N] -= X[s + c * N];
+    } else if (o->bath_model == 1) {
+      const cplx* b = o->bath_ops.data() + (size_t)m * N;
+      for (int c = 0; c < N; ++c)
+        for (int r = 0; r < N; ++r) drho[r + c * N] -= (b[r] - b[c]) * X[r + c * N];
+    } else {
+      const cplx* F = o->bath_ops.data() + (size_t)m * NE;
+      Gemm(N, cplx(-1, 0), F, X.data(), cplx(1, 0), drho);  // drho -= F X
+      Gemm(N, cplx(1, 0), X.data(), F, cplx(1, 0), drho);   // drho += X F
+    }
+  }
+}
+
 // hierarchy_updater.h:36-75 and hierarchy_updater.cc:129-144, 216-304.
 // drho for node i from the ADO array x (PHI id order, column-major blocks).
-// Arithmetic order follows the NOBLAS Gemm (blas_wrapper.cc:11-31).
+// Arithmetic order follows the NOBLAS Gemm (blas_wrapper.cc:11-31).  With
+// time-local truncation and one tier PHI's NoHierarchy* updaters apply
+// DoUnitary, DoMatsubara and DoTimelocal (hierarchy_updater.h:266-330).
 void NodeRhs(const Oracle* o, const cplx* x, int i, cplx* drho) {
   const int N = o->N;
   const int NE = N * N;
@@ -344,12 +382,14 @@ void NodeRhs(const Oracle* o, const cplx* x, int i, cplx* drho) {
         drho[a + b * N] = c;
       }
     }
-    if (o->vacuum) return;
+    if (o->vacuum && !o->tl_on) return;
+    const bool tl = o->tl_on && nd.tier == o->Lh - 1;
     // DoDephasing.
     const cplx mu(nd.dephasing, 0);
     for (int n = 0; n < NE; ++n) drho[n] -= mu * rho[n];
     if (o->bath_model != 0) {
       NodeRhsBath(o, x, nd, rho, drho);
+      if (tl) NodeTimeLocal(o, rho, drho);
       return;
     }
     // IndependentBath::DoMatsubara (single bath per state).
@@ -361,6 +401,7 @@ void NodeRhs(const Oracle* o, const cplx* x, int i, cplx* drho) {
       const int d = (N + 1) * m;
       drho[d] += cplx(2, 0) * o->matsubara[m] * rho[d];
     }
+    if (tl) NodeTimeLocal(o, rho, drho);
   }
   // DoNext: column m += p rho_next(:, m); row m -= p rho_next(m, :).
   const size_t nn = nd.next_node.size();
@@ -638,6 +679,18 @@ void oracle_set_hamiltonian(void* h, const double* H) {
   std::memcpy((void*)o->H.data(), H, (size_t)N * N * sizeof(cplx));
   for (int i = 0; i < N; ++i)
     for (int j = 0; j < N; ++j) o->Hadj[i * N + j] = std::conj(o->H[j * N + i]);
+}
+
+// Time-local truncation matrices Q_m (M x N x N column-major interleaved
+// complex), as UpdateTimeLocalTruncation leaves them between stages
+// (hierarchy_integrator.cc:1675-1731); null switches the term off.
+void oracle_set_timelocal(void* h, const double* Q) {
+  Oracle* o = (Oracle*)h;
+  o->tl_on = Q != nullptr;
+  const size_t n = (size_t)o->M * o->N * o->N;
+  o->tl_Q.assign(n, cplx(0, 0));
+  if (Q)
+    for (size_t e = 0; e < n; ++e) o->tl_Q[e] = cplx(Q[2 * e], Q[2 * e + 1]);
 }
 
 void oracle_get_active(void* h, int* out) {

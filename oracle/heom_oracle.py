@@ -75,6 +75,7 @@ def lib():
         L.oracle_get_active.argtypes = [vp, ip]
         L.oracle_set_hamiltonian.argtypes = [vp, dp]
         L.oracle_set_bath.argtypes = [vp, ctypes.c_int, dp]
+        L.oracle_set_timelocal.argtypes = [vp, dp]
         L.oracle_rhs.argtypes = [vp, dp, dp]
         L.oracle_steady_matvec.argtypes = [vp, dp, dp]
         L.oracle_rk4.restype = ctypes.c_longlong
@@ -215,6 +216,15 @@ class Oracle:
         """Replace the N x N system Hamiltonian (H[i, j] = element (i, j))."""
         self._hnew = np.ascontiguousarray(np.asarray(H, dtype=complex).T).view(np.float64)
         lib().oracle_set_hamiltonian(self.h, _dp(self._hnew.reshape(-1)))
+
+    def set_timelocal(self, Q):
+        """Time-local truncation matrices Q (M, N, N) with Q[m, i, j] =
+        element (i, j); None switches the term off."""
+        if Q is None:
+            lib().oracle_set_timelocal(self.h, None)
+            return
+        self._tl = np.ascontiguousarray(np.asarray(Q, dtype=complex).transpose(0, 2, 1))
+        lib().oracle_set_timelocal(self.h, _dp(self._tl.view(np.float64).reshape(-1)))
 
     def active(self):
         out = np.zeros(self.num_nodes, dtype=np.int32)

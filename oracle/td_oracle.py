@@ -149,9 +149,70 @@ class Hamiltonian:
         return self.V @ a @ self.V.conj().T
 
 
-def rk4(orc, ham, t_start, t_end, dt):
+class TimeLocal:
+    """PHI's time-local truncation matrices (hierarchy.cc:264-341,
+    hierarchy_integrator.cc:1675-1731).  For bath m and exponential term k
+
+        T_mk <- exp(-nu_k ts) P^dagger (T_mk + ts c_mk V_m) P,  P = 1 - i H ts,
+
+    with H the current (truncated) Hamiltonian as stored (PHI does not
+    divide it by hbar here), nu_0 = gamma_m, nu_k = 2 pi k kB T / hbar and
+    c_mk the correlation constants; Q_m = sum_k T_mk.  V_m is the matrix
+    with a one at element (m, m) for IndependentBath (the bath's own index,
+    AddIndependentBathOperator), diag(b_m) for DiagonalBath and the current
+    F_m for FullBath.  Steps shorter than 1e-9 are skipped (:1678).  All
+    matrices start at zero."""
+
+    def __init__(self, orc):
+        prob = orc.prob
+        c, _, _ = orc.constants()
+        self.M, self.N = prob.M, prob.N
+        self.K = len(c) // self.M
+        self.c = c.reshape(self.M, self.K)
+        nu = 2 * np.pi * prob.kb * prob.temperature / prob.hbar
+        self.rate = np.array([[prob.gamma[m] if k == 0 else nu * k for k in range(self.K)]
+                              for m in range(self.M)])
+        self.model = prob.bath_model
+        self.ops = prob.bath_ops
+        self.T = np.zeros((self.M, self.K, self.N, self.N), dtype=complex)
+        self.saved = self.T.copy()
+
+    def _seed(self, m, ops):
+        if self.model == "independent":
+            V = np.zeros((self.N, self.N), dtype=complex)
+            V[m, m] = 1
+            return V
+        if self.model == "diagonal":
+            return np.diag(self.ops[m])
+        return (self.ops if ops is None else ops)[m]
+
+    def step(self, H, ts, ops=None):
+        if ts < 1e-9:
+            return
+        P = np.eye(self.N) - 1j * np.asarray(H) * ts
+        Pd = P.conj().T
+        for m in range(self.M):
+            V = self._seed(m, ops)
+            for k in range(self.K):
+                X = self.T[m, k] + (ts * self.c[m, k]) * V
+                self.T[m, k] = np.exp(-self.rate[m, k] * ts) * (Pd @ X @ P)
+
+    def Q(self):
+        return self.T.sum(axis=1)
+
+    def store(self):
+        self.saved = self.T.copy()
+
+    def restore(self):
+        self.T = self.saved.copy()
+
+
+def rk4(orc, ham, t_start, t_end, dt, tl=None):
     """PHI's RK4 loop with H updates (classic stage order); the oracle's
-    state is advanced in place.  Returns the number of steps."""
+    state is advanced in place.  With time-local truncation (tl, a
+    TimeLocal) Q advances by dt/2 after k0 (with H(t)) and after k2 (with
+    H(t + dt/2)), before the H updates (hierarchy_integrator.cc:2721,
+    2784).  Returns the number of steps."""
     rot = lambda x: ham.R @ x @ ham.R.conj().T if ham.truncated else x  # noqa: E731
 
     def push():
@@ -159,7 +220,14 @@ def rk4(orc, ham, t_start, t_end, dt):
         if ham.ops is not None:
             orc.set_bath("full", ham.ops)
 
+    def tl_step(ts):
+        if tl is not None:
+            tl.step(ham.H, ts, ham.ops)
+            orc.set_timelocal(tl.Q())
+
     push()
+    if tl is not None:
+        orc.set_timelocal(tl.Q())
     t = t_start
     steps = 0
     rho = orc.state()
@@ -170,6 +238,7 @@ def rk4(orc, ham, t_start, t_end, dt):
                 rho = rot(rho)
         k0 = orc.rhs(rho)
         y = rho + k0 * (dt / 2)
+        tl_step(dt / 2)
         if ham.update(t + dt / 2):
             push()
             if ham.truncated:
@@ -178,6 +247,7 @@ def rk4(orc, ham, t_start, t_end, dt):
         y = rho + k1 * (dt / 2)
         k2 = orc.rhs(y)
         y = rho + k2 * dt
+        tl_step(dt / 2)
         if ham.update(t + dt):
             push()
             if ham.truncated:
@@ -198,14 +268,19 @@ _C4 = [1. / 9, 9. / 20, 16. / 45, 1. / 12]              # k0, k2, k3, k4
 _C5 = [47. / 450, 12. / 25, 32. / 225, 1. / 30, 6. / 25]  # k0, k2, k3, k4, k5
 
 
-def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt):
+def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt, tl=None):
     """PHI's RKF45 loop (hierarchy_integrator.cc:1797-2573) with H updates
     at t and t + a_s dt before each stage, rotating rho, the stage input and
     the live k's (k1 is not rotated before the last stage, :2257-2263).
     The weights are multiplied by dt once and then rescaled in place by the
     same factors as dt, the end clamp replaces the growth factor, a step is
-    accepted when err < tol or dt * 0.5 < mindt.  Returns (attempts,
-    accepted, final time)."""
+    accepted when err < tol or dt * 0.5 < mindt.  With time-local
+    truncation (tl) Q advances before each H update of stages 1-4 by the
+    stage-time increment (a1, a2 - a1, a3 - a2, a4 - a3), is restored to its
+    value at t and advanced by a5 before stage 5, advances by dt - a5 and is
+    saved after an accepted step, and is restored after a rejected one
+    (:2028-2247, 2386-2389, 2476-2480).  Returns (attempts, accepted, final
+    time)."""
     rot = lambda x: ham.R @ x @ ham.R.conj().T if ham.truncated else x  # noqa: E731
 
     def push():
@@ -213,7 +288,14 @@ def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt):
         if ham.ops is not None:
             orc.set_bath("full", ham.ops)
 
+    def tl_step(ts):
+        if tl is not None:
+            tl.step(ham.H, ts, ham.ops)
+            orc.set_timelocal(tl.Q())
+
     push()
+    if tl is not None:
+        orc.set_timelocal(tl.Q())
     mydt = dt0
     dt_f = dt0
     dtup, dtdn = 1.1, 0.5
@@ -235,6 +317,11 @@ def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt):
             y = rho.copy()
             for j in range(s + 1):
                 y = y + k[j] * b[s][j]
+            if s < 4:
+                tl_step(a[s] - a[s - 1] if s > 0 else a[0])
+            elif tl is not None:
+                tl.restore()
+                tl_step(a[4])
             if ham.update(t + a[s]):
                 push()
                 if ham.truncated:
@@ -246,6 +333,9 @@ def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt):
         err = float(np.abs(y4 - y5).max())
         if err < tol or dt_f * dtdn < mindt:
             rho = y5
+            if tl is not None:
+                tl_step(mydt - a[4])
+                tl.store()
             t += mydt
             accepted += 1
             update = err < tol / 10
@@ -262,6 +352,9 @@ def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt):
                 c5 = [x * f for x in c5]
             dt_f = mydt
         else:
+            if tl is not None:
+                tl.restore()
+                orc.set_timelocal(tl.Q())
             mydt *= dtdn
             a = [x * dtdn for x in a]
             b = [[x * dtdn for x in row] for row in b]
