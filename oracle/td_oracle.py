@@ -293,6 +293,8 @@ def rkf45(orc, ham, t_start, t_end, dt0, tol, mindt, tl=None):
             tl.step(ham.H, ts, ham.ops)
             orc.set_timelocal(tl.Q())
 
+    if not dt0 > 0:
+        raise ValueError("the RKF45 timestep must be positive")
     push()
     if tl is not None:
         orc.set_timelocal(tl.Q())

@@ -24,7 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB_SOURCES = ["phi_parameters.cc", "hierarchy.cc", "engine.cc", "engine_closure.cc",
-               "engine_filter.cc", "engine_td.cc", "system_hamiltonian.cc", "mlp.cc",
+               "engine_filter.cc", "engine_td.cc", "engine_timelocal.cc", "timelocal.cc",
+               "system_hamiltonian.cc", "mlp.cc",
                "integrator.cc", "capi.cc",
                "heom_kernels.hip", "heom_rk4_f64.hip", "heom_rk4_f32.hip",
                "heom_rkf45_f64.hip", "heom_rkf45_f32.hip", "mlp_kernels.hip",

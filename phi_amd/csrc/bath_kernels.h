@@ -32,6 +32,23 @@ struct BathArgs {
 hipError_t LaunchRhsBath(const RhsArgs& a, const BathArgs& b, const double* x, double* y,
                          hipStream_t s);
 
+// Time-local truncation term of PHI's DoTimelocal (hierarchy_updater.cc:
+// 28-43, 94-107, 170-213), added onto y for the nodes [first, first+count)
+// (the deepest tier):
+//   X_m = Q_m x_n - x_n Q_m^dagger,   y_n -= sum_m [V_m, X_m],
+// V_m the projector on site[m] (model 0), diag(ops_m) (1) or ops_m (2).
+struct TimeLocalArgs {
+  int model;              // 0 site projectors, 1 diagonal, 2 full
+  int M;
+  int first, count;       // node range
+  const double2* Q;       // M * N * N, column-major
+  const double2* ops;     // M * N (diagonal) or M * N * N (full)
+  const int* site;        // M (site projectors)
+};
+
+hipError_t LaunchTimeLocal(int N, const TimeLocalArgs& t, const double* x, double* y,
+                           hipStream_t s);
+
 }  // namespace phi
 
 #endif  // PHI_AMD_BATH_KERNELS_H_

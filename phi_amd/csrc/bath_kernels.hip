@@ -137,7 +137,84 @@ __global__ void __launch_bounds__(256) rhs_bath(RhsArgs A, BathArgs B,
   }
 }
 
+// One block per node of the range; x_n, Q_m, X_m (and F_m) in LDS.
+__global__ void __launch_bounds__(256) timelocal_term(int N, TimeLocalArgs T,
+                                                      const double2* __restrict__ x,
+                                                      double2* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int NE = N * N;
+  double2* xs = reinterpret_cast<double2*>(smem);
+  double2* qs = xs + NE;
+  double2* Xs = qs + NE;
+  double2* fs = Xs + NE;
+  for (int b = blockIdx.x; b < T.count; b += gridDim.x) {
+    const size_t n = (size_t)T.first + b;
+    __syncthreads();
+    for (int e = threadIdx.x; e < NE; e += blockDim.x) xs[e] = x[n * NE + e];
+    double2 out[kMaxPerThread];
+#pragma unroll
+    for (int i = 0; i < kMaxPerThread; ++i) out[i] = make_double2(0, 0);
+    for (int m = 0; m < T.M; ++m) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+        qs[e] = T.Q[(size_t)m * NE + e];
+        if (T.model == 2) fs[e] = T.ops[(size_t)m * NE + e];
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+        const int r = e % N, c = e / N;
+        double2 s = make_double2(0, 0);
+        for (int k = 0; k < N; ++k) {
+          s = cadd(s, cmul(qs[r + k * N], xs[k + c * N]));
+          s = csub(s, cmul(xs[r + k * N], cconj(qs[c + k * N])));
+        }
+        Xs[e] = s;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kMaxPerThread; ++i) {
+        const int e = threadIdx.x + i * blockDim.x;
+        if (e >= NE) continue;
+        const int r = e % N, c = e / N;
+        if (T.model == 0) {
+          const int st = T.site[m];
+          if (c == st) out[i] = cadd(out[i], Xs[e]);
+          if (r == st) out[i] = csub(out[i], Xs[e]);
+        } else if (T.model == 1) {
+          const double2 d = csub(T.ops[(size_t)m * N + r], T.ops[(size_t)m * N + c]);
+          out[i] = csub(out[i], cmul(d, Xs[e]));
+        } else {
+          double2 s = make_double2(0, 0);
+          for (int k = 0; k < N; ++k) {
+            s = cadd(s, cmul(fs[r + k * N], Xs[k + c * N]));
+            s = csub(s, cmul(Xs[r + k * N], fs[k + c * N]));
+          }
+          out[i] = csub(out[i], s);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxPerThread; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < NE) y[n * NE + e] = cadd(y[n * NE + e], out[i]);
+    }
+  }
+}
+
 }  // namespace dev
+
+hipError_t LaunchTimeLocal(int N, const TimeLocalArgs& t, const double* x, double* y,
+                           hipStream_t s) {
+  if (N < 1 || N * N > 256 * dev::kMaxPerThread || t.model < 0 || t.model > 2 || t.M < 1 ||
+      !t.Q || (t.model != 0 && !t.ops) || (t.model == 0 && !t.site))
+    return hipErrorInvalidValue;
+  if (t.count <= 0) return hipSuccess;
+  const size_t sh = 4 * (size_t)N * N * sizeof(double2);
+  const int grid = t.count < 65535 ? t.count : 65535;
+  hipLaunchKernelGGL(dev::timelocal_term, dim3(grid), dim3(256), sh, s, N, t, (const double2*)x,
+                     (double2*)y);
+  return hipGetLastError();
+}
 
 hipError_t LaunchRhsBath(const RhsArgs& a, const BathArgs& b, const double* x, double* y,
                          hipStream_t s) {

@@ -131,6 +131,15 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
       if (!rc) rc = e->Synchronize();
     }
   }
+  e->h_hcur_.assign(p.time_indepenent_hamiltonian.begin(), p.time_indepenent_hamiltonian.end());
+  if (!rc && p.use_time_local_truncation) {
+    if ((int)e->h_hcur_.size() != NE) {
+      e->err_ = "time-local truncation needs the system Hamiltonian";
+      rc = kErrInvalidParameters;
+    } else {
+      rc = e->InitTimeLocal(p);
+    }
+  }
   if (rc) {
     *why = e->err_;
     delete e;
@@ -240,8 +249,8 @@ int Engine::CreateEnsemble(const PhiParameters& base, int members,
     return kErrUnsupported;
   }
   if (e->explicit_path()) {
-    *why = "FilterTolerance and diagonal/full coupling operators are not available for an "
-           "ensemble";
+    *why = "FilterTolerance, diagonal/full coupling operators and time-local truncation are "
+           "not available for an ensemble";
     delete e;
     return kErrUnsupported;
   }
@@ -361,10 +370,12 @@ Engine::~Engine() {
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
                   d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_active_, d_kf_[0], d_kf_[1],
                   d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_kraw_, d_ym_, d_rot_,
-                  d_ops_, d_alpha_, d_corr_};
+                  d_ops_, d_alpha_, d_corr_, d_tlq_, d_tlsite_};
   for (void* p : ptrs) Free(p);
   if (h_stage_) (void)hipHostFree(h_stage_);
   if (h_ops_) (void)hipHostFree(h_ops_);
+  if (h_tlstage_) (void)hipHostFree(h_tlstage_);
+  delete tl_;
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -476,8 +487,8 @@ std::map<TuneKey, TuneVal> g_tune;
 
 int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   if (explicit_path()) {
-    err_ = "the autotuner covers the fused stage kernels (not adaptive truncation or "
-           "diagonal/full coupling operators)";
+    err_ = "the autotuner covers the fused stage kernels (not adaptive truncation, "
+           "diagonal/full coupling operators or time-local truncation)";
     return kErrUnsupported;
   }
   if (steps < 1) steps = 1;
@@ -741,8 +752,8 @@ int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int sna
 
 int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   if (explicit_path()) {
-    err_ = "stage timing covers the fused stage kernels (not adaptive truncation or "
-           "diagonal/full coupling operators)";
+    err_ = "stage timing covers the fused stage kernels (not adaptive truncation, "
+           "diagonal/full coupling operators or time-local truncation)";
     return kErrUnsupported;
   }
   int rc = AllocBulk();
@@ -882,6 +893,12 @@ int Engine::Rkf45Attempt(const Rkf45Coefs& c, double* err, double t, const doubl
 
 int Engine::Rkf45Accept() {
   std::swap(d_rho_, d_aux_);
+  if (tl_) {
+    // Time-local matrices to t + dt (a4 = dt), saved for the next step.
+    int rc = TlStep(tl_a_[3] - tl_a_[4]);
+    tl_->Store();
+    if (rc) return rc;
+  }
   if (!filter_on()) return kOk;
   return Check(LaunchUpdateActive(d_rho_, d_active_, N() * N(), num_nodes(), filter_, stream_),
                "flag update");
@@ -892,6 +909,12 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   if (members_) return Unsupported("the steady state");
   if (bath_model_ != 0) {
     err_ = "the steady state with diagonal/full coupling operators is not supported";
+    return kErrUnsupported;
+  }
+  if (tl_) {
+    // PHI switches the option off for the steady state (phi_parameters.cc:
+    // 833-837); the C ABI can still ask for it.
+    err_ = "the steady state with time-local truncation is not supported";
     return kErrUnsupported;
   }
   // Unpreconditioned complex BiCGSTAB (van der Vorst 1992) on A x = b with

@@ -14,6 +14,7 @@
 #include "heom_kernels.h"
 #include "hierarchy.h"
 #include "phi_parameters.h"
+#include "timelocal.h"
 
 namespace phi {
 
@@ -171,7 +172,15 @@ class Engine {
   int bath_model() const { return bath_model_; }
   int SetBathOps(const std::complex<double>* ops);
   // Runs that take the explicit-k RK4/RKF45 path (engine_filter.cc).
-  bool explicit_path() const { return filter_on() || bath_model_ != 0; }
+  bool explicit_path() const { return filter_on() || bath_model_ != 0 || tl_ != nullptr; }
+
+  // Time-local truncation (PHI's TimeLocal; engine_timelocal.cc): the
+  // deepest tier gets the time-local term, the matrices advance on PHI's
+  // stage schedule inside the explicit-k RK4 step and RKF45 attempt.
+  // Null without the option.  The saved matrices are what restart files
+  // hold.
+  TimeLocal* timelocal() { return tl_; }
+  int TlUpload();
 
   // Steady state: BiCGSTAB on A x = b (A = L with the trace row), from the
   // current state.  Returns kOk or kErrNotConverged; *iters, *resid report.
@@ -296,6 +305,21 @@ class Engine {
   int ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSource* src);
   int ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err, double t, const double* a,
                            HamiltonianSource* src);
+
+  // Time-local truncation: host matrices, device Q (M N^2), site of each
+  // projector bath, pinned staging ring, the current H and coupling
+  // operators on the host (the matrices' propagator and seeds), and the
+  // stage times of the last RKF45 attempt.
+  TimeLocal* tl_ = nullptr;
+  double2* d_tlq_ = nullptr;
+  int* d_tlsite_ = nullptr;
+  std::complex<double>* h_tlstage_ = nullptr;
+  int tl_slot_ = 0;
+  std::vector<std::complex<double>> h_hcur_, h_opscur_;
+  double tl_a_[5] = {0, 0, 0, 0, 0};
+  int InitTimeLocal(const PhiParameters& p);
+  int TlStep(double ts);
+  int TlTerm(const double* x, double* y);
 
   // Time-dependent Hamiltonian: pinned staging of H (three slots per step,
   // reused after the step's synchronisation) and the device rotation.

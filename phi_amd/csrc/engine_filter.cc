@@ -47,11 +47,15 @@ int Engine::ExplicitAlloc(int nk) {
 }
 
 int Engine::OperatorRhs(const double* x, double* y) {
+  int rc;
   if (bath_model_ != 0) {
     BathArgs b{bath_model_, h_.M, h_.K, d_ops_, d_alpha_, d_corr_};
-    return Check(LaunchRhsBath(args_, b, x, y, stream_), "rhs_bath");
+    rc = Check(LaunchRhsBath(args_, b, x, y, stream_), "rhs_bath");
+  } else {
+    rc = Check(LaunchRhsGeneral(args_, x, y, 0, stream_), "rhs_general");
   }
-  return Check(LaunchRhsGeneral(args_, x, y, 0, stream_), "rhs_general");
+  if (!rc && tl_) rc = TlTerm(x, y);
+  return rc;
 }
 
 int Engine::ExplicitRhs(const double* in, int s) {
@@ -78,8 +82,8 @@ int Engine::ExplicitUpdate(HamiltonianSource* src, double t, int nbuf, void* con
 
 int Engine::ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSource* src) {
   if (fp32_) {
-    err_ = "adaptive truncation and diagonal/full coupling operators run in FP64 only "
-           "(BulkPrecision=fp64)";
+    err_ = "adaptive truncation, diagonal/full coupling operators and time-local "
+           "truncation run in FP64 only (BulkPrecision=fp64)";
     return kErrUnsupported;
   }
   int rc = ExplicitAlloc(4);
@@ -95,6 +99,7 @@ int Engine::ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSourc
   if (!rc) rc = ExplicitRhs(d_rho_, 0);
   const double* k0[1] = {d_kf_[0]};
   if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k0, &half, 1, y, n, stream_), "combine");
+  if (!rc && tl_) rc = TlStep(dt / 2);  // Q(t + dt/2), with H(t)
   if (!rc) {
     void* b[3] = {d_rho_, y, d_kf_[0]};
     rc = ExplicitUpdate(src, t + dt / 2, 3, b);
@@ -105,6 +110,7 @@ int Engine::ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSourc
   if (!rc) rc = ExplicitRhs(y, 2);
   const double* k2[1] = {d_kf_[2]};
   if (!rc) rc = Check(LaunchFilterCombine(d_rho_, k2, &dt, 1, y, n, stream_), "combine");
+  if (!rc && tl_) rc = TlStep(dt / 2);  // Q(t + dt), with H(t + dt/2)
   if (!rc) {
     void* b[5] = {d_rho_, y, d_kf_[0], d_kf_[1], d_kf_[2]};
     rc = ExplicitUpdate(src, t + dt, 5, b);
@@ -118,10 +124,11 @@ int Engine::ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSourc
   if (!rc && snapshot)
     rc = Check(LaunchSnapshot(d_rho_, d_hist_, d_hist_count_, N() * N(), hist_capacity_, stream_),
                "snapshot");
-  if (!rc && src) {
+  if (!rc && (src || tl_)) {
     // The step's staging slots are reused by the next step.
     rc = Synchronize();
     stage_slot_ = 0;
+    tl_slot_ = 0;
   }
   return rc;
 }
@@ -129,8 +136,8 @@ int Engine::ExplicitRk4Step(double t, double dt, bool snapshot, HamiltonianSourc
 int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err, double t, const double* a,
                                  HamiltonianSource* src) {
   if (fp32_) {
-    err_ = "adaptive truncation and diagonal/full coupling operators run in FP64 only "
-           "(BulkPrecision=fp64)";
+    err_ = "adaptive truncation, diagonal/full coupling operators and time-local "
+           "truncation run in FP64 only (BulkPrecision=fp64)";
     return kErrUnsupported;
   }
   int rc = ExplicitAlloc(6);
@@ -138,10 +145,30 @@ int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err, double t, con
   const size_t n = state_elems();
   double* y = (double*)d_y_[0];
   rc = Check(LaunchFillU64(d_err_, 0ull, 1, stream_), "zero err");
+  if (tl_) {
+    if (!a) {
+      err_ = "time-local truncation needs the RKF45 stage times";
+      return kErrInvalidArgument;
+    }
+    for (int i = 0; i < 5; ++i) tl_a_[i] = a[i];
+    tl_->Restore();
+    if (!rc) rc = TlUpload();
+  }
+  // Time-local matrices before the H update of stage s: advanced by the
+  // stage-time increment (s = 1..4), restored and advanced by a5 (s = 5).
+  auto tl_advance = [&](int s) -> int {
+    if (!tl_ || s == 0) return kOk;
+    if (s == 5) {
+      tl_->Restore();
+      return TlStep(a[4]);
+    }
+    return TlStep(s == 1 ? a[0] : a[s - 1] - a[s - 2]);
+  };
   // H at t (rho live) and at t + a[s-1] before stage s (rho, Y and
   // k0..k(s-1) live, k1 excepted before stage 5).
   auto update = [&](int s) -> int {
-    if (!src) return kOk;
+    int e = tl_advance(s);
+    if (e || !src) return e;
     void* b[6];
     int nb = 0;
     b[nb++] = d_rho_;
@@ -176,6 +203,7 @@ int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err, double t, con
   if (!rc) rc = Synchronize();
   // The attempt's staging slots are free again.
   stage_slot_ = 0;
+  tl_slot_ = 0;
   if (rc) return rc;
   std::memcpy(err, &bits, sizeof(double));
   return kOk;
@@ -184,6 +212,7 @@ int Engine::ExplicitRkf45Attempt(const Rkf45Coefs& c, double* err, double t, con
 int Engine::SetBathOps(const std::complex<double>* ops) {
   if (bath_model_ == 0) return kOk;
   const size_t n = (size_t)h_.M * (bath_model_ == 1 ? N() : N() * N());
+  h_opscur_.assign(ops, ops + n);
   int rc = kOk;
   if (!h_ops_) {
     rc = Check(hipHostMalloc((void**)&h_ops_, 3 * n * sizeof(double2)), "pinned operators");

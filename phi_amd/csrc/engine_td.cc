@@ -39,6 +39,7 @@ static constexpr int kSlotsPerStep = 3;
 
 int Engine::SetHamiltonian(const std::complex<double>* H) {
   const int N = this->N(), NE = N * N;
+  h_hcur_.assign(H, H + NE);
   int rc = kOk;
   if (!h_stage_) {
     rc = Check(hipHostMalloc((void**)&h_stage_, (size_t)kSlotsPerStep * 4 * NE * sizeof(double2)),
@@ -96,9 +97,9 @@ int Engine::Rk4StepsTD(long long nsteps, const double* t_before, double dt, long
   }
   int rc = AllocBulk();
   if (rc) return rc;
-  if (bath_model_ != 0) {
-    // Diagonal / full coupling operators: the explicit-k step applies the
-    // updates itself (engine_filter.cc).
+  if (explicit_path()) {
+    // Diagonal / full coupling operators, time-local truncation: the
+    // explicit-k step applies the updates itself (engine_filter.cc).
     for (long long s = 0; s < nsteps && !rc; ++s)
       rc = ExplicitRk4Step(t_before[s], dt, snap_every > 0 && (step_offset + s) % snap_every == 0,
                            src);

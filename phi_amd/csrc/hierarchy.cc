@@ -39,10 +39,13 @@ std::string BuildHierarchy(const PhiParameters& p, HierarchyTables* h) {
   h->D = M * K;
   h->Lh = p.hierarchy_truncation_level;
   h->L = h->Lh;
-  h->vacuum = h->Lh == 1;
+  // One tier is PHI's Vacuum updater, unless time-local truncation turns it
+  // into the NoHierarchy* updaters (DoUnitary, DoMatsubara, DoTimelocal;
+  // hierarchy_integrator.cc:1741-1758).
+  h->vacuum = h->Lh == 1 && !p.use_time_local_truncation;
   const int D = h->D;
   const int Lh = h->Lh;
-  if (!h->vacuum && M < N)
+  if (!h->vacuum && M < N && !p.is_diagonal_bath_coupling && !p.is_full_bath_coupling)
     return "IndependentBath needs one bath per state (NumCouplingTerms >= NumStates)";
   if (Lh > 255) return "HierarchyTruncation above 255 is not supported";
 

@@ -208,10 +208,6 @@ void HierarchyIntegrator::Launch() {
 
 int HierarchyIntegrator::Initialize() {
   const PhiParameters& p = *p_;
-  if (p.use_time_local_truncation) {
-    Fail(kErrUnsupported, "time-local truncation is not supported");
-    return info_.status;
-  }
   std::string why;
   const PhiParameters* engine_params = &p;
   if (p.time_indepenent_hamiltonian.empty() || p.is_truncated_system) {
@@ -450,6 +446,14 @@ int HierarchyIntegrator::RunRkf45() {
   double mydt = p.timestep, dt_f = p.timestep;
   double dtup = 1.1;
   const double dtdn = 0.5;
+  if (!(mydt > 0)) {
+    // time_now += dt would never reach the end.
+    Fail(kErrInvalidParameters, "the RKF45 timestep must be positive");
+    return info_.status;
+  }
+  // Step size before the end clamp shrank it (the restart file's dt).
+  double free_dt = mydt;
+  bool clamped = false;
   // Fehlberg weights (hierarchy_integrator.cc:1926-1975), each pre-multiplied
   // by the initial dt and then rescaled in place exactly as PHI does.
   double b[5][5] = {{2. / 9., 0, 0, 0, 0},
@@ -499,6 +503,8 @@ int HierarchyIntegrator::RunRkf45() {
       accepted += 1;
       bool update_dt = max_diff < tol / 10;
       if (time_now + mydt > end_time) {
+        if (!clamped) free_dt = update_dt ? mydt * dtup : mydt;
+        clamped = true;
         update_dt = true;
         dtup = (end_time - time_now) / mydt;
       }
@@ -563,7 +569,10 @@ int HierarchyIntegrator::RunRkf45() {
   }
   WriteFinal(r0.data(), time_now);
   FullPrecisionLine(time_now, r0.data());
-  if (p.restart_output_time > 0) return WriteRestart(time_now, dt_f);
+  // PHI writes dt_f, which the end clamp has shrunk to about (end - t) ~ 0,
+  // so a run resumed from its file does not advance; the step size the
+  // controller held before the clamp is written instead.
+  if (p.restart_output_time > 0) return WriteRestart(time_now, clamped ? free_dt : dt_f);
   return kOk;
 }
 
@@ -639,6 +648,16 @@ int HierarchyIntegrator::WriteRestart(double t, double dt) {
     for (size_t i = 0; i < ne; ++i) o << (i ? " " : "") << s[n * ne + i];
     o << "\n";
   }
+  // Time-local truncation: the saved matrices T_mk, one line each, bath
+  // major (restart_helper.cc:43-52).  PHI saves them after accepted RKF45
+  // steps only, so an RK4 run writes zeros here, as PHI's does.
+  if (TimeLocal* tl = engine_->timelocal()) {
+    const std::vector<std::complex<double>>& T = tl->saved();
+    for (size_t mk = 0; mk < (size_t)tl->M() * tl->K(); ++mk) {
+      for (size_t i = 0; i < ne; ++i) o << (i ? " " : "") << T[mk * ne + i];
+      o << "\n";
+    }
+  }
   return o.good() ? kOk : kErrIo;
 }
 
@@ -651,11 +670,20 @@ int HierarchyIntegrator::ReadRestart(double* t, double* dt) {
   in >> *t >> *dt;
   std::vector<std::complex<double>> s(engine_->state_elems());
   for (auto& v : s) in >> v;
+  // Time-local matrices: read as saved, then current = saved
+  // (hierarchy_integrator.cc:1060-1068).
+  TimeLocal* tl = engine_->timelocal();
+  if (tl)
+    for (auto& v : tl->saved()) in >> v;
   if (!in) {
     Fail(kErrIo, "restart file " + p_->restart_input_filename + " is truncated or malformed");
     return kErrIo;
   }
   int rc = engine_->SetState(s.data());
+  if (!rc && tl) {
+    tl->Restore();
+    rc = engine_->TlUpload();
+  }
   if (rc) {
     Fail(rc, engine_->last_error());
     return rc;
