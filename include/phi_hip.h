@@ -200,8 +200,9 @@ int phi_hip_get_state(phi_ctx* c, double* state, size_t n_complex);
 int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n_complex);
 /* out = L(in) for every node (PHI's Get, any complex input); host arrays.
  * With FilterTolerance > 0 the current node flags apply: inactive nodes and
- * neighbours contribute nothing, and an inactive node's out holds only its
- * neighbour terms (PHI's Get adds them onto drho, here onto zero). */
+ * neighbour terms (PHI's Get adds them onto drho, here onto zero).  With
+ * TimeLocal the deepest tier includes the time-local term with the current
+ * matrices (zero until an integrator run advances them). */
 int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
 /* nsteps RK4 steps of size dt on the device state. */
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
@@ -212,7 +213,9 @@ int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
 int phi_hip_rk4_stage_times(phi_ctx* c, long long nsteps, double dt, double* ms);
 /* One RKF45 attempt.  coefs: 15 b's (b10, b20, b21, ..., b54), 4 c's of y4
  * (c50, c52, c53, c54) and 5 of y5 (c60, c62, c63, c64, c65), all already
- * multiplied by dt.  *err = max |y4 - y5|.  Accept keeps y5. */
+ * multiplied by dt.  *err = max |y4 - y5|.  Accept keeps y5.  Time-local
+ * truncation needs the stage times and runs through the integrator only
+ * (PHI_HIP_EINVAL here). */
 int phi_hip_rkf45_attempt(phi_ctx* c, const double* coefs, double* err);
 int phi_hip_rkf45_accept(phi_ctx* c);
 /* Adaptive hierarchy truncation (FilterTolerance > 0): the activity flag of
