@@ -15,8 +15,14 @@
 //                                              hierarchy_updater.cc:129-144,
 //                                              216-254 (NOBLAS Gemm order,
 //                                              blas_wrapper.cc:11-56)
-//   * Shi filter RHS (adaptive truncation) ... hierarchy_updater.h:217-245,
-//                                              hierarchy_updater.cc:256-304
+//   * Shi filter RHS (adaptive truncation) ... hierarchy_updater.h:36-52,
+//                                              hierarchy_updater.cc:256-304:
+//     HierarchyUpdater::Get is not virtual and the integrator calls it
+//     through a HierarchyUpdater* (hierarchy_integrator.cc:1798, 2011), so
+//     the base Get runs: every node's own terms (DoUnitary overwrites drho),
+//     and the adaptive DoNext / DoPrevious, which skip inactive neighbours.
+//     IndependentBathAdaptiveTruncation::Get (hierarchy_updater.h:227-239)
+//     is never called.
 //   * Vacuum updater (Lh == 1)             ... hierarchy_updater.h:251-265
 //   * RK4 loop (step count, update order)  ... hierarchy_integrator.cc:2578-2939
 //   * RKF45 loop and dt controller         ... hierarchy_integrator.cc:1797-2573
@@ -80,9 +86,7 @@ struct Oracle {
   std::vector<cplx> matsubara;    // Delta_j, M entries
   std::vector<Node> nodes;
   std::vector<cplx> rho;          // nodes * N * N, PHI id order
-  // Persistent per-stage k buffers: PHI keeps k0..k5 across steps, and the
-  // Shi-filter updater adds onto them without clearing for inactive nodes
-  // (hierarchy_updater.h:226-240), so their contents carry over.
+  // Per-stage k buffers (PHI's k0..k5; Get overwrites them every stage).
   std::vector<cplx> kbuf[6];
   int threads = 1;
 };
@@ -352,15 +356,18 @@ void NodeTimeLocal(const Oracle* o, const cplx* rho, cplx* drho) {
 // hierarchy_updater.h:36-75 and hierarchy_updater.cc:129-144, 216-304.
 // drho for node i from the ADO array x (PHI id order, column-major blocks).
 // Arithmetic order follows the NOBLAS Gemm (blas_wrapper.cc:11-31).  With
-// time-local truncation and one tier PHI's NoHierarchy* updaters apply
-// DoUnitary, DoMatsubara and DoTimelocal (hierarchy_updater.h:266-330).
+// time-local truncation and one tier the base Get applies DoUnitary,
+// DoDephasing (mu_0 = 0), DoMatsubara and DoTimelocal, which is also what
+// the (never called) NoHierarchy* Get would (hierarchy_updater.h:266-345).
+// Under the Shi filter every node keeps its own terms and inactive
+// neighbours are skipped.
 void NodeRhs(const Oracle* o, const cplx* x, int i, cplx* drho) {
   const int N = o->N;
   const int NE = N * N;
   const Node& nd = o->nodes[i];
   const cplx* rho = x + (size_t)i * NE;
   const bool adaptive = o->filter > 0 && !o->vacuum;
-  if (!adaptive || nd.active) {
+  {
     // DoUnitary: Hemm(Right, -i/hbar, rho, H) -> Gemm(H, rho);
     //            Hemm(Left, +i/hbar, rho, H^dagger, beta=1) -> Gemm(rho, H^dag).
     const cplx alpha(0, -1. / o->hbar), beta(0, 1. / o->hbar);
@@ -698,8 +705,8 @@ void oracle_get_active(void* h, int* out) {
   for (size_t i = 0; i < o->nodes.size(); ++i) out[i] = o->nodes[i].active ? 1 : 0;
 }
 
-// One HierarchyUpdater::Get over every node: out = L(in).  For inactive
-// nodes under the Shi filter, Get adds onto whatever `out` holds.
+// One HierarchyUpdater::Get over every node: out = L(in) (under the Shi
+// filter with the current flags).
 void oracle_rhs(void* h, const double* in, double* out) {
   Oracle* o = (Oracle*)h;
   AllRhs(o, (const cplx*)in, (cplx*)out);

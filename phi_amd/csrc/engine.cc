@@ -369,7 +369,7 @@ Engine::~Engine() {
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
                   d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_active_, d_kf_[0], d_kf_[1],
-                  d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_kraw_, d_ym_, d_rot_,
+                  d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_rot_,
                   d_ops_, d_alpha_, d_corr_, d_tlq_, d_tlsite_};
   for (void* p : ptrs) Free(p);
   if (h_stage_) (void)hipHostFree(h_stage_);
@@ -637,15 +637,8 @@ int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
   int rc = Check(hipMalloc((void**)&dx, bytes), "alloc rhs in");
   if (!rc) rc = Check(hipMalloc((void**)&dy, bytes), "alloc rhs out");
   if (!rc) rc = Check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, stream_), "rhs upload");
-  if (filter_on()) {
-    // Current flags: the plain operator on the masked input.
-    if (!rc) rc = ExplicitAlloc(0);
-    if (!rc) rc = Check(LaunchMaskNodes(dx, d_active_, d_ym_, N() * N(), num_nodes(), stream_),
-                        "mask nodes");
-    if (!rc) rc = OperatorRhs(d_ym_, dy);
-  } else if (!rc) {
-    rc = OperatorRhs(dx, dy);
-  }
+  // Under the filter the current flags apply.
+  if (!rc) rc = OperatorRhs(dx, dy);
   if (!rc) rc = Check(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, stream_), "rhs download");
   if (!rc) rc = Synchronize();
   Free(dx);

@@ -155,11 +155,11 @@ class Engine {
   int Rkf45Accept();
 
   // Adaptive hierarchy truncation (PHI's FilterTolerance > 0 with a
-  // hierarchy): RK4 and RKF45 run the filtered operator (filter_kernels.h)
-  // in FP64 with PHI's classic stage order and per-stage k buffers, and the
-  // node flags are updated after every accepted step.  The per-call RHS
-  // applies the current flags (an inactive node's result holds its
-  // neighbour terms only); the steady state uses the plain operator.
+  // hierarchy): RK4 and RKF45 run the filtered operator (the general RHS
+  // with inactive neighbours' terms skipped, every node's own terms kept)
+  // in FP64 with PHI's classic stage order, and the node flags are updated
+  // after every accepted step (filter_kernels.h).  The per-call RHS applies
+  // the current flags; the steady state uses the plain operator.
   bool filter_on() const { return filter_ > 0; }
   // Current flags (num_nodes entries, 1 = active); returns the active count.
   int ActiveFlags(int* out);
@@ -291,8 +291,6 @@ class Engine {
   double filter_ = 0;
   unsigned char* d_active_ = nullptr;
   double* d_kf_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  double* d_kraw_ = nullptr;
-  double* d_ym_ = nullptr;
   int bath_model_ = 0;
   double2* d_ops_ = nullptr;
   double* d_alpha_ = nullptr;

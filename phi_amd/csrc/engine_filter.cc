@@ -2,16 +2,15 @@
 // not cover.  The RHS of a stage is evaluated into its k buffer, then PHI's
 // classic combinations run as separate FP64 kernels.  Two cases:
 //
-// * Adaptive truncation (FilterTolerance > 0; IndependentBathAdaptiveTruncation,
-//   hierarchy_updater.h:217-245, hierarchy_updater.cc:256-304; flag loops
-//   hierarchy_integrator.cc:2358-2385 for RKF45 and 2828-2851 for RK4).
-//   With the flags fixed during a step,
-//     kraw = L(input with the ADOs of inactive nodes zeroed)   (plain operator)
-//     k_s  = kraw on active nodes, k_s(previous contents) + kraw on inactive
-//            ones (PHI's per-node k buffers persist across steps and attempts,
-//            and the adaptive Get skips DoUnitary, the call that overwrites
-//            drho);
-//   flags change after each accepted step.  PHI's RK4 updates a thread's
+// * Adaptive truncation (FilterTolerance > 0; IndependentBathAdaptiveTruncation's
+//   DoNext / DoPrevious, hierarchy_updater.cc:256-304, called from the base
+//   Get, hierarchy_updater.h:36-52 -- Get is not virtual and the integrator
+//   holds a HierarchyUpdater*, hierarchy_integrator.cc:1798, 2011, so the
+//   class's own Get is never called; flag loops hierarchy_integrator.cc:
+//   2358-2385 for RKF45 and 2828-2851 for RK4).  With the flags fixed during
+//   a step, k_s = L(input) with the terms of inactive neighbours skipped;
+//   every node keeps its own terms.  Flags change after each accepted step.
+//   PHI's RK4 updates a thread's
 //   flags right after its own final update while other threads may still
 //   evaluate k3 (no barrier, hierarchy_integrator.cc:2810-2851); with one
 //   thread the update follows the whole step, the order used here.
@@ -34,10 +33,6 @@ namespace phi {
 int Engine::ExplicitAlloc(int nk) {
   const size_t bytes = state_elems() * 2 * sizeof(double);
   int rc = kOk;
-  if (filter_on()) {
-    if (!d_ym_) rc = Check(hipMalloc((void**)&d_ym_, bytes), "alloc filter input");
-    if (!rc && !d_kraw_) rc = Check(hipMalloc((void**)&d_kraw_, bytes), "alloc filter rhs");
-  }
   for (int s = 0; s < nk && !rc; ++s) {
     if (d_kf_[s]) continue;
     rc = Check(hipMalloc((void**)&d_kf_[s], bytes), "alloc stage k");
@@ -52,20 +47,14 @@ int Engine::OperatorRhs(const double* x, double* y) {
     BathArgs b{bath_model_, h_.M, h_.K, d_ops_, d_alpha_, d_corr_};
     rc = Check(LaunchRhsBath(args_, b, x, y, stream_), "rhs_bath");
   } else {
-    rc = Check(LaunchRhsGeneral(args_, x, y, 0, stream_), "rhs_general");
+    rc = Check(LaunchRhsGeneral(args_, x, y, 0, stream_, filter_on() ? d_active_ : nullptr),
+               "rhs_general");
   }
   if (!rc && tl_) rc = TlTerm(x, y);
   return rc;
 }
 
-int Engine::ExplicitRhs(const double* in, int s) {
-  if (!filter_on()) return OperatorRhs(in, d_kf_[s]);
-  const int ne = N() * N(), nodes = num_nodes();
-  int rc = Check(LaunchMaskNodes(in, d_active_, d_ym_, ne, nodes, stream_), "mask nodes");
-  if (!rc) rc = OperatorRhs(d_ym_, d_kraw_);
-  if (!rc) rc = Check(LaunchFilterK(d_kraw_, d_kf_[s], d_active_, ne, nodes, stream_), "filter k");
-  return rc;
-}
+int Engine::ExplicitRhs(const double* in, int s) { return OperatorRhs(in, d_kf_[s]); }
 
 int Engine::ExplicitUpdate(HamiltonianSource* src, double t, int nbuf, void* const* bufs) {
   if (!src || !src->Update(t)) return kOk;

@@ -1,14 +1,13 @@
-// Kernels of the adaptive hierarchy truncation (PHI's FilterTolerance,
-// hierarchy_updater.h:217-245, hierarchy_updater.cc:256-304,
-// hierarchy_integrator.cc:2358-2385, 2828-2851).
+// Kernels of the explicit-k RK4 / RKF45 path (adaptive hierarchy truncation,
+// PHI's FilterTolerance, hierarchy_updater.cc:256-304,
+// hierarchy_integrator.cc:2358-2385, 2828-2851; diagonal/full coupling
+// operators; time-local truncation).
 //
 // With the filter on, every node carries an activity flag, fixed during a
-// step.  The RHS of a stage is the plain operator applied to the stage input
-// with the ADOs of inactive nodes zeroed (that drops an inactive node's own
-// terms and every inactive neighbour's terms), and an inactive node's k
-// keeps the previous contents of its per-stage buffer and adds the neighbour
-// terms to them: PHI's adaptive Get skips DoUnitary, the call that
-// overwrites drho.  After each accepted step a node is switched off when
+// step.  The RHS of a stage is PHI's base Get with the adaptive DoNext /
+// DoPrevious (the general RHS kernel with the flags, heom_kernels.h):
+// every node keeps its own terms and the terms of inactive neighbours are
+// skipped.  After each accepted step a node is switched off when
 // max |rho_n(i,j)| < tol and on when it is > tol.
 #ifndef PHI_AMD_FILTER_KERNELS_H_
 #define PHI_AMD_FILTER_KERNELS_H_
@@ -19,12 +18,6 @@
 
 namespace phi {
 
-// y = x with the ADOs of inactive nodes zeroed (N*N = ne complex per node).
-hipError_t LaunchMaskNodes(const double* x, const unsigned char* active, double* y, int ne,
-                           int nodes, hipStream_t s);
-// k = kraw on active nodes, k += kraw on inactive ones.
-hipError_t LaunchFilterK(const double* kraw, double* k, const unsigned char* active, int ne,
-                         int nodes, hipStream_t s);
 // out = rho + c[0] k[0] + ... + c[nk-1] k[nk-1] (left to right, nk <= 6).
 hipError_t LaunchFilterCombine(const double* rho, const double* const* k, const double* c, int nk,
                                double* out, size_t n, hipStream_t s);

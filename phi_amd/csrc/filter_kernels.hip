@@ -1,4 +1,4 @@
-// Kernels of the adaptive hierarchy truncation (FilterTolerance > 0); the
+// Combination, final-update and flag kernels of the explicit-k path; the
 // semantics are stated in filter_kernels.h.  All arithmetic is FP64.
 #include "filter_kernels.h"
 
@@ -6,30 +6,6 @@
 
 namespace phi {
 namespace dev {
-
-__global__ void __launch_bounds__(256) mask_nodes(const double2* __restrict__ x,
-                                                  const unsigned char* __restrict__ act,
-                                                  double2* __restrict__ y, int ne, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    y[i] = act[i / ne] ? x[i] : make_double2(0., 0.);
-}
-
-__global__ void __launch_bounds__(256) filter_k(const double2* __restrict__ kraw,
-                                                double2* __restrict__ k,
-                                                const unsigned char* __restrict__ act, int ne,
-                                                size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const double2 r = kraw[i];
-    if (act[i / ne]) {
-      k[i] = r;
-    } else {
-      const double2 o = k[i];
-      k[i] = make_double2(o.x + r.x, o.y + r.y);
-    }
-  }
-}
 
 struct KList {
   const double2* k[6];
@@ -129,22 +105,6 @@ int GridFor(size_t n) {
   return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
 }
 }  // namespace
-
-hipError_t LaunchMaskNodes(const double* x, const unsigned char* active, double* y, int ne,
-                           int nodes, hipStream_t s) {
-  const size_t n = (size_t)ne * nodes;
-  hipLaunchKernelGGL(dev::mask_nodes, dim3(GridFor(n)), dim3(256), 0, s, (const double2*)x,
-                     active, (double2*)y, ne, n);
-  return hipGetLastError();
-}
-
-hipError_t LaunchFilterK(const double* kraw, double* k, const unsigned char* active, int ne,
-                         int nodes, hipStream_t s) {
-  const size_t n = (size_t)ne * nodes;
-  hipLaunchKernelGGL(dev::filter_k, dim3(GridFor(n)), dim3(256), 0, s, (const double2*)kraw,
-                     (double2*)k, active, ne, n);
-  return hipGetLastError();
-}
 
 hipError_t LaunchFilterCombine(const double* rho, const double* const* k, const double* c, int nk,
                                double* out, size_t n, hipStream_t s) {

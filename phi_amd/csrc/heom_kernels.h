@@ -91,8 +91,10 @@ hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, bool horner, c
 hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* in,
                             const double* rho, const Rkf45Bufs<float>& bf,
                             const Rkf45Bufs<double>& bd, const Rkf45Coefs& c, hipStream_t s);
+// PHI's Get on any complex input, FP64.  active: the adaptive truncation's
+// node flags (inactive neighbours' terms skipped), or null.
 hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int trace_row,
-                            hipStream_t s);
+                            hipStream_t s, const unsigned char* active = nullptr);
 hipError_t LaunchSnapshot(const double* src, double* hist, int* counter, int ne, int capacity,
                           hipStream_t s);
 // hist[slot] = mean over the members of their node 0 (members in order).

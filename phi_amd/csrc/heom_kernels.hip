@@ -18,9 +18,13 @@ namespace dev {
 //   drho = -(i/hbar)(H x - x H^dag) - mu x - (Delta_a + Delta_b)(a != b) x
 //          + sum_next p (x' Q_m - Q_m x') - sum_prev (col x'' Q_m + row Q_m x'')
 // With trace_row set, element (0,0) of node 0 is replaced by trace(x_0)
-// (the steady-state operator of this integrator).
+// (the steady-state operator of this integrator).  With act non-null (the
+// adaptive truncation's node flags) the terms of inactive neighbours are
+// skipped and every node keeps its own terms (PHI's base Get with the
+// adaptive DoNext / DoPrevious, hierarchy_updater.cc:256-304).
 __global__ void __launch_bounds__(256) rhs_general(RhsArgs A, const double* __restrict__ x,
-                                                   double* __restrict__ y, int trace_row) {
+                                                   double* __restrict__ y, int trace_row,
+                                                   const unsigned char* __restrict__ act) {
   const size_t tot = (size_t)A.num_nodes * A.NE;
   const int N = A.N, NE = A.NE, D = A.D, L = A.L;
   for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < tot;
@@ -58,7 +62,7 @@ __global__ void __launch_bounds__(256) rhs_general(RhsArgs A, const double* __re
           const int j = A.site_modes[q];
           const int k = A.occ[(size_t)n * D + j];
           const int nx = A.next[(size_t)n * D + j];
-          if (nx >= 0) {
+          if (nx >= 0 && (!act || act[nx])) {
             const double s = A.next_s[j * L + k];
             const double2 u = reinterpret_cast<const double2*>(x)[(size_t)nx * NE + r];
             // column: + i s u ; row: - i s u
@@ -67,7 +71,7 @@ __global__ void __launch_bounds__(256) rhs_general(RhsArgs A, const double* __re
             oy += sg * u.x;
           }
           const int pv = A.prev[(size_t)n * D + j];
-          if (pv >= 0) {
+          if (pv >= 0 && (!act || act[pv])) {
             const double2 c = side == 0 ? A.prev_col[j * L + k] : A.prev_row[j * L + k];
             const double2 u = reinterpret_cast<const double2*>(x)[(size_t)pv * NE + r];
             ox -= c.x * u.x - c.y * u.y;
@@ -332,11 +336,12 @@ hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* 
 }
 
 hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int trace_row,
-                            hipStream_t s) {
+                            hipStream_t s, const unsigned char* active) {
   const size_t tot = (size_t)a.num_nodes * a.NE;
   int blocks = (int)((tot + 255) / 256);
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(dev::rhs_general, dim3(blocks), dim3(256), 0, s, a, x, y, trace_row);
+  hipLaunchKernelGGL(dev::rhs_general, dim3(blocks), dim3(256), 0, s, a, x, y, trace_row,
+                     active);
   return hipGetLastError();
 }
 

@@ -58,18 +58,18 @@ def test_rk4_filter_matches_oracle(path, lh, tol, steps):
     ctx.close()
 
 
-def test_rk4_filter_all_inactive_straight_line():
+def test_rk4_filter_all_inactive_nodes_decouple():
+    # a tolerance above every entry: after step 1 no node is active and each
+    # evolves under its own terms only (pinned on the CPU by the node's
+    # diagonal block of the operator); the GPU follows the oracle
     op, orc, p = _pair(C1, 10.0)
     dt = op.timestep
     ctx = P.Context(p)
-    rho0 = ctx.state()
     ctx.rk4_steps(1, dt)
-    rho1 = ctx.state()
     assert not ctx.active_flags().any()
     ctx.rk4_steps(29, dt)
-    want = rho0 + 30 * (rho1 - rho0)
-    assert np.abs(ctx.state() - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
     orc.rk4(30 * dt * (1 - 1e-9), dt, 1.0)
+    assert not orc.active().any()
     assert np.abs(ctx.state() - orc.state()).max() <= 1e-10
     ctx.close()
 
