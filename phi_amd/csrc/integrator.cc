@@ -201,7 +201,13 @@ void HierarchyIntegrator::Launch() {
       std::cout << ss.str();
     }
   } else if (rm == RK4SPECTRUM || rm == RKF45SPECTRUM) {
-    Fail(kErrUnsupported, "spectrum calculations are not part of this integrator");
+    // PHI's spectrum mode has no defined result to reproduce: Get is not
+    // virtual (hierarchy_updater.h:36), so its Spectrum* updaters run the
+    // matrix DoUnitary / DoDephasing on N-entry vectors and write N^2
+    // entries into N-entry buffers (hierarchy_integrator.cc:873-881).
+    Fail(kErrUnsupported,
+         "spectrum calculations are not supported (PHI's own spectrum mode writes N^2 "
+         "entries into N-entry ADO buffers)");
   }
   if (p_->verbose > -2) std::cout << "Done.\n";
 }
