@@ -129,7 +129,8 @@ def run_rk4(P, p, prec, steps, warmup, shape=None):
     ms = ctx.rk4_stage_times(min(200, max(steps, 20)), dt) / min(200, max(steps, 20))
     out = {"seconds": el, "node_steps_per_s": ctx.num_nodes * steps / el,
            "ms_per_step": 1e3 * el / steps, "stage_ms": ms.tolist(), "rho0": rho0,
-           "nodes": ctx.num_nodes, "shape": shape}
+           "nodes": ctx.num_nodes, "shape": shape,
+           "launch": "hipGraph blocks" if ctx.graphs() else "eager"}
     ctx.close()
     return out
 
@@ -334,6 +335,8 @@ def main():
                    "nodes": nodes, "num_states": N, "dt_ps": dt, "rk4_form": "nested",
                    "kernel_shape_fp64": res["fp64"]["shape"],
                    "kernel_shape_fp32": res["fp32"]["shape"],
+                   "launch_fp64": res["fp64"]["launch"],
+                   "launch_fp32": res["fp32"]["launch"],
                    "time_to_1ps_s_fp64": res["fp64"]["ms_per_step"] * steps_per_ps * 1e-3,
                    "time_to_1ps_s_fp32_bulk": res["fp32"]["ms_per_step"] * steps_per_ps * 1e-3},
         "fp32_bulk": {"value": res["fp32"]["node_steps_per_s"], "unit": "node-steps/s",

@@ -243,12 +243,17 @@ int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks);
  * linear RHS, with half the FP64 traffic.  Equal to rounding. */
 int phi_hip_set_rk4_form(phi_ctx* c, int form);
 /* RK4 step blocks replayed from a captured hipGraph (on = 1, the default)
- * or launched kernel by kernel (0).  Same results either way. */
+ * or launched kernel by kernel (0).  Same results either way.  Once called,
+ * phi_hip_autotune keeps this choice; otherwise it times both. */
 int phi_hip_set_graphs(phi_ctx* c, int on);
+/* The current choice (1 = graph blocks, 0 = eager launches). */
+int phi_hip_get_graphs(phi_ctx* c, int* on);
 /* Autotuner of the stage-kernel shape (no counterpart in PHI, whose
  * partitioning is PartitionHierarchy, hierarchy_integrator.cc:636-808):
  * times `steps` RK4 steps of size dt for every candidate shape in the
- * context's precision and RK4 form, keeps the fastest and leaves the state
+ * context's precision and RK4 form, keeps the fastest, then (unless
+ * phi_hip_set_graphs was called) times 129-step runs with graph blocks and
+ * with eager launches and keeps the faster, and leaves the state
  * unchanged.  Cached per device and problem shape for the process.
  * *best_ms = milliseconds per step; shape[4] = {gemm, tile, threads,
  * blocks} as in phi_hip_set_launch. */

@@ -92,6 +92,7 @@ def lib():
                                               ctypes.c_int]),
         "phi_hip_set_rk4_form": (ctypes.c_int, [vp, ctypes.c_int]),
         "phi_hip_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
+        "phi_hip_get_graphs": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
         "phi_hip_ensemble_create": (ctypes.c_int, [vp, ctypes.c_int, dp, ip, ctypes.c_int,
                                                    ctypes.POINTER(vp)]),
         "phi_hip_ensemble_members": (ctypes.c_int, [vp, ip, ctypes.c_int]),
@@ -423,6 +424,12 @@ class Context:
         """RK4 step blocks replayed from a captured hipGraph (True) or
         launched kernel by kernel (False)."""
         _check(lib().phi_hip_set_graphs(self._h, 1 if on else 0))
+
+    def graphs(self):
+        """True when RK4 step blocks replay a captured hipGraph."""
+        on = ctypes.c_int()
+        _check(lib().phi_hip_get_graphs(self._h, ctypes.byref(on)))
+        return bool(on.value)
 
     def autotune(self, dt, steps=5):
         """Pick the fastest stage-kernel shape for this problem, precision

@@ -653,6 +653,12 @@ int phi_hip_set_graphs(phi_ctx* c, int on) {
   return Ok();
 }
 
+int phi_hip_get_graphs(phi_ctx* c, int* on) {
+  if (!c || !on) return Err(PHI_HIP_EINVAL, "null argument");
+  *on = c->e->graphs() ? 1 : 0;
+  return Ok();
+}
+
 int phi_hip_autotune(phi_ctx* c, int steps, double dt, double* best_ms, int* shape) {
   if (!c || !best_ms || !shape || steps < 1 || !(dt > 0)) return Err(PHI_HIP_EINVAL, "bad argument");
   int rc = c->e->Autotune(steps, dt, best_ms, shape);

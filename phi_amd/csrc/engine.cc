@@ -479,6 +479,7 @@ struct TuneKey {
 struct TuneVal {
   int col_nm, tile, threads, blocks;
   double ms;
+  bool graphs;
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune;
@@ -504,6 +505,10 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
     if (it != g_tune.end()) {
       const TuneVal& v = it->second;
       rc = SetLaunch(v.col_nm, v.tile, v.threads, v.blocks);
+      if (!rc && !graphs_pinned_) {
+        use_graphs_ = v.graphs;
+        if (!use_graphs_) DropGraph();
+      }
       if (!rc) {
         *best_ms = v.ms;
         shape_out[0] = v.col_nm;
@@ -541,8 +546,9 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   if (!rc) rc = Check(hipEventCreate(&e0), "event");
   if (!rc) rc = Check(hipEventCreate(&e1), "event");
   const RhsArgs keep = args_;
+  const bool keep_graphs = use_graphs_;
   TuneVal best{keep.col_nm, keep.col_tile[fp32_ ? 1 : 0], keep.col_threads[fp32_ ? 1 : 0],
-               keep.col_grid[fp32_ ? 1 : 0], 1e300};
+               keep.col_grid[fp32_ ? 1 : 0], 1e300, use_graphs_};
   for (const Cand& c : cands) {
     if (rc) break;
     if (SetLaunch(c.col_nm, c.tile, c.threads, c.blocks) != kOk) continue;
@@ -556,7 +562,34 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
     if (!rc && ms / steps < best.ms) {
       const int f = fp32_ ? 1 : 0;
       best = TuneVal{c.col_nm, args_.col_tile[f], args_.col_threads[f], args_.col_grid[f],
-                     ms / steps};
+                     ms / steps, use_graphs_};
+    }
+  }
+  // Graph blocks or eager launches for the chosen shape: short stage kernels
+  // can run faster eagerly (the graph's node-to-node latency), long ones
+  // do not care, tiny ones need the graph when the host cannot keep up.
+  if (!rc && !graphs_pinned_) rc = SetLaunch(best.col_nm, best.tile, best.threads, best.blocks);
+  if (!rc && !graphs_pinned_) {
+    // One eager head step, then four graph blocks (Rk4Steps' layout).
+    const long long n = 4LL * kGraphSteps + 1;
+    double t[2] = {0, 0};
+    for (int g = 0; g < 2 && !rc; ++g) {
+      use_graphs_ = g == 1;
+      if (!use_graphs_) DropGraph();
+      rc = Rk4Steps(n, dt, 0, 0);  // warm-up (and graph capture)
+      if (!rc) rc = Check(hipEventRecord(e0, stream_), "record");
+      if (!rc) rc = Rk4Steps(n, dt, 0, 0);
+      if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");
+      if (!rc) rc = Check(hipEventSynchronize(e1), "event sync");
+      float ms = 0;
+      if (!rc) rc = Check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      t[g] = ms / n;
+    }
+    if (!rc) {
+      best.graphs = t[1] <= t[0];
+      best.ms = best.graphs ? t[1] : t[0];
+      use_graphs_ = best.graphs;
+      if (!use_graphs_) DropGraph();
     }
   }
   if (e0) (void)hipEventDestroy(e0);
@@ -568,6 +601,8 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   if (!rc) rc = rc_restore ? rc_restore : rc_sync;
   if (rc) {
     args_ = keep;
+    use_graphs_ = keep_graphs;
+    DropGraph();
     return rc;
   }
   rc = SetLaunch(best.col_nm, best.tile, best.threads, best.blocks);

@@ -190,14 +190,19 @@ class Engine {
   int Synchronize();
   std::string last_error() const { return err_; }
 
-  // RK4 blocks replayed from a captured hipGraph (default on).
+  // RK4 blocks replayed from a captured hipGraph (default on; the autotuner
+  // picks the faster of graphs and eager launches unless this was called).
   void set_graphs(bool v) {
     use_graphs_ = v;
+    graphs_pinned_ = true;
     if (!v) DropGraph();
   }
+  bool graphs() const { return use_graphs_; }
   // Autotuner: times `steps` RK4 steps (after one warm-up step) for each
   // candidate shape of the persistent kernels (tile x grid) in the current
-  // precision and RK4 form, keeps the fastest, and restores the state.
+  // precision and RK4 form, keeps the fastest, then (graphs not pinned)
+  // times graph blocks against eager launches for it, and restores the
+  // state.
   // Results are cached per (device, problem shape, precision, form) for the
   // process.  *best_ms = milliseconds per step of the winner; shape_out =
   // {form, tile, threads, blocks}.
@@ -228,6 +233,7 @@ class Engine {
   };
   static constexpr int kGraphSteps = 32;
   bool use_graphs_ = true;
+  bool graphs_pinned_ = false;
   hipGraphExec_t graph_exec_ = nullptr;
   GraphKey graph_key_{};
 
