@@ -43,6 +43,10 @@ int Engine::ExplicitAlloc(int nk) {
 
 int Engine::OperatorRhs(const double* x, double* y) {
   int rc;
+  if (bath_model_ != 0 && !d_ops_) {
+    err_ = "diagonal/full coupling operators were not given (bath_coupling_op)";
+    return kErrInvalidParameters;
+  }
   if (bath_model_ != 0) {
     BathArgs b{bath_model_, h_.M, h_.K, d_ops_, d_alpha_, d_corr_};
     rc = Check(LaunchRhsBath(args_, b, x, y, stream_), "rhs_bath");

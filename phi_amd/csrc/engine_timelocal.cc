@@ -77,6 +77,10 @@ int Engine::TlUpload() {
 }
 
 int Engine::TlStep(double ts) {
+  if (bath_model_ != 0 && h_opscur_.empty()) {
+    err_ = "diagonal/full coupling operators were not given (bath_coupling_op)";
+    return kErrInvalidParameters;
+  }
   tl_->Step(h_hcur_.data(), ts, h_opscur_.empty() ? nullptr : h_opscur_.data());
   return TlUpload();
 }
