@@ -199,8 +199,10 @@ int phi_hip_set_state(phi_ctx* c, const double* state, size_t n_complex);
 int phi_hip_get_state(phi_ctx* c, double* state, size_t n_complex);
 int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n_complex);
 /* out = L(in) for every node (PHI's Get, any complex input); host arrays.
- * With FilterTolerance > 0 the current node flags apply: inactive nodes and
- * neighbour terms (PHI's Get adds them onto drho, here onto zero).  With
+ * With FilterTolerance > 0 the current node flags apply: the terms of
+ * inactive neighbours are skipped and every node keeps its own terms
+ * (PHI's base Get with the adaptive DoNext / DoPrevious,
+ * hierarchy_updater.h:36-52, hierarchy_updater.cc:256-304).  With
  * TimeLocal the deepest tier includes the time-local term with the current
  * matrices (zero until an integrator run advances them). */
 int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
