@@ -990,55 +990,68 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   if (!rc) rc = Check(hipMemcpyAsync(rh, r, bytes, hipMemcpyDeviceToDevice, stream_), "rhat");
   if (!rc) rc = Check(hipMemsetAsync(pp, 0, bytes, stream_), "p0");
   if (!rc) rc = Check(hipMemsetAsync(vv, 0, bytes, stream_), "v0");
-  Z rho_old(1, 0), alpha(1, 0), omega(1, 0);
   Z rn2;
   if (!rc) rc = dot(r, r, &rn2);
   double res = std::sqrt(rn2.real());
-  if (history) history->push_back(res);
-  int it = 0;
-  while (!rc && res > tol && it < max_iter) {
-    ++it;
-    Z rho_new;
-    rc = dot(rh, r, &rho_new);
-    if (rc) break;
-    const Z beta = (rho_new / rho_old) * (alpha / omega);
-    {  // p = r + beta (p - omega v)
-      const double* xs[3] = {r, pp, vv};
-      Z cs[3] = {Z(1, 0), beta, -beta * omega};
-      rc = lin(pp, 3, xs, cs);
+  // The iterations keep their scalars on the device (BicgScalars): the dot
+  // products land there, the update kernels derive beta, alpha and omega
+  // from them, a one-thread kernel closes each iteration.  The host checks
+  // after every kBatch iterations; once converged (or broken down, or at
+  // max_iter) the x/r update and the closing kernel do nothing, so x and
+  // the count are those of stopping right there.
+  constexpr int kBatch = 16;
+  BicgScalars* st = nullptr;
+  double* hist = nullptr;
+  if (!rc) rc = Check(hipMalloc((void**)&st, sizeof(BicgScalars)), "alloc bicgstab scalars");
+  if (!rc)
+    rc = Check(hipMalloc((void**)&hist, (size_t)(std::max(max_iter, 0) + 1) * sizeof(double)),
+               "alloc bicgstab history");
+  BicgScalars hs{};
+  hs.rho_old = hs.alpha = hs.omega = make_double2(1., 0.);
+  hs.res = res;
+  hs.done = !(res > tol) || max_iter <= 0 ? 1 : 0;
+  hs.it = 0;
+  if (!rc) rc = Check(hipMemcpyAsync(st, &hs, sizeof(hs), hipMemcpyHostToDevice, stream_),
+                      "upload bicgstab scalars");
+  if (!rc) rc = Check(hipMemcpyAsync(hist, &res, sizeof(double), hipMemcpyHostToDevice, stream_),
+                      "upload residual");
+  auto dev_dot = [&](const double* a, const double* b, double2* out) -> int {
+    return Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, reinterpret_cast<double*>(out),
+                           stream_), "dot");
+  };
+  while (!rc && !hs.done) {
+    for (int b = 0; b < kBatch && !rc; ++b) {
+      rc = dev_dot(rh, r, &st->rho_new);
+      // p = r + beta (p - omega v)
+      if (!rc) rc = Check(LaunchBicgUpdate(0, pp, n, r, pp, vv, st, stream_), "p update");
+      if (!rc) rc = RhsDevice(pp, vv, 1);
+      if (!rc) rc = dev_dot(rh, vv, &st->rv);
+      // s = r - alpha v
+      if (!rc) rc = Check(LaunchBicgUpdate(1, s, n, r, vv, nullptr, st, stream_), "s update");
+      if (!rc) rc = RhsDevice(s, t, 1);
+      if (!rc) rc = dev_dot(t, s, &st->ts);
+      if (!rc) rc = dev_dot(t, t, &st->tt);
+      // x = x + alpha p + omega s, r = s - omega t, |r|^2
+      if (!rc)
+        rc = Check(LaunchBicgXR(x, r, pp, s, t, n, st, d_partial_, red_blocks_, stream_),
+                   "x/r update");
+      if (!rc) rc = Check(LaunchBicgFinish(st, hist, tol, max_iter, stream_), "finish");
     }
-    if (!rc) rc = RhsDevice(pp, vv, 1);
-    Z rv;
-    if (!rc) rc = dot(rh, vv, &rv);
-    if (rc) break;
-    alpha = rho_new / rv;
-    {  // s = r - alpha v
-      const double* xs[2] = {r, vv};
-      Z cs[2] = {Z(1, 0), -alpha};
-      rc = lin(s, 2, xs, cs);
-    }
-    if (!rc) rc = RhsDevice(s, t, 1);
-    Z ts, tt;
-    if (!rc) rc = dot(t, s, &ts);
-    if (!rc) rc = dot(t, t, &tt);
-    if (rc) break;
-    omega = tt.real() > 0 ? ts / tt : Z(0, 0);
-    {  // x = x + alpha p + omega s
-      const double* xs[3] = {x, pp, s};
-      Z cs[3] = {Z(1, 0), alpha, omega};
-      rc = lin(x, 3, xs, cs);
-    }
-    {  // r = s - omega t
-      const double* xs[2] = {s, t};
-      Z cs[2] = {Z(1, 0), -omega};
-      if (!rc) rc = lin(r, 2, xs, cs);
-    }
-    if (!rc) rc = dot(r, r, &rn2);
-    res = std::sqrt(rn2.real());
-    if (history) history->push_back(res);
-    rho_old = rho_new;
-    if (omega == Z(0, 0)) break;
+    if (!rc) rc = Check(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_),
+                        "bicgstab scalars");
+    if (!rc) rc = Synchronize();
   }
+  const int it = hs.it;
+  if (!rc) res = hs.res;
+  if (!rc && history) {
+    std::vector<double> h(it + 1);
+    rc = Check(hipMemcpyAsync(h.data(), hist, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              stream_), "residual history");
+    if (!rc) rc = Synchronize();
+    if (!rc) history->insert(history->end(), h.begin(), h.end());
+  }
+  Free(st);
+  Free(hist);
   cleanup();
   if (iters) *iters = it;
   if (resid) *resid = res;

@@ -112,6 +112,31 @@ hipError_t LaunchDot(const double* u, const double* v, size_t n, int mode, doubl
 hipError_t LaunchLincomb(double* y, size_t n, int nv, const double* const* x, const double2* c,
                          hipStream_t s);
 
+// BiCGSTAB with its scalars on the device (no host round trip inside an
+// iteration).  The dot products land in this struct; the update kernels
+// derive their coefficients from it; the finish kernel closes an iteration.
+struct BicgScalars {
+  double2 rho_old, alpha, omega;  // from the previous iteration (1, 1, 1 at start)
+  double2 rho_new, rv, ts, tt, rn2;  // dot products of the current iteration
+  double res;                     // |r| after the last finished iteration
+  int done;                       // converged, breakdown (omega = 0) or max_iter
+  int it;                         // finished iterations
+};
+// which = 0: y = x0 + beta x1 - beta omega x2 (p update; beta from the
+// previous iteration's scalars and rho_new); which = 1: y = x0 - alpha x1
+// (s update; alpha = rho_new / rv).
+hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
+                            const double* x2, const BicgScalars* st, hipStream_t s);
+// x += alpha p + omega s, r = s - omega t (omega = ts / tt, 0 if tt = 0) and
+// st->rn2 = |r|^2 (two-pass, nblocks partials); nothing once st->done.
+hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
+                        const double* t, size_t n, BicgScalars* st, double* partial, int nblocks,
+                        hipStream_t s);
+// One thread: alpha, omega, rho_old <- this iteration's; res = |r|;
+// hist[it] = res; done when !(res > tol), omega = 0 or it = max_iter.
+hipError_t LaunchBicgFinish(BicgScalars* st, double* hist, double tol, int max_iter,
+                            hipStream_t s);
+
 }  // namespace phi
 
 #endif  // PHI_AMD_HEOM_KERNELS_H_

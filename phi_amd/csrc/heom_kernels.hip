@@ -266,6 +266,107 @@ __global__ void __launch_bounds__(256) zlincomb(double* __restrict__ y, size_t n
   }
 }
 
+// --- BiCGSTAB with device scalars ------------------------------------------------
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a / b = a conj(b) / |b|^2
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double2 bicg_omega(const BicgScalars* st) {
+  return st->tt.x > 0 ? zdiv(st->ts, st->tt) : make_double2(0., 0.);
+}
+
+__global__ void __launch_bounds__(256) bicg_update(int which, double* __restrict__ y, size_t n,
+                                                   const double* x0, const double* x1,
+                                                   const double* x2, const BicgScalars* st) {
+  double2 c[3];
+  int nv;
+  if (which == 0) {
+    const double2 beta = zmul(zdiv(st->rho_new, st->rho_old), zdiv(st->alpha, st->omega));
+    const double2 bo = zmul(beta, st->omega);
+    c[0] = make_double2(1., 0.);
+    c[1] = beta;
+    c[2] = make_double2(-bo.x, -bo.y);
+    nv = 3;
+  } else {
+    const double2 al = zdiv(st->rho_new, st->rv);
+    c[0] = make_double2(1., 0.);
+    c[1] = make_double2(-al.x, -al.y);
+    c[2] = make_double2(0., 0.);
+    nv = 2;
+  }
+  const double* xs[3] = {x0, x1, x2};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double ar = 0, ai = 0;
+    for (int t = 0; t < nv; ++t) {
+      const double2 v = reinterpret_cast<const double2*>(xs[t])[i];
+      ar += c[t].x * v.x - c[t].y * v.y;
+      ai += c[t].x * v.y + c[t].y * v.x;
+    }
+    reinterpret_cast<double2*>(y)[i] = make_double2(ar, ai);
+  }
+}
+
+__global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* __restrict__ r,
+                                               const double* __restrict__ p,
+                                               const double* __restrict__ sv,
+                                               const double* __restrict__ t, size_t n,
+                                               const BicgScalars* st,
+                                               double* __restrict__ partial) {
+  __shared__ double sr[256];
+  if (st->done) return;  // uniform: every block returns
+  const double2 al = zdiv(st->rho_new, st->rv), om = bicg_omega(st);
+  double acc = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double2 xv = reinterpret_cast<const double2*>(x)[i];
+    const double2 pv = reinterpret_cast<const double2*>(p)[i];
+    const double2 s = reinterpret_cast<const double2*>(sv)[i];
+    const double2 tv = reinterpret_cast<const double2*>(t)[i];
+    // x = 1 x + alpha p + omega s ; r = 1 s - omega t (left to right)
+    double xr = xv.x, xi = xv.y;
+    xr += al.x * pv.x - al.y * pv.y;
+    xi += al.x * pv.y + al.y * pv.x;
+    xr += om.x * s.x - om.y * s.y;
+    xi += om.x * s.y + om.y * s.x;
+    reinterpret_cast<double2*>(x)[i] = make_double2(xr, xi);
+    double rr = s.x, ri = s.y;
+    rr += -om.x * tv.x + om.y * tv.y;
+    ri += -om.x * tv.y - om.y * tv.x;
+    reinterpret_cast<double2*>(r)[i] = make_double2(rr, ri);
+    acc += rr * rr + ri * ri;
+  }
+  sr[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sr[threadIdx.x] += sr[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sr[0];
+    partial[2 * blockIdx.x + 1] = 0;
+  }
+}
+
+__global__ void bicg_finish(BicgScalars* st, double* hist, double tol, int max_iter) {
+  if (threadIdx.x != 0 || st->done) return;
+  const double2 al = zdiv(st->rho_new, st->rv), om = bicg_omega(st);
+  st->alpha = al;
+  st->omega = om;
+  st->rho_old = st->rho_new;
+  const double res = sqrt(st->rn2.x);
+  st->res = res;
+  const int it = st->it + 1;
+  st->it = it;
+  hist[it] = res;
+  if (!(res > tol) || (om.x == 0 && om.y == 0) || it >= max_iter) st->done = 1;
+}
+
 }  // namespace dev
 
 // --- Host-side launchers -------------------------------------------------------
@@ -386,6 +487,31 @@ hipError_t LaunchDot(const double* u, const double* v, size_t n, int mode, doubl
                      int nblocks, double* out, hipStream_t s) {
   hipLaunchKernelGGL(dev::dot_partial, dim3(nblocks), dim3(256), 0, s, u, v, n, mode, partial);
   hipLaunchKernelGGL(dev::dot_final, dim3(1), dim3(256), 0, s, partial, nblocks, out);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
+                            const double* x2, const BicgScalars* st, hipStream_t s) {
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(dev::bicg_update, dim3(blocks), dim3(256), 0, s, which, y, n, x0, x1, x2,
+                     st);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
+                        const double* t, size_t n, BicgScalars* st, double* partial, int nblocks,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_xr, dim3(nblocks), dim3(256), 0, s, x, r, p, sv, t, n, st,
+                     partial);
+  hipLaunchKernelGGL(dev::dot_final, dim3(1), dim3(256), 0, s, partial, nblocks,
+                     reinterpret_cast<double*>(&st->rn2));
+  return hipGetLastError();
+}
+
+hipError_t LaunchBicgFinish(BicgScalars* st, double* hist, double tol, int max_iter,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_finish, dim3(1), dim3(64), 0, s, st, hist, tol, max_iter);
   return hipGetLastError();
 }
 

@@ -211,6 +211,57 @@ def test_steady_state_c1_small():
     assert np.abs(Lx).max() < 1e-8
 
 
+def test_steady_state_iterations_match_dense_bicgstab():
+    """The device-scalar BiCGSTAB takes the textbook iterates (van der
+    Vorst 1992): residuals after k = 1..12 iterations equal those of a dense
+    numpy BiCGSTAB on the oracle's steady-state operator (A = L with the
+    trace row, b = e_0, x_0 = the initial state)."""
+    op = PO.read_command_line(os.path.join(CFG, "c1.prm"), "bicgstab")
+    kw = dict(gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature, lh=3,
+              num_terms=op.num_matsubara_terms)
+    p = P.params_from_problem(op.time_indepenent_hamiltonian, op.initial_density_matrix,
+                              method="bicgstab", **kw)
+    orc = HO.Oracle(HO.Problem(op.time_indepenent_hamiltonian, op.initial_density_matrix, **kw))
+    ctx = P.Context(p)
+    x0 = ctx.state()
+    ctx.close()
+    shape = x0.shape
+    m = x0.size
+    A = np.zeros((m, m), dtype=complex)
+    for c in range(m):
+        e = np.zeros(m, dtype=complex)
+        e[c] = 1
+        A[:, c] = orc.steady_matvec(e.reshape(shape)).reshape(-1)
+    b = np.zeros(m, dtype=complex)
+    b[0] = 1
+    x = x0.reshape(-1).copy()
+    r = b - A @ x
+    rh = r.copy()
+    pv = np.zeros(m, dtype=complex)
+    v = np.zeros(m, dtype=complex)
+    rho_old = alpha = omega = 1.0 + 0j
+    res = [np.linalg.norm(r)]
+    for _ in range(12):
+        rho_new = np.vdot(rh, r)
+        beta = (rho_new / rho_old) * (alpha / omega)
+        pv = r + beta * (pv - omega * v)
+        v = A @ pv
+        alpha = rho_new / np.vdot(rh, v)
+        s = r - alpha * v
+        t = A @ s
+        omega = np.vdot(t, s) / np.vdot(t, t)
+        x = x + alpha * pv + omega * s
+        r = s - omega * t
+        rho_old = rho_new
+        res.append(np.linalg.norm(r))
+    for k in (1, 2, 3, 5, 8, 12):
+        ctx = P.Context(p)
+        rc, it, got = ctx.steady_state(1e-30, k)
+        ctx.close()
+        assert it == k
+        assert abs(got - res[k]) <= 1e-9 * res[0], (k, got, res[k])
+
+
 def _subset_params(cfg, lh):
     path = os.path.join(CFG, cfg + ".prm")
     op = PO.read_command_line(path, "rk4")
