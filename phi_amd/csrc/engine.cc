@@ -1002,7 +1002,11 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   constexpr int kBatch = 16;
   BicgScalars* st = nullptr;
   double* hist = nullptr;
+  double* part3 = nullptr;
   if (!rc) rc = Check(hipMalloc((void**)&st, sizeof(BicgScalars)), "alloc bicgstab scalars");
+  if (!rc)
+    rc = Check(hipMalloc((void**)&part3, (size_t)red_blocks_ * 3 * sizeof(double)),
+               "alloc bicgstab partials");
   if (!rc)
     rc = Check(hipMalloc((void**)&hist, (size_t)(std::max(max_iter, 0) + 1) * sizeof(double)),
                "alloc bicgstab history");
@@ -1019,21 +1023,21 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
     return Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, reinterpret_cast<double*>(out),
                            stream_), "dot");
   };
+  // rho for the first iteration; later ones come with the r update.
+  if (!rc && !hs.done) rc = dev_dot(rh, r, &st->rho_new);
   while (!rc && !hs.done) {
     for (int b = 0; b < kBatch && !rc; ++b) {
-      rc = dev_dot(rh, r, &st->rho_new);
       // p = r + beta (p - omega v)
-      if (!rc) rc = Check(LaunchBicgUpdate(0, pp, n, r, pp, vv, st, stream_), "p update");
+      rc = Check(LaunchBicgUpdate(0, pp, n, r, pp, vv, st, stream_), "p update");
       if (!rc) rc = RhsDevice(pp, vv, 1);
       if (!rc) rc = dev_dot(rh, vv, &st->rv);
       // s = r - alpha v
       if (!rc) rc = Check(LaunchBicgUpdate(1, s, n, r, vv, nullptr, st, stream_), "s update");
       if (!rc) rc = RhsDevice(s, t, 1);
-      if (!rc) rc = dev_dot(t, s, &st->ts);
-      if (!rc) rc = dev_dot(t, t, &st->tt);
-      // x = x + alpha p + omega s, r = s - omega t, |r|^2
+      if (!rc) rc = Check(LaunchBicgDotTS(t, s, n, st, part3, red_blocks_, stream_), "t.s, t.t");
+      // x = x + alpha p + omega s, r = s - omega t, |r|^2, rhat . r
       if (!rc)
-        rc = Check(LaunchBicgXR(x, r, pp, s, t, n, st, d_partial_, red_blocks_, stream_),
+        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, st, part3, red_blocks_, stream_),
                    "x/r update");
       if (!rc) rc = Check(LaunchBicgFinish(st, hist, tol, max_iter, stream_), "finish");
     }
@@ -1052,6 +1056,7 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   }
   Free(st);
   Free(hist);
+  Free(part3);
   cleanup();
   if (iters) *iters = it;
   if (resid) *resid = res;

@@ -118,6 +118,7 @@ hipError_t LaunchLincomb(double* y, size_t n, int nv, const double* const* x, co
 struct BicgScalars {
   double2 rho_old, alpha, omega;  // from the previous iteration (1, 1, 1 at start)
   double2 rho_new, rv, ts, tt, rn2;  // dot products of the current iteration
+  double2 rho_next;               // rhat . r of the new r (the next rho_new)
   double res;                     // |r| after the last finished iteration
   int done;                       // converged, breakdown (omega = 0) or max_iter
   int it;                         // finished iterations
@@ -127,13 +128,19 @@ struct BicgScalars {
 // (s update; alpha = rho_new / rv).
 hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
                             const double* x2, const BicgScalars* st, hipStream_t s);
-// x += alpha p + omega s, r = s - omega t (omega = ts / tt, 0 if tt = 0) and
-// st->rn2 = |r|^2 (two-pass, nblocks partials); nothing once st->done.
+// st->ts = t . s (conjugating t) and st->tt = |t|^2 in one pass (two-pass
+// reduction over nblocks partials of 3 doubles).
+hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, BicgScalars* st,
+                           double* partial3, int nblocks, hipStream_t s);
+// x += alpha p + omega s, r = s - omega t (omega = ts / tt, 0 if tt = 0),
+// st->rn2 = |r|^2 and st->rho_next = rhat . r in the same pass (nblocks
+// partials of 3 doubles); nothing once st->done.
 hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
-                        const double* t, size_t n, BicgScalars* st, double* partial, int nblocks,
-                        hipStream_t s);
-// One thread: alpha, omega, rho_old <- this iteration's; res = |r|;
-// hist[it] = res; done when !(res > tol), omega = 0 or it = max_iter.
+                        const double* t, const double* rhat, size_t n, BicgScalars* st,
+                        double* partial3, int nblocks, hipStream_t s);
+// One thread: alpha, omega, rho_old <- this iteration's, rho_new <-
+// rho_next; res = |r|; hist[it] = res; done when !(res > tol), omega = 0
+// or it = max_iter.
 hipError_t LaunchBicgFinish(BicgScalars* st, double* hist, double tol, int max_iter,
                             hipStream_t s);
 

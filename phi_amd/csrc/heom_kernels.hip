@@ -22,11 +22,88 @@ namespace dev {
 // adaptive truncation's node flags) the terms of inactive neighbours are
 // skipped and every node keeps its own terms (PHI's base Get with the
 // adaptive DoNext / DoPrevious, hierarchy_updater.cc:256-304).
+// One element (a, b) of node n; xn points at the node's ADO, H / Hd at H
+// and H^dagger (global memory or LDS copies: same arithmetic either way).
+template <typename XP, typename HP>
+__device__ __forceinline__ double2 general_elem(const RhsArgs& A, const double* __restrict__ x,
+                                                XP xn, HP H, HP Hd, int n, int r, int a, int b,
+                                                const unsigned char* __restrict__ act,
+                                                bool trace) {
+  const int N = A.N, NE = A.NE, D = A.D, L = A.L;
+  // The unshifted H: the mean-diagonal shift only cancels in the
+  // Hermitian form H x - (H x)^dagger, not in H x - x H^dagger for general x.
+  double hr = 0, hi = 0;
+  for (int c = 0; c < N; ++c) {
+    const double2 h = H[a + c * N], v = xn[b * N + c];
+    hr += h.x * v.x - h.y * v.y;
+    hi += h.x * v.y + h.y * v.x;
+    const double2 w = xn[a + c * N], hd = Hd[c + b * N];
+    hr -= w.x * hd.x - w.y * hd.y;
+    hi -= w.x * hd.y + w.y * hd.x;
+  }
+  // Unitary part with the unshifted H: -(i/hbar)(Hx - xH^dag).
+  double ox = A.inv_hbar * hi, oy = -A.inv_hbar * hr;
+  if (!A.vacuum) {
+    const double2 v = xn[r];
+    double f = A.mu[n];
+    if (a != b) f += A.delta[a] + A.delta[b];
+    ox -= f * v.x;
+    oy -= f * v.y;
+    // Column b gathers (site b), row a gathers (site a): the modes of the
+    // site in site_modes order, from the edge records {next id, prev id,
+    // j L + n_j} when the engine keeps them (one load instead of the chain
+    // site_ptr -> site_modes -> occ / next / prev), else from the tables.
+    for (int side = 0; side < 2; ++side) {
+      const int site = side == 0 ? b : a;
+      const int q0 = A.edge ? 0 : A.site_ptr[site];
+      const int q1 = A.edge ? A.mps : A.site_ptr[site + 1];
+      for (int q = q0; q < q1; ++q) {
+        int nx, pv, z;
+        if (A.edge) {
+          const int4 c = A.edge[((size_t)n * N + site) * A.mps + q];
+          nx = c.x;
+          pv = c.y;
+          z = c.z;
+        } else {
+          const int j = A.site_modes[q];
+          const size_t o = (size_t)n * D + j;
+          nx = A.next[o];
+          pv = A.prev[o];
+          z = j * L + A.occ[o];
+        }
+        if (nx >= 0 && (!act || act[nx])) {
+          const double s = A.next_s[z];
+          const double2 u = reinterpret_cast<const double2*>(x)[(size_t)nx * NE + r];
+          // column: + i s u ; row: - i s u
+          const double sg = side == 0 ? s : -s;
+          ox -= sg * u.y;
+          oy += sg * u.x;
+        }
+        if (pv >= 0 && (!act || act[pv])) {
+          const double2 c = side == 0 ? A.prev_col[z] : A.prev_row[z];
+          const double2 u = reinterpret_cast<const double2*>(x)[(size_t)pv * NE + r];
+          ox -= c.x * u.x - c.y * u.y;
+          oy -= c.x * u.y + c.y * u.x;
+        }
+      }
+    }
+  }
+  if (trace) {
+    ox = 0;
+    oy = 0;
+    for (int c = 0; c < N; ++c) {
+      ox += xn[c * (N + 1)].x;
+      oy += xn[c * (N + 1)].y;
+    }
+  }
+  return make_double2(ox, oy);
+}
+
 __global__ void __launch_bounds__(256) rhs_general(RhsArgs A, const double* __restrict__ x,
                                                    double* __restrict__ y, int trace_row,
                                                    const unsigned char* __restrict__ act) {
   const size_t tot = (size_t)A.num_nodes * A.NE;
-  const int N = A.N, NE = A.NE, D = A.D, L = A.L;
+  const int N = A.N, NE = A.NE;
   for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < tot;
        g += (size_t)gridDim.x * blockDim.x) {
     const int n = (int)(g / NE);
@@ -34,61 +111,43 @@ __global__ void __launch_bounds__(256) rhs_general(RhsArgs A, const double* __re
     const int b = r / N;
     const int a = r - b * N;
     const double2* xn = reinterpret_cast<const double2*>(x) + (size_t)n * NE;
-    // The unshifted H: the mean-diagonal shift only cancels in the
-    // Hermitian form H x - (H x)^dagger, not in H x - x H^dagger for general x.
-    const double2* H = A.Hraw;
-    const double2* Hd = A.Hadj;
-    double hr = 0, hi = 0;
-    for (int c = 0; c < N; ++c) {
-      const double2 h = H[a + c * N], v = xn[b * N + c];
-      hr += h.x * v.x - h.y * v.y;
-      hi += h.x * v.y + h.y * v.x;
-      const double2 w = xn[a + c * N], hd = Hd[c + b * N];
-      hr -= w.x * hd.x - w.y * hd.y;
-      hi -= w.x * hd.y + w.y * hd.x;
+    reinterpret_cast<double2*>(y)[g] =
+        general_elem(A, x, xn, A.Hraw, A.Hadj, n, r, a, b, act, trace_row && g == 0);
+  }
+}
+
+// The same with H, H^dagger and the ADOs of T nodes per block in LDS (one
+// load of each per element instead of 2N): for N*N*(T+2) complex within
+// the LDS budget.
+__global__ void __launch_bounds__(256) rhs_general_lds(RhsArgs A, const double* __restrict__ x,
+                                                       double* __restrict__ y, int trace_row,
+                                                       const unsigned char* __restrict__ act,
+                                                       int T) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = A.N, NE = A.NE;
+  double2* sH = reinterpret_cast<double2*>(smem);
+  double2* sHd = sH + NE;
+  double2* sx = sHd + NE;
+  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+    sH[e] = A.Hraw[e];
+    sHd[e] = A.Hadj[e];
+  }
+  for (int n0 = blockIdx.x * T; n0 < A.num_nodes; n0 += gridDim.x * T) {
+    const int nt = min(T, A.num_nodes - n0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nt * NE; e += blockDim.x)
+      sx[e] = reinterpret_cast<const double2*>(x)[(size_t)n0 * NE + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nt * NE; e += blockDim.x) {
+      const int te = e / NE;
+      const int r = e - te * NE;
+      const int b = r / N;
+      const int a = r - b * N;
+      const int n = n0 + te;
+      const size_t g = (size_t)n * NE + r;
+      reinterpret_cast<double2*>(y)[g] = general_elem(A, x, sx + te * NE, sH, sHd, n, r, a, b,
+                                                      act, trace_row && g == 0);
     }
-    // Unitary part with the unshifted H: -(i/hbar)(Hx - xH^dag).
-    double ox = A.inv_hbar * hi, oy = -A.inv_hbar * hr;
-    if (!A.vacuum) {
-      const double2 v = xn[r];
-      double f = A.mu[n];
-      if (a != b) f += A.delta[a] + A.delta[b];
-      ox -= f * v.x;
-      oy -= f * v.y;
-      // Column b gathers (site b), row a gathers (site a).
-      for (int side = 0; side < 2; ++side) {
-        const int site = side == 0 ? b : a;
-        for (int q = A.site_ptr[site]; q < A.site_ptr[site + 1]; ++q) {
-          const int j = A.site_modes[q];
-          const int k = A.occ[(size_t)n * D + j];
-          const int nx = A.next[(size_t)n * D + j];
-          if (nx >= 0 && (!act || act[nx])) {
-            const double s = A.next_s[j * L + k];
-            const double2 u = reinterpret_cast<const double2*>(x)[(size_t)nx * NE + r];
-            // column: + i s u ; row: - i s u
-            const double sg = side == 0 ? s : -s;
-            ox -= sg * u.y;
-            oy += sg * u.x;
-          }
-          const int pv = A.prev[(size_t)n * D + j];
-          if (pv >= 0 && (!act || act[pv])) {
-            const double2 c = side == 0 ? A.prev_col[j * L + k] : A.prev_row[j * L + k];
-            const double2 u = reinterpret_cast<const double2*>(x)[(size_t)pv * NE + r];
-            ox -= c.x * u.x - c.y * u.y;
-            oy -= c.x * u.y + c.y * u.x;
-          }
-        }
-      }
-    }
-    if (trace_row && g == 0) {
-      ox = 0;
-      oy = 0;
-      for (int c = 0; c < N; ++c) {
-        ox += xn[c * (N + 1)].x;
-        oy += xn[c * (N + 1)].y;
-      }
-    }
-    reinterpret_cast<double2*>(y)[g] = make_double2(ox, oy);
   }
 }
 
@@ -280,48 +339,116 @@ __device__ __forceinline__ double2 bicg_omega(const BicgScalars* st) {
   return st->tt.x > 0 ? zdiv(st->ts, st->tt) : make_double2(0., 0.);
 }
 
-__global__ void __launch_bounds__(256) bicg_update(int which, double* __restrict__ y, size_t n,
-                                                   const double* x0, const double* x1,
-                                                   const double* x2, const BicgScalars* st) {
-  double2 c[3];
-  int nv;
-  if (which == 0) {
+// y = 1 x0 + c1 x1 (+ c2 x2), terms added left to right; y may be x1 (the
+// p update is in place), so no pointer is restrict.
+template <int WHICH>
+__global__ void __launch_bounds__(256) bicg_update(double* y, size_t n, const double* x0,
+                                                   const double* x1, const double* x2,
+                                                   const BicgScalars* st) {
+  double2 c1, c2 = make_double2(0., 0.);
+  if (WHICH == 0) {
     const double2 beta = zmul(zdiv(st->rho_new, st->rho_old), zdiv(st->alpha, st->omega));
     const double2 bo = zmul(beta, st->omega);
-    c[0] = make_double2(1., 0.);
-    c[1] = beta;
-    c[2] = make_double2(-bo.x, -bo.y);
-    nv = 3;
+    c1 = beta;
+    c2 = make_double2(-bo.x, -bo.y);
   } else {
     const double2 al = zdiv(st->rho_new, st->rv);
-    c[0] = make_double2(1., 0.);
-    c[1] = make_double2(-al.x, -al.y);
-    c[2] = make_double2(0., 0.);
-    nv = 2;
+    c1 = make_double2(-al.x, -al.y);
   }
-  const double* xs[3] = {x0, x1, x2};
+  const double2* a = reinterpret_cast<const double2*>(x0);
+  const double2* b = reinterpret_cast<const double2*>(x1);
+  const double2* d = reinterpret_cast<const double2*>(x2);
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    double ar = 0, ai = 0;
-    for (int t = 0; t < nv; ++t) {
-      const double2 v = reinterpret_cast<const double2*>(xs[t])[i];
-      ar += c[t].x * v.x - c[t].y * v.y;
-      ai += c[t].x * v.y + c[t].y * v.x;
+    const double2 u = a[i], v = b[i];
+    double ar = u.x, ai = u.y;
+    ar += c1.x * v.x - c1.y * v.y;
+    ai += c1.x * v.y + c1.y * v.x;
+    if (WHICH == 0) {
+      const double2 w = d[i];
+      ar += c2.x * w.x - c2.y * w.y;
+      ai += c2.x * w.y + c2.y * w.x;
     }
     reinterpret_cast<double2*>(y)[i] = make_double2(ar, ai);
+  }
+}
+
+// Block sum of three values per thread into partial[3 b + c].
+__device__ __forceinline__ void block_sum3(double a0, double a1, double a2,
+                                           double* __restrict__ partial) {
+  __shared__ double s3[3][256];
+  s3[0][threadIdx.x] = a0;
+  s3[1][threadIdx.x] = a1;
+  s3[2][threadIdx.x] = a2;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 3; ++c) s3[c][threadIdx.x] += s3[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = s3[threadIdx.x][0];
+}
+
+// ts = sum conj(t) s, tt = sum |t|^2
+__global__ void __launch_bounds__(256) bicg_dot_ts(const double* __restrict__ t,
+                                                   const double* __restrict__ sv, size_t n,
+                                                   double* __restrict__ partial) {
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double2 p = reinterpret_cast<const double2*>(t)[i];
+    const double2 q = reinterpret_cast<const double2*>(sv)[i];
+    a0 += p.x * q.x + p.y * q.y;
+    a1 += p.x * q.y - p.y * q.x;
+    a2 += p.x * p.x + p.y * p.y;
+  }
+  block_sum3(a0, a1, a2, partial);
+}
+
+// which 0: st->ts = (P0, P1), st->tt = (P2, 0); which 1: st->rn2 = (P0, 0),
+// st->rho_next = (P1, P2); P = sums of the nb partials (fixed order).
+__global__ void __launch_bounds__(256) bicg_final3(const double* __restrict__ partial, int nb,
+                                                   BicgScalars* st, int which) {
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    a0 += partial[3 * i];
+    a1 += partial[3 * i + 1];
+    a2 += partial[3 * i + 2];
+  }
+  __shared__ double out[3];
+  __shared__ double s3[3][256];
+  s3[0][threadIdx.x] = a0;
+  s3[1][threadIdx.x] = a1;
+  s3[2][threadIdx.x] = a2;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 3; ++c) s3[c][threadIdx.x] += s3[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[threadIdx.x] = s3[threadIdx.x][0];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (which == 0) {
+      st->ts = make_double2(out[0], out[1]);
+      st->tt = make_double2(out[2], 0.);
+    } else {
+      st->rn2 = make_double2(out[0], 0.);
+      st->rho_next = make_double2(out[1], out[2]);
+    }
   }
 }
 
 __global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* __restrict__ r,
                                                const double* __restrict__ p,
                                                const double* __restrict__ sv,
-                                               const double* __restrict__ t, size_t n,
+                                               const double* __restrict__ t,
+                                               const double* __restrict__ rhat, size_t n,
                                                const BicgScalars* st,
                                                double* __restrict__ partial) {
-  __shared__ double sr[256];
   if (st->done) return;  // uniform: every block returns
   const double2 al = zdiv(st->rho_new, st->rv), om = bicg_omega(st);
-  double acc = 0;
+  double acc = 0, hr = 0, hi = 0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const double2 xv = reinterpret_cast<const double2*>(x)[i];
@@ -340,17 +467,11 @@ __global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* _
     ri += -om.x * tv.y - om.y * tv.x;
     reinterpret_cast<double2*>(r)[i] = make_double2(rr, ri);
     acc += rr * rr + ri * ri;
+    const double2 h = reinterpret_cast<const double2*>(rhat)[i];
+    hr += h.x * rr + h.y * ri;
+    hi += h.x * ri - h.y * rr;
   }
-  sr[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sr[threadIdx.x] += sr[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = sr[0];
-    partial[2 * blockIdx.x + 1] = 0;
-  }
+  block_sum3(acc, hr, hi, partial);
 }
 
 __global__ void bicg_finish(BicgScalars* st, double* hist, double tol, int max_iter) {
@@ -359,6 +480,7 @@ __global__ void bicg_finish(BicgScalars* st, double* hist, double tol, int max_i
   st->alpha = al;
   st->omega = om;
   st->rho_old = st->rho_new;
+  st->rho_new = st->rho_next;
   const double res = sqrt(st->rn2.x);
   st->res = res;
   const int it = st->it + 1;
@@ -438,6 +560,19 @@ hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* 
 
 hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int trace_row,
                             hipStream_t s, const unsigned char* active) {
+  // LDS form when H, H^dagger and at least one node fit in 64 KB.
+  int T = a.NE > 0 ? 256 / a.NE : 0;
+  if (T < 1) T = 1;
+  while (T > 1 && (size_t)(T + 2) * a.NE * sizeof(double2) > 64 * 1024) --T;
+  const size_t sh = (size_t)(T + 2) * a.NE * sizeof(double2);
+  if (sh <= 64 * 1024) {
+    long long blocks = ((long long)a.num_nodes + T - 1) / T;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(dev::rhs_general_lds, dim3((unsigned)blocks), dim3(256), sh, s, a, x, y,
+                       trace_row, active, T);
+    return hipGetLastError();
+  }
   const size_t tot = (size_t)a.num_nodes * a.NE;
   int blocks = (int)((tot + 255) / 256);
   if (blocks > 65536) blocks = 65536;
@@ -494,18 +629,26 @@ hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, co
                             const double* x2, const BicgScalars* st, hipStream_t s) {
   int blocks = (int)((n + 255) / 256);
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(dev::bicg_update, dim3(blocks), dim3(256), 0, s, which, y, n, x0, x1, x2,
-                     st);
+  if (which == 0)
+    hipLaunchKernelGGL(dev::bicg_update<0>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st);
+  else
+    hipLaunchKernelGGL(dev::bicg_update<1>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, BicgScalars* st,
+                           double* partial3, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_dot_ts, dim3(nblocks), dim3(256), 0, s, t, sv, n, partial3);
+  hipLaunchKernelGGL(dev::bicg_final3, dim3(1), dim3(256), 0, s, partial3, nblocks, st, 0);
   return hipGetLastError();
 }
 
 hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
-                        const double* t, size_t n, BicgScalars* st, double* partial, int nblocks,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(dev::bicg_xr, dim3(nblocks), dim3(256), 0, s, x, r, p, sv, t, n, st,
-                     partial);
-  hipLaunchKernelGGL(dev::dot_final, dim3(1), dim3(256), 0, s, partial, nblocks,
-                     reinterpret_cast<double*>(&st->rn2));
+                        const double* t, const double* rhat, size_t n, BicgScalars* st,
+                        double* partial3, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_xr, dim3(nblocks), dim3(256), 0, s, x, r, p, sv, t, rhat, n, st,
+                     partial3);
+  hipLaunchKernelGGL(dev::bicg_final3, dim3(1), dim3(256), 0, s, partial3, nblocks, st, 1);
   return hipGetLastError();
 }
 
