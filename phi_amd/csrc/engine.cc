@@ -1004,9 +1004,14 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   double* hist = nullptr;
   double* part3 = nullptr;
   if (!rc) rc = Check(hipMalloc((void**)&st, sizeof(BicgScalars)), "alloc bicgstab scalars");
+  // Partials: rhat.v (2 per block), t.s / t.t (3), |r|^2 / rhat.r (3).
+  constexpr int kRedBlocks = 512;
   if (!rc)
-    rc = Check(hipMalloc((void**)&part3, (size_t)red_blocks_ * 3 * sizeof(double)),
+    rc = Check(hipMalloc((void**)&part3, (size_t)kRedBlocks * 8 * sizeof(double)),
                "alloc bicgstab partials");
+  double* p_rv = part3;
+  double* p_ts = part3 + 2 * kRedBlocks;
+  double* p_xr = part3 + 5 * kRedBlocks;
   if (!rc)
     rc = Check(hipMalloc((void**)&hist, (size_t)(std::max(max_iter, 0) + 1) * sizeof(double)),
                "alloc bicgstab history");
@@ -1028,18 +1033,22 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   while (!rc && !hs.done) {
     for (int b = 0; b < kBatch && !rc; ++b) {
       // p = r + beta (p - omega v)
-      rc = Check(LaunchBicgUpdate(0, pp, n, r, pp, vv, st, stream_), "p update");
+      rc = Check(LaunchBicgUpdate(0, pp, n, r, pp, vv, st, nullptr, 0, stream_), "p update");
       if (!rc) rc = RhsDevice(pp, vv, 1);
-      if (!rc) rc = dev_dot(rh, vv, &st->rv);
-      // s = r - alpha v
-      if (!rc) rc = Check(LaunchBicgUpdate(1, s, n, r, vv, nullptr, st, stream_), "s update");
-      if (!rc) rc = RhsDevice(s, t, 1);
-      if (!rc) rc = Check(LaunchBicgDotTS(t, s, n, st, part3, red_blocks_, stream_), "t.s, t.t");
-      // x = x + alpha p + omega s, r = s - omega t, |r|^2, rhat . r
+      if (!rc) rc = Check(LaunchDotPartial(rh, vv, n, p_rv, kRedBlocks, stream_), "rhat.v");
+      // s = r - alpha v (alpha from the rhat.v partials)
       if (!rc)
-        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, st, part3, red_blocks_, stream_),
+        rc = Check(LaunchBicgUpdate(1, s, n, r, vv, nullptr, st, p_rv, kRedBlocks, stream_),
+                   "s update");
+      if (!rc) rc = RhsDevice(s, t, 1);
+      if (!rc) rc = Check(LaunchBicgDotTS(t, s, n, p_ts, kRedBlocks, stream_), "t.s, t.t");
+      // x = x + alpha p + omega s, r = s - omega t, partials of |r|^2, rhat.r
+      if (!rc)
+        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, st, p_ts, p_xr, kRedBlocks, stream_),
                    "x/r update");
-      if (!rc) rc = Check(LaunchBicgFinish(st, hist, tol, max_iter, stream_), "finish");
+      if (!rc)
+        rc = Check(LaunchBicgFinish(st, p_xr, kRedBlocks, hist, tol, max_iter, stream_),
+                   "finish");
     }
     if (!rc) rc = Check(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_),
                         "bicgstab scalars");

@@ -123,26 +123,34 @@ struct BicgScalars {
   int done;                       // converged, breakdown (omega = 0) or max_iter
   int it;                         // finished iterations
 };
+// Reductions are two-pass with a fixed order: a kernel writes one partial
+// per block, and the kernel that needs the value sums the partials itself
+// (every block the same way), so no single-block reduction kernel runs.
+//
+// Partials of conj(u) . v (2 doubles per block).
+hipError_t LaunchDotPartial(const double* u, const double* v, size_t n, double* partial,
+                            int nblocks, hipStream_t s);
 // which = 0: y = x0 + beta x1 - beta omega x2 (p update; beta from the
 // previous iteration's scalars and rho_new); which = 1: y = x0 - alpha x1
-// (s update; alpha = rho_new / rv).
+// (s update; rv = sum of the nb partials of rhat . v, kept in st,
+// alpha = rho_new / rv).
 hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
-                            const double* x2, const BicgScalars* st, hipStream_t s);
-// st->ts = t . s (conjugating t) and st->tt = |t|^2 in one pass (two-pass
-// reduction over nblocks partials of 3 doubles).
-hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, BicgScalars* st,
-                           double* partial3, int nblocks, hipStream_t s);
-// x += alpha p + omega s, r = s - omega t (omega = ts / tt, 0 if tt = 0),
-// st->rn2 = |r|^2 and st->rho_next = rhat . r in the same pass (nblocks
-// partials of 3 doubles); nothing once st->done.
+                            const double* x2, BicgScalars* st, const double* rv_part, int nb,
+                            hipStream_t s);
+// Partials of t . s (conjugating t) and |t|^2 (3 doubles per block).
+hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, double* partial3,
+                           int nblocks, hipStream_t s);
+// Sums the t.s / t.t partials into st->ts, st->tt; x += alpha p + omega s,
+// r = s - omega t (omega = ts / tt, 0 if tt = 0), and partials of |r|^2
+// and rhat . r; nothing once st->done.
 hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
                         const double* t, const double* rhat, size_t n, BicgScalars* st,
-                        double* partial3, int nblocks, hipStream_t s);
-// One thread: alpha, omega, rho_old <- this iteration's, rho_new <-
-// rho_next; res = |r|; hist[it] = res; done when !(res > tol), omega = 0
-// or it = max_iter.
-hipError_t LaunchBicgFinish(BicgScalars* st, double* hist, double tol, int max_iter,
-                            hipStream_t s);
+                        const double* ts_part, double* partial3, int nblocks, hipStream_t s);
+// One block: st->rn2, st->rho_next from the x/r partials; alpha, omega,
+// rho_old <- this iteration's, rho_new <- rho_next; res = |r|; hist[it] =
+// res; done when !(res > tol), omega = 0 or it = max_iter.
+hipError_t LaunchBicgFinish(BicgScalars* st, const double* xr_part, int nblocks, double* hist,
+                            double tol, int max_iter, hipStream_t s);
 
 }  // namespace phi
 

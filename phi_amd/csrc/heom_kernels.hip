@@ -339,12 +339,58 @@ __device__ __forceinline__ double2 bicg_omega(const BicgScalars* st) {
   return st->tt.x > 0 ? zdiv(st->ts, st->tt) : make_double2(0., 0.);
 }
 
+// Sum of component c of nb partials (stride NC), fixed order; every
+// thread of the (256-thread) block gets the NC results.  Called by all.
+template <int NC>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ partial, int nb,
+                                                double* out) {
+  __shared__ double sp[NC][256];
+  __shared__ double so[NC];
+  double acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] += partial[NC * i + c];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) sp[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sp[c][threadIdx.x] += sp[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < NC) so[threadIdx.x] = sp[threadIdx.x][0];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = so[c];
+}
+
+// Block sum of three values per thread into partial[3 b + c].
+__device__ __forceinline__ void block_sum3(double a0, double a1, double a2,
+                                           double* __restrict__ partial) {
+  __shared__ double s3[3][256];
+  s3[0][threadIdx.x] = a0;
+  s3[1][threadIdx.x] = a1;
+  s3[2][threadIdx.x] = a2;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 3; ++c) s3[c][threadIdx.x] += s3[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = s3[threadIdx.x][0];
+}
+
 // y = 1 x0 + c1 x1 (+ c2 x2), terms added left to right; y may be x1 (the
-// p update is in place), so no pointer is restrict.
+// p update is in place), so no pointer is restrict.  WHICH 1 first reduces
+// the partials of rhat . v into rv (block 0 keeps it in st).
 template <int WHICH>
 __global__ void __launch_bounds__(256) bicg_update(double* y, size_t n, const double* x0,
                                                    const double* x1, const double* x2,
-                                                   const BicgScalars* st) {
+                                                   BicgScalars* st, const double* rv_part,
+                                                   int nb) {
   double2 c1, c2 = make_double2(0., 0.);
   if (WHICH == 0) {
     const double2 beta = zmul(zdiv(st->rho_new, st->rho_old), zdiv(st->alpha, st->omega));
@@ -352,7 +398,11 @@ __global__ void __launch_bounds__(256) bicg_update(double* y, size_t n, const do
     c1 = beta;
     c2 = make_double2(-bo.x, -bo.y);
   } else {
-    const double2 al = zdiv(st->rho_new, st->rv);
+    double pr[2];
+    reduce_partials<2>(rv_part, nb, pr);
+    const double2 rv = make_double2(pr[0], pr[1]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rv = rv;
+    const double2 al = zdiv(st->rho_new, rv);
     c1 = make_double2(-al.x, -al.y);
   }
   const double2* a = reinterpret_cast<const double2*>(x0);
@@ -373,22 +423,6 @@ __global__ void __launch_bounds__(256) bicg_update(double* y, size_t n, const do
   }
 }
 
-// Block sum of three values per thread into partial[3 b + c].
-__device__ __forceinline__ void block_sum3(double a0, double a1, double a2,
-                                           double* __restrict__ partial) {
-  __shared__ double s3[3][256];
-  s3[0][threadIdx.x] = a0;
-  s3[1][threadIdx.x] = a1;
-  s3[2][threadIdx.x] = a2;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int c = 0; c < 3; ++c) s3[c][threadIdx.x] += s3[c][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = s3[threadIdx.x][0];
-}
-
 // ts = sum conj(t) s, tt = sum |t|^2
 __global__ void __launch_bounds__(256) bicg_dot_ts(const double* __restrict__ t,
                                                    const double* __restrict__ sv, size_t n,
@@ -405,49 +439,26 @@ __global__ void __launch_bounds__(256) bicg_dot_ts(const double* __restrict__ t,
   block_sum3(a0, a1, a2, partial);
 }
 
-// which 0: st->ts = (P0, P1), st->tt = (P2, 0); which 1: st->rn2 = (P0, 0),
-// st->rho_next = (P1, P2); P = sums of the nb partials (fixed order).
-__global__ void __launch_bounds__(256) bicg_final3(const double* __restrict__ partial, int nb,
-                                                   BicgScalars* st, int which) {
-  double a0 = 0, a1 = 0, a2 = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    a0 += partial[3 * i];
-    a1 += partial[3 * i + 1];
-    a2 += partial[3 * i + 2];
-  }
-  __shared__ double out[3];
-  __shared__ double s3[3][256];
-  s3[0][threadIdx.x] = a0;
-  s3[1][threadIdx.x] = a1;
-  s3[2][threadIdx.x] = a2;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int c = 0; c < 3; ++c) s3[c][threadIdx.x] += s3[c][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x < 3) out[threadIdx.x] = s3[threadIdx.x][0];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (which == 0) {
-      st->ts = make_double2(out[0], out[1]);
-      st->tt = make_double2(out[2], 0.);
-    } else {
-      st->rn2 = make_double2(out[0], 0.);
-      st->rho_next = make_double2(out[1], out[2]);
-    }
-  }
-}
-
+// Reduces the t.s / t.t partials (block 0 keeps ts and tt in st), then
+// x = x + alpha p + omega s, r = s - omega t and the partials of |r|^2 and
+// rhat . r.
 __global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* __restrict__ r,
                                                const double* __restrict__ p,
                                                const double* __restrict__ sv,
                                                const double* __restrict__ t,
                                                const double* __restrict__ rhat, size_t n,
-                                               const BicgScalars* st,
-                                               double* __restrict__ partial) {
+                                               BicgScalars* st, const double* __restrict__ ts_part,
+                                               int nb, double* __restrict__ partial) {
   if (st->done) return;  // uniform: every block returns
-  const double2 al = zdiv(st->rho_new, st->rv), om = bicg_omega(st);
+  double q[3];
+  reduce_partials<3>(ts_part, nb, q);
+  const double2 ts = make_double2(q[0], q[1]), tt = make_double2(q[2], 0.);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->ts = ts;
+    st->tt = tt;
+  }
+  const double2 al = zdiv(st->rho_new, st->rv);
+  const double2 om = tt.x > 0 ? zdiv(ts, tt) : make_double2(0., 0.);
   double acc = 0, hr = 0, hi = 0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -474,8 +485,16 @@ __global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* _
   block_sum3(acc, hr, hi, partial);
 }
 
-__global__ void bicg_finish(BicgScalars* st, double* hist, double tol, int max_iter) {
-  if (threadIdx.x != 0 || st->done) return;
+// One block: reduces the |r|^2 / rhat . r partials, then closes the
+// iteration.
+__global__ void __launch_bounds__(256) bicg_finish(BicgScalars* st, const double* xr_part, int nb,
+                                                   double* hist, double tol, int max_iter) {
+  if (st->done) return;  // uniform
+  double q[3];
+  reduce_partials<3>(xr_part, nb, q);
+  if (threadIdx.x != 0) return;
+  st->rn2 = make_double2(q[0], 0.);
+  st->rho_next = make_double2(q[1], q[2]);
   const double2 al = zdiv(st->rho_new, st->rv), om = bicg_omega(st);
   st->alpha = al;
   st->omega = om;
@@ -625,36 +644,44 @@ hipError_t LaunchDot(const double* u, const double* v, size_t n, int mode, doubl
   return hipGetLastError();
 }
 
-hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
-                            const double* x2, const BicgScalars* st, hipStream_t s) {
-  int blocks = (int)((n + 255) / 256);
-  if (blocks > 8192) blocks = 8192;
-  if (which == 0)
-    hipLaunchKernelGGL(dev::bicg_update<0>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st);
-  else
-    hipLaunchKernelGGL(dev::bicg_update<1>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st);
+hipError_t LaunchDotPartial(const double* u, const double* v, size_t n, double* partial,
+                            int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(dev::dot_partial, dim3(nblocks), dim3(256), 0, s, u, v, n, 0, partial);
   return hipGetLastError();
 }
 
-hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, BicgScalars* st,
-                           double* partial3, int nblocks, hipStream_t s) {
+hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
+                            const double* x2, BicgScalars* st, const double* rv_part, int nb,
+                            hipStream_t s) {
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  if (which == 0)
+    hipLaunchKernelGGL(dev::bicg_update<0>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st,
+                       rv_part, nb);
+  else
+    hipLaunchKernelGGL(dev::bicg_update<1>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st,
+                       rv_part, nb);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, double* partial3,
+                           int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(dev::bicg_dot_ts, dim3(nblocks), dim3(256), 0, s, t, sv, n, partial3);
-  hipLaunchKernelGGL(dev::bicg_final3, dim3(1), dim3(256), 0, s, partial3, nblocks, st, 0);
   return hipGetLastError();
 }
 
 hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
                         const double* t, const double* rhat, size_t n, BicgScalars* st,
-                        double* partial3, int nblocks, hipStream_t s) {
+                        const double* ts_part, double* partial3, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(dev::bicg_xr, dim3(nblocks), dim3(256), 0, s, x, r, p, sv, t, rhat, n, st,
-                     partial3);
-  hipLaunchKernelGGL(dev::bicg_final3, dim3(1), dim3(256), 0, s, partial3, nblocks, st, 1);
+                     ts_part, nblocks, partial3);
   return hipGetLastError();
 }
 
-hipError_t LaunchBicgFinish(BicgScalars* st, double* hist, double tol, int max_iter,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(dev::bicg_finish, dim3(1), dim3(64), 0, s, st, hist, tol, max_iter);
+hipError_t LaunchBicgFinish(BicgScalars* st, const double* xr_part, int nblocks, double* hist,
+                            double tol, int max_iter, hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_finish, dim3(1), dim3(256), 0, s, st, xr_part, nblocks, hist, tol,
+                     max_iter);
   return hipGetLastError();
 }
 
