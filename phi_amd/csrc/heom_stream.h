@@ -255,19 +255,26 @@ __device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restr
   }
 }
 
+// Element f of the MFMA A-fragments of H (zero padding outside N x N).
+__device__ __forceinline__ double2 stream_h_frag(const RhsArgs& A, const double2* H, int f) {
+  const int N = A.N;
+  const int KS = (N + 3) >> 2;
+  const int lane = f & 63, ms = f >> 6;
+  const int mt = ms / KS, st = ms - mt * KS;
+  const int row = 16 * mt + (lane & 15), k = (lane >> 4) * KS + st;
+  double2 h = make_double2(0., 0.);
+  if (row < N && k < N) h = H[row + k * N];
+  return h;
+}
+
 // H of `member` (shifted by its mean diagonal) as MFMA A-fragments in LDS.
 template <typename TB>
 __device__ __forceinline__ void stream_load_h(const RhsArgs& A, const StreamLds<TB>& s,
                                               int member) {
-  const int N = A.N;
-  const int KS = (N + 3) >> 2, nf = StreamFragCount(N);
+  const int nf = StreamFragCount(A.N);
   const double2* H = A.H + (size_t)member * A.NE;
   for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-    const int lane = f & 63, ms = f >> 6;
-    const int mt = ms / KS, st = ms - mt * KS;
-    const int row = 16 * mt + (lane & 15), k = (lane >> 4) * KS + st;
-    double2 h = make_double2(0., 0.);
-    if (row < N && k < N) h = H[row + k * N];
+    const double2 h = stream_h_frag(A, H, f);
     s.hfr[f] = (TB)h.x;
     s.hfi[f] = (TB)h.y;
   }
@@ -301,19 +308,47 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
   }
   if (tile >= tile_end) return;
 
-  // Per-block constants.
-  for (int e = threadIdx.x; e < D * L; e += blockDim.x) {
-    s.pcol[e] = ld<TB, double>(reinterpret_cast<const double*>(A.prev_col), e);
-    s.ns[e] = (TB)A.next_s[e];
-  }
-  for (int e = threadIdx.x; e < N; e += blockDim.x) s.delta[e] = (TB)A.delta[e];
-
+  // Prologue: every load that needs no other load is issued first (the
+  // first tile's elements, mu and edge records; each thread's first share
+  // of the per-block constants and of H's fragments), then the LDS stores,
+  // then the neighbour gathers.  Two trips to memory before the first
+  // barrier instead of one per table: short kernels are mostly prologue.
   double emax = 0;
   Regs cur;
   TileInfo ti = stream_tile<ENS>(A, tile);
   int hmember = ti.member;
-  stream_load_h<TB>(A, s, hmember);
   stream_load1<TB, TI, EPT, MPS>(A, in, ti, cur);
+  const int DL = D * L, nf = StreamFragCount(N);
+  const double2* Hm = A.H + (size_t)hmember * NE;
+  const int e0 = threadIdx.x;
+  V pc0 = mk<TB>((TB)0, (TB)0);
+  double ns0 = 0, dl0 = 0;
+  double2 h0 = make_double2(0., 0.);
+  if (e0 < DL) {
+    pc0 = ld<TB, double>(reinterpret_cast<const double*>(A.prev_col), e0);
+    ns0 = A.next_s[e0];
+  }
+  if (e0 < N) dl0 = A.delta[e0];
+  if (e0 < nf) h0 = stream_h_frag(A, Hm, e0);
+  if (e0 < DL) {
+    s.pcol[e0] = pc0;
+    s.ns[e0] = (TB)ns0;
+  }
+  if (e0 < N) s.delta[e0] = (TB)dl0;
+  if (e0 < nf) {
+    s.hfr[e0] = (TB)h0.x;
+    s.hfi[e0] = (TB)h0.y;
+  }
+  for (int e = e0 + (int)blockDim.x; e < DL; e += blockDim.x) {
+    s.pcol[e] = ld<TB, double>(reinterpret_cast<const double*>(A.prev_col), e);
+    s.ns[e] = (TB)A.next_s[e];
+  }
+  for (int e = e0 + (int)blockDim.x; e < N; e += blockDim.x) s.delta[e] = (TB)A.delta[e];
+  for (int f = e0 + (int)blockDim.x; f < nf; f += blockDim.x) {
+    const double2 h = stream_h_frag(A, Hm, f);
+    s.hfr[f] = (TB)h.x;
+    s.hfi[f] = (TB)h.y;
+  }
   stream_load2<TB, TI, EPT, MPS, ENS>(A, in, ti, cur);
   __syncthreads();  // per-block tables visible
 
