@@ -208,6 +208,11 @@ int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n_complex);
 int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
 /* nsteps RK4 steps of size dt on the device state. */
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
+/* phi_hip_rk4_steps bracketed by two HIP events on the context's stream;
+ * *ms = milliseconds between them (returns after the second event).  No
+ * PHI counterpart: PHI times its loop with the host clock (Timer,
+ * timer.cc:50-75). */
+int phi_hip_rk4_steps_timed(phi_ctx* c, long long nsteps, double dt, double* ms);
 /* Stage-kernel timing: stage s (RK4 order, RK4's buffers) launched nsteps
  * times back to back between two HIP events on the context's stream;
  * ms[s] = total milliseconds of those nsteps launches.  The state is

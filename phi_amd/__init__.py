@@ -83,6 +83,7 @@ def lib():
         "phi_hip_rhs": (ctypes.c_int, [vp, dp, dp, sz]),
         "phi_hip_rk4_steps": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double]),
         "phi_hip_rk4_stage_times": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
+        "phi_hip_rk4_steps_timed": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
         "phi_hip_rkf45_attempt": (ctypes.c_int, [vp, dp, dp]),
         "phi_hip_rkf45_accept": (ctypes.c_int, [vp]),
         "phi_hip_active_flags": (ctypes.c_int, [vp, ip, ctypes.c_int, ip]),
@@ -376,6 +377,13 @@ class Context:
 
     def rk4_steps(self, nsteps, dt):
         _check(lib().phi_hip_rk4_steps(self._h, int(nsteps), float(dt)))
+
+    def rk4_steps_timed(self, nsteps, dt):
+        """rk4_steps between two HIP events on the context's stream;
+        returns the milliseconds between them."""
+        ms = np.zeros(1)
+        _check(lib().phi_hip_rk4_steps_timed(self._h, int(nsteps), float(dt), _dp(ms)))
+        return float(ms[0])
 
     def rk4_stage_times(self, nsteps, dt):
         """Milliseconds of each stage kernel (4 values) summed over nsteps

@@ -601,6 +601,12 @@ int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt) {
   return rc ? Err(rc, c->e->last_error()) : Ok();
 }
 
+int phi_hip_rk4_steps_timed(phi_ctx* c, long long nsteps, double dt, double* ms) {
+  if (!c || !ms || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
+  int rc = c->e->Rk4StepsTimed(nsteps, dt, ms);
+  return rc ? Err(rc, c->e->last_error()) : Ok();
+}
+
 int phi_hip_rk4_stage_times(phi_ctx* c, long long nsteps, double dt, double* ms) {
   if (!c || !ms || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
   int rc = c->e->Rk4StageTimes(nsteps, dt, ms);

@@ -778,6 +778,23 @@ int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int sna
   return kOk;
 }
 
+int Engine::Rk4StepsTimed(long long nsteps, double dt, double* ms) {
+  *ms = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = Check(hipEventCreate(&e0), "event");
+  if (!rc) rc = Check(hipEventCreate(&e1), "event");
+  if (!rc) rc = Check(hipEventRecord(e0, stream_), "record");
+  if (!rc) rc = Rk4Steps(nsteps, dt, 0, 0);
+  if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");
+  if (!rc) rc = Check(hipEventSynchronize(e1), "event sync");
+  float t = 0;
+  if (!rc) rc = Check(hipEventElapsedTime(&t, e0, e1), "elapsed");
+  if (!rc) *ms = t;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
 int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   if (explicit_path()) {
     err_ = "stage timing covers the fused stage kernels (not adaptive truncation, "

@@ -139,6 +139,8 @@ class Engine {
   // Stage-kernel timing: stage s launched nsteps times back to back between
   // two HIP events; ms[s] = their total.  The state is restored.
   int Rk4StageTimes(long long nsteps, double dt, double ms[4]);
+  // Rk4Steps between two HIP events on the engine's stream; *ms = elapsed.
+  int Rk4StepsTimed(long long nsteps, double dt, double* ms);
   int HistoryCount();
   int history_capacity() const { return hist_capacity_; }
   int DrainHistory(std::vector<std::complex<double>>* out);  // and resets it

@@ -381,6 +381,11 @@ def test_stage_times_keep_state(prec):
     ctx.rk4_steps(5, dt)
     ref.rk4_steps(5, dt)
     assert np.array_equal(ctx.state(), ref.state())
+    # The event-timed loop runs the same steps.
+    ms = ctx.rk4_steps_timed(9, dt)
+    ref.rk4_steps(9, dt)
+    assert ms > 0
+    assert np.array_equal(ctx.state(), ref.state())
     ctx.close()
     ref.close()
 
