@@ -294,7 +294,11 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
   const StreamLds<TB> s = stream_carve<TB>(smem, N, D, L, T);
   // Tiles of this block: a contiguous range of the descriptor list for an
   // ensemble (members change rarely, so does H), else every gridDim.x-th
-  // tile of the hierarchy (neighbouring tiles in flight together).
+  // tile of the hierarchy (neighbouring tiles in flight together), starting
+  // from an XCD-aware slot: workgroups are dealt to the 8 XCDs round robin
+  // (block b on XCD b % 8), so the slots of XCD x form one contiguous run
+  // [start_x, start_x + count_x) and the neighbour ADOs that nearby tiles
+  // share are fetched into one XCD's L2 instead of several.
   const int ntiles = ENS ? A.ntiles : (A.num_nodes + T - 1) / T;
   int tile, tile_end, step;
   if constexpr (ENS) {
@@ -302,9 +306,10 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     tile_end = (int)(((long long)(blockIdx.x + 1) * ntiles) / gridDim.x);
     step = 1;
   } else {
-    tile = blockIdx.x;
+    const int G = gridDim.x, x = blockIdx.x & 7, base = G >> 3, r = G & 7;
+    tile = x * base + min(x, r) + (int)(blockIdx.x >> 3);
     tile_end = ntiles;
-    step = gridDim.x;
+    step = G;
   }
   if (tile >= tile_end) return;
 
