@@ -53,8 +53,8 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        # (re)built when missing or older than its source; the GPU box gets
-        # the library built here (no compiler step there)
+        # (re)built when missing or older than its source; a machine that
+        # only runs the tests uses the library built beforehand
         if not os.path.exists(LIB) or (os.path.exists(SRC) and
                                        os.path.getmtime(LIB) < os.path.getmtime(SRC)):
             build()

@@ -446,7 +446,7 @@ int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
       t = tile > 0 ? tile : std::max(1, std::min(64, th / NE));
       while (t > 1 && TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) --t;
       if (TileLdsBytes(N, D, L, t, f == 1, with_h) > kLdsBudget) {
-        err_ = "NumStates too large for the LDS tile of this build";
+        err_ = "NumStates too large for one node's LDS tile (64 KB budget)";
         return kErrUnsupported;
       }
       a.v1_tile[f] = t;
