@@ -1022,13 +1022,17 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   double* part3 = nullptr;
   if (!rc) rc = Check(hipMalloc((void**)&st, sizeof(BicgScalars)), "alloc bicgstab scalars");
   // Partials: rhat.v (2 per block), t.s / t.t (3), |r|^2 / rhat.r (3).
+  // When the matvec's LDS form fits, it writes the rhat.v and t.s / t.t
+  // partials itself (one record per matvec block; mb blocks).
   constexpr int kRedBlocks = 512;
+  const int mb = RhsGeneralDotBlocks(args_);
+  const int nb_dot = mb > 0 ? mb : kRedBlocks;
+  const size_t cap = (size_t)std::max(kRedBlocks, mb);
   if (!rc)
-    rc = Check(hipMalloc((void**)&part3, (size_t)kRedBlocks * 8 * sizeof(double)),
-               "alloc bicgstab partials");
+    rc = Check(hipMalloc((void**)&part3, cap * 8 * sizeof(double)), "alloc bicgstab partials");
   double* p_rv = part3;
-  double* p_ts = part3 + 2 * kRedBlocks;
-  double* p_xr = part3 + 5 * kRedBlocks;
+  double* p_ts = part3 + 2 * cap;
+  double* p_xr = part3 + 5 * cap;
   if (!rc)
     rc = Check(hipMalloc((void**)&hist, (size_t)(std::max(max_iter, 0) + 1) * sizeof(double)),
                "alloc bicgstab history");
@@ -1051,17 +1055,26 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
     for (int b = 0; b < kBatch && !rc; ++b) {
       // p = r + beta (p - omega v)
       rc = Check(LaunchBicgUpdate(0, pp, n, r, pp, vv, st, nullptr, 0, stream_), "p update");
-      if (!rc) rc = RhsDevice(pp, vv, 1);
-      if (!rc) rc = Check(LaunchDotPartial(rh, vv, n, p_rv, kRedBlocks, stream_), "rhat.v");
+      // v = A p with the rhat.v partials
+      if (!rc && mb > 0)
+        rc = Check(LaunchRhsGeneralDot(args_, pp, vv, 1, 1, rh, p_rv, stream_), "A p, rhat.v");
+      if (!rc && mb == 0) rc = RhsDevice(pp, vv, 1);
+      if (!rc && mb == 0)
+        rc = Check(LaunchDotPartial(rh, vv, n, p_rv, kRedBlocks, stream_), "rhat.v");
       // s = r - alpha v (alpha from the rhat.v partials)
       if (!rc)
-        rc = Check(LaunchBicgUpdate(1, s, n, r, vv, nullptr, st, p_rv, kRedBlocks, stream_),
+        rc = Check(LaunchBicgUpdate(1, s, n, r, vv, nullptr, st, p_rv, nb_dot, stream_),
                    "s update");
-      if (!rc) rc = RhsDevice(s, t, 1);
-      if (!rc) rc = Check(LaunchBicgDotTS(t, s, n, p_ts, kRedBlocks, stream_), "t.s, t.t");
+      // t = A s with the t.s / t.t partials
+      if (!rc && mb > 0)
+        rc = Check(LaunchRhsGeneralDot(args_, s, t, 1, 2, s, p_ts, stream_), "A s, t.s, t.t");
+      if (!rc && mb == 0) rc = RhsDevice(s, t, 1);
+      if (!rc && mb == 0)
+        rc = Check(LaunchBicgDotTS(t, s, n, p_ts, kRedBlocks, stream_), "t.s, t.t");
       // x = x + alpha p + omega s, r = s - omega t, partials of |r|^2, rhat.r
       if (!rc)
-        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, st, p_ts, p_xr, kRedBlocks, stream_),
+        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, st, p_ts, nb_dot, p_xr, kRedBlocks,
+                                stream_),
                    "x/r update");
       if (!rc)
         rc = Check(LaunchBicgFinish(st, p_xr, kRedBlocks, hist, tol, max_iter, stream_),

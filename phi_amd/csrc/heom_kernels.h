@@ -95,6 +95,14 @@ hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* 
 // node flags (inactive neighbours' terms skipped), or null.
 hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int trace_row,
                             hipStream_t s, const unsigned char* active = nullptr);
+// Blocks of LaunchRhsGeneralDot (one partial record each), 0 when its LDS
+// form does not fit (then it returns hipErrorInvalidValue).
+int RhsGeneralDotBlocks(const RhsArgs& a);
+// LaunchRhsGeneral (no active flags) fused with the next reduction of the
+// steady-state solver, per block: dot 1 = sum conj(w) y (2 doubles),
+// dot 2 = sum conj(y) w and |y|^2 (3 doubles).
+hipError_t LaunchRhsGeneralDot(const RhsArgs& a, const double* x, double* y, int trace_row,
+                               int dot, const double* w, double* partial, hipStream_t s);
 hipError_t LaunchSnapshot(const double* src, double* hist, int* counter, int ne, int capacity,
                           hipStream_t s);
 // hist[slot] = mean over the members of their node 0 (members in order).
@@ -145,7 +153,8 @@ hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, double* 
 // and rhat . r; nothing once st->done.
 hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
                         const double* t, const double* rhat, size_t n, BicgScalars* st,
-                        const double* ts_part, double* partial3, int nblocks, hipStream_t s);
+                        const double* ts_part, int nb_ts, double* partial3, int nblocks,
+                        hipStream_t s);
 // One block: st->rn2, st->rho_next from the x/r partials; alpha, omega,
 // rho_old <- this iteration's, rho_new <- rho_next; res = |r|; hist[it] =
 // res; done when !(res > tol), omega = 0 or it = max_iter.

@@ -116,13 +116,34 @@ __global__ void __launch_bounds__(256) rhs_general(RhsArgs A, const double* __re
   }
 }
 
+// partial[NC * block + c] = sum over the (256-thread) block of a[c], in a
+// fixed order.  Called by every thread of the block.
+template <int NC>
+__device__ __forceinline__ void block_sum(const double* a, double* __restrict__ partial) {
+  __shared__ double sb[NC][256];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) sb[c][threadIdx.x] = a[c];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sb[c][threadIdx.x] += sb[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < NC) partial[NC * blockIdx.x + threadIdx.x] = sb[threadIdx.x][0];
+}
+
 // The same with H, H^dagger and the ADOs of T nodes per block in LDS (one
 // load of each per element instead of 2N): for N*N*(T+2) complex within
-// the LDS budget.
+// the LDS budget.  DOT fuses the solver's next reduction into the matvec:
+// 1 = the block's sum of conj(w) y (2 doubles per block), 2 = the block's
+// sums of conj(y) w and |y|^2 (3 doubles per block).
+template <int DOT>
 __global__ void __launch_bounds__(256) rhs_general_lds(RhsArgs A, const double* __restrict__ x,
                                                        double* __restrict__ y, int trace_row,
                                                        const unsigned char* __restrict__ act,
-                                                       int T) {
+                                                       int T, const double* __restrict__ w,
+                                                       double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = A.N, NE = A.NE;
   double2* sH = reinterpret_cast<double2*>(smem);
@@ -132,6 +153,7 @@ __global__ void __launch_bounds__(256) rhs_general_lds(RhsArgs A, const double* 
     sH[e] = A.Hraw[e];
     sHd[e] = A.Hadj[e];
   }
+  double acc[3] = {0, 0, 0};
   for (int n0 = blockIdx.x * T; n0 < A.num_nodes; n0 += gridDim.x * T) {
     const int nt = min(T, A.num_nodes - n0);
     __syncthreads();
@@ -145,10 +167,23 @@ __global__ void __launch_bounds__(256) rhs_general_lds(RhsArgs A, const double* 
       const int a = r - b * N;
       const int n = n0 + te;
       const size_t g = (size_t)n * NE + r;
-      reinterpret_cast<double2*>(y)[g] = general_elem(A, x, sx + te * NE, sH, sHd, n, r, a, b,
-                                                      act, trace_row && g == 0);
+      const double2 v = general_elem(A, x, sx + te * NE, sH, sHd, n, r, a, b, act,
+                                     trace_row && g == 0);
+      reinterpret_cast<double2*>(y)[g] = v;
+      if constexpr (DOT == 1) {
+        const double2 q = reinterpret_cast<const double2*>(w)[g];
+        acc[0] += q.x * v.x + q.y * v.y;
+        acc[1] += q.x * v.y - q.y * v.x;
+      } else if constexpr (DOT == 2) {
+        const double2 q = reinterpret_cast<const double2*>(w)[g];
+        acc[0] += v.x * q.x + v.y * q.y;
+        acc[1] += v.x * q.y - v.y * q.x;
+        acc[2] += v.x * v.x + v.y * v.y;
+      }
     }
   }
+  if constexpr (DOT == 1) block_sum<2>(acc, partial);
+  if constexpr (DOT == 2) block_sum<3>(acc, partial);
 }
 
 // --- Small utility kernels ----------------------------------------------------
@@ -577,26 +612,60 @@ hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* 
               : LaunchRkf45F64(a, stage, in, rho, bd, c, s);
 }
 
-hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int trace_row,
-                            hipStream_t s, const unsigned char* active) {
-  // LDS form when H, H^dagger and at least one node fit in 64 KB.
+// Nodes per block and blocks of rhs_general_lds; blocks = 0 when H,
+// H^dagger and one node do not fit in 64 KB of LDS.
+static void RhsLdsShape(const RhsArgs& a, int* T_out, int* blocks_out) {
   int T = a.NE > 0 ? 256 / a.NE : 0;
   if (T < 1) T = 1;
   while (T > 1 && (size_t)(T + 2) * a.NE * sizeof(double2) > 64 * 1024) --T;
-  const size_t sh = (size_t)(T + 2) * a.NE * sizeof(double2);
-  if (sh <= 64 * 1024) {
-    long long blocks = ((long long)a.num_nodes + T - 1) / T;
+  long long blocks = 0;
+  if ((size_t)(T + 2) * a.NE * sizeof(double2) <= 64 * 1024) {
+    blocks = ((long long)a.num_nodes + T - 1) / T;
     if (blocks > 16384) blocks = 16384;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(dev::rhs_general_lds, dim3((unsigned)blocks), dim3(256), sh, s, a, x, y,
-                       trace_row, active, T);
+  }
+  *T_out = T;
+  *blocks_out = (int)blocks;
+}
+
+// The fused form keeps the plain form's blocks: capping them at 512 (the
+// consumers' partial count otherwise) cost the 7-site matvec 9.6 -> 16.7 us.
+int RhsGeneralDotBlocks(const RhsArgs& a) {
+  int T, blocks;
+  RhsLdsShape(a, &T, &blocks);
+  return blocks;
+}
+
+hipError_t LaunchRhsGeneralDot(const RhsArgs& a, const double* x, double* y, int trace_row,
+                               int dot, const double* w, double* partial, hipStream_t s) {
+  int T, blocks;
+  RhsLdsShape(a, &T, &blocks);
+  if (blocks == 0 || (dot != 1 && dot != 2)) return hipErrorInvalidValue;
+  const size_t sh = (size_t)(T + 2) * a.NE * sizeof(double2);
+  if (dot == 1)
+    hipLaunchKernelGGL(dev::rhs_general_lds<1>, dim3((unsigned)blocks), dim3(256), sh, s, a, x,
+                       y, trace_row, (const unsigned char*)nullptr, T, w, partial);
+  else
+    hipLaunchKernelGGL(dev::rhs_general_lds<2>, dim3((unsigned)blocks), dim3(256), sh, s, a, x,
+                       y, trace_row, (const unsigned char*)nullptr, T, w, partial);
+  return hipGetLastError();
+}
+
+hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int trace_row,
+                            hipStream_t s, const unsigned char* active) {
+  // LDS form when H, H^dagger and at least one node fit in 64 KB.
+  int T, blocks;
+  RhsLdsShape(a, &T, &blocks);
+  if (blocks > 0) {
+    const size_t sh = (size_t)(T + 2) * a.NE * sizeof(double2);
+    hipLaunchKernelGGL(dev::rhs_general_lds<0>, dim3((unsigned)blocks), dim3(256), sh, s, a, x,
+                       y, trace_row, active, T, (const double*)nullptr, (double*)nullptr);
     return hipGetLastError();
   }
   const size_t tot = (size_t)a.num_nodes * a.NE;
-  int blocks = (int)((tot + 255) / 256);
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(dev::rhs_general, dim3(blocks), dim3(256), 0, s, a, x, y, trace_row,
-                     active);
+  int gb = (int)((tot + 255) / 256);
+  if (gb > 65536) gb = 65536;
+  hipLaunchKernelGGL(dev::rhs_general, dim3(gb), dim3(256), 0, s, a, x, y, trace_row, active);
   return hipGetLastError();
 }
 
@@ -672,9 +741,10 @@ hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, double* 
 
 hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
                         const double* t, const double* rhat, size_t n, BicgScalars* st,
-                        const double* ts_part, double* partial3, int nblocks, hipStream_t s) {
+                        const double* ts_part, int nb_ts, double* partial3, int nblocks,
+                        hipStream_t s) {
   hipLaunchKernelGGL(dev::bicg_xr, dim3(nblocks), dim3(256), 0, s, x, r, p, sv, t, rhat, n, st,
-                     ts_part, nblocks, partial3);
+                     ts_part, nb_ts, partial3);
   return hipGetLastError();
 }
 
