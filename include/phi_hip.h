@@ -2,9 +2,11 @@
  * phi_hip.h - C ABI of the MI355X HEOM integrator (drop-in for PHI's hot path).
  *
  * Every entry point returns 0 on success or a negative status (PHI_HIP_E*),
- * never throws, takes plain pointers and sizes, and is single-threaded per
- * handle.  Complex arrays are interleaved doubles (re, im); matrices are
- * column-major (element (i, j) at index i + j*N), as in PHI.  ADO state
+ * never throws (each body runs inside a catch-all that maps exceptions to
+ * PHI_HIP_ENOMEM / PHI_HIP_EINTERNAL), takes plain pointers and sizes, and
+ * is single-threaded per handle.  Complex arrays are interleaved doubles
+ * (re, im); matrices are column-major (element (i, j) at index i + j*N), as
+ * in PHI.  ADO state
  * arrays hold every node in PHI's id order (hierarchy.cc:45-155), N*N complex
  * per node.
  *
@@ -53,6 +55,8 @@ extern "C" {
 #define PHI_HIP_EIO (-5)
 #define PHI_HIP_ENOTCONVERGED (-6)
 #define PHI_HIP_ENODEVICE (-7)
+#define PHI_HIP_ENOMEM (-8)    /* host memory exhausted */
+#define PHI_HIP_EINTERNAL (-9) /* an unexpected internal error (message says which) */
 
 typedef struct phi_params phi_params;
 typedef struct phi_integrator phi_integrator;

@@ -17,7 +17,10 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libphi_hip.so")
+# PHI_HIP_LIBRARY names another build of the same library (the host-code
+# sanitizer build of phi_amd/build_sanitized.py); the default is the
+# in-tree build.
+LIB_PATH = os.environ.get("PHI_HIP_LIBRARY") or os.path.join(HERE, "libphi_hip.so")
 EXE_PATH = os.path.join(HERE, "phi")
 
 RUN_METHODS = ["BROKENINPUT", "NONE", "RK4", "RKF45", "RK4SPECTRUM", "RKF45SPECTRUM",

@@ -129,6 +129,22 @@ std::string* StringField(phi::PhiParameters& p, const std::string& n) {
   return nullptr;
 }
 
+// Every entry point runs its body through Guarded: no C++ exception crosses
+// the C boundary (allocation failures of host tables, for instance, become
+// PHI_HIP_ENOMEM with a message).
+template <class F>
+int Guarded(const char* what, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return Err(PHI_HIP_ENOMEM, std::string(what) + ": out of host memory");
+  } catch (const std::exception& e) {
+    return Err(PHI_HIP_EINTERNAL, std::string(what) + ": " + e.what());
+  } catch (...) {
+    return Err(PHI_HIP_EINTERNAL, std::string(what) + ": unknown exception");
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -143,357 +159,420 @@ int phi_hip_device_count(void) {
 }
 
 int phi_hip_main(int argc, const char* const* argv) {
-  phi::PhiParameters p;
-  p.ReadCommandLine(argc, argv);
-  phi::HierarchyIntegrator it(&p);
-  it.Launch();
-  if (!p.are_parameters_valid) return Err(PHI_HIP_EPARAMS, "invalid parameters");
-  if (it.status()) return Err(it.status(), it.error());
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    phi::PhiParameters p;
+    p.ReadCommandLine(argc, argv);
+    phi::HierarchyIntegrator it(&p);
+    it.Launch();
+    if (!p.are_parameters_valid) return Err(PHI_HIP_EPARAMS, "invalid parameters");
+    if (it.status()) return Err(it.status(), it.error());
+    return Ok();
+  });
 }
 
 int phi_hip_params_new(phi_params** out) {
-  if (!out) return Err(PHI_HIP_EINVAL, "null output pointer");
-  *out = new phi_params;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!out) return Err(PHI_HIP_EINVAL, "null output pointer");
+    *out = new phi_params;
+    return Ok();
+  });
 }
 
 int phi_hip_params_read(const char* path, const char* method, int threads, phi_params** out) {
-  if (!path || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  phi_params* h = new phi_params;
-  *out = h;
-  {
-    std::ifstream probe(path);
-    if (!probe.is_open()) {
-      h->p.are_parameters_valid = false;
-      return Err(PHI_HIP_EIO, std::string("cannot open ") + path);
+  return Guarded(__func__, [&]() -> int {
+    if (!path || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    phi_params* h = new phi_params;
+    *out = h;
+    {
+      std::ifstream probe(path);
+      if (!probe.is_open()) {
+        h->p.are_parameters_valid = false;
+        return Err(PHI_HIP_EIO, std::string("cannot open ") + path);
+      }
     }
-  }
-  if (threads > 0) h->p.num_threads = threads;
-  if (method) {
-    bool known;
-    h->p.run_method = MethodFromWord(method, &known);
-  }
-  h->p.ReadInputFile(path);
-  if (!h->p.are_parameters_valid || h->p.run_method == phi::BROKENINPUT)
-    return Err(PHI_HIP_EPARAMS, "invalid parameters");
-  return Ok();
+    if (threads > 0) h->p.num_threads = threads;
+    if (method) {
+      bool known;
+      h->p.run_method = MethodFromWord(method, &known);
+    }
+    h->p.ReadInputFile(path);
+    if (!h->p.are_parameters_valid || h->p.run_method == phi::BROKENINPUT)
+      return Err(PHI_HIP_EPARAMS, "invalid parameters");
+    return Ok();
+  });
 }
 
 void phi_hip_params_free(phi_params* p) { delete p; }
 
 int phi_hip_params_get_double(const phi_params* h, const char* name, double* out) {
-  if (!h || !name || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  double* f = DoubleField(const_cast<phi::PhiParameters&>(h->p), name);
-  if (!f) return Err(PHI_HIP_EINVAL, std::string("unknown double field ") + name);
-  *out = *f;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    double* f = DoubleField(const_cast<phi::PhiParameters&>(h->p), name);
+    if (!f) return Err(PHI_HIP_EINVAL, std::string("unknown double field ") + name);
+    *out = *f;
+    return Ok();
+  });
 }
 
 int phi_hip_params_set_double(phi_params* h, const char* name, double v) {
-  if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
-  double* f = DoubleField(h->p, name);
-  if (!f) return Err(PHI_HIP_EINVAL, std::string("unknown double field ") + name);
-  *f = v;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
+    double* f = DoubleField(h->p, name);
+    if (!f) return Err(PHI_HIP_EINVAL, std::string("unknown double field ") + name);
+    *f = v;
+    return Ok();
+  });
 }
 
 int phi_hip_params_get_int(const phi_params* h, const char* name, int* out) {
-  if (!h || !name || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  phi::PhiParameters& p = const_cast<phi::PhiParameters&>(h->p);
-  const std::string n = name;
-  if (int* f = IntField(p, n)) { *out = *f; return Ok(); }
-  if (bool* b = BoolField(p, n)) { *out = *b ? 1 : 0; return Ok(); }
-  if (n == "run_method") { *out = (int)p.run_method; return Ok(); }
-  if (n == "bulk_precision") { *out = (int)p.bulk_precision; return Ok(); }
-  return Err(PHI_HIP_EINVAL, "unknown int field " + n);
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    phi::PhiParameters& p = const_cast<phi::PhiParameters&>(h->p);
+    const std::string n = name;
+    if (int* f = IntField(p, n)) { *out = *f; return Ok(); }
+    if (bool* b = BoolField(p, n)) { *out = *b ? 1 : 0; return Ok(); }
+    if (n == "run_method") { *out = (int)p.run_method; return Ok(); }
+    if (n == "bulk_precision") { *out = (int)p.bulk_precision; return Ok(); }
+    return Err(PHI_HIP_EINVAL, "unknown int field " + n);
+  });
 }
 
 int phi_hip_params_set_int(phi_params* h, const char* name, int v) {
-  if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
-  const std::string n = name;
-  if (int* f = IntField(h->p, n)) { *f = v; return Ok(); }
-  if (bool* b = BoolField(h->p, n)) { *b = v != 0; return Ok(); }
-  if (n == "run_method") { h->p.run_method = (phi::RunMethod)v; return Ok(); }
-  if (n == "bulk_precision") {
-    h->p.bulk_precision = v ? phi::kBulkFp32 : phi::kBulkFp64;
-    return Ok();
-  }
-  return Err(PHI_HIP_EINVAL, "unknown int field " + n);
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
+    const std::string n = name;
+    if (int* f = IntField(h->p, n)) { *f = v; return Ok(); }
+    if (bool* b = BoolField(h->p, n)) { *b = v != 0; return Ok(); }
+    if (n == "run_method") { h->p.run_method = (phi::RunMethod)v; return Ok(); }
+    if (n == "bulk_precision") {
+      h->p.bulk_precision = v ? phi::kBulkFp32 : phi::kBulkFp64;
+      return Ok();
+    }
+    return Err(PHI_HIP_EINVAL, "unknown int field " + n);
+  });
 }
 
 int phi_hip_params_get_array(const phi_params* h, const char* name, double* out, size_t cap,
                              size_t* n) {
-  if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
-  phi::PhiParameters& p = const_cast<phi::PhiParameters&>(h->p);
-  if (std::string(name) == "multiple_independent_bath_indices") {
-    const auto& v = p.multiple_independent_bath_indices;
-    if (n) *n = v.size();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
+    phi::PhiParameters& p = const_cast<phi::PhiParameters&>(h->p);
+    if (std::string(name) == "multiple_independent_bath_indices") {
+      const auto& v = p.multiple_independent_bath_indices;
+      if (n) *n = v.size();
+      if (out) {
+        if (cap < v.size()) return Err(PHI_HIP_EINVAL, "buffer too small");
+        for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+      }
+      return Ok();
+    }
+    std::vector<double>* a = RealArray(p, name);
+    if (!a) return Err(PHI_HIP_EINVAL, std::string("unknown array field ") + name);
+    if (n) *n = a->size();
     if (out) {
-      if (cap < v.size()) return Err(PHI_HIP_EINVAL, "buffer too small");
-      for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+      if (cap < a->size()) return Err(PHI_HIP_EINVAL, "buffer too small");
+      std::memcpy(out, a->data(), a->size() * sizeof(double));
     }
     return Ok();
-  }
-  std::vector<double>* a = RealArray(p, name);
-  if (!a) return Err(PHI_HIP_EINVAL, std::string("unknown array field ") + name);
-  if (n) *n = a->size();
-  if (out) {
-    if (cap < a->size()) return Err(PHI_HIP_EINVAL, "buffer too small");
-    std::memcpy(out, a->data(), a->size() * sizeof(double));
-  }
-  return Ok();
+  });
 }
 
 int phi_hip_params_set_array(phi_params* h, const char* name, const double* v, size_t n) {
-  if (!h || !name || (n && !v)) return Err(PHI_HIP_EINVAL, "null argument");
-  if (std::string(name) == "multiple_independent_bath_indices") {
-    h->p.multiple_independent_bath_indices.assign(n, 0);
-    for (size_t i = 0; i < n; ++i) h->p.multiple_independent_bath_indices[i] = (int)v[i];
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name || (n && !v)) return Err(PHI_HIP_EINVAL, "null argument");
+    if (std::string(name) == "multiple_independent_bath_indices") {
+      h->p.multiple_independent_bath_indices.assign(n, 0);
+      for (size_t i = 0; i < n; ++i) h->p.multiple_independent_bath_indices[i] = (int)v[i];
+      return Ok();
+    }
+    std::vector<double>* a = RealArray(h->p, name);
+    if (!a) return Err(PHI_HIP_EINVAL, std::string("unknown array field ") + name);
+    a->assign(v, v + n);
     return Ok();
-  }
-  std::vector<double>* a = RealArray(h->p, name);
-  if (!a) return Err(PHI_HIP_EINVAL, std::string("unknown array field ") + name);
-  a->assign(v, v + n);
-  return Ok();
+  });
 }
 
 int phi_hip_params_get_matrix(const phi_params* h, const char* name, double* out, size_t cap,
                               size_t* n) {
-  if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
-  auto* m = Matrix(const_cast<phi::PhiParameters&>(h->p), name);
-  if (!m) return Err(PHI_HIP_EINVAL, std::string("unknown matrix field ") + name);
-  if (n) *n = m->size();
-  if (out) {
-    if (cap < m->size()) return Err(PHI_HIP_EINVAL, "buffer too small");
-    std::memcpy(out, (const void*)m->data(), m->size() * sizeof(std::complex<double>));
-  }
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name) return Err(PHI_HIP_EINVAL, "null argument");
+    auto* m = Matrix(const_cast<phi::PhiParameters&>(h->p), name);
+    if (!m) return Err(PHI_HIP_EINVAL, std::string("unknown matrix field ") + name);
+    if (n) *n = m->size();
+    if (out) {
+      if (cap < m->size()) return Err(PHI_HIP_EINVAL, "buffer too small");
+      std::memcpy(out, (const void*)m->data(), m->size() * sizeof(std::complex<double>));
+    }
+    return Ok();
+  });
 }
 
 int phi_hip_params_set_matrix(phi_params* h, const char* name, const double* v, size_t n) {
-  if (!h || !name || (n && !v)) return Err(PHI_HIP_EINVAL, "null argument");
-  auto* m = Matrix(h->p, name);
-  if (!m) return Err(PHI_HIP_EINVAL, std::string("unknown matrix field ") + name);
-  m->resize(n);
-  for (size_t i = 0; i < n; ++i) (*m)[i] = std::complex<double>(v[2 * i], v[2 * i + 1]);
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name || (n && !v)) return Err(PHI_HIP_EINVAL, "null argument");
+    auto* m = Matrix(h->p, name);
+    if (!m) return Err(PHI_HIP_EINVAL, std::string("unknown matrix field ") + name);
+    m->resize(n);
+    for (size_t i = 0; i < n; ++i) (*m)[i] = std::complex<double>(v[2 * i], v[2 * i + 1]);
+    return Ok();
+  });
 }
 
 int phi_hip_params_get_string(const phi_params* h, const char* name, char* out, size_t cap) {
-  if (!h || !name || !out || cap == 0) return Err(PHI_HIP_EINVAL, "null argument");
-  std::string* s = StringField(const_cast<phi::PhiParameters&>(h->p), name);
-  if (!s) return Err(PHI_HIP_EINVAL, std::string("unknown string field ") + name);
-  if (s->size() + 1 > cap) return Err(PHI_HIP_EINVAL, "buffer too small");
-  std::memcpy(out, s->c_str(), s->size() + 1);
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name || !out || cap == 0) return Err(PHI_HIP_EINVAL, "null argument");
+    std::string* s = StringField(const_cast<phi::PhiParameters&>(h->p), name);
+    if (!s) return Err(PHI_HIP_EINVAL, std::string("unknown string field ") + name);
+    if (s->size() + 1 > cap) return Err(PHI_HIP_EINVAL, "buffer too small");
+    std::memcpy(out, s->c_str(), s->size() + 1);
+    return Ok();
+  });
 }
 
 int phi_hip_params_set_string(phi_params* h, const char* name, const char* v) {
-  if (!h || !name || !v) return Err(PHI_HIP_EINVAL, "null argument");
-  std::string* s = StringField(h->p, name);
-  if (!s) return Err(PHI_HIP_EINVAL, std::string("unknown string field ") + name);
-  *s = v;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !name || !v) return Err(PHI_HIP_EINVAL, "null argument");
+    std::string* s = StringField(h->p, name);
+    if (!s) return Err(PHI_HIP_EINVAL, std::string("unknown string field ") + name);
+    *s = v;
+    return Ok();
+  });
 }
 
 int phi_hip_params_fix(phi_params* h, const char* method) {
-  if (!h) return Err(PHI_HIP_EINVAL, "null argument");
-  if (method) {
-    bool known;
-    h->p.run_method = MethodFromWord(method, &known);
-    if (!known) return Err(PHI_HIP_EINVAL, std::string("unknown method ") + method);
-  }
-  h->p.FixIncompatibleCombinations();
-  h->p.matrix_count = phi::GetHierarchyNodeCountEstimate(
-      h->p.hierarchy_truncation_level, h->p.num_bath_couplings * h->p.num_matsubara_terms);
-  return h->p.are_parameters_valid ? Ok() : Err(PHI_HIP_EPARAMS, "invalid parameters");
+  return Guarded(__func__, [&]() -> int {
+    if (!h) return Err(PHI_HIP_EINVAL, "null argument");
+    if (method) {
+      bool known;
+      h->p.run_method = MethodFromWord(method, &known);
+      if (!known) return Err(PHI_HIP_EINVAL, std::string("unknown method ") + method);
+    }
+    h->p.FixIncompatibleCombinations();
+    h->p.matrix_count = phi::GetHierarchyNodeCountEstimate(
+        h->p.hierarchy_truncation_level, h->p.num_bath_couplings * h->p.num_matsubara_terms);
+    return h->p.are_parameters_valid ? Ok() : Err(PHI_HIP_EPARAMS, "invalid parameters");
+  });
 }
 
 int phi_hip_params_header(const phi_params* h, int num_nodes, char* out, size_t cap, size_t* len) {
-  if (!h) return Err(PHI_HIP_EINVAL, "null argument");
-  std::ostringstream o;
-  h->p.Write(&o, num_nodes);
-  o << "#Total Number of matrices:" << num_nodes << "\n";
-  o << phi::MemoryTable(h->p, num_nodes);
-  o << "#-------------\n";
-  const std::string s = o.str();
-  if (len) *len = s.size();
-  if (out) {
-    if (cap < s.size() + 1) return Err(PHI_HIP_EINVAL, "buffer too small");
-    std::memcpy(out, s.c_str(), s.size() + 1);
-  }
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h) return Err(PHI_HIP_EINVAL, "null argument");
+    std::ostringstream o;
+    h->p.Write(&o, num_nodes);
+    o << "#Total Number of matrices:" << num_nodes << "\n";
+    o << phi::MemoryTable(h->p, num_nodes);
+    o << "#-------------\n";
+    const std::string s = o.str();
+    if (len) *len = s.size();
+    if (out) {
+      if (cap < s.size() + 1) return Err(PHI_HIP_EINVAL, "buffer too small");
+      std::memcpy(out, s.c_str(), s.size() + 1);
+    }
+    return Ok();
+  });
 }
 
 int phi_hip_validate(const char* kind, const char* text, int* valid) {
-  if (!kind || !text || !valid) return Err(PHI_HIP_EINVAL, "null argument");
-  const std::string k = kind, t = text;
-  bool ok = true, v;
-  if (k == "int") v = phi::prm::ValidInteger(t, &ok);
-  else if (k == "float") v = phi::prm::ValidFloat(t, &ok);
-  else if (k == "bool") v = phi::prm::ValidBool(t, &ok);
-  else if (k == "reals") v = phi::prm::ValidRealList(t, &ok);
-  else if (k == "complexes") v = phi::prm::ValidComplexList(t, &ok);
-  else return Err(PHI_HIP_EINVAL, "unknown kind " + k);
-  *valid = (v && ok) ? 1 : 0;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!kind || !text || !valid) return Err(PHI_HIP_EINVAL, "null argument");
+    const std::string k = kind, t = text;
+    bool ok = true, v;
+    if (k == "int") v = phi::prm::ValidInteger(t, &ok);
+    else if (k == "float") v = phi::prm::ValidFloat(t, &ok);
+    else if (k == "bool") v = phi::prm::ValidBool(t, &ok);
+    else if (k == "reals") v = phi::prm::ValidRealList(t, &ok);
+    else if (k == "complexes") v = phi::prm::ValidComplexList(t, &ok);
+    else return Err(PHI_HIP_EINVAL, "unknown kind " + k);
+    *valid = (v && ok) ? 1 : 0;
+    return Ok();
+  });
 }
 
 int phi_hip_integrator_create(phi_params* h, phi_integrator** out) {
-  if (!h || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  phi_integrator* it = new phi_integrator;
-  it->p = &h->p;
-  it->it = new phi::HierarchyIntegrator(&h->p);
-  *out = it;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    phi_integrator* it = new phi_integrator;
+    it->p = &h->p;
+    it->it = new phi::HierarchyIntegrator(&h->p);
+    *out = it;
+    return Ok();
+  });
 }
 
 int phi_hip_integrator_launch(phi_integrator* it) {
-  if (!it) return Err(PHI_HIP_EINVAL, "null argument");
-  it->it->Launch();
-  if (it->it->status()) return Err(it->it->status(), it->it->error());
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!it) return Err(PHI_HIP_EINVAL, "null argument");
+    it->it->Launch();
+    if (it->it->status()) return Err(it->it->status(), it->it->error());
+    return Ok();
+  });
 }
 
 int phi_hip_integrator_density_matrix(phi_integrator* it, double* out, size_t cap) {
-  if (!it || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  const size_t dim = (size_t)it->it->density_matrix_dim();
-  const size_t ne = dim * dim;
-  if (cap < ne) return Err(PHI_HIP_EINVAL, "buffer too small");
-  std::complex<double>* d = it->it->density_matrix();
-  std::memcpy(out, (const void*)d, ne * sizeof(std::complex<double>));
-  delete[] d;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!it || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    const size_t dim = (size_t)it->it->density_matrix_dim();
+    const size_t ne = dim * dim;
+    if (cap < ne) return Err(PHI_HIP_EINVAL, "buffer too small");
+    std::complex<double>* d = it->it->density_matrix();
+    std::memcpy(out, (const void*)d, ne * sizeof(std::complex<double>));
+    delete[] d;
+    return Ok();
+  });
 }
 
 int phi_hip_integrator_state(phi_integrator* it, double* out, size_t cap) {
-  if (!it || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  std::vector<std::complex<double>> s;
-  int rc = it->it->GetState(&s);
-  if (rc) return Err(rc, "no device state (Launch not run or failed)");
-  if (cap < s.size()) return Err(PHI_HIP_EINVAL, "buffer too small");
-  std::memcpy(out, (const void*)s.data(), s.size() * sizeof(std::complex<double>));
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!it || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    std::vector<std::complex<double>> s;
+    int rc = it->it->GetState(&s);
+    if (rc) return Err(rc, "no device state (Launch not run or failed)");
+    if (cap < s.size()) return Err(PHI_HIP_EINVAL, "buffer too small");
+    std::memcpy(out, (const void*)s.data(), s.size() * sizeof(std::complex<double>));
+    return Ok();
+  });
 }
 
 int phi_hip_integrator_info(phi_integrator* it, double* out, size_t cap) {
-  if (!it || !out || cap < 8) return Err(PHI_HIP_EINVAL, "bad argument");
-  const phi::RunInfo& r = it->it->info();
-  out[0] = (double)r.steps;
-  out[1] = (double)r.accepted;
-  out[2] = r.final_time;
-  out[3] = r.final_dt;
-  out[4] = r.seconds;
-  out[5] = r.bicgstab_iterations;
-  out[6] = r.bicgstab_residual;
-  out[7] = r.num_nodes;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!it || !out || cap < 8) return Err(PHI_HIP_EINVAL, "bad argument");
+    const phi::RunInfo& r = it->it->info();
+    out[0] = (double)r.steps;
+    out[1] = (double)r.accepted;
+    out[2] = r.final_time;
+    out[3] = r.final_dt;
+    out[4] = r.seconds;
+    out[5] = r.bicgstab_iterations;
+    out[6] = r.bicgstab_residual;
+    out[7] = r.num_nodes;
+    return Ok();
+  });
 }
 
 void phi_hip_integrator_destroy(phi_integrator* it) {
-  if (!it) return;
-  delete it->it;
-  delete it;
+  try {
+    if (!it) return;
+    delete it->it;
+    delete it;
+  } catch (...) {
+  }
 }
 
 int phi_hip_create(const phi_params* h, int precision, phi_ctx** out) {
-  if (!h || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  phi::PhiParameters p = h->p;
-  p.bulk_precision = precision ? phi::kBulkFp32 : phi::kBulkFp64;
-  phi::Engine* e = nullptr;
-  std::string why;
-  int rc = phi::Engine::Create(p, &e, &why);
-  if (rc) return Err(rc, why);
-  // Node 0 starts at rho0 (InitializeDensityMatrix), all other ADOs at 0.
-  const size_t ne = (size_t)p.num_states * p.num_states;
-  if (p.initial_density_matrix.size() == ne) {
-    std::vector<std::complex<double>> s(e->state_elems(), 0);
-    for (size_t i = 0; i < ne; ++i) s[i] = p.initial_density_matrix[i];
-    rc = e->SetState(s.data());
-    if (rc) {
-      std::string m = e->last_error();
-      delete e;
-      return Err(rc, m);
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    phi::PhiParameters p = h->p;
+    p.bulk_precision = precision ? phi::kBulkFp32 : phi::kBulkFp64;
+    phi::Engine* e = nullptr;
+    std::string why;
+    int rc = phi::Engine::Create(p, &e, &why);
+    if (rc) return Err(rc, why);
+    // Node 0 starts at rho0 (InitializeDensityMatrix), all other ADOs at 0.
+    const size_t ne = (size_t)p.num_states * p.num_states;
+    if (p.initial_density_matrix.size() == ne) {
+      std::vector<std::complex<double>> s(e->state_elems(), 0);
+      for (size_t i = 0; i < ne; ++i) s[i] = p.initial_density_matrix[i];
+      rc = e->SetState(s.data());
+      if (rc) {
+        std::string m = e->last_error();
+        delete e;
+        return Err(rc, m);
+      }
     }
-  }
-  *out = new phi_ctx{e};
-  return Ok();
+    *out = new phi_ctx{e};
+    return Ok();
+  });
 }
 
 int phi_hip_ensemble_create(const phi_params* h, int members, const double* H, const int* depths,
                             int precision, phi_ctx** out) {
-  if (!h || !H || !depths || !out || members < 1) return Err(PHI_HIP_EINVAL, "bad argument");
-  phi::PhiParameters p = h->p;
-  p.bulk_precision = precision ? phi::kBulkFp32 : phi::kBulkFp64;
-  phi::Engine* e = nullptr;
-  std::string why;
-  int rc = phi::Engine::CreateEnsemble(p, members, reinterpret_cast<const std::complex<double>*>(H),
-                                       depths, &e, &why);
-  if (rc) return Err(rc, why);
-  *out = new phi_ctx{e};
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!h || !H || !depths || !out || members < 1) return Err(PHI_HIP_EINVAL, "bad argument");
+    phi::PhiParameters p = h->p;
+    p.bulk_precision = precision ? phi::kBulkFp32 : phi::kBulkFp64;
+    phi::Engine* e = nullptr;
+    std::string why;
+    int rc = phi::Engine::CreateEnsemble(p, members, reinterpret_cast<const std::complex<double>*>(H),
+                                         depths, &e, &why);
+    if (rc) return Err(rc, why);
+    *out = new phi_ctx{e};
+    return Ok();
+  });
 }
 
 int phi_hip_ensemble_members(const phi_ctx* c, int* counts, int cap) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  const int m = c->e->members();
-  if (counts) {
-    const auto& n = c->e->member_counts();
-    for (int i = 0; i < m && i < cap; ++i) counts[i] = n[i];
-  }
-  return m;
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    const int m = c->e->members();
+    if (counts) {
+      const auto& n = c->e->member_counts();
+      for (int i = 0; i < m && i < cap; ++i) counts[i] = n[i];
+    }
+    return m;
+  });
 }
 
 int phi_hip_ensemble_rho0(phi_ctx* c, double* out, size_t n) {
-  if (!c || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  const size_t m = c->e->members() > 0 ? c->e->members() : 1;
-  if (n < m * c->e->N() * c->e->N()) return Err(PHI_HIP_EINVAL, "buffer too small");
-  int rc = c->e->GetMemberRho0(reinterpret_cast<std::complex<double>*>(out));
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    const size_t m = c->e->members() > 0 ? c->e->members() : 1;
+    if (n < m * c->e->N() * c->e->N()) return Err(PHI_HIP_EINVAL, "buffer too small");
+    int rc = c->e->GetMemberRho0(reinterpret_cast<std::complex<double>*>(out));
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_rk4_history(phi_ctx* c, long long nsteps, double dt, int every, double* out,
                         size_t n, int* count) {
-  if (!c || !out || !count || nsteps < 0 || every < 1) return Err(PHI_HIP_EINVAL, "bad argument");
-  phi::Engine* e = c->e;
-  const size_t ne = (size_t)e->N() * e->N();
-  if ((nsteps + every - 1) / every > e->history_capacity())
-    return Err(PHI_HIP_EINVAL, "more snapshots than the device history holds");
-  std::vector<std::complex<double>> h;
-  int rc = e->DrainHistory(&h);  // discard anything recorded before
-  if (rc >= 0) rc = e->Rk4Steps(nsteps, dt, 0, every);
-  if (rc >= 0) rc = e->DrainHistory(&h);
-  if (rc < 0) return Err(rc, e->last_error());
-  const size_t k = h.size() / ne;
-  if (k * ne > n) return Err(PHI_HIP_EINVAL, "history buffer too small");
-  std::memcpy(out, h.data(), h.size() * sizeof(std::complex<double>));
-  *count = (int)k;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !out || !count || nsteps < 0 || every < 1) return Err(PHI_HIP_EINVAL, "bad argument");
+    phi::Engine* e = c->e;
+    const size_t ne = (size_t)e->N() * e->N();
+    if ((nsteps + every - 1) / every > e->history_capacity())
+      return Err(PHI_HIP_EINVAL, "more snapshots than the device history holds");
+    std::vector<std::complex<double>> h;
+    int rc = e->DrainHistory(&h);  // discard anything recorded before
+    if (rc >= 0) rc = e->Rk4Steps(nsteps, dt, 0, every);
+    if (rc >= 0) rc = e->DrainHistory(&h);
+    if (rc < 0) return Err(rc, e->last_error());
+    const size_t k = h.size() / ne;
+    if (k * ne > n) return Err(PHI_HIP_EINVAL, "history buffer too small");
+    std::memcpy(out, h.data(), h.size() * sizeof(std::complex<double>));
+    *count = (int)k;
+    return Ok();
+  });
 }
 
 void phi_hip_destroy(phi_ctx* c) {
-  if (!c) return;
-  delete c->e;
-  delete c;
+  try {
+    if (!c) return;
+    delete c->e;
+    delete c;
+  } catch (...) {
+  }
 }
 
 // ---- Closure network ----------------------------------------------------------
 
 int phi_hip_mlp_create(int n_in, int n_hidden, int n_out, int max_batch, phi_mlp** out) {
-  if (!out) return Err(PHI_HIP_EINVAL, "null argument");
-  phi::Mlp* m = nullptr;
-  std::string why;
-  int rc = phi::Mlp::Create(n_in, n_hidden, n_out, max_batch, &m, &why);
-  if (rc) return Err(rc, why);
-  *out = new phi_mlp{m};
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!out) return Err(PHI_HIP_EINVAL, "null argument");
+    phi::Mlp* m = nullptr;
+    std::string why;
+    int rc = phi::Mlp::Create(n_in, n_hidden, n_out, max_batch, &m, &why);
+    if (rc) return Err(rc, why);
+    *out = new phi_mlp{m};
+    return Ok();
+  });
 }
 
 void phi_hip_mlp_destroy(phi_mlp* m) {
-  if (!m) return;
-  delete m->m;
-  delete m;
+  try {
+    if (!m) return;
+    delete m->m;
+    delete m;
+  } catch (...) {
+  }
 }
 
 long long phi_hip_mlp_param_count(const phi_mlp* m) {
@@ -501,57 +580,73 @@ long long phi_hip_mlp_param_count(const phi_mlp* m) {
 }
 
 int phi_hip_mlp_set_params(phi_mlp* m, const float* flat, size_t n) {
-  if (!m || !flat) return Err(PHI_HIP_EINVAL, "null argument");
-  if ((long long)n != m->m->param_count()) return Err(PHI_HIP_EINVAL, "parameter count mismatch");
-  int rc = m->m->SetParams(flat);
-  return rc ? Err(rc, m->m->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!m || !flat) return Err(PHI_HIP_EINVAL, "null argument");
+    if ((long long)n != m->m->param_count()) return Err(PHI_HIP_EINVAL, "parameter count mismatch");
+    int rc = m->m->SetParams(flat);
+    return rc ? Err(rc, m->m->last_error()) : Ok();
+  });
 }
 
 int phi_hip_mlp_get_params(phi_mlp* m, float* flat, size_t n) {
-  if (!m || !flat) return Err(PHI_HIP_EINVAL, "null argument");
-  if ((long long)n < m->m->param_count()) return Err(PHI_HIP_EINVAL, "buffer too small");
-  int rc = m->m->GetParams(flat);
-  return rc ? Err(rc, m->m->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!m || !flat) return Err(PHI_HIP_EINVAL, "null argument");
+    if ((long long)n < m->m->param_count()) return Err(PHI_HIP_EINVAL, "buffer too small");
+    int rc = m->m->GetParams(flat);
+    return rc ? Err(rc, m->m->last_error()) : Ok();
+  });
 }
 
 int phi_hip_mlp_forward(phi_mlp* m, const float* x, int batch, float* y) {
-  if (!m || !x || !y) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = m->m->Forward(x, batch, y);
-  return rc ? Err(rc, m->m->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!m || !x || !y) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = m->m->Forward(x, batch, y);
+    return rc ? Err(rc, m->m->last_error()) : Ok();
+  });
 }
 
 int phi_hip_mlp_gradients(phi_mlp* m, const float* x, const float* t, int batch, double* loss,
                           float* grads, size_t n) {
-  if (!m || !x || !t || !loss || !grads) return Err(PHI_HIP_EINVAL, "null argument");
-  if ((long long)n < m->m->param_count()) return Err(PHI_HIP_EINVAL, "buffer too small");
-  int rc = m->m->Gradients(x, t, batch, loss, grads);
-  return rc ? Err(rc, m->m->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!m || !x || !t || !loss || !grads) return Err(PHI_HIP_EINVAL, "null argument");
+    if ((long long)n < m->m->param_count()) return Err(PHI_HIP_EINVAL, "buffer too small");
+    int rc = m->m->Gradients(x, t, batch, loss, grads);
+    return rc ? Err(rc, m->m->last_error()) : Ok();
+  });
 }
 
 int phi_hip_mlp_train(phi_mlp* m, const float* X, const float* T, long long samples, int batch,
                       int epochs, float lr, unsigned seed, double* epoch_loss) {
-  if (!m || !X || !T) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = m->m->Train(X, T, samples, batch, epochs, lr, seed, epoch_loss);
-  return rc ? Err(rc, m->m->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!m || !X || !T) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = m->m->Train(X, T, samples, batch, epochs, lr, seed, epoch_loss);
+    return rc ? Err(rc, m->m->last_error()) : Ok();
+  });
 }
 
 int phi_hip_closure_samples(phi_ctx* c, int shallow_depth, float* X, float* T,
                             long long cap_rows, long long* rows) {
-  if (!c || !X || !T || !rows) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = c->e->ClosureSamples(shallow_depth, X, T, cap_rows, rows);
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !X || !T || !rows) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->ClosureSamples(shallow_depth, X, T, cap_rows, rows);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m) {
-  if (!c || !m) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = c->e->AttachClosure(m->m);
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !m) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->AttachClosure(m->m);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_closure_fused(phi_ctx* c, int on) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  c->e->set_closure_fused(on != 0);
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    c->e->set_closure_fused(on != 0);
+    return Ok();
+  });
 }
 
 int phi_hip_num_nodes(const phi_ctx* c) { return c ? c->e->num_nodes() : PHI_HIP_EINVAL; }
@@ -566,122 +661,156 @@ int phi_hip_get_index(const phi_ctx* c, int* out, size_t cap) {
 }
 
 int phi_hip_set_state(phi_ctx* c, const double* s, size_t n) {
-  if (!c || !s) return Err(PHI_HIP_EINVAL, "null argument");
-  if (n != c->e->state_elems()) return Err(PHI_HIP_EINVAL, "state size mismatch");
-  int rc = c->e->SetState(reinterpret_cast<const std::complex<double>*>(s));
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !s) return Err(PHI_HIP_EINVAL, "null argument");
+    if (n != c->e->state_elems()) return Err(PHI_HIP_EINVAL, "state size mismatch");
+    int rc = c->e->SetState(reinterpret_cast<const std::complex<double>*>(s));
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_get_state(phi_ctx* c, double* s, size_t n) {
-  if (!c || !s) return Err(PHI_HIP_EINVAL, "null argument");
-  if (n != c->e->state_elems()) return Err(PHI_HIP_EINVAL, "state size mismatch");
-  int rc = c->e->GetState(reinterpret_cast<std::complex<double>*>(s));
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !s) return Err(PHI_HIP_EINVAL, "null argument");
+    if (n != c->e->state_elems()) return Err(PHI_HIP_EINVAL, "state size mismatch");
+    int rc = c->e->GetState(reinterpret_cast<std::complex<double>*>(s));
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n) {
-  if (!c || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  if (n < (size_t)c->e->N() * c->e->N()) return Err(PHI_HIP_EINVAL, "buffer too small");
-  int rc = c->e->GetNode0(reinterpret_cast<std::complex<double>*>(out));
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    if (n < (size_t)c->e->N() * c->e->N()) return Err(PHI_HIP_EINVAL, "buffer too small");
+    int rc = c->e->GetNode0(reinterpret_cast<std::complex<double>*>(out));
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n) {
-  if (!c || !in || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  if (n != c->e->state_elems()) return Err(PHI_HIP_EINVAL, "state size mismatch");
-  int rc = c->e->Rhs(reinterpret_cast<const std::complex<double>*>(in),
-                     reinterpret_cast<std::complex<double>*>(out));
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !in || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    if (n != c->e->state_elems()) return Err(PHI_HIP_EINVAL, "state size mismatch");
+    int rc = c->e->Rhs(reinterpret_cast<const std::complex<double>*>(in),
+                       reinterpret_cast<std::complex<double>*>(out));
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt) {
-  if (!c || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
-  int rc = c->e->Rk4Steps(nsteps, dt, 0, 0);
-  if (!rc) rc = c->e->Synchronize();
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
+    int rc = c->e->Rk4Steps(nsteps, dt, 0, 0);
+    if (!rc) rc = c->e->Synchronize();
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_rk4_steps_timed(phi_ctx* c, long long nsteps, double dt, double* ms) {
-  if (!c || !ms || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
-  int rc = c->e->Rk4StepsTimed(nsteps, dt, ms);
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !ms || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
+    int rc = c->e->Rk4StepsTimed(nsteps, dt, ms);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_rk4_stage_times(phi_ctx* c, long long nsteps, double dt, double* ms) {
-  if (!c || !ms || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
-  int rc = c->e->Rk4StageTimes(nsteps, dt, ms);
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !ms || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");
+    int rc = c->e->Rk4StageTimes(nsteps, dt, ms);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_rkf45_attempt(phi_ctx* c, const double* coefs, double* err) {
-  if (!c || !coefs || !err) return Err(PHI_HIP_EINVAL, "null argument");
-  phi::Rkf45Coefs cf{};
-  int q = 0;
-  for (int s = 0; s < 5; ++s)
-    for (int i = 0; i <= s; ++i) cf.b[s][i] = coefs[q++];
-  for (int i = 0; i < 4; ++i) cf.c4[i] = coefs[q++];
-  for (int i = 0; i < 5; ++i) cf.c5[i] = coefs[q++];
-  int rc = c->e->Rkf45Attempt(cf, err);
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !coefs || !err) return Err(PHI_HIP_EINVAL, "null argument");
+    phi::Rkf45Coefs cf{};
+    int q = 0;
+    for (int s = 0; s < 5; ++s)
+      for (int i = 0; i <= s; ++i) cf.b[s][i] = coefs[q++];
+    for (int i = 0; i < 4; ++i) cf.c4[i] = coefs[q++];
+    for (int i = 0; i < 5; ++i) cf.c5[i] = coefs[q++];
+    int rc = c->e->Rkf45Attempt(cf, err);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_rkf45_accept(phi_ctx* c) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = c->e->Rkf45Accept();
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->Rkf45Accept();
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_active_flags(phi_ctx* c, int* flags, int n, int* count) {
-  if (!c || !flags || !count) return Err(PHI_HIP_EINVAL, "null argument");
-  if (n < c->e->num_nodes()) return Err(PHI_HIP_EINVAL, "flags buffer too small");
-  const int rc = c->e->ActiveFlags(flags);
-  if (rc < 0) return Err(rc, c->e->last_error());
-  *count = rc;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !flags || !count) return Err(PHI_HIP_EINVAL, "null argument");
+    if (n < c->e->num_nodes()) return Err(PHI_HIP_EINVAL, "flags buffer too small");
+    const int rc = c->e->ActiveFlags(flags);
+    if (rc < 0) return Err(rc, c->e->last_error());
+    *count = rc;
+    return Ok();
+  });
 }
 
 int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, double* resid) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = c->e->SteadyState(tol, max_iter, iters, resid, nullptr);
-  if (rc == phi::kErrNotConverged) return Err(rc, "BiCGSTAB did not converge");
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->SteadyState(tol, max_iter, iters, resid, nullptr);
+    if (rc == phi::kErrNotConverged) return Err(rc, "BiCGSTAB did not converge");
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = c->e->SetLaunch(gemm, tile, threads, blocks);
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->SetLaunch(gemm, tile, threads, blocks);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_set_graphs(phi_ctx* c, int on) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  c->e->set_graphs(on != 0);
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    c->e->set_graphs(on != 0);
+    return Ok();
+  });
 }
 
 int phi_hip_get_graphs(phi_ctx* c, int* on) {
-  if (!c || !on) return Err(PHI_HIP_EINVAL, "null argument");
-  *on = c->e->graphs() ? 1 : 0;
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !on) return Err(PHI_HIP_EINVAL, "null argument");
+    *on = c->e->graphs() ? 1 : 0;
+    return Ok();
+  });
 }
 
 int phi_hip_autotune(phi_ctx* c, int steps, double dt, double* best_ms, int* shape) {
-  if (!c || !best_ms || !shape || steps < 1 || !(dt > 0)) return Err(PHI_HIP_EINVAL, "bad argument");
-  int rc = c->e->Autotune(steps, dt, best_ms, shape);
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !best_ms || !shape || steps < 1 || !(dt > 0)) return Err(PHI_HIP_EINVAL, "bad argument");
+    int rc = c->e->Autotune(steps, dt, best_ms, shape);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 int phi_hip_set_rk4_form(phi_ctx* c, int form) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  if (form != 0 && form != 1) return Err(PHI_HIP_EINVAL, "RK4 form must be 0 or 1");
-  c->e->set_rk4_nested(form == 1);
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    if (form != 0 && form != 1) return Err(PHI_HIP_EINVAL, "RK4 form must be 0 or 1");
+    c->e->set_rk4_nested(form == 1);
+    return Ok();
+  });
 }
 
 int phi_hip_synchronize(phi_ctx* c) {
-  if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-  int rc = c->e->Synchronize();
-  return rc ? Err(rc, c->e->last_error()) : Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->Synchronize();
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
 }
 
 }  // extern "C"

@@ -38,17 +38,19 @@ int Engine::Check(hipError_t e, const char* where) {
 
 int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   *out = nullptr;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-    *why = "no HIP device is visible";
-    return kErrNoDevice;
-  }
+  // The host tables first (sizes are bounded there), then the device.
   Engine* e = new Engine;
   std::string msg = BuildHierarchy(p, &e->h_);
   if (!msg.empty()) {
     *why = msg;
     delete e;
     return kErrInvalidParameters;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    *why = "no HIP device is visible";
+    delete e;
+    return kErrNoDevice;
   }
   const HierarchyTables& h = e->h_;
   const int N = h.N, NE = N * N;
@@ -188,8 +190,9 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
 
 // Edge records: node n's q-th mode j of site b -> {n + e_j, n - e_j, j*L + n_j}.
 // With a closure, the nodes first .. first + nb - 1 of the last tier point
-// their (missing) n + e_j at virtual slot vbase + (n - first)*D + j.
-int Engine::UploadEdges(int first, int nb, int vbase) {
+// their (missing) n + e_j at virtual slot vbase + (n - first)*D + j.  The
+// records go to a new device array, *out.
+int Engine::BuildEdges(int first, int nb, int vbase, int4** out) {
   const HierarchyTables& h = h_;
   const int N = h.N, mps = mps_;
   std::vector<int4> edge((size_t)h.num_nodes * N * mps);
@@ -207,11 +210,23 @@ int Engine::UploadEdges(int first, int nb, int vbase) {
         }
         edge[((size_t)n * N + b) * mps + q] = r;
       }
-  Free(d_edge_);
-  d_edge_ = nullptr;
-  const int rc = Check(Upload(&d_edge_, edge.data(), edge.size()), "upload edges");
-  args_.edge = d_edge_;
+  *out = nullptr;
+  const int rc = Check(Upload(out, edge.data(), edge.size()), "upload edges");
+  if (rc) {
+    Free(*out);
+    *out = nullptr;
+  }
   return rc;
+}
+
+int Engine::UploadEdges(int first, int nb, int vbase) {
+  int4* e = nullptr;
+  const int rc = BuildEdges(first, nb, vbase, &e);
+  if (rc) return rc;
+  Free(d_edge_);
+  d_edge_ = e;
+  args_.edge = d_edge_;
+  return kOk;
 }
 
 int Engine::CreateEnsemble(const PhiParameters& base, int members,
@@ -368,7 +383,7 @@ Engine::~Engine() {
                   d_prev_col_, d_prev_row_, d_delta_, d_H_, d_Hraw_, d_Hadj_, d_edge_, d_rho_, d_aux_,
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
-                  d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_active_, d_kf_[0], d_kf_[1],
+                  d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_cpack_, d_active_, d_kf_[0], d_kf_[1],
                   d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_rot_,
                   d_ops_, d_alpha_, d_corr_, d_tlq_, d_tlsite_};
   for (void* p : ptrs) Free(p);

@@ -28,6 +28,8 @@ enum Status {
   kErrIo = -5,
   kErrNotConverged = -6,
   kErrNoDevice = -7,
+  kErrNoMemory = -8,
+  kErrInternal = -9,
 };
 
 // A time-dependent system Hamiltonian seen by the RK4 step: Update(t) is
@@ -280,7 +282,9 @@ class Engine {
   float* d_cx_ = nullptr;
   float* d_cs_ = nullptr;
   float* d_cy_ = nullptr;
+  void* d_cpack_ = nullptr;  // the attached network's packed parameters
   int ClosurePredict(const void* in, bool in_fp32);
+  int BuildEdges(int closure_first, int closure_nb, int closure_vbase, int4** out);
   int UploadEdges(int closure_first, int closure_nb, int closure_vbase);
   int* d_moff_ = nullptr;
   double2* d_Hm_ = nullptr;

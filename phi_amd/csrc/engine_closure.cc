@@ -123,21 +123,53 @@ int Engine::AttachClosure(Mlp* mlp) {
   const int vbase = h.num_nodes;
   const int span = h.num_nodes + nb * D;
   const long long total = (long long)members_ * span;
-  if (total * NE > (1LL << 31) - 1) {
+  if (total * NE > kMaxStateElems) {
     err_ = "ensemble slab with closure slots too large";
     return kErrUnsupported;
   }
-  DropGraph();
-  int rc = Synchronize();
-  if (rc) return rc;
-  // New slab: each member's nodes followed by its virtual slots.
-  for (int m = 0; m < members_; ++m) {
-    moff_[m] = m * span;
-    mcount_[m] = span;
+  // Everything that can fail runs first, into new buffers; the engine
+  // changes only once all of them exist, so a failed attach leaves it as
+  // it was (state, offsets, edges and tiles).
+  std::vector<int> moff(members_), mcount(members_, span);
+  for (int m = 0; m < members_; ++m) moff[m] = m * span;
+  FusedNet fnet{};
+  void* pack = nullptr;
+  double* rho = nullptr;
+  int* dmoff = nullptr;
+  int4* edge = nullptr;
+  float *cbmi = nullptr, *cx = nullptr, *cs = nullptr, *cy = nullptr;
+  int rc = mlp->Pack(&fnet, &pack);
+  if (rc) {
+    err_ = "closure network: " + mlp->last_error();
+    return rc == kErrUnsupported ? kErrInvalidArgument : rc;
   }
-  slab_nodes_ = (int)total;
+  const size_t bytes = (size_t)total * NE * 2 * sizeof(double);
+  rc = Check(hipMalloc((void**)&rho, bytes), "alloc closure rho");
+  if (!rc) rc = Check(hipMemsetAsync(rho, 0, bytes, stream_), "zero closure rho");
+  if (!rc && (int)rho0_init_.size() == NE)
+    for (int m = 0; m < members_ && !rc; ++m)
+      rc = Check(hipMemcpyAsync(rho + (size_t)moff[m] * NE * 2, rho0_init_.data(),
+                                NE * sizeof(double2), hipMemcpyHostToDevice, stream_),
+                 "initial state");
+  if (!rc) rc = Check(Up(&dmoff, moff.data(), moff.size()), "upload offsets");
+  if (!rc) rc = BuildEdges(first, nb, vbase, &edge);
+  const std::vector<float> bmi = BoundaryIndex(h, first, nb, B);
+  const long long nrows = (long long)members_ * nb;
+  if (!rc) rc = Check(Up(&cbmi, bmi.data(), bmi.size()), "upload boundary indices");
+  if (!rc) rc = Check(hipMalloc((void**)&cx, (size_t)nrows * mlp->n_in() * 4), "alloc X");
+  if (!rc) rc = Check(hipMalloc((void**)&cs, (size_t)nrows * 4), "alloc S");
+  if (!rc) rc = Check(hipMalloc((void**)&cy, (size_t)nrows * mlp->n_out() * 4), "alloc Y");
+  if (!rc) rc = Synchronize();
+  if (rc) {
+    void* ptrs[] = {pack, rho, dmoff, edge, cbmi, cx, cs, cy};
+    for (void* q : ptrs) FreeD(q);
+    return rc;
+  }
+  // Commit: the new slab (each member's nodes followed by its virtual
+  // slots) replaces the old one; stage buffers are reallocated on demand.
+  DropGraph();
   FreeD(d_rho_);
-  d_rho_ = nullptr;
+  d_rho_ = rho;
   FreeD(d_aux_);
   d_aux_ = nullptr;
   FreeD(d_y_[0]);
@@ -151,27 +183,22 @@ int Engine::AttachClosure(Mlp* mlp) {
   d_tmp_ = nullptr;
   bulk_alloc_ = false;
   rkf45_alloc_ = false;
-  const size_t bytes = state_elems() * 2 * sizeof(double);
-  rc = Check(hipMalloc((void**)&d_rho_, bytes), "alloc closure rho");
-  if (!rc) rc = Check(hipMemset(d_rho_, 0, bytes), "zero closure rho");
-  if (!rc && (int)rho0_init_.size() == NE)
-    for (int m = 0; m < members_ && !rc; ++m)
-      rc = Check(hipMemcpy(d_rho_ + (size_t)moff_[m] * NE * 2, rho0_init_.data(),
-                           NE * sizeof(double2), hipMemcpyHostToDevice), "initial state");
-  if (!rc) rc = Check(hipMemcpy(d_moff_, moff_.data(), members_ * sizeof(int),
-                                hipMemcpyHostToDevice), "upload offsets");
-  if (!rc) rc = UploadEdges(first, nb, vbase);
-  const std::vector<float> bmi = BoundaryIndex(h, first, nb, B);
-  const long long nrows = (long long)members_ * nb;
-  if (!rc) rc = Check(Up(&d_cbmi_, bmi.data(), bmi.size()), "upload boundary indices");
-  if (!rc) rc = Check(hipMalloc((void**)&d_cx_, (size_t)nrows * mlp->n_in() * 4), "alloc X");
-  if (!rc) rc = Check(hipMalloc((void**)&d_cs_, (size_t)nrows * 4), "alloc S");
-  if (!rc) rc = Check(hipMalloc((void**)&d_cy_, (size_t)nrows * mlp->n_out() * 4), "alloc Y");
-  if (!rc) {
-    rc = mlp->Pack(&fused_);
-    if (rc) err_ = "closure network: " + mlp->last_error();
-  }
-  if (rc) return rc;
+  FreeD(d_moff_);
+  d_moff_ = dmoff;
+  FreeD(d_edge_);
+  d_edge_ = edge;
+  args_.edge = d_edge_;
+  void* old[] = {d_cbmi_, d_cx_, d_cs_, d_cy_, d_cpack_};
+  for (void* q : old) FreeD(q);
+  d_cbmi_ = cbmi;
+  d_cx_ = cx;
+  d_cs_ = cs;
+  d_cy_ = cy;
+  d_cpack_ = pack;
+  fused_ = fnet;
+  moff_ = moff;
+  mcount_ = mcount;
+  slab_nodes_ = (int)total;
   args_.num_nodes = slab_nodes_;
   closure_ = mlp;
   closure_first_ = first;

@@ -48,6 +48,23 @@ std::string BuildHierarchy(const PhiParameters& p, HierarchyTables* h) {
   if (!h->vacuum && M < N && !p.is_diagonal_bath_coupling && !p.is_full_bath_coupling)
     return "IndependentBath needs one bath per state (NumCouplingTerms >= NumStates)";
   if (Lh > 255) return "HierarchyTruncation above 255 is not supported";
+  // The mode -> site map reads one index per bath.
+  if (!p.multiple_independent_bath_indices.empty() &&
+      (int)p.multiple_independent_bath_indices.size() < M)
+    return "MultipleIndependentBaths needs one index per bath (NumCouplingTerms entries)";
+  // Size bound before anything is enumerated: C(D + Lh - 1, D) nodes (every
+  // multi-index of D modes with at most Lh - 1 quanta), each of N^2 complex
+  // elements.  Node ids are 32-bit and every state buffer holds at most
+  // kMaxStateElems elements.
+  {
+    double count = 1;
+    for (int i = 1; i < Lh && count <= 1e18; ++i) count = count * (D + i) / i;
+    if (count > (double)kMaxNodes || count * (double)N * N > (double)kMaxStateElems)
+      return "hierarchy too large (" + std::to_string(count) + " nodes of " +
+             std::to_string(N) + "x" + std::to_string(N) + "; at most " +
+             std::to_string(kMaxNodes) + " nodes and " + std::to_string(kMaxStateElems) +
+             " complex elements per state)";
+  }
 
   // --- index graph, PHI id order (hierarchy.cc:45-155) ----------------------
   std::vector<std::string> idx;

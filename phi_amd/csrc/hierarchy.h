@@ -22,6 +22,11 @@
 
 namespace phi {
 
+// Size limits of one hierarchy (or ensemble slab): node ids are int32, and
+// a state buffer holds at most 2^31 - 1 complex elements (32 GiB in FP64).
+constexpr long long kMaxNodes = 1LL << 28;
+constexpr long long kMaxStateElems = (1LL << 31) - 1;
+
 struct HierarchyTables {
   int N = 0;        // system size
   int M = 0;        // baths

@@ -158,7 +158,8 @@ void HierarchyIntegrator::Launch() {
     info_.status = kErrInvalidParameters;
     error_ = "invalid parameters";
   } else if (rm == RK4 || rm == RKF45 || rm == BICGSTAB || rm == BICGSTABL || rm == BICGSTABU) {
-    int rc = Initialize();
+    int rc = CheckRun();
+    if (!rc) rc = Initialize();
     if (!rc) rc = InitialState();
     if (!rc) {
       if (rm == RK4) rc = RunRk4();
@@ -210,6 +211,36 @@ void HierarchyIntegrator::Launch() {
          "entries into N-entry ADO buffers)");
   }
   if (p_->verbose > -2) std::cout << "Done.\n";
+}
+
+// Values the run loops need that PHI's reader guarantees but the C ABI's
+// setters do not (a zero timestep or output period would loop forever),
+// checked before anything is allocated.
+int HierarchyIntegrator::CheckRun() {
+  const PhiParameters& p = *p_;
+  const RunMethod rm = p.run_method;
+  std::string why;
+  if (rm == RK4 || rm == RKF45) {
+    if (!std::isfinite(p.integration_time) || !std::isfinite(p.start_time))
+      why = "the start and end times must be finite";
+    else if (!(p.timestep > 0) || !std::isfinite(p.timestep))
+      why = "the timestep must be positive and finite";
+    else if (!(p.output_minimum_timestep > 0))
+      why = "OutputMinimumTimestep must be positive";
+    else if (rm == RK4 && std::ceil(p.output_minimum_timestep / p.timestep) > 1e9)
+      why = "OutputMinimumTimestep must be at most 1e9 timesteps";
+    else if (rm == RK4 && (p.integration_time - p.start_time) / p.timestep > 1e15)
+      why = "more than 1e15 RK4 steps requested";
+    else if (rm == RKF45 && !(p.rkf45_tolerance > 0))
+      why = "RKF45tolerance must be positive";
+    else if (rm == RKF45 && !(p.rkf45_minimum_timestep > 0))
+      why = "RKF45mindt must be positive";
+  } else if (!(p.bicgstab_tolerance > 0)) {
+    why = "the BiCGSTAB tolerance must be positive";
+  }
+  if (why.empty()) return kOk;
+  Fail(kErrInvalidParameters, why);
+  return info_.status;
 }
 
 int HierarchyIntegrator::Initialize() {
@@ -303,6 +334,17 @@ int HierarchyIntegrator::InitialState() {
     double t = 0, dt = 0;
     int rc = ReadRestart(&t, &dt);
     if (rc) return rc;
+    // A steady-state run writes dt = 0 (its file holds the solution, not a
+    // point of a trajectory); time stepping cannot resume from it.
+    if ((p.run_method == RK4 || p.run_method == RKF45) && !(dt > 0 && std::isfinite(dt))) {
+      Fail(kErrInvalidParameters, "the restart file's timestep is not positive (a steady-state "
+                                  "restart file cannot resume time stepping)");
+      return info_.status;
+    }
+    if (!std::isfinite(t)) {
+      Fail(kErrInvalidParameters, "the restart file's time is not finite");
+      return info_.status;
+    }
     p_->start_time = t;
     p_->timestep = dt;
     return kOk;
@@ -369,50 +411,83 @@ int HierarchyIntegrator::RunRk4() {
   const int NE = p.num_states * p.num_states;
   const double dt = p.timestep;
   const double end_time = p.integration_time;
-  const int output_factor = (int)std::ceil(p.output_minimum_timestep / dt);
-  // Step count and times exactly as PHI accumulates them (F7: time_now +=
-  // dt in double until time_now >= end).
-  std::vector<double> times;
-  {
-    double t = p.start_time;
-    while (t < end_time) {
+  // time_now += dt must reach the end, and the output period must be at
+  // least one step (PHI loops forever or divides by zero otherwise).
+  if (!(dt > 0) || !std::isfinite(dt)) {
+    Fail(kErrInvalidParameters, "the RK4 timestep must be positive and finite");
+    return info_.status;
+  }
+  if (!std::isfinite(end_time) || !std::isfinite(p.start_time)) {
+    Fail(kErrInvalidParameters, "the start and end times must be finite");
+    return info_.status;
+  }
+  const double of = std::ceil(p.output_minimum_timestep / dt);
+  if (!(of >= 1) || of > 1e9) {
+    Fail(kErrInvalidParameters,
+         "OutputMinimumTimestep must be positive and at most 1e9 timesteps");
+    return info_.status;
+  }
+  const long long output_factor = (long long)of;
+  if (end_time > p.start_time && (end_time - p.start_time) / dt > 1e15) {
+    Fail(kErrInvalidParameters, "more than 1e15 RK4 steps requested");
+    return info_.status;
+  }
+  if (p.verbose > -1) std::cout << "\tIntegrating using RK4.\n";
+  const double t0 = Now();
+  const std::time_t w0 = std::time(nullptr);
+  std::time_t last_restart = w0;
+  const bool restart = RestartOn();
+  // Blocks of steps between host checks: the node-0 history never
+  // overflows (2048 snapshots), and with restart output the wall clock is
+  // read at least every 256 steps (PHI reads it after every step,
+  // hierarchy_integrator.cc:2881-2895).
+  long long chunk = output_factor * 2048;
+  if (restart) chunk = std::min<long long>(chunk, 256);
+  std::vector<std::complex<double>> hist;
+  // Step times exactly as PHI accumulates them (F7: time_now += dt in
+  // double until time_now >= end), generated one block at a time; t_before
+  // is the time at the start of each step (a time-dependent H is evaluated
+  // there).
+  std::vector<double> times, t_before;
+  double t = p.start_time;
+  long long s = 0;
+  while (t < end_time) {
+    times.clear();
+    t_before.clear();
+    while (t < end_time && (long long)times.size() < chunk) {
+      t_before.push_back(t);
       t += dt;
       times.push_back(t);
     }
-  }
-  const long long steps = (long long)times.size();
-  if (p.verbose > -1) std::cout << "\tIntegrating using RK4.\n";
-  const double t0 = Now();
-  // Chunks sized so the node-0 history never overflows.
-  const long long chunk = (long long)output_factor * 2048;
-  std::vector<std::complex<double>> hist;
-  // Time at the start of each step (a time-dependent H is evaluated there).
-  std::vector<double> t_before(steps);
-  for (long long i = 0; i < steps; ++i) t_before[i] = i == 0 ? p.start_time : times[i - 1];
-  for (long long s = 0; s < steps; s += chunk) {
-    const long long n = std::min(chunk, steps - s);
+    const long long n = (long long)times.size();
     int rc = sh_ && sh_->time_dependent()
-                 ? engine_->Rk4StepsTD(n, t_before.data() + s, dt, s, output_factor, sh_)
-                 : engine_->Rk4Steps(n, dt, s, output_factor);
+                 ? engine_->Rk4StepsTD(n, t_before.data(), dt, s, (int)output_factor, sh_)
+                 : engine_->Rk4Steps(n, dt, s, (int)output_factor);
     if (!rc) rc = engine_->DrainHistory(&hist);
     if (rc < 0) {
       Fail(rc, engine_->last_error());
       return rc;
     }
-    const double el = Now() - t0;
+    // PHI's timing lines use whole seconds (time_t, difftime).
+    const double el = std::difftime(std::time(nullptr), w0);
     long long k = 0;
     for (long long i = s; i < s + n; ++i) {
       if (i % output_factor != 0) continue;
       if (out_.is_open() && i % 100 == 0) {
-        const double per = el / (double)(s + n);
-        out_ << "#Time for 1st timestep: " << per << " s.\n"
-             << "#Estimated time for 1 time unit: " << per / dt / 60 / 60 << " hrs.\n"
+        out_ << "#Time for 1st timestep: " << el / (double)(i + 1) << " s.\n"
+             << "#Estimated time for 1 time unit: " << el / dt / 60 / 60 / (double)(i + 1)
+             << " hrs.\n"
              << "#Total running time: " << 0 << " days.\n";
       }
       const std::complex<double>* r = hist.data() + (size_t)k * NE;
-      WriteRho0Line(r, times[i]);
-      FullPrecisionLine(times[i], r);
+      WriteRho0Line(r, times[i - s]);
+      FullPrecisionLine(times[i - s], r);
       ++k;
+    }
+    s += n;
+    if (restart && t < end_time) {
+      rc = MaybeWriteRestart(t, dt, &last_restart);
+      if (rc) return rc;
     }
   }
   int rc = engine_->Synchronize();
@@ -421,8 +496,8 @@ int HierarchyIntegrator::RunRk4() {
     return rc;
   }
   info_.seconds = Now() - t0;
-  info_.steps = steps;
-  info_.final_time = steps ? times.back() : p.start_time;
+  info_.steps = s;
+  info_.final_time = t;
   info_.final_dt = dt;
   std::vector<std::complex<double>> r0(NE);
   rc = engine_->GetNode0(r0.data());
@@ -433,15 +508,27 @@ int HierarchyIntegrator::RunRk4() {
   if (p.verbose > -1) {
     std::ostringstream ss;
     ActiveLine(&ss);
-    ss << "#Total number of integration steps = " << steps << "\n"
-       << "#Total integration time = " << info_.seconds / 60 << " mins\n";
+    ss << "#Total number of integration steps = " << s << "\n"
+       << "#Total integration time = " << std::difftime(std::time(nullptr), w0) / 60
+       << " mins\n";
     std::cout << ss.str();
     if (out_.is_open()) out_ << ss.str();
   }
   WriteFinal(r0.data(), info_.final_time);
   FullPrecisionLine(info_.final_time, r0.data());
-  if (p.restart_output_time > 0) return WriteRestart(info_.final_time, dt);
+  if (restart) return WriteRestart(info_.final_time, dt);
   return kOk;
+}
+
+// A restart file when more than RestartOutputTimeSecs whole seconds of wall
+// clock passed since the last one (PHI's difftime(now, last) > time,
+// hierarchy_integrator.cc:2512-2526, 2881-2895).
+int HierarchyIntegrator::MaybeWriteRestart(double t, double dt, std::time_t* last) {
+  const std::time_t now = std::time(nullptr);
+  if (!(std::difftime(now, *last) > p_->restart_output_time)) return kOk;
+  const int rc = WriteRestart(t, dt);
+  *last = std::time(nullptr);
+  return rc;
 }
 
 int HierarchyIntegrator::RunRkf45() {
@@ -481,10 +568,31 @@ int HierarchyIntegrator::RunRkf45() {
   scale_all(dt_f);
   HamiltonianSource* src = sh_ && sh_->time_dependent() ? sh_ : nullptr;
   double time_now = p.start_time;
-  double next_write_time = time_now;
+  if (!std::isfinite(end_time) || !std::isfinite(time_now)) {
+    Fail(kErrInvalidParameters, "the start and end times must be finite");
+    return info_.status;
+  }
+  // Dynamics lines of the output file.  PHI's writer compares an
+  // uninitialised time_t `now` with next_write_time = start time
+  // (hierarchy_integrator.cc:1808, 1840, 2500-2510); what it does in
+  // practice, and what its example output shows (one_qubit_100.txt:49-53,
+  // 447), is defined here: one line after the first accepted step; with
+  // restart output on, `now` holds the wall clock of the previous accepted
+  // step, so a further line follows whenever that wall clock passed the
+  // previous line's by more than OutputMinimumTimestep seconds; plus the
+  // t = 0 line and the final line.  The full-precision side file instead
+  // takes a line after each accepted step at or past next_full_time
+  // (simulated time), which then advances by OutputMinimumTimestep.
+  const bool restart = RestartOn();
+  const bool wall_clock = p.restart_output_time > 0;  // PHI then reads `now`
+  bool first_line = true;
+  std::time_t wall_prev = 0, wall_line = 0;
+  double next_full_time = time_now;
   long long i = -1, accepted = 0, inext = 0;
   std::vector<std::complex<double>> r0(NE);
   const double t0 = Now();
+  const std::time_t w0 = std::time(nullptr);
+  std::time_t last_restart = w0;
   if (p.verbose > -1) std::cout << "\tIntegrating using adaptive timestepping RKF45.\n";
   while (time_now < end_time) {
     i += 1;
@@ -519,11 +627,8 @@ int HierarchyIntegrator::RunRkf45() {
         scale_all(dtup);
       }
       dt_f = mydt;
-      // PHI's writer cadence is undefined (it tests an uninitialised wall
-      // clock); here a line is written after each accepted step at or past
-      // next_write_time, which then advances by OutputMinimumTimestep.
       if (out_.is_open() && i >= inext) {
-        const double el = Now() - t0;
+        const double el = std::difftime(std::time(nullptr), w0);
         out_ << "#Info: Time for single timestep: " << el / (i + 1) << " s.\n"
              << "#Info: Estimated time for 1 time unit: "
              << (time_now > p.start_time ? el / (time_now - p.start_time) / 3600 : 0) << " hrs.\n"
@@ -532,20 +637,39 @@ int HierarchyIntegrator::RunRkf45() {
              << " days.\n";
         inext += 100;
       }
-      if (time_now >= next_write_time) {
+      bool line = first_line;
+      if (!first_line && wall_clock &&
+          std::difftime(wall_prev, wall_line) > p.output_minimum_timestep)
+        line = true;
+      const bool full_line = full_.is_open() && time_now >= next_full_time;
+      if (line || full_line) {
         rc = engine_->GetNode0(r0.data());
         if (rc) {
           Fail(rc, engine_->last_error());
           return rc;
         }
+      }
+      if (line) {
         if (out_.is_open()) {
           out_ << std::setw(20) << std::setprecision(6) << time_now << std::setw(6)
                << std::setprecision(4);
           for (int j = 0; j < NE; ++j) out_ << " " << r0[j];
           out_ << std::endl;
         }
+        wall_line = first_line ? std::time(nullptr) : wall_prev;
+        first_line = false;
+      }
+      if (full_line) {
         FullPrecisionLine(time_now, r0.data());
-        next_write_time = time_now + p.output_minimum_timestep;
+        next_full_time = time_now + p.output_minimum_timestep;
+      }
+      if (wall_clock) wall_prev = std::time(nullptr);
+      if (restart) {
+        if (time_now < end_time) {
+          // The step size the controller holds for the next attempt.
+          rc = MaybeWriteRestart(time_now, clamped ? free_dt : dt_f, &last_restart);
+          if (rc) return rc;
+        }
       }
     } else {
       mydt *= dtdn;
@@ -569,7 +693,8 @@ int HierarchyIntegrator::RunRkf45() {
     ActiveLine(&ss);
     ss << "#Total number of integration steps = " << i + 1 << "\n"
        << "#Total number of accepted integration steps = " << accepted << "\n"
-       << "#Total integration time = " << info_.seconds / 60 << " mins.\n";
+       << "#Total integration time = " << std::difftime(std::time(nullptr), w0) / 60
+       << " mins.\n";
     std::cout << ss.str();
     if (out_.is_open()) out_ << ss.str();
   }
@@ -578,7 +703,7 @@ int HierarchyIntegrator::RunRkf45() {
   // PHI writes dt_f, which the end clamp has shrunk to about (end - t) ~ 0,
   // so a run resumed from its file does not advance; the step size the
   // controller held before the clamp is written instead.
-  if (p.restart_output_time > 0) return WriteRestart(time_now, clamped ? free_dt : dt_f);
+  if (restart) return WriteRestart(time_now, clamped ? free_dt : dt_f);
   return kOk;
 }
 
@@ -621,7 +746,7 @@ int HierarchyIntegrator::RunSteadyState() {
   final_rho0_ = r0;
   have_final_ = true;
   FullPrecisionLine((double)iters, r0.data());
-  if (p.restart_output_time > 0) {
+  if (RestartOn()) {
     int w = WriteRestart((double)iters, 0);
     if (w) return w;
   }
@@ -643,28 +768,43 @@ int HierarchyIntegrator::WriteRestart(double t, double dt) {
     Fail(rc, engine_->last_error());
     return rc;
   }
-  std::ofstream o(p_->restart_output_filename.c_str());
-  if (!o.is_open()) {
-    Fail(kErrIo, "cannot write restart file " + p_->restart_output_filename);
-    return kErrIo;
-  }
-  o << std::setprecision(17) << t << "\n" << dt << "\n";
-  const size_t ne = (size_t)p_->num_states * p_->num_states;
-  for (int n = 0; n < engine_->num_nodes(); ++n) {
-    for (size_t i = 0; i < ne; ++i) o << (i ? " " : "") << s[n * ne + i];
-    o << "\n";
-  }
-  // Time-local truncation: the saved matrices T_mk, one line each, bath
-  // major (restart_helper.cc:43-52).  PHI saves them after accepted RKF45
-  // steps only, so an RK4 run writes zeros here, as PHI's does.
-  if (TimeLocal* tl = engine_->timelocal()) {
-    const std::vector<std::complex<double>>& T = tl->saved();
-    for (size_t mk = 0; mk < (size_t)tl->M() * tl->K(); ++mk) {
-      for (size_t i = 0; i < ne; ++i) o << (i ? " " : "") << T[mk * ne + i];
+  // Written next to the target and renamed over it, so that a run killed
+  // while writing leaves the previous file whole.
+  const std::string name = p_->restart_output_filename;
+  const std::string tmp = name + ".tmp";
+  {
+    std::ofstream o(tmp.c_str());
+    if (!o.is_open()) {
+      Fail(kErrIo, "cannot write restart file " + tmp);
+      return kErrIo;
+    }
+    o << std::setprecision(17) << t << "\n" << dt << "\n";
+    const size_t ne = (size_t)p_->num_states * p_->num_states;
+    for (int n = 0; n < engine_->num_nodes(); ++n) {
+      for (size_t i = 0; i < ne; ++i) o << (i ? " " : "") << s[n * ne + i];
       o << "\n";
     }
+    // Time-local truncation: the saved matrices T_mk, one line each, bath
+    // major (restart_helper.cc:43-52).  PHI saves them after accepted RKF45
+    // steps only, so an RK4 run writes zeros here, as PHI's does.
+    if (TimeLocal* tl = engine_->timelocal()) {
+      const std::vector<std::complex<double>>& T = tl->saved();
+      for (size_t mk = 0; mk < (size_t)tl->M() * tl->K(); ++mk) {
+        for (size_t i = 0; i < ne; ++i) o << (i ? " " : "") << T[mk * ne + i];
+        o << "\n";
+      }
+    }
+    o.flush();
+    if (!o.good()) {
+      Fail(kErrIo, "cannot write restart file " + tmp);
+      return kErrIo;
+    }
   }
-  return o.good() ? kOk : kErrIo;
+  if (std::rename(tmp.c_str(), name.c_str()) != 0) {
+    Fail(kErrIo, "cannot replace restart file " + name);
+    return kErrIo;
+  }
+  return kOk;
 }
 
 int HierarchyIntegrator::ReadRestart(double* t, double* dt) {

@@ -9,6 +9,7 @@
 #define PHI_AMD_INTEGRATOR_H_
 
 #include <complex>
+#include <ctime>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -52,6 +53,7 @@ class HierarchyIntegrator {
   int GetState(std::vector<std::complex<double>>* out);
 
  private:
+  int CheckRun();
   int Initialize();
   void WriteHeader();
   int InitialState();
@@ -64,7 +66,15 @@ class HierarchyIntegrator {
   // "#Total number of active matrices = a/n" with the filter on and
   // verbose > 0 (hierarchy_integrator.cc:2540-2543, 2908-2911).
   void ActiveLine(std::ostream* os);
+  // Restart files (PHI's text layout, restart_helper.cc:30-55): written at
+  // the end of a run and, during it, on PHI's wall-clock cadence.
   int WriteRestart(double t, double dt);
+  // Restart output is on with RestartOutputTimeSecs > 0 and a RestartFile
+  // (PHI opens an empty name, which fails silently and writes nothing).
+  bool RestartOn() const {
+    return p_->restart_output_time > 0 && !p_->restart_output_filename.empty();
+  }
+  int MaybeWriteRestart(double t, double dt, std::time_t* last);
   int ReadRestart(double* t, double* dt);
   void Fail(int code, const std::string& msg);
 

@@ -81,7 +81,7 @@ int Mlp::Create(int n_in, int n_hidden, int n_out, int max_batch, Mlp** out, std
 Mlp::~Mlp() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   void* ptrs[] = {d_p_, d_g_, d_m_, d_v_, d_x_, d_t_, d_z1_, d_h1_, d_z2_, d_h2_, d_y_,
-                  d_dy_, d_dh2_, d_dh1_, d_partial_, d_pack_};
+                  d_dy_, d_dh2_, d_dh1_, d_partial_};
   for (void* p : ptrs) FreeDev(p);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -99,7 +99,8 @@ unsigned short Bf16(float f) {
 
 }  // namespace
 
-int Mlp::Pack(FusedNet* out) {
+int Mlp::Pack(FusedNet* out, void** owned) {
+  *owned = nullptr;
   if (n_hid_ % 32 != 0) {
     err_ = "the fused closure pass needs a hidden width that is a multiple of 32";
     return -3;
@@ -112,7 +113,11 @@ int Mlp::Pack(FusedNet* out) {
   auto rows64 = [](int n) { return (n + 63) / 64 * 64; };
   const size_t n1 = (size_t)rows64(hid) * k1, n2 = (size_t)rows64(hid) * hid,
                n3 = (size_t)rows64(n_out_) * hid;
-  std::vector<unsigned short> w(n1 + n2 + n3, 0);
+  // One buffer: the bfloat16 weights, then (16-byte aligned) the float32
+  // biases b1, b2, b3.
+  const size_t nw = (n1 + n2 + n3 + 7) / 8 * 8;
+  const size_t nb = (size_t)2 * hid + n_out_;
+  std::vector<unsigned short> w(nw + 2 * nb, 0);
   for (int o = 0; o < hid; ++o)
     for (int i = 0; i < n_in_; ++i) w[(size_t)o * k1 + i] = Bf16(p[off_[0] + (size_t)o * n_in_ + i]);
   for (int o = 0; o < hid; ++o)
@@ -120,22 +125,30 @@ int Mlp::Pack(FusedNet* out) {
   for (int o = 0; o < n_out_; ++o)
     for (int i = 0; i < hid; ++i)
       w[n1 + n2 + (size_t)o * hid + i] = Bf16(p[off_[4] + (size_t)o * hid + i]);
-  FreeDev(d_pack_);
-  d_pack_ = nullptr;
-  rc = Check(hipMalloc((void**)&d_pack_, w.size() * sizeof(unsigned short)), "alloc pack");
-  if (!rc) rc = Check(hipMemcpy(d_pack_, w.data(), w.size() * sizeof(unsigned short),
+  float* bias = reinterpret_cast<float*>(w.data() + nw);
+  std::memcpy(bias, p.data() + off_[1], hid * sizeof(float));
+  std::memcpy(bias + hid, p.data() + off_[3], hid * sizeof(float));
+  std::memcpy(bias + 2 * hid, p.data() + off_[5], n_out_ * sizeof(float));
+  unsigned short* d = nullptr;
+  rc = Check(hipMalloc((void**)&d, w.size() * sizeof(unsigned short)), "alloc pack");
+  if (!rc) rc = Check(hipMemcpy(d, w.data(), w.size() * sizeof(unsigned short),
                                 hipMemcpyHostToDevice), "upload pack");
-  if (rc) return rc;
-  out->w1 = d_pack_;
-  out->w2 = d_pack_ + n1;
-  out->w3 = d_pack_ + n1 + n2;
-  out->b1 = d_p_ + off_[1];
-  out->b2 = d_p_ + off_[3];
-  out->b3 = d_p_ + off_[5];
+  if (rc) {
+    FreeDev(d);
+    return rc;
+  }
+  const float* db = reinterpret_cast<const float*>(d + nw);
+  out->w1 = d;
+  out->w2 = d + n1;
+  out->w3 = d + n1 + n2;
+  out->b1 = db;
+  out->b2 = db + hid;
+  out->b3 = db + 2 * hid;
   out->k1 = k1;
   out->hid = hid;
   out->n_out = n_out_;
   out->width = std::max(k1, hid);
+  *owned = d;
   return 0;
 }
 

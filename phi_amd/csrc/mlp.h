@@ -46,9 +46,10 @@ class Mlp {
   // given stream; activations of the first max_batch rows live in this
   // object's buffers, larger batches run in chunks.
   int ForwardDevice(const float* x, int batch, float* y, hipStream_t s);
-  // The current parameters packed for the fused closure pass (bfloat16
-  // weights owned by this object; valid until the next Pack).
-  int Pack(struct FusedNet* out);
+  // A snapshot of the current parameters packed for the fused closure pass
+  // (bfloat16 weights and float32 biases in one device buffer, *owned, which
+  // the caller frees with hipFree; later training does not change it).
+  int Pack(struct FusedNet* out, void** owned);
 
  private:
   Mlp() = default;
@@ -73,7 +74,6 @@ class Mlp {
   double* d_partial_ = nullptr;
   int partial_blocks_ = 64;
   int partial_slots_ = 0;
-  unsigned short* d_pack_ = nullptr;  // packed bfloat16 weights
 };
 
 }  // namespace phi

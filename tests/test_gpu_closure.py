@@ -252,3 +252,32 @@ def test_fused_closure_pass_equals_unfused_and_network(prec):
             slot = states[True][m * span + n_real + bi * 7 + j]
             tol = (1e-5 if prec == "fp64" else 1e-5) * max(1e-30, np.abs(kids[j]).max())
             assert np.abs(slot - kids[j]).max() <= tol
+
+
+def test_failed_attach_leaves_the_ensemble_unchanged():
+    """A network the fused pass cannot take (hidden width 48, not a multiple
+    of 32) is refused, and the ensemble it was offered to still steps exactly
+    as before: same slab, offsets and tiles, equal to its members run alone."""
+    from phi_amd.configs.make_configs import c4_member
+    p = _c2_params()
+    dt = p.get_double("timestep")
+    Hs = np.array([c4_member(r)[0] for r in range(3)])
+    ens = P.Ensemble(p, Hs, [5] * 3)
+    ens.rk4_steps(7, dt)
+    before = ens.state()
+    net = P.Mlp(105, 48, 686, max_batch=256)
+    with pytest.raises(P.PhiError):
+        ens.attach_closure(net)
+    assert np.array_equal(ens.state(), before)
+    ens.rk4_steps(40, dt)
+    slab = ens.state()
+    off = 0
+    for m in range(3):
+        ctx = P.Context(_member_param(p, Hs[m], 5))
+        ctx.rk4_steps(47, dt)
+        assert np.abs(slab[off:off + ctx.num_nodes] - ctx.state()).max() <= 1e-12
+        off += ctx.num_nodes
+        ctx.close()
+    assert off == ens.num_nodes
+    net.close()
+    ens.close()
