@@ -639,16 +639,27 @@ def fix_incompatible_combinations(p):
         p.is_truncated_system = True
 
 
+INT_MAX = 2 ** 31 - 1
+
+
 def node_count_estimate(max_level, terms):
-    """GetHierarchyNodeCountEstimate: sum_l (l+D-1)!/(l!(D-1)!) in double."""
+    """GetHierarchyNodeCountEstimate (phi_parameters.cc:1138-1147):
+    sum_l (l+D-1)!/(l!(D-1)!) in double.  Past 170! the factorials are
+    infinite and inf/inf is NaN; PHI's integer conversion of a NaN or
+    out-of-range sum is undefined, and such a sum is taken as INT_MAX (the
+    loops stop once that is known)."""
     def ff(a, b):
         acc = 1.0
         for i in range(a, b + 1):
             acc *= i
+            if acc == float("inf"):
+                break
         return acc
     count = 0.0
     for l in range(max_level):
         count += ff(1, l + terms - 1) / ff(1, l) / ff(1, terms - 1)
+        if not count <= INT_MAX:
+            return INT_MAX
     return int(count)
 
 

@@ -9,6 +9,9 @@
 // same output header.  The code is this project's own.
 #include "phi_parameters.h"
 
+#include <climits>
+#include <cmath>
+
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -766,15 +769,22 @@ void PhiParameters::FixIncompatibleCombinations() {
   }
 }
 
+// PHI sums (l + terms - 1)! / (l! (terms - 1)!) over l < max_level in
+// double and converts the sum to an integer (phi_parameters.cc:1138-1147).
+// Beyond 170! the factorials are infinite, inf / inf is NaN, and PHI's
+// conversion of a NaN or out-of-range sum is undefined; here such a sum
+// saturates at INT_MAX, and the loops stop once the result is known.
 int GetHierarchyNodeCountEstimate(int max_level, int terms) {
   auto product = [](int a, int b) {
     double acc = 1;
-    for (int i = a; i <= b; ++i) acc *= i;
+    for (int i = a; i <= b && std::isfinite(acc); ++i) acc *= i;
     return acc;
   };
   double count = 0;
-  for (int l = 0; l < max_level; ++l)
+  for (int l = 0; l < max_level; ++l) {
     count += product(1, l + terms - 1) / product(1, l) / product(1, terms - 1);
+    if (!(count <= (double)INT_MAX)) return INT_MAX;
+  }
   return (int)count;
 }
 

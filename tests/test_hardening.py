@@ -137,3 +137,22 @@ def test_binary_junk_prm_never_crashes(tmp_path):
         path.write_bytes(data)
         p = P.PhiParameters.read(str(path), "rk4", check=False)
         assert p.get_int("are_parameters_valid") in (0, 1)
+
+
+def test_node_count_estimate_of_huge_hierarchies_saturates(tmp_path):
+    """PHI's double sum of factorials turns into NaN past 170! and its
+    integer conversion is then undefined: the reader and the oracle both
+    stop early and report INT_MAX (HierarchyTruncation 5000 with 255 * 256
+    modes would otherwise take minutes in the oracle)."""
+    import time
+    text = open(C1).read().replace("HierarchyTruncation=9", "HierarchyTruncation=5000")
+    text += "MatsubaraTerms=255\nNumCouplingTerms=255\n"
+    path = tmp_path / "huge.prm"
+    path.write_text(text)
+    t0 = time.perf_counter()
+    p = P.PhiParameters.read(str(path), "rk4", check=False)
+    o = PO.read_command_line(str(path), "rk4")
+    assert time.perf_counter() - t0 < 5
+    assert p.get_int("matrix_count") == o.matrix_count == PO.INT_MAX
+    assert PO.node_count_estimate(9, 4) == 495          # C(8+4, 4), the 2-site c1 case
+    assert PO.node_count_estimate(5, 24) == 20475       # the 24-site ring at depth 4
