@@ -13,7 +13,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "heom_oracle.cc")
-LIB = os.path.join(HERE, "libheom_oracle.so")
+# HEOM_ORACLE_LIBRARY names another build of the oracle (the sanitizer
+# build of phi_amd/build_sanitized.py); the default is the in-tree build.
+LIB = os.environ.get("HEOM_ORACLE_LIBRARY") or os.path.join(HERE, "libheom_oracle.so")
 
 
 def build(force=False):

@@ -70,8 +70,10 @@ def test_rk4_dump_during_run_kill_and_resume(tmp_path):
     assert dt == 0.001 and rows.shape == (495, 4)
     print("dumped at t = %.17g" % t, flush=True)
     t_end = t + 0.05
+    # PHI opens RestartInput relative to the working directory; this process
+    # runs elsewhere, so the path is absolute
     (tmp_path / "resume.prm").write_text(
-        c1_text(["RestartInput=resume.txt"], t_end))
+        c1_text(["RestartInput=%s" % (tmp_path / "resume.txt")], t_end))
     a = P.HierarchyIntegrator(P.PhiParameters.read(str(tmp_path / "resume.prm"), "rk4"))
     a.Launch()
     (tmp_path / "whole.prm").write_text(c1_text([], t_end))
@@ -100,7 +102,8 @@ def test_resume_from_phi_format_restart_file(tmp_path):
     t, dt, rows = read_restart(tmp_path / "phi_restart.txt")
     rounded = rows.reshape(-1, 2, 2).transpose(0, 2, 1)
     assert np.abs(rounded - state).max() > 0  # the file did lose digits
-    (tmp_path / "r.prm").write_text(c1_text(["RestartInput=phi_restart.txt"], 0.3))
+    (tmp_path / "r.prm").write_text(
+        c1_text(["RestartInput=%s" % (tmp_path / "phi_restart.txt")], 0.3))
     it = P.HierarchyIntegrator(P.PhiParameters.read(str(tmp_path / "r.prm"), "rk4"))
     it.Launch()
     orc.set_state(rounded)
