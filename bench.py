@@ -1,46 +1,48 @@
-"""Benchmark: HEOM RK4 on the 7-site exciton, one MI355X.
+"""Benchmark: HEOM RK4 over a static-disorder ensemble, one MI355X.
 
-Workload (the reference's headline configuration): 7-site exciton with 7
-projector baths, depth 8 (PHI HierarchyTruncation=9), 6435 nodes of 7x7
-complex ADOs, RK4 with dt = 1 fs; one step = one RK4 step over every node
-(4 fused RHS + stage kernels, nested form).  W warm-up steps, then K timed
-steps (default 1000 = 1 ps) with the state resident in HBM, bracketed by
-device synchronisation.  Prints one JSON line on stdout.
+Workload (the largest single-GPU configuration): 512 realisations of the
+7-site exciton (7 projector baths, site energies + N(0, 100 cm^-1) disorder,
+depths drawn from U{4..8}, i.e. PHI HierarchyTruncation 5..9, 330-6435 nodes
+each, 1.36 M nodes of 7x7 complex ADOs in one slab), RK4 with dt = 1 fs,
+FP32-bulk right-hand side on the matrix cores with the FP64 state, the
+member-averaged rho_0 recorded every 10 fs.  One step = one RK4 step over
+every node of every member (4 persistent stage-kernel launches, nested
+form).  W warm-up steps, then K timed steps (default 1000 = 1 ps, so the
+time to 1 ps is measured, not extrapolated) with the state resident in HBM,
+bracketed by device synchronisation.  Prints one JSON line on stdout.
 
-value    node-steps/s of the plain FP64 path (max |drho| <= 1e-10 against the
-         oracle in the GPU tests); the FP32-bulk path (FP32 RHS, FP64 state)
-         is reported beside it with its population error against FP64 after
-         the same W + K steps.
-roofline the dominant kernel, stream_stage (the four launches of each
-         FP64 RK4 step; nothing else runs in the loop): algorithmic bytes
-         per launch over the mean launch duration, i.e. a step's bytes over
-         the timed region's HIP-event time per step (events on the engine's
-         stream; launch gaps included).  Algorithmic bytes per node (nested
-         RK4 form, s = bytes per bulk complex element): stage 0 N^2 (16 + s),
-         stages 1-2 N^2 (16 + 2 s), stage 3 N^2 (32 + s), N^2 (80 + 6 s)
-         per step; neighbour columns count as cache hits.
-         isolated_launches: each stage launched back to back on the same
-         buffers after the timed region (no producer-consumer hand-off).
-         classic_accounting repeats the figures with the bytes of classic
-         RK4 (N^2 (160 + 6 s) per node-step), which the nested form beats.
-         traffic = mean HBM bytes per launch of the four stages from the
-         committed rocprofv3 counter summary (profiles/), when present.
---shape-fp64/--shape-fp32/--shape-c3 pin a leg's launch shape instead of
-autotuning it (profiles of the shape an earlier run chose).
-cpu_baseline  the oracle's pthread restatement of PHI's RK4 loop (FP64) on
-         the host cores, timed on a bounded sample of the same workload.
+value    node-steps/s of the FP32-bulk ensemble run (whole slab).  The same
+         ensemble in plain FP64 is reported beside it ("fp64"), and the
+         FP32-bulk result's population and coherence errors against it
+         (every member's rho_0 and the member average).
+roofline the dominant kernel, stream_stage (ensemble instantiation; the
+         four launches of each RK4 step, nothing else but the snapshot of
+         every 10th step runs in the loop): algorithmic bytes per launch over
+         the mean launch duration, i.e. a step's bytes over the timed
+         region's HIP-event time per step (events on the engine's stream).
+         Algorithmic bytes per node (nested RK4 form, s = bytes per bulk
+         complex element): stage 0 N^2 (16 + s), stages 1-2 N^2 (16 + 2 s),
+         stage 3 N^2 (32 + s), N^2 (80 + 6 s) per step; neighbour columns
+         count as cache hits.  classic_accounting repeats the figures with
+         the bytes of classic RK4, N^2 (160 + 6 s) per node-step.  traffic =
+         calibrated HBM bytes per launch (FETCH_SIZE + WRITE_SIZE) of the
+         four stages from the committed rocprofv3 counter summary.
+cpu_baseline  the oracle's pthread restatement of PHI's RK4 loop (FP64,
+         all host cores) on a bounded sample of the same ensemble: members
+         in order, a fixed number of steps each, about 12 s.
+c2_single  the 7-site depth-8 hierarchy alone (6435 nodes), FP64 and FP32
+         bulk, with its own roofline figures.
 c3_fp32  the 24-site ring at depth 4 (20475 nodes of 24x24), FP32-bulk RK4:
-         node-steps/s and the HBM fraction of its stage kernels.
+         node-steps/s, population / coherence error vs FP64, HBM fraction.
 c3_rkf45 the 24-site ring with RKF45 (abs tol 1e-8) to 1 ps, FP64 and FP32
          bulk: attempts, time to 1 ps, HBM fraction of N^2 (112 + 29 s) per
-         attempted node-step, FP32-vs-FP64 population error.
+         attempted node-step, FP32-vs-FP64 population / coherence error.
 c2_steady_state  BiCGSTAB to residual 1e-10 from the 7-site state at 1 ps.
-c4_ensemble  512 static-disorder realisations of the 7-site system with
-         depths U{4..8} in one slab (persistent kernels, autotuned shape,
-         hipGraph step blocks): time to 1 ps, FP32-bulk and FP64.
 c5_closure   the learned truncation closure: trained on depth-10 runs,
          depth 4 + closure vs plain depth 4 vs plain depth 10 on held-out
          Hamiltonians (populations and time to 1 ps).
+--shape-ens/--shape-fp64/--shape-fp32/--shape-c3 pin a leg's launch shape
+instead of autotuning it (profiles of the shape an earlier run chose).
 """
 
 import argparse
@@ -50,13 +52,16 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 CFG = os.path.join(ROOT, "phi_amd", "configs", "c2.prm")
 CFG3 = os.path.join(ROOT, "phi_amd", "configs", "c3.prm")
-PROFILE = os.path.join(ROOT, "profiles", "rk4_stage_7site_depth8_traffic.txt")
+PROFILE_C2 = os.path.join(ROOT, "profiles", "rk4_stage_7site_depth8_traffic.txt")
+PROFILE_ENS = os.path.join(ROOT, "profiles", "rk4_stage_ensemble_512x7site_traffic.txt")
 
 
 def log(*a):
@@ -80,37 +85,81 @@ def classic_stage_bytes(N, s):
     return [ne * (32 + s), ne * (48 + 2 * s), ne * (48 + 2 * s), ne * (32 + s)]
 
 
-def cpu_baseline(seconds=12.0):
+def host_cores():
+    """Threads this process may run on (its affinity set) and the host's
+    online CPU count."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return aff, os.cpu_count()
+
+
+def ensemble_members(members):
+    from phi_amd.configs.make_configs import c4_member
+    Hs, depths = [], []
+    for r in range(members):
+        H, lh = c4_member(r)
+        Hs.append(H)
+        depths.append(lh)
+    return Hs, depths
+
+
+def cpu_baseline(Hs, depths, seconds=12.0, steps_per_member=10):
+    """The oracle (FP64, all host threads) on the ensemble's members in
+    order, `steps_per_member` RK4 steps each, until `seconds` have passed."""
     from oracle import heom_oracle as HO
     from oracle import prm_oracle as PO
+    threads, ncpu = host_cores()
     op = PO.read_command_line(CFG, "rk4")
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
-    orc = HO.Oracle(HO.Problem.from_prm(op), threads=threads)
     dt = op.timestep
-    steps = 0
+    node_steps = 0
+    m = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        orc.rk4(10 * dt * (1 - 1e-9), dt, 1.0)
-        steps += 10
+    while time.perf_counter() - t0 < seconds and m < len(Hs):
+        op.time_indepenent_hamiltonian = Hs[m]
+        op.hierarchy_truncation_level = depths[m]
+        orc = HO.Oracle(HO.Problem.from_prm(op), threads=threads)
+        orc.rk4(steps_per_member * dt * (1 - 1e-9), dt, 1.0)
+        node_steps += orc.num_nodes * steps_per_member
+        orc.close()
+        m += 1
     el = time.perf_counter() - t0
-    return {"value": steps * orc.num_nodes / el, "unit": "node-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d RK4 steps of the 7-site depth-8 hierarchy (6435 nodes), FP64, "
-                      "%d pthreads, %.1f s" % (steps, threads, el)}
+    return {"value": node_steps / el, "unit": "node-steps/s", "cores": threads, "kind": "port",
+            "sample": "members 0..%d of the 512-member ensemble (%d node-steps), %d RK4 steps "
+                      "each, FP64, %d pthreads (affinity set of %d online CPUs), %.1f s"
+                      % (m - 1, node_steps, steps_per_member, threads, ncpu, el)}
 
 
-def profile_traffic(kernel_stage, prec):
-    """HBM bytes per launch of a stage kernel from the committed counter
+def profile_traffic(path, kernel_stage, prec):
+    """HBM bytes per launch of a stage kernel from a committed counter
     summary (lines 'stage=<s> prec=<p> fetch_bytes=<F> write_bytes=<W>')."""
-    if not os.path.exists(PROFILE):
+    if not os.path.exists(path):
         return None
-    for line in open(PROFILE):
+    for line in open(path):
         m = re.match(r"\s*stage=(\d) prec=(\w+) fetch_bytes=([\d.eE+]+) write_bytes=([\d.eE+]+)",
                      line)
         if m and int(m.group(1)) == kernel_stage and m.group(2) == prec:
             return float(m.group(3)) + float(m.group(4))
     return None
+
+
+def mean_traffic(path, prec):
+    tr = [profile_traffic(path, s, prec) for s in range(4)]
+    return sum(tr) / 4 if None not in tr else None
+
+
+def errors(r, ref):
+    """Population (diagonal) and coherence (off-diagonal) max |difference|
+    of density matrices r and ref (..., N, N)."""
+    d = np.asarray(r) - np.asarray(ref)
+    diag = np.einsum("...ii->...i", d)
+    off = d - np.einsum("...i,ij->...ij", diag, np.eye(d.shape[-1]))
+    return float(np.abs(diag.real).max()), float(np.abs(off).max())
+
+
+def launch_label(counts):
+    g, e = counts
+    if g == 0:
+        return "eager launches (%d steps)" % e
+    return "hipGraph blocks (%d steps) + eager head/tail (%d steps)" % (g, e)
 
 
 def run_rk4(P, p, prec, steps, warmup, shape=None):
@@ -129,13 +178,14 @@ def run_rk4(P, p, prec, steps, warmup, shape=None):
     ev_ms = ctx.rk4_steps_timed(steps, dt)
     ctx.synchronize()
     el = time.perf_counter() - t0
+    launch = launch_label(ctx.launch_counts())
     rho0 = ctx.rho0()
     ms = ctx.rk4_stage_times(min(200, max(steps, 20)), dt) / min(200, max(steps, 20))
     out = {"seconds": el, "node_steps_per_s": ctx.num_nodes * steps / el,
            "ms_per_step": 1e3 * el / steps, "event_ms_per_step": ev_ms / steps,
            "stage_ms": ms.tolist(), "rho0": rho0,
            "nodes": ctx.num_nodes, "shape": shape,
-           "launch": "hipGraph blocks" if ctx.graphs() else "eager"}
+           "launch": launch}
     ctx.close()
     return out
 
@@ -184,53 +234,44 @@ def run_c3_rkf45(P, np, log):
                      "node_steps_per_s": nodes * info["steps"] / info["seconds"],
                      "hbm_frac_attempt": bytes_attempt * info["steps"] / info["seconds"]
                                          / (HBM_PEAK_GBS * 1e9),
-                     "pops": np.real(np.diag(r))}
+                     "rho": r}
         log("bench: c3 rkf45 %s %d attempts %.3f s" % (prec, info["steps"], info["seconds"]))
-    err = float(np.abs(out["fp32"]["pops"] - out["fp64"]["pops"]).max())
+    pe, ce = errors(out["fp32"]["rho"], out["fp64"]["rho"])
     for v in out.values():
-        del v["pops"]
-    out["population_error_fp32_vs_fp64"] = err
+        del v["rho"]
+    out["population_error_fp32_vs_fp64"] = pe
+    out["coherence_error_fp32_vs_fp64"] = ce
     out["workload"] = ("24-site ring, depth 4 (20475 nodes of 24x24), RKF45 abs tol 1e-8, "
                        "initial dt 1 fs, to 1 ps")
     return out
 
 
-def run_ensemble(P, np, steps, dt, members, log):
-    """Static-disorder ensemble: `members` realisations of the 7-site system
-    (site energies + N(0, 100 cm^-1), depth U{4..8}) in one slab, FP32-bulk
-    and FP64 RK4 to `steps` steps with the member-averaged rho_0 recorded
-    every 10 steps."""
-    from phi_amd.configs.make_configs import c4_member
-    p = P.PhiParameters.read(CFG, "rk4")
-    Hs, depths = [], []
-    for r in range(members):
-        H, lh = c4_member(r)
-        Hs.append(H)
-        depths.append(lh)
-    out = {}
-    for prec in ("fp32", "fp64"):
-        ens = P.Ensemble(p, np.array(Hs), depths, precision=prec)
+def run_ensemble(P, p, Hs, depths, prec, steps, warmup, dt, shape=None):
+    """The ensemble in one slab: autotuned (or pinned) shape, W warm-up
+    steps, then K steps with the member-averaged rho_0 every 10 steps,
+    timed by the host clock (synchronised) and by HIP events on the
+    engine's stream.  Also the stage kernels back to back."""
+    ens = P.Ensemble(p, np.array(Hs), depths, precision=prec)
+    if shape is None:
         _, shape = ens.autotune(dt, steps=3)
-        ens.synchronize()
-        t0 = time.perf_counter()
-        hist = ens.rk4_history(steps, dt, 10)
-        ens.synchronize()
-        el = time.perf_counter() - t0
-        out[prec] = {"seconds": el, "nodes": ens.num_nodes, "hist": hist, "shape": shape}
-        log("bench: ensemble %s %d members %d nodes: %.3f s" % (prec, members, ens.num_nodes, el))
-        ens.close()
-    pops = {k: np.real(np.einsum("tii->ti", v["hist"])) for k, v in out.items()}
-    return {
-        "workload": "%d realisations of the 7-site exciton (site-energy disorder N(0,100), "
-                    "depth U{4..8}), one slab, RK4 dt=1 fs, member-averaged populations every 10 fs"
-                    % members,
-        "members": members, "nodes": out["fp32"]["nodes"],
-        "time_to_1ps_s_fp32_bulk": out["fp32"]["seconds"] * (1000.0 / steps),
-        "time_to_1ps_s_fp64": out["fp64"]["seconds"] * (1000.0 / steps),
-        "node_steps_per_s_fp32_bulk": out["fp32"]["nodes"] * steps / out["fp32"]["seconds"],
-        "node_steps_per_s_fp64": out["fp64"]["nodes"] * steps / out["fp64"]["seconds"],
-        "averaged_population_error_fp32_vs_fp64": float(np.abs(pops["fp32"] - pops["fp64"]).max()),
-        "kernel_shape_fp32": out["fp32"]["shape"]}
+    else:
+        ens.set_launch(*shape)
+        shape = tuple(shape)
+    ens.rk4_steps(warmup, dt)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    hist, ev_ms = ens.rk4_history(steps, dt, 10, timed=True)
+    ens.synchronize()
+    el = time.perf_counter() - t0
+    launch = launch_label(ens.launch_counts())
+    members_rho0 = ens.member_rho0()
+    n_st = min(50, max(steps, 10))
+    st_ms = ens.rk4_stage_times(n_st, dt) / n_st
+    out = {"seconds": el, "event_ms": ev_ms, "nodes": ens.num_nodes, "hist": hist,
+           "members_rho0": members_rho0, "shape": shape, "stage_ms": st_ms.tolist(),
+           "launch": launch}
+    ens.close()
+    return out
 
 
 def run_closure(P, np, steps, dt, train_members, test_members, log):
@@ -267,19 +308,20 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-ensemble-fp64", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--no-steady-state", action="store_true")
     ap.add_argument("--no-c3-rkf45", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=512)
-    ap.add_argument("--closure-train", type=int, default=512)
+    ap.add_argument("--closure-train", type=int, default=4096)
     ap.add_argument("--closure-test", type=int, default=256)
     ap.add_argument("--calibration-copy", action="store_true",
                     help="first copy 1 GiB device to device (a plain copy of known size, "
                          "the reference point of the HBM byte counters under rocprofv3)")
     shape = lambda s: tuple(int(v) for v in s.split(","))
-    for leg in ("fp64", "fp32", "c3"):
+    for leg in ("ens", "fp64", "fp32", "c3"):
         ap.add_argument("--shape-" + leg, type=shape, default=None,
                         help="launch shape form,tile,threads,blocks of the %s leg instead of "
                              "autotuning it (e.g. to profile the shape a previous run chose)" % leg)
@@ -289,7 +331,6 @@ def main():
               "(the multi-GPU sharding is a design, not a build).", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import phi_amd as P
 
     if args.calibration_copy:
@@ -304,99 +345,145 @@ def main():
     p = P.PhiParameters.read(CFG, "rk4")
     N = p.get_int("num_states")
     dt = p.get_double("timestep")
-    res = {}
-    for prec in ("fp64", "fp32"):
-        log("bench: 7-site depth 8, %s" % prec)
-        res[prec] = run_rk4(P, p, prec, args.steps, args.warmup,
-                            getattr(args, "shape_" + prec))
-        log("bench: %s %.4f ms/step, shape %s, stage ms %s"
-            % (prec, res[prec]["ms_per_step"], res[prec]["shape"], res[prec]["stage_ms"]))
-    pop64 = np.real(np.diag(res["fp64"]["rho0"]))
-    pop32 = np.real(np.diag(res["fp32"]["rho0"]))
-    pop_err = float(np.abs(pop64 - pop32).max())
+    K, W = args.steps, args.warmup
+    Hs, depths = ensemble_members(args.ensemble_members)
 
-    nodes = res["fp64"]["nodes"]
-    sb = stage_bytes(N, 16)
-    st_ms = res["fp64"]["stage_ms"]
-    dom = int(np.argmax(st_ms))
-    # The four stage launches of a step, timed over the timed region.
-    launch_ms = res["fp64"]["event_ms_per_step"] / 4
+    # ---- headline: the ensemble, FP32 bulk --------------------------------------
+    log("bench: ensemble fp32, %d members" % len(Hs))
+    e32 = run_ensemble(P, p, Hs, depths, "fp32", K, W, dt, args.shape_ens)
+    nodes = e32["nodes"]
+    log("bench: ensemble fp32 %d nodes: %.4f ms/step (events %.4f), shape %s, %s, stage ms %s"
+        % (nodes, 1e3 * e32["seconds"] / K, e32["event_ms"] / K, e32["shape"], e32["launch"],
+           e32["stage_ms"]))
+    sb = stage_bytes(N, 8)
+    launch_ms = e32["event_ms"] / K / 4
     per_launch = sum(sb) * nodes / 4
     achieved = per_launch / (launch_ms * 1e-3) / 1e9
-    tr = [profile_traffic(s, "fp64") for s in range(4)]
-    traffic = sum(tr) / 4 if None not in tr else None
+    classic = sum(classic_stage_bytes(N, 8)) * nodes / 4
+    st_ms = e32["stage_ms"]
+    dom = int(np.argmax(st_ms))
+    ms_step = 1e3 * e32["seconds"] / K
     steps_per_ps = round(1.0 / dt)
     line = {
-        "metric": "node-steps/s (RK4, 7-site exciton depth 8, FP64) + time-to-1ps",
-        "value": res["fp64"]["node_steps_per_s"],
+        "metric": "node-steps/s (RK4, 512-member static-disorder 7-site ensemble, FP32-bulk "
+                  "RHS / FP64 state) + time-to-1ps",
+        "value": nodes * K / e32["seconds"],
         "unit": "node-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": res["fp64"]["ms_per_step"],
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded 7-site exciton Hamiltonian, phi_amd/configs/c2.prm)",
-        "config": {"workload": "7-site exciton HEOM, depth 8 (6435 nodes of 7x7), RK4 dt=1 fs",
-                   "nodes": nodes, "num_states": N, "dt_ps": dt, "rk4_form": "nested",
-                   "kernel_shape_fp64": res["fp64"]["shape"],
-                   "kernel_shape_fp32": res["fp32"]["shape"],
-                   "launch_fp64": res["fp64"]["launch"],
-                   "launch_fp32": res["fp32"]["launch"],
-                   "time_to_1ps_s_fp64": res["fp64"]["ms_per_step"] * steps_per_ps * 1e-3,
-                   "time_to_1ps_s_fp32_bulk": res["fp32"]["ms_per_step"] * steps_per_ps * 1e-3},
-        "fp32_bulk": {"value": res["fp32"]["node_steps_per_s"], "unit": "node-steps/s",
-                      "ms_per_step": res["fp32"]["ms_per_step"],
-                      "population_error_vs_fp64": pop_err,
-                      "stage_ms": res["fp32"]["stage_ms"]},
+        "dtype": "f32",
+        "data": "synthetic (seeded static-disorder realisations of the 7-site exciton, "
+                "phi_amd/configs/make_configs.py c4_member)",
+        "config": {"workload": "%d static-disorder realisations of the 7-site exciton (depth "
+                               "U{4..8}, %d nodes of 7x7 in one slab), RK4 dt=1 fs, FP32-bulk "
+                               "RHS on the matrix cores, FP64 state, member-averaged rho_0 "
+                               "every 10 fs" % (len(Hs), nodes),
+                   "members": len(Hs), "nodes": nodes, "num_states": N, "dt_ps": dt,
+                   "rk4_form": "nested", "state_dtype": "f64",
+                   "kernel_shape": e32["shape"], "launch": e32["launch"],
+                   "time_to_1ps_s": ms_step * steps_per_ps * 1e-3,
+                   "time_to_1ps": "measured" if K == steps_per_ps else
+                                  "extrapolated from %d steps" % K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stream_stage (the 4 launches of a nested RK4 step, FP64)",
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": mean_traffic(PROFILE_ENS, "fp32"),
+                     "kernel": "stream_stage, ensemble instantiation (the 4 launches of a "
+                               "nested RK4 step, FP32 bulk)",
                      "bytes_per_launch": per_launch, "launch_ms": launch_ms,
-                     "isolated_launches": {
-                         "stage_ms_fp64": st_ms,
-                         "dominant_stage": dom,
-                         "achieved": sb[dom] * nodes / (st_ms[dom] * 1e-3) / 1e9},
+                     "accounting": "nested RK4, N^2 (80 + 6 s) per node-step, s = 8",
                      "classic_accounting": {
-                         "bytes_per_launch": sum(classic_stage_bytes(N, 16)) * nodes / 4,
-                         "achieved": sum(classic_stage_bytes(N, 16)) * nodes / 4
-                                     / (launch_ms * 1e-3) / 1e9,
-                         "frac_step": nodes * N * N * (160 + 6 * 16)
-                                      / (res["fp64"]["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9)}},
+                         "bytes_per_launch": classic,
+                         "achieved": classic / (launch_ms * 1e-3) / 1e9,
+                         "frac": classic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "isolated_launches": {
+                         "stage_ms": st_ms, "dominant_stage": dom,
+                         "achieved": sb[dom] * nodes / (st_ms[dom] * 1e-3) / 1e9}},
     }
+    if not args.no_ensemble_fp64:
+        log("bench: ensemble fp64")
+        e64 = run_ensemble(P, p, Hs, depths, "fp64", K, W, dt)
+        pe, ce = errors(e32["members_rho0"], e64["members_rho0"])
+        ape, ace = errors(e32["hist"], e64["hist"])
+        ms64 = 1e3 * e64["seconds"] / K
+        line["accuracy_vs_fp64"] = {
+            "population_error": pe, "coherence_error": ce,
+            "averaged_population_error": ape, "averaged_coherence_error": ace,
+            "what": "max |FP32 bulk - FP64| over every member's rho_0 after W + K steps "
+                    "(population: diagonal, coherence: off-diagonal), and over the "
+                    "member-averaged rho_0 at every 10 fs snapshot"}
+        line["fp64"] = {"node_steps_per_s": nodes * K / e64["seconds"], "ms_per_step": ms64,
+                        "time_to_1ps_s": ms64 * steps_per_ps * 1e-3,
+                        "kernel_shape": e64["shape"], "launch": e64["launch"],
+                        "hbm_frac": sum(stage_bytes(N, 16)) * nodes / (e64["event_ms"] / K * 1e-3)
+                                    / (HBM_PEAK_GBS * 1e9),
+                        "stage_ms": e64["stage_ms"]}
+        log("bench: ensemble fp64 %.4f ms/step; fp32 errors: population %.3e coherence %.3e"
+            % (ms64, pe, ce))
+
+    if not args.no_c2:
+        c2 = {}
+        for prec in ("fp64", "fp32"):
+            log("bench: 7-site depth 8, %s" % prec)
+            c2[prec] = run_rk4(P, p, prec, K, W, getattr(args, "shape_" + prec))
+            log("bench: %s %.4f ms/step, shape %s, stage ms %s"
+                % (prec, c2[prec]["ms_per_step"], c2[prec]["shape"], c2[prec]["stage_ms"]))
+        pe, ce = errors(c2["fp32"]["rho0"], c2["fp64"]["rho0"])
+        n2 = c2["fp64"]["nodes"]
+        lm = c2["fp64"]["event_ms_per_step"] / 4
+        pl = sum(stage_bytes(N, 16)) * n2 / 4
+        line["c2_single"] = {
+            "workload": "7-site exciton HEOM, depth 8 (6435 nodes of 7x7), RK4 dt=1 fs",
+            "fp64": {"node_steps_per_s": c2["fp64"]["node_steps_per_s"],
+                     "ms_per_step": c2["fp64"]["ms_per_step"],
+                     "time_to_1ps_s": c2["fp64"]["ms_per_step"] * steps_per_ps * 1e-3,
+                     "kernel_shape": c2["fp64"]["shape"], "launch": c2["fp64"]["launch"],
+                     "stage_ms": c2["fp64"]["stage_ms"]},
+            "fp32_bulk": {"node_steps_per_s": c2["fp32"]["node_steps_per_s"],
+                          "ms_per_step": c2["fp32"]["ms_per_step"],
+                          "kernel_shape": c2["fp32"]["shape"], "launch": c2["fp32"]["launch"],
+                          "population_error_vs_fp64": pe, "coherence_error_vs_fp64": ce},
+            "roofline_fp64": {"bound": "latency (state resident in MALL)",
+                              "achieved": pl / (lm * 1e-3) / 1e9, "launch_ms": lm,
+                              "frac_of_hbm": pl / (lm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "traffic": mean_traffic(PROFILE_C2, "fp64")}}
     if not args.no_c3:
-        log("bench: 24-site depth 4, fp32")
+        log("bench: 24-site depth 4, fp32 and fp64")
         p3 = P.PhiParameters.read(CFG3, "rk4")
         r3 = run_rk4(P, p3, "fp32", 50, 5, args.shape_c3)
+        r3d = run_rk4(P, p3, "fp64", 50, 5)
+        pe, ce = errors(r3["rho0"], r3d["rho0"])
         N3 = p3.get_int("num_states")
         b3 = stage_bytes(N3, 8)
         step_bytes = sum(b3) * r3["nodes"]
         line["c3_fp32"] = {
-            "workload": "24-site ring, depth 4 (20475 nodes of 24x24), FP32-bulk RK4 dt=1 fs",
+            "workload": "24-site ring, depth 4 (20475 nodes of 24x24), FP32-bulk RK4 dt=1 fs, "
+                        "55 steps",
             "node_steps_per_s": r3["node_steps_per_s"], "ms_per_step": r3["ms_per_step"],
-            "hbm_frac_step": step_bytes / (r3["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "population_error_vs_fp64": pe, "coherence_error_vs_fp64": ce,
+            "hbm_frac_step": step_bytes / (r3["event_ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "hbm_frac_step_classic_accounting":
-                r3["nodes"] * N3 * N3 * (160 + 6 * 8) / (r3["ms_per_step"] * 1e-3)
+                r3["nodes"] * N3 * N3 * (160 + 6 * 8) / (r3["event_ms_per_step"] * 1e-3)
                 / (HBM_PEAK_GBS * 1e9),
-            "stage_ms": r3["stage_ms"], "kernel_shape": r3["shape"]}
+            "stage_ms": r3["stage_ms"], "kernel_shape": r3["shape"], "launch": r3["launch"],
+            "fp64_ms_per_step": r3d["ms_per_step"]}
     if not args.no_c3 and not args.no_c3_rkf45:
         log("bench: 24-site depth 4, rkf45")
         line["c3_rkf45"] = run_c3_rkf45(P, np, log)
     if not args.no_steady_state:
         log("bench: 7-site steady state")
         line["c2_steady_state"] = run_steady_state(P, np, p, 1000)
-    if not args.no_ensemble:
-        log("bench: static-disorder ensemble")
-        line["c4_ensemble"] = run_ensemble(P, np, 1000, dt, args.ensemble_members, log)
     if not args.no_closure:
         log("bench: learned closure")
         line["c5_closure"] = run_closure(P, np, 1000, dt, args.closure_train, args.closure_test,
                                          log)
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = cpu_baseline(Hs, depths)
     print(json.dumps(line))
 
 

@@ -156,6 +156,12 @@ int phi_hip_ensemble_rho0(phi_ctx* c, double* out, size_t n_complex);
  * Rk4Integrate, hierarchy_integrator.cc:2859-2880). */
 int phi_hip_rk4_history(phi_ctx* c, long long nsteps, double dt, int every, double* out,
                         size_t n_complex, int* count);
+/* phi_hip_rk4_history with the steps (snapshots included) between two HIP
+ * events on the context's stream; *ms = milliseconds between them.
+ * PHI counterpart: PHI times its loop with the host clock (Timer,
+ * timer.cc:50-75). */
+int phi_hip_rk4_history_timed(phi_ctx* c, long long nsteps, double dt, int every, double* out,
+                              size_t n_complex, int* count, double* ms);
 
 /* ---- Learned truncation closure ------------------------------------------------ */
 /* No counterpart in PHI, which truncates the hierarchy by dropping the tier
@@ -259,6 +265,10 @@ int phi_hip_set_rk4_form(phi_ctx* c, int form);
 int phi_hip_set_graphs(phi_ctx* c, int on);
 /* The current choice (1 = graph blocks, 0 = eager launches). */
 int phi_hip_get_graphs(phi_ctx* c, int* on);
+/* How the last RK4 call (phi_hip_rk4_steps, _timed, _history) ran: steps
+ * replayed from a captured graph and steps launched eagerly.  No
+ * counterpart in PHI (no device graphs). */
+int phi_hip_launch_counts(const phi_ctx* c, long long* graph_steps, long long* eager_steps);
 /* Autotuner of the stage-kernel shape (no counterpart in PHI, whose
  * partitioning is PartitionHierarchy, hierarchy_integrator.cc:636-808):
  * times `steps` RK4 steps of size dt for every candidate shape in the

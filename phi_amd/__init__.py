@@ -103,6 +103,10 @@ def lib():
         "phi_hip_ensemble_rho0": (ctypes.c_int, [vp, dp, sz]),
         "phi_hip_rk4_history": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double,
                                                ctypes.c_int, dp, sz, ip]),
+        "phi_hip_launch_counts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_longlong),
+                                                 ctypes.POINTER(ctypes.c_longlong)]),
+        "phi_hip_rk4_history_timed": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double,
+                                                     ctypes.c_int, dp, sz, ip, dp]),
         "phi_hip_mlp_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int, ctypes.POINTER(vp)]),
         "phi_hip_mlp_destroy": (None, [vp]),
@@ -318,17 +322,26 @@ class Context:
         self.D = lib().phi_hip_num_modes(h)
         self.N = params.get_int("num_states")
 
-    def rk4_history(self, nsteps, dt, every):
+    def rk4_history(self, nsteps, dt, every, timed=False):
         """nsteps RK4 steps; rho_0 (an ensemble: the mean of the members'
-        rho_0) after every step i with i % every == 0.  Returns (k, N, N)."""
+        rho_0) after every step i with i % every == 0.  Returns (k, N, N);
+        with timed=True also the milliseconds between two HIP events on the
+        context's stream around the steps: (history, ms)."""
         k_max = (int(nsteps) + int(every) - 1) // int(every)
         out = np.zeros(2 * max(k_max, 1) * self.N * self.N)
         count = ctypes.c_int()
-        _check(lib().phi_hip_rk4_history(self._h, int(nsteps), float(dt), int(every), _dp(out),
-                                         max(k_max, 1) * self.N * self.N, ctypes.byref(count)))
+        cap = max(k_max, 1) * self.N * self.N
+        if timed:
+            ms = np.zeros(1)
+            _check(lib().phi_hip_rk4_history_timed(self._h, int(nsteps), float(dt), int(every),
+                                                   _dp(out), cap, ctypes.byref(count), _dp(ms)))
+        else:
+            _check(lib().phi_hip_rk4_history(self._h, int(nsteps), float(dt), int(every),
+                                             _dp(out), cap, ctypes.byref(count)))
         k = count.value
-        return out[:2 * k * self.N * self.N].view(complex).reshape(k, self.N, self.N).transpose(
+        hist = out[:2 * k * self.N * self.N].view(complex).reshape(k, self.N, self.N).transpose(
             0, 2, 1).copy()
+        return (hist, float(ms[0])) if timed else hist
 
     def close(self):
         if getattr(self, "_h", None):
@@ -441,6 +454,13 @@ class Context:
         on = ctypes.c_int()
         _check(lib().phi_hip_get_graphs(self._h, ctypes.byref(on)))
         return bool(on.value)
+
+    def launch_counts(self):
+        """(graph steps, eager steps) of the last RK4 call."""
+        g = ctypes.c_longlong()
+        e = ctypes.c_longlong()
+        _check(lib().phi_hip_launch_counts(self._h, ctypes.byref(g), ctypes.byref(e)))
+        return g.value, e.value
 
     def autotune(self, dt, steps=5):
         """Pick the fastest stage-kernel shape for this problem, precision

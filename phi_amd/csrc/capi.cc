@@ -522,24 +522,37 @@ int phi_hip_ensemble_rho0(phi_ctx* c, double* out, size_t n) {
   });
 }
 
+static int Rk4History(phi_ctx* c, long long nsteps, double dt, int every, double* out,
+                      size_t n, int* count, double* ms) {
+  if (!c || !out || !count || nsteps < 0 || every < 1) return Err(PHI_HIP_EINVAL, "bad argument");
+  phi::Engine* e = c->e;
+  const size_t ne = (size_t)e->N() * e->N();
+  if ((nsteps + every - 1) / every > e->history_capacity())
+    return Err(PHI_HIP_EINVAL, "more snapshots than the device history holds");
+  std::vector<std::complex<double>> h;
+  int rc = e->DrainHistory(&h);  // discard anything recorded before
+  if (rc >= 0) rc = ms ? e->Rk4StepsTimed(nsteps, dt, ms, every) : e->Rk4Steps(nsteps, dt, 0, every);
+  if (rc >= 0) rc = e->DrainHistory(&h);
+  if (rc < 0) return Err(rc, e->last_error());
+  const size_t k = h.size() / ne;
+  if (k * ne > n) return Err(PHI_HIP_EINVAL, "history buffer too small");
+  std::memcpy(out, h.data(), h.size() * sizeof(std::complex<double>));
+  *count = (int)k;
+  return Ok();
+}
+
 int phi_hip_rk4_history(phi_ctx* c, long long nsteps, double dt, int every, double* out,
                         size_t n, int* count) {
   return Guarded(__func__, [&]() -> int {
-    if (!c || !out || !count || nsteps < 0 || every < 1) return Err(PHI_HIP_EINVAL, "bad argument");
-    phi::Engine* e = c->e;
-    const size_t ne = (size_t)e->N() * e->N();
-    if ((nsteps + every - 1) / every > e->history_capacity())
-      return Err(PHI_HIP_EINVAL, "more snapshots than the device history holds");
-    std::vector<std::complex<double>> h;
-    int rc = e->DrainHistory(&h);  // discard anything recorded before
-    if (rc >= 0) rc = e->Rk4Steps(nsteps, dt, 0, every);
-    if (rc >= 0) rc = e->DrainHistory(&h);
-    if (rc < 0) return Err(rc, e->last_error());
-    const size_t k = h.size() / ne;
-    if (k * ne > n) return Err(PHI_HIP_EINVAL, "history buffer too small");
-    std::memcpy(out, h.data(), h.size() * sizeof(std::complex<double>));
-    *count = (int)k;
-    return Ok();
+    return Rk4History(c, nsteps, dt, every, out, n, count, nullptr);
+  });
+}
+
+int phi_hip_rk4_history_timed(phi_ctx* c, long long nsteps, double dt, int every, double* out,
+                              size_t n, int* count, double* ms) {
+  return Guarded(__func__, [&]() -> int {
+    if (!ms) return Err(PHI_HIP_EINVAL, "null argument");
+    return Rk4History(c, nsteps, dt, every, out, n, count, ms);
   });
 }
 
@@ -784,6 +797,15 @@ int phi_hip_get_graphs(phi_ctx* c, int* on) {
   return Guarded(__func__, [&]() -> int {
     if (!c || !on) return Err(PHI_HIP_EINVAL, "null argument");
     *on = c->e->graphs() ? 1 : 0;
+    return Ok();
+  });
+}
+
+int phi_hip_launch_counts(const phi_ctx* c, long long* graph_steps, long long* eager_steps) {
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !graph_steps || !eager_steps) return Err(PHI_HIP_EINVAL, "null argument");
+    *graph_steps = c->e->last_graph_steps();
+    *eager_steps = c->e->last_eager_steps();
     return Ok();
   });
 }

@@ -756,11 +756,13 @@ int Engine::GraphBlock(int steps, int snap_every, double dt) {
 }
 
 int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int snap_every) {
+  last_graph_steps_ = last_eager_steps_ = 0;
   int rc = AllocBulk();
   if (rc) return rc;
   auto snap = [&](long long s) { return snap_every > 0 && (step_offset + s) % snap_every == 0; };
   if (explicit_path()) {
     for (long long s = 0; s < nsteps && !rc; ++s) rc = ExplicitRk4Step(0.0, dt, snap(s), nullptr);
+    last_eager_steps_ = nsteps;
     return rc;
   }
   // Graph blocks of G steps, G a multiple of the snapshot period, aligned so
@@ -784,22 +786,24 @@ int Engine::Rk4Steps(long long nsteps, double dt, long long step_offset, int sna
     for (; s + G <= nsteps; s += G) {
       rc = Check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
       if (rc) return rc;
+      last_graph_steps_ += G;
     }
   }
   for (; s < nsteps; ++s) {
     rc = Rk4Step(dt, snap(s));
     if (rc) return rc;
   }
+  last_eager_steps_ = nsteps - last_graph_steps_;
   return kOk;
 }
 
-int Engine::Rk4StepsTimed(long long nsteps, double dt, double* ms) {
+int Engine::Rk4StepsTimed(long long nsteps, double dt, double* ms, int snap_every) {
   *ms = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = Check(hipEventCreate(&e0), "event");
   if (!rc) rc = Check(hipEventCreate(&e1), "event");
   if (!rc) rc = Check(hipEventRecord(e0, stream_), "record");
-  if (!rc) rc = Rk4Steps(nsteps, dt, 0, 0);
+  if (!rc) rc = Rk4Steps(nsteps, dt, 0, snap_every);
   if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");
   if (!rc) rc = Check(hipEventSynchronize(e1), "event sync");
   float t = 0;

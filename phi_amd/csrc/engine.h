@@ -142,7 +142,7 @@ class Engine {
   // two HIP events; ms[s] = their total.  The state is restored.
   int Rk4StageTimes(long long nsteps, double dt, double ms[4]);
   // Rk4Steps between two HIP events on the engine's stream; *ms = elapsed.
-  int Rk4StepsTimed(long long nsteps, double dt, double* ms);
+  int Rk4StepsTimed(long long nsteps, double dt, double* ms, int snap_every = 0);
   int HistoryCount();
   int history_capacity() const { return hist_capacity_; }
   int DrainHistory(std::vector<std::complex<double>>* out);  // and resets it
@@ -202,6 +202,9 @@ class Engine {
     if (!v) DropGraph();
   }
   bool graphs() const { return use_graphs_; }
+  // Steps of the last RK4 call that replayed a captured graph / ran eagerly.
+  long long last_graph_steps() const { return last_graph_steps_; }
+  long long last_eager_steps() const { return last_eager_steps_; }
   // Autotuner: times `steps` RK4 steps (after one warm-up step) for each
   // candidate shape of the persistent kernels (tile x grid) in the current
   // precision and RK4 form, keeps the fastest, then (graphs not pinned)
@@ -238,6 +241,7 @@ class Engine {
   static constexpr int kGraphSteps = 32;
   bool use_graphs_ = true;
   bool graphs_pinned_ = false;
+  long long last_graph_steps_ = 0, last_eager_steps_ = 0;
   hipGraphExec_t graph_exec_ = nullptr;
   GraphKey graph_key_{};
 

@@ -292,19 +292,27 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
   const int N = A.N, NE = A.NE, D = A.D, L = A.L, T = A.tile;
   const int NP = N * (N + 1);
   const StreamLds<TB> s = stream_carve<TB>(smem, N, D, L, T);
-  // Tiles of this block: a contiguous range of the descriptor list for an
-  // ensemble (members change rarely, so does H), else every gridDim.x-th
-  // tile of the hierarchy (neighbouring tiles in flight together), starting
-  // from an XCD-aware slot: workgroups are dealt to the 8 XCDs round robin
-  // (block b on XCD b % 8), so the slots of XCD x form one contiguous run
-  // [start_x, start_x + count_x) and the neighbour ADOs that nearby tiles
-  // share are fetched into one XCD's L2 instead of several.
+  // Tiles of this block.  Workgroups are dealt to the 8 XCDs round robin
+  // (block b on XCD b % 8), each XCD with its own L2.
+  //  * Ensemble: the descriptor list is cut into X = min(G, 8) contiguous
+  //    runs, run x for the blocks b with b % X == x, which take every G_x-th
+  //    tile of it (G_x blocks in the run).  The blocks of one XCD therefore
+  //    work together on a window of G_x consecutive tiles, inside one or two
+  //    members, and every neighbour gather of that window (a member's
+  //    neighbours lie anywhere in its tier-major id range) is served by that
+  //    XCD's L2, which holds the member's stage input.
+  //  * Single hierarchy: every G-th tile of the hierarchy, starting from an
+  //    XCD-aware slot, so the slots of XCD x form one contiguous run
+  //    [start_x, start_x + count_x) and the neighbour ADOs that nearby tiles
+  //    share are fetched into one XCD's L2 instead of several.
   const int ntiles = ENS ? A.ntiles : (A.num_nodes + T - 1) / T;
   int tile, tile_end, step;
   if constexpr (ENS) {
-    tile = (int)(((long long)blockIdx.x * ntiles) / gridDim.x);
-    tile_end = (int)(((long long)(blockIdx.x + 1) * ntiles) / gridDim.x);
-    step = 1;
+    const int G = gridDim.x, X = G < 8 ? G : 8;
+    const int x = blockIdx.x % X;
+    tile = (int)(((long long)x * ntiles) / X) + (int)(blockIdx.x / X);
+    tile_end = (int)(((long long)(x + 1) * ntiles) / X);
+    step = G / X + (x < G % X ? 1 : 0);
   } else {
     const int G = gridDim.x, x = blockIdx.x & 7, base = G >> 3, r = G & 7;
     tile = x * base + min(x, r) + (int)(blockIdx.x >> 3);
@@ -356,6 +364,8 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
   }
   stream_load2<TB, TI, EPT, MPS, ENS>(A, in, ti, cur);
   __syncthreads();  // per-block tables visible
+  double2 hp[2] = {make_double2(0., 0.), make_double2(0., 0.)};
+  bool hpre = false;
 
   for (int buf = 0; tile < tile_end; tile += step, buf ^= 1) {
     const int nt = ti.nt, base = ti.base;
@@ -363,10 +373,22 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     V* Yb = s.Y + buf * s.bstride;
     V* Wb = s.W + buf * s.bstride;
     // H of a new member (the previous tile's product finished before the
-    // last barrier; the next barrier publishes the new fragments).
+    // last barrier; the next barrier publishes the new fragments): from the
+    // registers loaded with the previous tile's prefetch, else from memory.
     if (ENS && ti.member != hmember) {
       hmember = ti.member;
-      stream_load_h<TB>(A, s, hmember);
+      if (hpre) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int f = threadIdx.x + j * blockDim.x;
+          if (f < nf) {
+            s.hfr[f] = (TB)hp[j].x;
+            s.hfi[f] = (TB)hp[j].y;
+          }
+        }
+      } else {
+        stream_load_h<TB>(A, s, hmember);
+      }
     }
     // (1) elementwise part of this tile -> LDS; epilogue operands of this tile
     Pre pre[EPT];
@@ -409,9 +431,21 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     const int ntile = tile + step;
     const bool more = ntile < tile_end;
     TileInfo nti = ti;
+    hpre = false;
     if (more) {
       nti = stream_tile<ENS>(A, ntile);
       stream_load1<TB, TI, EPT, MPS>(A, in, nti, cur);
+      // The next tile's H fragments, when its member differs and they fit
+      // two per thread (L2 hits, in flight with the tile's other loads).
+      if (ENS && nti.member != ti.member && nf <= 2 * (int)blockDim.x) {
+        const double2* Hn = A.H + (size_t)nti.member * NE;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int f = threadIdx.x + j * blockDim.x;
+          if (f < nf) hp[j] = stream_h_frag(A, Hn, f);
+        }
+        hpre = true;
+      }
     }
     __syncthreads();
     // (3) matrix-core product
