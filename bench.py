@@ -6,10 +6,10 @@ depths drawn from U{4..8}, i.e. PHI HierarchyTruncation 5..9, 330-6435 nodes
 each, 1.36 M nodes of 7x7 complex ADOs in one slab), RK4 with dt = 1 fs,
 FP32-bulk right-hand side on the matrix cores with the FP64 state, the
 member-averaged rho_0 recorded every 10 fs.  One step = one RK4 step over
-every node of every member (4 persistent stage-kernel launches, nested
-form).  W warm-up steps, then K timed steps (default 1000 = 1 ps, so the
-time to 1 ps is measured, not extrapolated) with the state resident in HBM,
-bracketed by device synchronisation.  Prints one JSON line on stdout.
+every node of every member (4 persistent stage-kernel launches, increment
+form of the nested step).  W warm-up steps, then K timed steps (default
+1000 = 1 ps, so the time to 1 ps is measured, not extrapolated) with the
+state resident in HBM, bracketed by device synchronisation.  Prints one JSON line on stdout.
 
 value    node-steps/s of the FP32-bulk ensemble run (whole slab).  The same
          ensemble in plain FP64 is reported beside it ("fp64"), and the
@@ -20,10 +20,10 @@ roofline the dominant kernel, stream_stage (ensemble instantiation; the
          every 10th step runs in the loop): algorithmic bytes per launch over
          the mean launch duration, i.e. a step's bytes over the timed
          region's HIP-event time per step (events on the engine's stream).
-         Algorithmic bytes per node (nested RK4 form, s = bytes per bulk
-         complex element): stage 0 N^2 (16 + s), stages 1-2 N^2 (16 + 2 s),
-         stage 3 N^2 (32 + s), N^2 (80 + 6 s) per step; neighbour columns
-         count as cache hits.  classic_accounting repeats the figures with
+         Algorithmic bytes per node (increment form of the nested RK4
+         step, s = bytes per bulk complex element): stage 0 N^2 (16 + s),
+         stage 1 N^2 2 s, stage 2 N^2 3 s, stage 3 N^2 (32 + 2 s),
+         N^2 (48 + 8 s) per step; neighbour columns count as cache hits.  classic_accounting repeats the figures with
          the bytes of classic RK4, N^2 (160 + 6 s) per node-step.  traffic =
          calibrated HBM bytes per launch (FETCH_SIZE + WRITE_SIZE) of the
          four stages from the committed rocprofv3 counter summary.
@@ -69,9 +69,13 @@ def log(*a):
 
 
 def stage_bytes(N, s):
-    """Algorithmic bytes per node of the four nested-form stage kernels."""
+    """Algorithmic bytes per node of the four stage kernels of the increment
+    form (the default RK4 order of the persistent kernels): stage 0 reads
+    rho and writes K, stage 1 reads K and writes D, stage 2 reads D and K
+    and writes D', stage 3 reads D', K and rho and writes rho:
+    N^2 (48 + 8 s) per node-step."""
     ne = N * N
-    return [ne * (16 + s), ne * (16 + 2 * s), ne * (16 + 2 * s), ne * (32 + s)]
+    return [ne * (16 + s), ne * (2 * s), ne * (3 * s), ne * (32 + 2 * s)]
 
 
 def classic_stage_bytes(N, s):
@@ -384,7 +388,7 @@ def main():
                                "RHS on the matrix cores, FP64 state, member-averaged rho_0 "
                                "every 10 fs" % (len(Hs), nodes),
                    "members": len(Hs), "nodes": nodes, "num_states": N, "dt_ps": dt,
-                   "rk4_form": "nested", "state_dtype": "f64",
+                   "rk4_form": "increment", "state_dtype": "f64",
                    "kernel_shape": e32["shape"], "launch": e32["launch"],
                    "time_to_1ps_s": ms_step * steps_per_ps * 1e-3,
                    "time_to_1ps": "measured" if K == steps_per_ps else
@@ -392,10 +396,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": mean_traffic(PROFILE_ENS, "fp32"),
-                     "kernel": "stream_stage, ensemble instantiation (the 4 launches of a "
-                               "nested RK4 step, FP32 bulk)",
+                     "kernel": "stream_stage, ensemble instantiation (the 4 launches of an "
+                               "RK4 step, increment form, FP32 bulk)",
                      "bytes_per_launch": per_launch, "launch_ms": launch_ms,
-                     "accounting": "nested RK4, N^2 (80 + 6 s) per node-step, s = 8",
+                     "accounting": "increment-form RK4, N^2 (48 + 8 s) per node-step, s = 8",
                      "classic_accounting": {
                          "bytes_per_launch": classic,
                          "achieved": classic / (launch_ms * 1e-3) / 1e9,

@@ -256,8 +256,12 @@ int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks);
 /* Stage order of phi_hip_rk4_steps (the RK4 loop body of Rk4Integrate,
  * hierarchy_integrator.cc:2709-2825): 0 = PHI's k0..k3 form with an
  * accumulator, 1 = nested form rho + dt L (rho + dt/2 L (rho + dt/3 L (rho +
- * dt/4 L rho))) (default), the same polynomial for this time-independent
- * linear RHS, with half the FP64 traffic.  Equal to rounding. */
+ * dt/4 L rho))), the same polynomial for this time-independent linear RHS,
+ * with half the FP64 traffic, 2 = its increment form (default: stages
+ * carry K = L rho and the increments dt/3 (K + dt/4 L K), dt/2 (K + L D);
+ * the FP64 state is read by the first and last stage only; with the
+ * one-shot kernels of large systems a step runs the nested form).  Equal to
+ * rounding. */
 int phi_hip_set_rk4_form(phi_ctx* c, int form);
 /* RK4 step blocks replayed from a captured hipGraph (on = 1, the default)
  * or launched kernel by kernel (0).  Same results either way.  Once called,

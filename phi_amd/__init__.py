@@ -471,9 +471,10 @@ class Context:
         return ms.value, tuple(shape)
 
     def set_rk4_form(self, form):
-        """RK4 stage order: "classic" (PHI's k0..k3 with an accumulator) or
-        "nested" (Taylor/Horner form, the default)."""
-        f = {"classic": 0, "nested": 1}[form]
+        """RK4 stage order: "classic" (PHI's k0..k3 with an accumulator),
+        "nested" (Taylor/Horner form) or "increment" (the nested form's
+        increments as stage inputs, the default)."""
+        f = {"classic": 0, "nested": 1, "increment": 2}[form]
         _check(lib().phi_hip_set_rk4_form(self._h, f))
 
 

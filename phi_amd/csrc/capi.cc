@@ -821,8 +821,8 @@ int phi_hip_autotune(phi_ctx* c, int steps, double dt, double* best_ms, int* sha
 int phi_hip_set_rk4_form(phi_ctx* c, int form) {
   return Guarded(__func__, [&]() -> int {
     if (!c) return Err(PHI_HIP_EINVAL, "null argument");
-    if (form != 0 && form != 1) return Err(PHI_HIP_EINVAL, "RK4 form must be 0 or 1");
-    c->e->set_rk4_nested(form == 1);
+    if (form < 0 || form > 2) return Err(PHI_HIP_EINVAL, "RK4 form must be 0, 1 or 2");
+    c->e->set_rk4_form(form);
     return Ok();
   });
 }

@@ -450,6 +450,13 @@ int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
                "256 threads when more than one)";
         return kErrInvalidArgument;
       }
+      // H's MFMA A-fragments: ceil(N/16) row tiles x ceil(N/4) k-steps x 64 lanes
+      const int frags = ((N + 15) / 16) * ((N + 3) / 4) * 64;
+      if (members_ && 2 * th < frags) {
+        err_ = "an ensemble block needs at least half as many threads as H has MFMA "
+               "fragment elements";
+        return kErrInvalidArgument;
+      }
       // Kernels without a persistent form (RKF45) run the one-shot
       // matrix-core kernels with 256 threads and their default tile.
       int t1 = std::max(1, std::min(64, 256 / NE));
@@ -485,10 +492,11 @@ namespace {
 
 struct TuneKey {
   int dev, N, D, L, nodes, mps, members;
-  bool fp32, nested;
+  bool fp32;
+  int form;
   bool operator<(const TuneKey& o) const {
-    return std::tie(dev, N, D, L, nodes, mps, members, fp32, nested) <
-           std::tie(o.dev, o.N, o.D, o.L, o.nodes, o.mps, o.members, o.fp32, o.nested);
+    return std::tie(dev, N, D, L, nodes, mps, members, fp32, form) <
+           std::tie(o.dev, o.N, o.D, o.L, o.nodes, o.mps, o.members, o.fp32, o.form);
   }
 };
 struct TuneVal {
@@ -513,7 +521,7 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   const int N = h_.N, NE = N * N;
   int dev = 0;
   (void)hipGetDevice(&dev);
-  const TuneKey key{dev, N, h_.D, h_.L, slab_nodes_, mps_, members_, fp32_, rk4_nested_};
+  const TuneKey key{dev, N, h_.D, h_.L, slab_nodes_, mps_, members_, fp32_, rk4_form_};
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune.find(key);
@@ -696,18 +704,38 @@ int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
   return rc;
 }
 
-int Engine::Rk4Step(double dt, bool snapshot) {
+int Engine::StepForm() const {
   // The closed RHS is not linear: PHI's classic stage order then.
-  const bool hn = rk4_nested_ && !closure_;
-  // Stage inputs alternate between the two bulk buffers so that a stage
-  // never overwrites what its own neighbour gathers read.
-  const void* in[4] = {d_rho_, d_y_[0], d_y_[1], d_y_[0]};
-  void* out[4] = {d_y_[0], d_y_[1], d_y_[0], nullptr};
+  if (closure_) return kRk4Classic;
+  if (rk4_form_ == kRk4Increment && args_.col_nm != 2) return kRk4Nested;
+  return rk4_form_;
+}
+
+void Engine::StageBuffers(int form, const void* in[4], void* out[4], void** aux) const {
+  if (form == kRk4Increment) {
+    // K = L rho in y0 (read again by stages 2 and 3), D in y1, D' in aux
+    // (FP64-sized, unused by this form otherwise).
+    in[0] = d_rho_, in[1] = d_y_[0], in[2] = d_y_[1], in[3] = d_aux_;
+    out[0] = d_y_[0], out[1] = d_y_[1], out[2] = d_aux_, out[3] = nullptr;
+    *aux = d_y_[0];
+    return;
+  }
+  in[0] = d_rho_, in[1] = d_y_[0], in[2] = d_y_[1], in[3] = d_y_[0];
+  out[0] = d_y_[0], out[1] = d_y_[1], out[2] = d_y_[0], out[3] = nullptr;
+  *aux = d_aux_;
+}
+
+int Engine::Rk4Step(double dt, bool snapshot) {
+  const int form = StepForm();
+  const void* in[4];
+  void* out[4];
+  void* aux;
+  StageBuffers(form, in, out, &aux);
   int rc = kOk;
   for (int s = 0; s < 4 && !rc; ++s) {
     if (closure_) rc = ClosurePredict(in[s], s > 0 && fp32_);
     if (!rc)
-      rc = Check(LaunchRk4Stage(args_, fp32_, s, hn, in[s], d_rho_, d_aux_, out[s], dt, stream_),
+      rc = Check(LaunchRk4Stage(args_, fp32_, s, form, in[s], d_rho_, aux, out[s], dt, stream_),
                  "rk4 stage");
   }
   if (!rc && snapshot) {
@@ -731,7 +759,7 @@ void Engine::DropGraph() {
 // j % snap_every == 0) captured once and replayed; re-captured when any
 // launch argument changes.  Only kernels are captured.
 int Engine::GraphBlock(int steps, int snap_every, double dt) {
-  GraphKey key{steps, snap_every, dt, fp32_, rk4_nested_, args_.col_nm,
+  GraphKey key{steps, snap_every, dt, fp32_, StepForm(), args_.col_nm,
                args_.col_tile[0], args_.col_tile[1], args_.col_threads[0], args_.col_threads[1],
                args_.col_grid[0], args_.col_grid[1], d_y_[0], d_rho_, d_tiles_};
   if (graph_exec_ && key == graph_key_) return kOk;
@@ -836,13 +864,15 @@ int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = Check(hipEventCreate(&e0), "event");
   if (!rc) rc = Check(hipEventCreate(&e1), "event");
-  const bool hn = rk4_nested_ && !closure_;
-  const void* in[4] = {d_rho_, d_y_[0], d_y_[1], d_y_[0]};
-  void* out[4] = {d_y_[0], d_y_[1], d_y_[0], nullptr};
+  const int form = StepForm();
+  const void* in[4];
+  void* out[4];
+  void* aux;
+  StageBuffers(form, in, out, &aux);
   for (int s = 0; s < 4 && !rc; ++s) {
     for (long long i = 0; i <= nsteps && !rc; ++i) {
       if (i == 1) rc = Check(hipEventRecord(e0, stream_), "record");
-      if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, s, hn, in[s], d_rho_, d_aux_, out[s], dt,
+      if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, s, form, in[s], d_rho_, aux, out[s], dt,
                                          stream_), "rk4 stage");
     }
     if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");

@@ -108,8 +108,11 @@ class Engine {
   int SetLaunch(int col_nm, int tile, int threads, int blocks);
   // RK4 stage order: true = nested (Taylor/Horner) form, false = PHI's
   // classic k0..k3 form.  Both give PHI's step to rounding.
-  void set_rk4_nested(bool v) { rk4_nested_ = v; }
-  bool rk4_nested() const { return rk4_nested_; }
+  // RK4 stage order (kRk4Classic, kRk4Nested, kRk4Increment).  The
+  // increment form needs the persistent kernels; with the one-shot kernels
+  // a step runs nested instead, and the closure always runs classic.
+  void set_rk4_form(int f) { rk4_form_ = f; }
+  int rk4_form() const { return rk4_form_; }
   const RhsArgs& args() const { return args_; }
   hipStream_t stream() const { return stream_; }
 
@@ -220,20 +223,25 @@ class Engine {
   int Check(hipError_t e, const char* where);
   int AllocBulk();
   int Rk4Step(double dt, bool snapshot);
+  // The stage order a step runs now, and each stage's input, output and
+  // auxiliary buffer for it (inputs alternate so that a stage never
+  // overwrites what its own neighbour gathers read).
+  int StepForm() const;
+  void StageBuffers(int form, const void* in[4], void* out[4], void** aux) const;
   int GraphBlock(int steps, int snap_every, double dt);
   void DropGraph();
 
   struct GraphKey {
     int steps, snap_every;
     double dt;
-    bool fp32, nested;
-    int col_nm, tile0, tile1, th0, th1, grid0, grid1;
+    bool fp32;
+    int form, col_nm, tile0, tile1, th0, th1, grid0, grid1;
     const void* y0;
     const void* rho;
     const void* tiles;
     bool operator==(const GraphKey& o) const {
       return steps == o.steps && snap_every == o.snap_every && dt == o.dt && fp32 == o.fp32 &&
-             nested == o.nested && col_nm == o.col_nm && tile0 == o.tile0 && tile1 == o.tile1 &&
+             form == o.form && col_nm == o.col_nm && tile0 == o.tile0 && tile1 == o.tile1 &&
              th0 == o.th0 && th1 == o.th1 && grid0 == o.grid0 && grid1 == o.grid1 &&
              y0 == o.y0 && rho == o.rho && tiles == o.tiles;
     }
@@ -248,7 +256,7 @@ class Engine {
   HierarchyTables h_;
   RhsArgs args_{};
   bool fp32_ = false;
-  bool rk4_nested_ = true;
+  int rk4_form_ = kRk4Increment;
   hipStream_t stream_ = nullptr;
   std::string err_;
 

@@ -387,6 +387,71 @@ __device__ __forceinline__ double2 ld2(const double* p, size_t e) {
   return reinterpret_cast<const double2*>(p)[e];
 }
 
+// Streaming accesses: elements a kernel reads or writes exactly once (the
+// FP64 state and accumulators, the stage output).  The non-temporal hint
+// lets L2 evict these lines first, which keeps resident the stage input
+// that neighbouring nodes gather from.
+typedef double V2dNt __attribute__((ext_vector_type(2)));
+typedef float V2fNt __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double2 ld2_nt(const double* p, size_t e) {
+  const V2dNt v = __builtin_nontemporal_load(reinterpret_cast<const V2dNt*>(p) + e);
+  return make_double2(v.x, v.y);
+}
+
+// Element e of a bulk slab in its own precision, non-temporal.
+template <typename TB>
+__device__ __forceinline__ typename C2<TB>::type ldv_nt(const TB* p, size_t e);
+
+template <>
+__device__ __forceinline__ double2 ldv_nt<double>(const double* p, size_t e) {
+  return ld2_nt(p, e);
+}
+
+template <>
+__device__ __forceinline__ float2 ldv_nt<float>(const float* p, size_t e) {
+  const V2fNt v = __builtin_nontemporal_load(reinterpret_cast<const V2fNt*>(p) + e);
+  return make_float2(v.x, v.y);
+}
+
+template <typename T>
+__device__ __forceinline__ void st_nt(T* __restrict__ p, size_t e, double re, double im);
+
+template <>
+__device__ __forceinline__ void st_nt<double>(double* __restrict__ p, size_t e, double re,
+                                              double im) {
+  V2dNt v;
+  v.x = re;
+  v.y = im;
+  __builtin_nontemporal_store(v, reinterpret_cast<V2dNt*>(p) + e);
+}
+
+template <>
+__device__ __forceinline__ void st_nt<float>(float* __restrict__ p, size_t e, double re,
+                                             double im) {
+  V2fNt v;
+  v.x = (float)re;
+  v.y = (float)im;
+  __builtin_nontemporal_store(v, reinterpret_cast<V2fNt*>(p) + e);
+}
+
+// A bulk element in its own precision, non-temporal.
+template <typename TB>
+__device__ __forceinline__ void stv_nt(TB* __restrict__ p, size_t e, typename C2<TB>::type v);
+
+template <>
+__device__ __forceinline__ void stv_nt<double>(double* __restrict__ p, size_t e, double2 v) {
+  st_nt<double>(p, e, v.x, v.y);
+}
+
+template <>
+__device__ __forceinline__ void stv_nt<float>(float* __restrict__ p, size_t e, float2 v) {
+  V2fNt w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<V2fNt*>(p) + e);
+}
+
 // --- RK4 -------------------------------------------------------------------
 // Two stage orders of the same step.  Classic (PHI's form,
 // hierarchy_integrator.cc:2714-2825), buffers rho, acc (FP64), Yin/Yout (TB):

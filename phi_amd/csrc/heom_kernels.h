@@ -85,9 +85,14 @@ size_t TileLdsBytes(int N, int D, int L, int tile, bool fp32, bool with_h);
 // LDS bytes of a block of the persistent kernels (two tiles in flight).
 size_t StreamTileLdsBytes(int N, int D, int L, int tile, bool fp32);
 
-// One RK4 stage; horner selects the nested stage order (no accumulator).
-hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, bool horner, const void* in,
-                          double* rho, double* acc, void* yout, double dt, hipStream_t s);
+// Stage orders of an RK4 step: PHI's classic order (FP64 accumulator), the
+// nested (Horner) order and its increment form (heom_stream.h; persistent
+// kernels only).
+enum { kRk4Classic = 0, kRk4Nested = 1, kRk4Increment = 2 };
+// One RK4 stage.  aux: the classic form's FP64 accumulator, or the
+// increment form's K (bulk precision); unused by the nested form.
+hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, int form, const void* in,
+                          double* rho, void* aux, void* yout, double dt, hipStream_t s);
 hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* in,
                             const double* rho, const Rkf45Bufs<float>& bf,
                             const Rkf45Bufs<double>& bd, const Rkf45Coefs& c, hipStream_t s);

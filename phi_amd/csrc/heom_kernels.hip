@@ -590,19 +590,19 @@ size_t StreamTileLdsBytes(int N, int D, int L, int tile, bool fp32) {
   return dev::StreamLdsBytes(N, D, L, tile, fp32 ? sizeof(float) : sizeof(double));
 }
 
-hipError_t LaunchRk4F64(const RhsArgs& a, int stage, bool horner, const void* in, double* rho,
-                        double* acc, void* yout, double dt, hipStream_t s);
-hipError_t LaunchRk4F32(const RhsArgs& a, int stage, bool horner, const void* in, double* rho,
-                        double* acc, void* yout, double dt, hipStream_t s);
+hipError_t LaunchRk4F64(const RhsArgs& a, int stage, int form, const void* in, double* rho,
+                        void* aux, void* yout, double dt, hipStream_t s);
+hipError_t LaunchRk4F32(const RhsArgs& a, int stage, int form, const void* in, double* rho,
+                        void* aux, void* yout, double dt, hipStream_t s);
 hipError_t LaunchRkf45F64(const RhsArgs& a, int stage, const void* in, const double* rho,
                          const Rkf45Bufs<double>& b, const Rkf45Coefs& c, hipStream_t s);
 hipError_t LaunchRkf45F32(const RhsArgs& a, int stage, const void* in, const double* rho,
                          const Rkf45Bufs<float>& b, const Rkf45Coefs& c, hipStream_t s);
 
-hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, bool horner, const void* in,
-                          double* rho, double* acc, void* yout, double dt, hipStream_t s) {
-  return fp32 ? LaunchRk4F32(a, stage, horner, in, rho, acc, yout, dt, s)
-              : LaunchRk4F64(a, stage, horner, in, rho, acc, yout, dt, s);
+hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, int form, const void* in,
+                          double* rho, void* aux, void* yout, double dt, hipStream_t s) {
+  return fp32 ? LaunchRk4F32(a, stage, form, in, rho, aux, yout, dt, s)
+              : LaunchRk4F64(a, stage, form, in, rho, aux, yout, dt, s);
 }
 
 hipError_t LaunchRkf45Stage(const RhsArgs& a, bool fp32, int stage, const void* in,

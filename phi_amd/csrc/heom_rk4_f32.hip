@@ -3,12 +3,12 @@
 
 namespace phi {
 
-hipError_t LaunchRk4F32(const RhsArgs& a, int stage, bool horner, const void* in, double* rho,
-                        double* acc, void* yout, double dt, hipStream_t s) {
+hipError_t LaunchRk4F32(const RhsArgs& a, int stage, int form, const void* in, double* rho,
+                        void* aux, void* yout, double dt, hipStream_t s) {
   if (stage == 0)
-    return LaunchRk4T<float, double>(a, 0, horner, (const double*)in, rho, acc, (float*)yout,
+    return LaunchRk4T<float, double>(a, 0, form, (const double*)in, rho, aux, (float*)yout,
                                      dt, s);
-  return LaunchRk4T<float, float>(a, stage, horner, (const float*)in, rho, acc, (float*)yout,
+  return LaunchRk4T<float, float>(a, stage, form, (const float*)in, rho, aux, (float*)yout,
                                   dt, s);
 }
 

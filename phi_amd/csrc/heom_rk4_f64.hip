@@ -3,9 +3,9 @@
 
 namespace phi {
 
-hipError_t LaunchRk4F64(const RhsArgs& a, int stage, bool horner, const void* in, double* rho,
-                        double* acc, void* yout, double dt, hipStream_t s) {
-  return LaunchRk4T<double, double>(a, stage, horner, (const double*)in, rho, acc,
+hipError_t LaunchRk4F64(const RhsArgs& a, int stage, int form, const void* in, double* rho,
+                        void* aux, void* yout, double dt, hipStream_t s) {
+  return LaunchRk4T<double, double>(a, stage, form, (const double*)in, rho, aux,
                                     (double*)yout, dt, s);
 }
 

@@ -17,36 +17,39 @@ int ResidentBlocks(const void* kernel, int threads, size_t lds);
 // One launch of the persistent stage kernel: `elems` elements per thread
 // (1..4) and `mps` modes per site (1..2) select the instantiation.
 template <typename TB, typename TI, int EPT, int MPS, bool ENS, class EPI>
-inline void LaunchStreamKE(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
-                           hipStream_t s) {
+inline hipError_t LaunchStreamKE(const RhsArgs& a, const TI* in, const EPI& epi, int threads,
+                                 int grid, hipStream_t s) {
   const size_t sh = dev::StreamLdsBytes(a.N, a.D, a.L, a.tile, sizeof(TB));
   const void* k = reinterpret_cast<const void*>(&dev::stream_stage<TB, TI, EPT, MPS, ENS, EPI>);
   const int ntiles = ENS ? a.ntiles : (a.num_nodes + a.tile - 1) / a.tile;
+  // An ensemble block prefetches a new member's H fragments two per thread.
+  if (ENS && dev::StreamFragCount(a.N) > 2 * threads) return hipErrorInvalidConfiguration;
   int g = grid > 0 ? grid : ResidentBlocks(k, threads, sh);
   if (g > ntiles) g = ntiles;
   if (g < 1) g = 1;
   hipLaunchKernelGGL((dev::stream_stage<TB, TI, EPT, MPS, ENS, EPI>), dim3(g), dim3(threads), sh,
                      s, a, in, epi);
+  return hipGetLastError();
 }
 
 // Ensembles (a descriptor list) and single hierarchies are separate
 // instantiations, so the single-hierarchy loop carries no descriptor code.
 template <typename TB, typename TI, int EPT, int MPS, class EPI>
-inline void LaunchStreamK(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
-                          hipStream_t s) {
-  if (a.tiles) LaunchStreamKE<TB, TI, EPT, MPS, true, EPI>(a, in, epi, threads, grid, s);
-  else LaunchStreamKE<TB, TI, EPT, MPS, false, EPI>(a, in, epi, threads, grid, s);
+inline hipError_t LaunchStreamK(const RhsArgs& a, const TI* in, const EPI& epi, int threads,
+                                int grid, hipStream_t s) {
+  if (a.tiles) return LaunchStreamKE<TB, TI, EPT, MPS, true, EPI>(a, in, epi, threads, grid, s);
+  return LaunchStreamKE<TB, TI, EPT, MPS, false, EPI>(a, in, epi, threads, grid, s);
 }
 
 template <typename TB, typename TI, int MPS, class EPI>
-inline void LaunchStreamE(const RhsArgs& a, const TI* in, const EPI& epi, int threads, int grid,
-                          hipStream_t s) {
+inline hipError_t LaunchStreamE(const RhsArgs& a, const TI* in, const EPI& epi, int threads,
+                                int grid, hipStream_t s) {
   const int ept = (a.tile * a.NE + threads - 1) / threads;
   switch (ept) {
-    case 1: LaunchStreamK<TB, TI, 1, MPS, EPI>(a, in, epi, threads, grid, s); break;
-    case 2: LaunchStreamK<TB, TI, 2, MPS, EPI>(a, in, epi, threads, grid, s); break;
-    case 3: LaunchStreamK<TB, TI, 3, MPS, EPI>(a, in, epi, threads, grid, s); break;
-    default: LaunchStreamK<TB, TI, 4, MPS, EPI>(a, in, epi, threads, grid, s); break;
+    case 1: return LaunchStreamK<TB, TI, 1, MPS, EPI>(a, in, epi, threads, grid, s);
+    case 2: return LaunchStreamK<TB, TI, 2, MPS, EPI>(a, in, epi, threads, grid, s);
+    case 3: return LaunchStreamK<TB, TI, 3, MPS, EPI>(a, in, epi, threads, grid, s);
+    default: return LaunchStreamK<TB, TI, 4, MPS, EPI>(a, in, epi, threads, grid, s);
   }
 }
 
@@ -57,18 +60,29 @@ inline hipError_t LaunchStream(const RhsArgs& a0, const TI* in, const EPI& epi, 
   a.tile = a0.col_tile[fp32 ? 1 : 0];
   const int threads = a0.col_threads[fp32 ? 1 : 0];
   const int grid = a0.col_grid[fp32 ? 1 : 0];
-  if (a.mps <= 1) LaunchStreamE<TB, TI, 1, EPI>(a, in, epi, threads, grid, s);
-  else LaunchStreamE<TB, TI, 2, EPI>(a, in, epi, threads, grid, s);
-  return hipGetLastError();
+  if (a.mps <= 1) return LaunchStreamE<TB, TI, 1, EPI>(a, in, epi, threads, grid, s);
+  return LaunchStreamE<TB, TI, 2, EPI>(a, in, epi, threads, grid, s);
 }
 
 // RK4 stage through the persistent kernels (stage 0 reads the FP64 state).
+// form: kRk4Classic (aux = FP64 accumulator), kRk4Nested, kRk4Increment
+// (aux = K, bulk precision).
 template <typename TB>
-inline hipError_t LaunchRk4Stream(const RhsArgs& a, int stage, bool horner, const void* in,
-                                  double* rho, double* acc, TB* yout, double dt, hipStream_t s) {
+inline hipError_t LaunchRk4Stream(const RhsArgs& a, int stage, int form, const void* in,
+                                  double* rho, void* aux, TB* yout, double dt, hipStream_t s) {
   const double* x = (const double*)in;
   const TB* y = (const TB*)in;
-  if (horner) {
+  double* acc = (double*)aux;
+  if (form == kRk4Increment) {
+    const TB* kk = (const TB*)aux;
+    switch (stage) {
+      case 0: return LaunchStream<TB, double>(a, x, dev::EpiIncr<TB, 0>{rho, kk, yout, dt}, s);
+      case 1: return LaunchStream<TB, TB>(a, y, dev::EpiIncr<TB, 1>{rho, kk, yout, dt}, s);
+      case 2: return LaunchStream<TB, TB>(a, y, dev::EpiIncr<TB, 2>{rho, kk, yout, dt}, s);
+      default: return LaunchStream<TB, TB>(a, y, dev::EpiIncr<TB, 3>{rho, kk, yout, dt}, s);
+    }
+  }
+  if (form == kRk4Nested) {
     switch (stage) {
       case 0: return LaunchStream<TB, double>(a, x, dev::EpiNested<TB, 0, true>{rho, yout, dt / 4}, s);
       case 1: return LaunchStream<TB, TB>(a, y, dev::EpiNested<TB, 1, false>{rho, yout, dt / 3}, s);
@@ -111,11 +125,15 @@ inline void LaunchRk4NM(const RhsArgs& a, int stage, const TI* in, double* rho, 
   }
 }
 
-// `horner` selects the nested stage order (see rk4_stage).
+// `form` selects the stage order (the one-shot kernels have the classic and
+// the nested one, see rk4_stage).
 template <typename TB, typename TI>
-inline hipError_t LaunchRk4T(const RhsArgs& a0, int stage, bool horner, const TI* in, double* rho,
-                             double* acc, TB* yout, double dt, hipStream_t s) {
-  if (a0.col_nm == 2) return LaunchRk4Stream<TB>(a0, stage, horner, in, rho, acc, yout, dt, s);
+inline hipError_t LaunchRk4T(const RhsArgs& a0, int stage, int form, const TI* in, double* rho,
+                             void* aux, TB* yout, double dt, hipStream_t s) {
+  if (a0.col_nm == 2) return LaunchRk4Stream<TB>(a0, stage, form, in, rho, aux, yout, dt, s);
+  if (form == kRk4Increment) return hipErrorInvalidValue;
+  const bool horner = form == kRk4Nested;
+  double* acc = (double*)aux;
   int threads;
   const RhsArgs a = Shape(a0, sizeof(TB) == 4, &threads);
   if (a.col_nm >= 1) {

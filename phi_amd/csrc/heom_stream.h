@@ -167,6 +167,21 @@ __device__ __forceinline__ void stream_coords(const RhsArgs& A, int i, int* te, 
   *a = *r - *b * A.N;
 }
 
+// The coordinates of a thread's elements, the same in every tile (computed
+// once per kernel).
+template <int EPT>
+struct StreamCoords {
+  int te[EPT], r[EPT], b[EPT], a[EPT];
+};
+
+template <int EPT>
+__device__ __forceinline__ StreamCoords<EPT> stream_coords_all(const RhsArgs& A) {
+  StreamCoords<EPT> c;
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) stream_coords(A, i, &c.te[i], &c.r[i], &c.b[i], &c.a[i]);
+  return c;
+}
+
 // A tile: slab node `base` holds local node `local` of its member (the
 // member's nodes are slab nodes base - local .. base - local + count - 1),
 // nt nodes.  A single hierarchy is one member with count = num_nodes.
@@ -200,27 +215,29 @@ __device__ __forceinline__ TileInfo stream_tile(const RhsArgs& A, int tile) {
   return t;
 }
 
+// Loads are unconditional: an element beyond the tile (a short last tile,
+// or a thread past the tile's elements) loads those of the tile's last node
+// instead, and an absent neighbour's gather loads the node's own element;
+// the values are never used.  Divergent branches around loads would make
+// the compiler's wait counting assume the worst at every join and drain all
+// loads in flight, prefetches included.
+
 // Loads of a tile that need no other load first: its input elements, mu
 // and the edge records (tables indexed by the member-local node id).
 template <typename TB, typename TI, int EPT, int MPS>
 __device__ __forceinline__ void stream_load1(const RhsArgs& A, const TI* __restrict__ in,
-                                             const TileInfo& t,
+                                             const TileInfo& t, const StreamCoords<EPT>& C,
                                              StreamRegs<TB, TI, EPT, MPS>& R) {
   typedef typename C2<TI>::type VI;
-  const int tot = t.nt * A.NE;
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
-    const int e = threadIdx.x + i * blockDim.x;
-    if (e < tot) {
-      int te, r, b, a;
-      stream_coords(A, i, &te, &r, &b, &a);
-      const int ln = t.local + te;
-      R.y[i] = reinterpret_cast<const VI*>(in)[(size_t)(t.base + te) * A.NE + r];
-      R.mu[i] = A.mu[ln];
-      const int4* ed = A.edge + ((size_t)ln * A.N + b) * MPS;
+    const int te = min(C.te[i], t.nt - 1);
+    const int ln = t.local + te;
+    R.y[i] = reinterpret_cast<const VI*>(in)[(size_t)(t.base + te) * A.NE + C.r[i]];
+    R.mu[i] = A.mu[ln];
+    const int4* ed = A.edge + ((size_t)ln * A.N + C.b[i]) * MPS;
 #pragma unroll
-      for (int q = 0; q < MPS; ++q) R.rec[i][q] = ed[q];
-    }
+    for (int q = 0; q < MPS; ++q) R.rec[i][q] = ed[q];
   }
 }
 
@@ -229,28 +246,26 @@ __device__ __forceinline__ void stream_load1(const RhsArgs& A, const TI* __restr
 // depth do not exist; a single hierarchy's ids are slab ids already.
 template <typename TB, typename TI, int EPT, int MPS, bool ENS>
 __device__ __forceinline__ void stream_load2(const RhsArgs& A, const TI* __restrict__ in,
-                                             const TileInfo& t,
+                                             const TileInfo& t, const StreamCoords<EPT>& C,
                                              StreamRegs<TB, TI, EPT, MPS>& R) {
   typedef typename C2<TI>::type VI;
   const VI* v = reinterpret_cast<const VI*>(in);
-  const int tot = t.nt * A.NE;
   const int off = t.base - t.local;
 #pragma unroll
   for (int i = 0; i < EPT; ++i) {
-    const int e = threadIdx.x + i * blockDim.x;
-    if (e < tot) {
-      const int r = e - fdiv(e, A.mNE) * A.NE;
+    const int self = t.base + min(C.te[i], t.nt - 1);
 #pragma unroll
-      for (int q = 0; q < MPS; ++q) {
-        int4 c = R.rec[i][q];
-        if constexpr (ENS) {
-          c.x = (c.x >= 0 && c.x < t.count) ? c.x + off : -1;
-          c.y = (c.y >= 0 && c.y < t.count) ? c.y + off : -1;
-          R.rec[i][q] = c;
-        }
-        if (c.x >= 0) R.gn[i][q] = v[(size_t)c.x * A.NE + r];
-        if (c.y >= 0) R.gp[i][q] = v[(size_t)c.y * A.NE + r];
+    for (int q = 0; q < MPS; ++q) {
+      int4 c = R.rec[i][q];
+      if constexpr (ENS) {
+        c.x = (c.x >= 0 && c.x < t.count) ? c.x + off : -1;
+        c.y = (c.y >= 0 && c.y < t.count) ? c.y + off : -1;
+        R.rec[i][q] = c;
       }
+      const int jn = c.x >= 0 ? c.x : self;
+      const int jp = c.y >= 0 ? c.y : self;
+      R.gn[i][q] = v[(size_t)jn * A.NE + C.r[i]];
+      R.gp[i][q] = v[(size_t)jp * A.NE + C.r[i]];
     }
   }
 }
@@ -262,9 +277,9 @@ __device__ __forceinline__ double2 stream_h_frag(const RhsArgs& A, const double2
   const int lane = f & 63, ms = f >> 6;
   const int mt = ms / KS, st = ms - mt * KS;
   const int row = 16 * mt + (lane & 15), k = (lane >> 4) * KS + st;
-  double2 h = make_double2(0., 0.);
-  if (row < N && k < N) h = H[row + k * N];
-  return h;
+  const bool ok = row < N && k < N;
+  const double2 h = H[ok ? row + k * N : 0];
+  return ok ? h : make_double2(0., 0.);
 }
 
 // H of `member` (shifted by its mean diagonal) as MFMA A-fragments in LDS.
@@ -327,10 +342,11 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
   // then the neighbour gathers.  Two trips to memory before the first
   // barrier instead of one per table: short kernels are mostly prologue.
   double emax = 0;
+  const StreamCoords<EPT> C = stream_coords_all<EPT>(A);
   Regs cur;
   TileInfo ti = stream_tile<ENS>(A, tile);
   int hmember = ti.member;
-  stream_load1<TB, TI, EPT, MPS>(A, in, ti, cur);
+  stream_load1<TB, TI, EPT, MPS>(A, in, ti, C, cur);
   const int DL = D * L, nf = StreamFragCount(N);
   const double2* Hm = A.H + (size_t)hmember * NE;
   const int e0 = threadIdx.x;
@@ -362,10 +378,26 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     s.hfr[f] = (TB)h.x;
     s.hfi[f] = (TB)h.y;
   }
-  stream_load2<TB, TI, EPT, MPS, ENS>(A, in, ti, cur);
+  stream_load2<TB, TI, EPT, MPS, ENS>(A, in, ti, C, cur);
+  // Every prologue load has arrived before the loop (the table loops above
+  // have data-dependent trip counts; without this wait the compiler's load
+  // counting could not tell, inside the loop, which loads are still in
+  // flight, and would drain them all before each use).
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();  // per-block tables visible
+  // Per-element constants of every tile: the Matsubara correction
+  // (Delta_a + Delta_b)/2 off the diagonal and the LDS offsets of the
+  // element and of its transpose.
+  TB fd[EPT];
+  int oo[EPT], ot[EPT];
+#pragma unroll
+  for (int i = 0; i < EPT; ++i) {
+    const int te = min(C.te[i], T - 1), b = C.b[i], a = C.a[i];
+    fd[i] = a != b ? (TB)0.5 * (s.delta[a] + s.delta[b]) : (TB)0;
+    oo[i] = te * NP + b * (N + 1) + a;
+    ot[i] = te * NP + a * (N + 1) + b;
+  }
   double2 hp[2] = {make_double2(0., 0.), make_double2(0., 0.)};
-  bool hpre = false;
 
   for (int buf = 0; tile < tile_end; tile += step, buf ^= 1) {
     const int nt = ti.nt, base = ti.base;
@@ -373,99 +405,85 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     V* Yb = s.Y + buf * s.bstride;
     V* Wb = s.W + buf * s.bstride;
     // H of a new member (the previous tile's product finished before the
-    // last barrier; the next barrier publishes the new fragments): from the
-    // registers loaded with the previous tile's prefetch, else from memory.
+    // last barrier; the next barrier publishes the new fragments), from the
+    // registers the previous iteration loaded (ensemble launches have at
+    // least half as many threads as fragment elements).
     if (ENS && ti.member != hmember) {
       hmember = ti.member;
-      if (hpre) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int f = threadIdx.x + j * blockDim.x;
-          if (f < nf) {
-            s.hfr[f] = (TB)hp[j].x;
-            s.hfi[f] = (TB)hp[j].y;
-          }
+      for (int j = 0; j < 2; ++j) {
+        const int f = threadIdx.x + j * blockDim.x;
+        if (f < nf) {
+          s.hfr[f] = (TB)hp[j].x;
+          s.hfi[f] = (TB)hp[j].y;
         }
-      } else {
-        stream_load_h<TB>(A, s, hmember);
       }
     }
-    // (1) elementwise part of this tile -> LDS; epilogue operands of this tile
+    // (1) epilogue operands of every owned element (clamped), then the
+    // elementwise part of this tile -> LDS
     Pre pre[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
-      const int e = threadIdx.x + i * blockDim.x;
-      if (e < tot) {
-        int te, r, b, a;
-        stream_coords(A, i, &te, &r, &b, &a);
-        const size_t g = (size_t)(base + te) * NE + r;
-        if (EPI::kInIsState) pre[i] = epi.from_state(make_double2((double)cur.y[i].x,
-                                                                  (double)cur.y[i].y));
-        else pre[i] = epi.pre(g);
-        const V y = mk<TB>((TB)cur.y[i].x, (TB)cur.y[i].y);
-        TB f = (TB)(0.5 * cur.mu[i]);
-        if (a != b) f += (TB)0.5 * (s.delta[a] + s.delta[b]);
-        TB ax = -f * y.x, ay = -f * y.y;
+      const size_t g = (size_t)(base + min(C.te[i], nt - 1)) * NE + C.r[i];
+      if (EPI::kInIsState) pre[i] = epi.from_state(make_double2((double)cur.y[i].x,
+                                                                (double)cur.y[i].y));
+      else pre[i] = epi.pre(g);
+    }
 #pragma unroll
-        for (int q = 0; q < MPS; ++q) {
-          const int4 c = cur.rec[i][q];
-          if (c.x >= 0) {
-            const TB sc = s.ns[c.z];
-            const TB gx = (TB)cur.gn[i][q].x, gy = (TB)cur.gn[i][q].y;
-            ax -= sc * gy;
-            ay += sc * gx;
-          }
-          if (c.y >= 0) {
-            const V pc = s.pcol[c.z];
-            const TB gx = (TB)cur.gp[i][q].x, gy = (TB)cur.gp[i][q].y;
-            ax -= pc.x * gx - pc.y * gy;
-            ay -= pc.x * gy + pc.y * gx;
-          }
-        }
-        const int o = te * NP + b * (N + 1) + a;
-        Yb[o] = y;
-        Wb[o] = mk<TB>(ax, ay);
+    for (int i = 0; i < EPT; ++i) {
+      const V y = mk<TB>((TB)cur.y[i].x, (TB)cur.y[i].y);
+      const TB f = (TB)(0.5 * cur.mu[i]) + fd[i];
+      TB ax = -f * y.x, ay = -f * y.y;
+#pragma unroll
+      for (int q = 0; q < MPS; ++q) {
+        // absent neighbours (id -1) contribute nothing; their table index
+        // is -1 too, read as entry 0 and multiplied by 0
+        const int4 c = cur.rec[i][q];
+        const int z = max(c.z, 0);
+        const TB sc = c.x >= 0 ? s.ns[z] : (TB)0;
+        V pc = s.pcol[z];
+        if (c.y < 0) pc = mk<TB>((TB)0, (TB)0);
+        const TB gx = (TB)cur.gn[i][q].x, gy = (TB)cur.gn[i][q].y;
+        const TB hx = (TB)cur.gp[i][q].x, hy = (TB)cur.gp[i][q].y;
+        ax -= sc * gy + (pc.x * hx - pc.y * hy);
+        ay += sc * gx - (pc.x * hy + pc.y * hx);
+      }
+      if (threadIdx.x + i * blockDim.x < tot) {
+        Yb[oo[i]] = y;
+        Wb[oo[i]] = mk<TB>(ax, ay);
       }
     }
-    // (2) first loads of the next tile
+    // (2) first loads of the next tile (the current one again after the
+    // last, loaded and never combined, so that no load depends on a branch)
     const int ntile = tile + step;
     const bool more = ntile < tile_end;
-    TileInfo nti = ti;
-    hpre = false;
-    if (more) {
-      nti = stream_tile<ENS>(A, ntile);
-      stream_load1<TB, TI, EPT, MPS>(A, in, nti, cur);
-      // The next tile's H fragments, when its member differs and they fit
-      // two per thread (L2 hits, in flight with the tile's other loads).
-      if (ENS && nti.member != ti.member && nf <= 2 * (int)blockDim.x) {
-        const double2* Hn = A.H + (size_t)nti.member * NE;
+    const TileInfo nti = stream_tile<ENS>(A, more ? ntile : tile);
+    stream_load1<TB, TI, EPT, MPS>(A, in, nti, C, cur);
+    if constexpr (ENS) {
+      // The next tile's H fragments (L2 hits, in flight with the tile's
+      // other loads), used when its member differs.
+      const double2* Hn = A.H + (size_t)nti.member * NE;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int f = threadIdx.x + j * blockDim.x;
-          if (f < nf) hp[j] = stream_h_frag(A, Hn, f);
-        }
-        hpre = true;
-      }
+      for (int j = 0; j < 2; ++j)
+        hp[j] = stream_h_frag(A, Hn, min((int)threadIdx.x + j * (int)blockDim.x, nf - 1));
     }
     __syncthreads();
     // (3) matrix-core product
     stream_gemm<TB>(A, s, Yb, Wb, nt);
     __syncthreads();
-    // (4) neighbour gathers of the next tile
-    if (more) stream_load2<TB, TI, EPT, MPS, ENS>(A, in, nti, cur);
+    // (4) neighbour gathers of the next tile (its edge records arrived
+    // during the product)
+    stream_load2<TB, TI, EPT, MPS, ENS>(A, in, nti, C, cur);
     // (5) epilogue of this tile
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
-      const int e = threadIdx.x + i * blockDim.x;
-      if (e < tot) {
-        int te, r, b, a;
-        stream_coords(A, i, &te, &r, &b, &a);
-        const V p = Wb[te * NP + b * (N + 1) + a];
-        const V q = Wb[te * NP + a * (N + 1) + b];
+      if (threadIdx.x + i * blockDim.x < tot) {
+        const V p = Wb[oo[i]];
+        const V q = Wb[ot[i]];
         double2 k;
         k.x = (double)(p.x + q.x);
         k.y = (double)(p.y - q.y);
-        epi.apply((size_t)(base + te) * NE + r, k, pre[i], &emax);
+        epi.apply((size_t)(base + C.te[i]) * NE + C.r[i], k, pre[i], &emax);
       }
     }
     ti = nti;
@@ -499,11 +517,64 @@ struct EpiNested {
   double* rho;
   TB* yout;
   double c;
-  __device__ __forceinline__ Pre pre(size_t g) const { return Pre{ld2(rho, g)}; }
+  __device__ __forceinline__ Pre pre(size_t g) const { return Pre{ld2_nt(rho, g)}; }
   __device__ __forceinline__ Pre from_state(double2 x) const { return Pre{x}; }
   __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p, double*) const {
-    if (STAGE < 3) st<TB>(yout, g, p.x.x + c * k.x, p.x.y + c * k.y);
-    else st<double>(rho, g, p.x.x + c * k.x, p.x.y + c * k.y);
+    if (STAGE < 3) st_nt<TB>(yout, g, p.x.x + c * k.x, p.x.y + c * k.y);
+    else st_nt<double>(rho, g, p.x.x + c * k.x, p.x.y + c * k.y);
+  }
+  __device__ __forceinline__ void finalize(double) const {}
+};
+
+// Increment form of the same step: with K = L rho, the stage inputs are
+// increments instead of rho + increment,
+//   s0: K = L rho                       s1: D = dt/3 (K + dt/4 L K)
+//   s2: D' = dt/2 (K + L D)             s3: rho += dt (K + L D'),
+// which is the nested step (Y1 = rho + dt/4 K, Y2 = rho + D, Y3 = rho + D')
+// rearranged.  The FP64 state is read by the first and last stage only
+// (N^2 (48 + 8 s) bytes per node-step against N^2 (80 + 6 s): fewer for
+// FP32 bulk, the same for FP64), and the bulk copies hold increments, which
+// round far less than rho + increment.  K, D and D' are Hermitian when rho
+// is (linear combinations of L applied to Hermitian ADOs).  Stages 0 and 1
+// take their own input element as the epilogue operand (kInIsState).
+template <typename TB, int STAGE>
+struct EpiIncr {
+  typedef typename C2<TB>::type V;
+  struct Pre { double2 x; V q; };
+  static constexpr bool kInIsState = STAGE <= 1;
+  static constexpr bool kReduce = false;
+  double* rho;
+  const TB* kk;   // K (stages 2 and 3)
+  TB* yout;
+  double dt;
+  __device__ __forceinline__ Pre pre(size_t g) const {
+    Pre p;
+    p.q = ldv_nt<TB>(kk, g);
+    if (STAGE == 3) p.x = ld2_nt(rho, g);
+    return p;
+  }
+  __device__ __forceinline__ Pre from_state(double2 x) const {
+    Pre p;
+    p.x = x;
+    return p;
+  }
+  // Stages 0-2 write bulk values and compute in the bulk precision; the
+  // state update (stage 3) is FP64.
+  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p, double*) const {
+    const TB kx = (TB)k.x, ky = (TB)k.y;
+    if (STAGE == 0) {
+      stv_nt<TB>(yout, g, mk<TB>(kx, ky));
+    } else if (STAGE == 1) {
+      const TB c1 = (TB)(dt / 3), c0 = (TB)(dt / 4);
+      const TB x = (TB)p.x.x, y = (TB)p.x.y;
+      stv_nt<TB>(yout, g, mk<TB>(c1 * (x + c0 * kx), c1 * (y + c0 * ky)));
+    } else if (STAGE == 2) {
+      const TB c2 = (TB)(dt / 2);
+      stv_nt<TB>(yout, g, mk<TB>(c2 * (p.q.x + kx), c2 * (p.q.y + ky)));
+    } else {
+      st_nt<double>(rho, g, p.x.x + dt * ((double)p.q.x + k.x),
+                    p.x.y + dt * ((double)p.q.y + k.y));
+    }
   }
   __device__ __forceinline__ void finalize(double) const {}
 };

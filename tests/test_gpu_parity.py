@@ -273,9 +273,9 @@ def _subset_params(cfg, lh):
 
 @pytest.mark.parametrize("cfg,lh,steps", [("c2", 5, 30), ("c3", 3, 20)])
 def test_rk4_forms_and_launch_shapes_match_oracle(cfg, lh, steps):
-    """Both RK4 stage orders (PHI's classic k0..k3 and the nested Taylor
-    form), both GEMM forms (matrix cores / element) and several block
-    shapes give the oracle's state within 1e-10 (FP64)."""
+    """Every RK4 stage order (PHI's classic k0..k3, the nested Taylor form
+    and its increment form), both GEMM forms (matrix cores / element) and
+    several block shapes give the oracle's state within 1e-10 (FP64)."""
     op, p = _subset_params(cfg, lh)
     orc = HO.Oracle(HO.Problem.from_prm(op), threads=8)
     dt = op.timestep
@@ -283,7 +283,7 @@ def test_rk4_forms_and_launch_shapes_match_oracle(cfg, lh, steps):
     ref = orc.state()
     shapes = ((2, 0, 0, 0), (2, 1, 0, 0), (2, 3, 0, 7), (2, 1, 64, 0), (2, 2, 256, 5),
               (1, 0, 0, 0), (0, 0, 0, 0), (1, 1, 64, 0), (1, 2, 128, 0), (0, 3, 192, 0))
-    for form in ("nested", "classic"):
+    for form in ("increment", "nested", "classic"):
         for gemm, tile, threads, blocks in shapes:
             print("case", cfg, form, gemm, tile, threads, blocks, flush=True)
             ctx = P.Context(p)
