@@ -216,6 +216,14 @@ int phi_hip_get_rho0(phi_ctx* c, double* out, size_t n_complex);
  * TimeLocal the deepest tier includes the time-local term with the current
  * matrices (zero until an integrator run advances them). */
 int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
+/* phi_hip_rhs without any device allocation.  No counterpart in PHI: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The input and output are staged in a
+ * fixed per-device array of 32 MiB held until the process ends, so
+ * n_complex is limited to 1048576 (PHI_HIP_EINVAL beyond; phi_hip_rhs has
+ * no such limit).  Calls that overlap on one device share that array and
+ * race: serialise them. */
+int phi_hip_rhs_noalloc(phi_ctx* c, const double* in, double* out, size_t n_complex);
 /* nsteps RK4 steps of size dt on the device state. */
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
 /* phi_hip_rk4_steps bracketed by two HIP events on the context's stream;

@@ -84,6 +84,7 @@ def lib():
         "phi_hip_get_state": (ctypes.c_int, [vp, dp, sz]),
         "phi_hip_get_rho0": (ctypes.c_int, [vp, dp, sz]),
         "phi_hip_rhs": (ctypes.c_int, [vp, dp, dp, sz]),
+        "phi_hip_rhs_noalloc": (ctypes.c_int, [vp, dp, dp, sz]),
         "phi_hip_rk4_steps": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double]),
         "phi_hip_rk4_stage_times": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
         "phi_hip_rk4_steps_timed": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
@@ -389,6 +390,13 @@ class Context:
         f = self._to_flat(x)
         out = np.zeros_like(f)
         _check(lib().phi_hip_rhs(self._h, _dp(f), _dp(out), self.n_complex))
+        return self._from_flat(out)
+
+    def rhs_noalloc(self, x):
+        """rhs() through the allocation-free entry point (phi_hip_rhs_noalloc)."""
+        f = self._to_flat(x)
+        out = np.zeros_like(f)
+        _check(lib().phi_hip_rhs_noalloc(self._h, _dp(f), _dp(out), self.n_complex))
         return self._from_flat(out)
 
     def rk4_steps(self, nsteps, dt):

@@ -29,7 +29,8 @@ LIB_SOURCES = ["phi_parameters.cc", "hierarchy.cc", "engine.cc", "engine_closure
                "integrator.cc", "capi.cc",
                "heom_kernels.hip", "heom_rk4_f64.hip", "heom_rk4_f32.hip",
                "heom_rkf45_f64.hip", "heom_rkf45_f32.hip", "mlp_kernels.hip",
-               "closure_kernels.hip", "filter_kernels.hip", "bath_kernels.hip"]
+               "closure_kernels.hip", "filter_kernels.hip", "bath_kernels.hip",
+               "rhs_noalloc.hip"]
 
 
 def _compile(src):

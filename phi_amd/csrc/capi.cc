@@ -7,6 +7,7 @@
 #include <sstream>
 #include <string>
 
+#include "capi_internal.h"
 #include "engine.h"
 #include "mlp.h"
 #include "integrator.h"
@@ -19,27 +20,26 @@ struct phi_integrator {
   phi::HierarchyIntegrator* it;
   phi::PhiParameters* p;
 };
-struct phi_ctx {
-  phi::Engine* e;
-};
-
 struct phi_mlp {
   phi::Mlp* m;
 };
 
+namespace phi {
+namespace capi {
+
+std::string& LastError() {
+  thread_local std::string error;
+  return error;
+}
+
+}  // namespace capi
+}  // namespace phi
+
 namespace {
 
-thread_local std::string g_error;
-
-int Err(int code, const std::string& msg) {
-  g_error = msg;
-  return code;
-}
-
-int Ok() {
-  g_error.clear();
-  return PHI_HIP_OK;
-}
+using phi::capi::Err;
+using phi::capi::Guarded;
+using phi::capi::Ok;
 
 phi::RunMethod MethodFromWord(const std::string& m, bool* known) {
   *known = true;
@@ -129,27 +129,11 @@ std::string* StringField(phi::PhiParameters& p, const std::string& n) {
   return nullptr;
 }
 
-// Every entry point runs its body through Guarded: no C++ exception crosses
-// the C boundary (allocation failures of host tables, for instance, become
-// PHI_HIP_ENOMEM with a message).
-template <class F>
-int Guarded(const char* what, F&& f) {
-  try {
-    return f();
-  } catch (const std::bad_alloc&) {
-    return Err(PHI_HIP_ENOMEM, std::string(what) + ": out of host memory");
-  } catch (const std::exception& e) {
-    return Err(PHI_HIP_EINTERNAL, std::string(what) + ": " + e.what());
-  } catch (...) {
-    return Err(PHI_HIP_EINTERNAL, std::string(what) + ": unknown exception");
-  }
-}
-
 }  // namespace
 
 extern "C" {
 
-const char* phi_hip_last_error(void) { return g_error.c_str(); }
+const char* phi_hip_last_error(void) { return phi::capi::LastError().c_str(); }
 int phi_hip_version(void) { return 100; }
 
 int phi_hip_device_count(void) {

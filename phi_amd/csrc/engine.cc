@@ -60,6 +60,7 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
     return kErrUnsupported;
   }
   int rc = e->Check(hipStreamCreateWithFlags(&e->stream_, hipStreamNonBlocking), "stream");
+  if (!rc) rc = e->Check(hipGetDevice(&e->device_), "device");
   // H as given, its adjoint, and H shifted by the mean of its diagonal: the
   // shift leaves H rho - rho H^dag unchanged and removes the large common
   // site energy before the bulk GEMM (FP32 cancellation).
@@ -694,13 +695,21 @@ int Engine::Rhs(const std::complex<double>* x, std::complex<double>* y) {
   double *dx = nullptr, *dy = nullptr;
   int rc = Check(hipMalloc((void**)&dx, bytes), "alloc rhs in");
   if (!rc) rc = Check(hipMalloc((void**)&dy, bytes), "alloc rhs out");
-  if (!rc) rc = Check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, stream_), "rhs upload");
+  if (!rc) rc = RhsStaged(x, y, dx, dy);
+  Free(dx);
+  Free(dy);
+  return rc;
+}
+
+int Engine::RhsStaged(const std::complex<double>* x, std::complex<double>* y, double* dx,
+                      double* dy) {
+  if (members_) return Unsupported("the general RHS");
+  const size_t bytes = state_elems() * sizeof(std::complex<double>);
+  int rc = Check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, stream_), "rhs upload");
   // Under the filter the current flags apply.
   if (!rc) rc = OperatorRhs(dx, dy);
   if (!rc) rc = Check(hipMemcpyAsync(y, dy, bytes, hipMemcpyDeviceToHost, stream_), "rhs download");
   if (!rc) rc = Synchronize();
-  Free(dx);
-  Free(dy);
   return rc;
 }
 

@@ -123,6 +123,12 @@ class Engine {
 
   // y = L(x) with PHI's Get semantics (any complex input), host pointers.
   int Rhs(const std::complex<double>* x, std::complex<double>* y);
+  // Same through caller-provided device staging buffers dx, dy (each
+  // state_elems() complex doubles on this engine's device); allocates
+  // nothing.
+  int RhsStaged(const std::complex<double>* x, std::complex<double>* y, double* dx, double* dy);
+  // The device this engine's buffers and stream live on.
+  int device() const { return device_; }
   // Same on device pointers (interleaved doubles).
   int RhsDevice(const double* x, double* y, int trace_row);
 
@@ -257,6 +263,7 @@ class Engine {
   RhsArgs args_{};
   bool fp32_ = false;
   int rk4_form_ = kRk4Increment;
+  int device_ = 0;
   hipStream_t stream_ = nullptr;
   std::string err_;
 
