@@ -61,7 +61,7 @@ HBM_PEAK_GBS = 8000.0
 CFG = os.path.join(ROOT, "phi_amd", "configs", "c2.prm")
 CFG3 = os.path.join(ROOT, "phi_amd", "configs", "c3.prm")
 PROFILE_C2 = os.path.join(ROOT, "profiles", "rk4_stage_7site_depth8_traffic.txt")
-PROFILE_ENS = os.path.join(ROOT, "profiles", "rk4_stage_ensemble_512x7site_traffic.txt")
+PROFILE_ENS = os.path.join(ROOT, "profiles", "rk4_stage_ensemble_512x7site.txt")
 
 
 def log(*a):
