@@ -289,11 +289,13 @@ def run_closure(P, np, steps, dt, train_members, test_members, log):
     t_data = time.perf_counter() - t0
     net, losses = C.train(X, T, epochs=20, batch=512, lr=1e-3, seed=2, log=log)
     res = C.evaluate(C.hamiltonians(100000, test_members), net, deep=11, shallow=5, steps=steps,
-                     dt=dt, every=10, log=log)
+                     dt=dt, every=10, precision="fp32", deep_precision="fp64", log=log)
     scale = 1000.0 / steps
     return {
         "workload": "7-site exciton, random Hamiltonians; network trained on %d depth-10 "
-                    "trajectories (%d rows), evaluated on %d held-out ones, FP64 RK4 dt=1 fs"
+                    "trajectories (%d rows), evaluated on %d held-out ones, RK4 dt=1 fs: "
+                    "depth 10 plain FP64 (increment form), depth 4 and depth 4 + closure "
+                    "FP32 bulk (the closure run in PHI's classic order)"
                     % (train_members, len(X), test_members),
         "train_seconds_data": t_data, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
         "time_to_1ps_s_depth10": res["deep"]["seconds"] * scale,

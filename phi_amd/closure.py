@@ -114,7 +114,11 @@ def evaluate(Hs, net, deep, shallow, steps, dt, every, precision="fp64", deep_pr
              log=print):
     """Population histories (time, member, site) of plain deep, plain
     shallow and shallow + closure runs of Hs, their errors against the deep
-    run and the wall time of each (after a warm-up step and autotuning)."""
+    run and the wall time of each (after autotuning).  The plain runs take
+    the default RK4 order (the fastest for a linear right-hand side); the
+    closure makes the right-hand side nonlinear, so its run takes PHI's
+    classic order.  `precision` is the bulk precision of the shallow runs,
+    `deep_precision` that of the deep reference run."""
     p = base_params()
     out = {}
     for name, lh, closed, prec in (("deep", deep, False, deep_precision),
@@ -123,8 +127,6 @@ def evaluate(Hs, net, deep, shallow, steps, dt, every, precision="fp64", deep_pr
         ens = P.Ensemble(p, Hs, [lh] * len(Hs), precision=prec)
         if closed:
             ens.attach_closure(net)
-        else:
-            ens.set_rk4_form("classic")
         # time a copy of the run after autotuning on a scratch ensemble
         ens.autotune(dt, steps=3)
         pops, el = _run(ens, steps, dt, every)

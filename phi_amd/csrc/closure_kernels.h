@@ -23,7 +23,9 @@ struct FusedNet {
 };
 
 size_t FusedClosureLds(const FusedNet& net);
-// Inputs, the three layers and the scatter in one launch (32 rows per block).
+// Whether the fused pass's LDS fits one CU (hidden widths up to 256).
+bool FusedClosureFits(const FusedNet& net);
+// Inputs, the three layers and the scatter in one launch (128 rows per block).
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
                               int members, int first, int nb, int ne, int D, const float* bmi,
                               int vbase, hipStream_t s);

@@ -116,18 +116,22 @@ __global__ void __launch_bounds__(256) scatter(const float* __restrict__ Y,
 }
 
 // --- Fused closure pass ------------------------------------------------------------
-// One block of 256 threads takes kRows rows end to end: network inputs from
-// the stage input (as `inputs`), the three layers on the matrix cores
-// (v_mfma_f32_16x16x32_bf16; activations in LDS as bfloat16, weights packed
-// as bfloat16 rows streamed from L2), and the scatter of the outputs into
-// the virtual slots (as `scatter`).  Same arithmetic as the unfused path:
-// operands rounded to bfloat16, float32 sums in the same K order, bias and
-// GELU in float32.
-constexpr int kRows = 32;
+// One block of 512 threads (8 wavefronts) takes kRows = 128 rows end to
+// end: network inputs from the stage input (as `inputs`), the three layers
+// on the matrix cores (v_mfma_f32_16x16x32_bf16; activations in LDS as
+// bfloat16, weights packed as bfloat16 rows streamed from L2), and the
+// scatter of the outputs into the virtual slots (as `scatter`).  Same
+// arithmetic as the unfused path: operands rounded to bfloat16, float32
+// sums in the same K order, bias and GELU in float32.
+//
+// A layer's output tile of a wavefront is 64 rows x 64 columns (4 x 4 MFMA
+// tiles): wave w takes row half w & 1 and column chunks w >> 1, + 4, ...
+// Each weight fragment feeds 4 row tiles (128 rows share one read of the
+// weights), and the fragments of the next K step are loaded while the
+// current step's 16 products run.
+constexpr int kRows = 128;
+constexpr int kThreads = 512;
 
-// One layer for the block's rows: Out = act(In W^T + b).  Each wavefront
-// computes column chunks of 64 (4 MFMA tiles) for all kRows rows (2 tiles).
-// W is (ncols_padded x K) bfloat16 row-major with K a multiple of 32.
 template <class EPI>
 __device__ __forceinline__ void fused_layer(const unsigned short* In, int stride_in,
                                             const unsigned short* __restrict__ W, int K,
@@ -135,37 +139,50 @@ __device__ __forceinline__ void fused_layer(const unsigned short* In, int stride
                                             const EPI& epi) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, c = lane & 15;
-  for (int chunk = wave; chunk * 64 < ncols; chunk += 4) {
-    floatx4 acc[2][4];
+  const int rh = wave & 1;
+  for (int chunk = wave >> 1; chunk * 64 < ncols; chunk += 4) {
+    floatx4 acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[a][b] = (floatx4){0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-      bf16x8 av[2], bv[4];
+    const unsigned short* wrow = W + (size_t)(chunk * 64 + c) * K + 8 * q;
+    const unsigned short* arow = In + (64 * rh + c) * stride_in + 8 * q;
+    // Two weight buffers alternate over K steps of 32 (K is a multiple of
+    // 64): the next step's fragments load while this step's 16 products run.
+    bf16x8 b0[4], b1[4];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
-        av[a] = *reinterpret_cast<const bf16x8*>(In + (16 * a + c) * stride_in + k0 + 8 * q);
+    for (int b = 0; b < 4; ++b) b0[b] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * b * K);
+    auto step = [&](int k0, const bf16x8* bv, bf16x8* bnext, int kn) {
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-        bv[b] = *reinterpret_cast<const bf16x8*>(W + (size_t)(chunk * 64 + 16 * b + c) * K + k0 +
-                                                 8 * q);
+        bnext[b] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * b * K + kn);
+      bf16x8 av[4];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < 4; ++a)
+        av[a] = *reinterpret_cast<const bf16x8*>(arow + 16 * a * stride_in + k0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+    };
+    for (int k0 = 0; k0 < K; k0 += 64) {
+      step(k0, b0, b1, k0 + 32);
+      // (the last step reloads its own fragments: no branch around loads)
+      step(k0 + 32, b1, b0, k0 + 64 < K ? k0 + 64 : k0 + 32);
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const int col = chunk * 64 + 16 * b + c;
+      if (col < ncols) {
+        const float bc = bias[col];
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = 16 * a + 4 * q + r;
-          const int col = chunk * 64 + 16 * b + c;
-          if (col < ncols) epi(row, col, acc[a][b][r] + bias[col]);
-        }
+          for (int r = 0; r < 4; ++r) epi(64 * rh + 16 * a + 4 * q + r, col, acc[a][b][r] + bc);
+      }
+    }
   }
 }
 
@@ -177,77 +194,105 @@ struct ToLds {
   }
 };
 
+// Outputs straight into the virtual slots: the packed output neurons are in
+// slot order (output 2 (j ne + e) + part is element e of slot j, real or
+// imaginary part), so output col of a row is float col of its slots.
 template <typename TO>
 struct ToSlots {
   TO* slab;
-  const int* rowbase;  // per block row: first virtual slot's element offset, or -1
+  const long long* rowbase;  // per block row: first virtual slot's real offset, or -1
   const float* srow;
-  int ne;
   __device__ __forceinline__ void operator()(int row, int col, float v) const {
-    const int base = rowbase[row];
-    if (base < 0) return;
-    const int j = col / (2 * ne), w = col - j * 2 * ne;
-    const int part = w >= ne, e = w - part * ne;
-    // Complex element e of slot j: real part at 2*(...), imaginary at +1.
-    slab[2 * ((size_t)base + (size_t)j * ne + e) + part] = (TO)(v * srow[row]);
+    const long long base = rowbase[row];
+    if (base >= 0) slab[base + col] = (TO)(v * srow[row]);
   }
 };
 
 template <typename TI>
-__global__ void __launch_bounds__(256) fused(FusedNet net, TI* slab, const int* __restrict__ moff,
-                                             int members, int first, int nb, int ne, int D,
-                                             const float* __restrict__ bmi, int vbase) {
+__global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
+                                                  const int* __restrict__ moff, int members,
+                                                  int first, int nb, int ne, int D,
+                                                  const float* __restrict__ bmi, int vbase) {
   typedef typename C2T<TI>::type V;
   extern __shared__ __align__(16) unsigned short lds[];
   __shared__ float srow[kRows];
-  __shared__ int rowbase[kRows];
+  __shared__ long long rowbase[kRows];
+  __shared__ long long rowsrc[kRows];
+  __shared__ unsigned smax[kRows];
   const int stride = net.width + 8;
   unsigned short* A0 = lds;
   unsigned short* A1 = lds + kRows * stride;
   const long long R = (long long)members * nb;
   const long long row0 = (long long)blockIdx.x * kRows;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // Inputs: one wavefront per row.
-  for (int r = wave; r < kRows; r += 4) {
-    const long long row = row0 + r;
-    unsigned short* x = A0 + r * stride;
+  const int tid = threadIdx.x;
+  // Inputs, for all the block's rows at once (one trip to memory for the
+  // member offsets, one for the ADOs): each row's source node and slots.
+  if (tid < kRows) {
+    const long long row = row0 + tid;
+    smax[tid] = 0u;
     if (row < R) {
       const int m = (int)(row / nb), b = (int)(row - (long long)m * nb);
-      const V* src = reinterpret_cast<const V*>(slab) + ((size_t)moff[m] + first + b) * ne;
-      float mx = 0.f;
-      for (int e = lane; e < ne; e += 64) {
-        const V v = src[e];
-        mx = fmaxf(mx, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
-      }
-      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-      const float s = fmaxf(mx, 1e-30f);
-      const float inv = 1.0f / s;
-      for (int e = lane; e < ne; e += 64) {
-        const V v = src[e];
-        x[e] = to_bf16((float)v.x * inv);
-        x[ne + e] = to_bf16((float)v.y * inv);
-      }
-      for (int j = lane; j < D; j += 64) x[2 * ne + j] = to_bf16(bmi[(size_t)b * D + j]);
-      for (int k = 2 * ne + D + lane; k < net.k1; k += 64) x[k] = 0;
-      if (lane == 0) {
-        srow[r] = s;
-        rowbase[r] = (int)(((size_t)moff[m] + vbase + (size_t)b * D) * ne);
-      }
+      const long long mo = moff[m];
+      rowsrc[tid] = (mo + first + b) * ne;
+      rowbase[tid] = 2 * ((mo + vbase + (long long)b * D) * ne);
     } else {
-      for (int k = lane; k < net.k1; k += 64) x[k] = 0;
-      if (lane == 0) {
-        srow[r] = 0.f;
-        rowbase[r] = -1;
-      }
+      rowsrc[tid] = -1;
+      rowbase[tid] = -1;
     }
+  }
+  __syncthreads();
+  // Elements (row, e) of the tile, several per thread: largest |Re|, |Im|
+  // of each row (non-negative floats order as their bits).
+  const int tot = kRows * ne;
+  constexpr int kPer = 16;  // enough for ne <= 64 (N <= 8)
+  V v[kPer];
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    const int i = tid + t * kThreads;
+    const int r = i / ne;
+    const long long src = i < tot ? rowsrc[r] : -1;
+    v[t] = src >= 0 ? reinterpret_cast<const V*>(slab)[src + (i - r * ne)] : V{};
+  }
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    const int i = tid + t * kThreads;
+    if (i < tot) {
+      const float mx = fmaxf(fabsf((float)v[t].x), fabsf((float)v[t].y));
+      atomicMax(&smax[i / ne], __float_as_uint(mx));
+    }
+  }
+  __syncthreads();
+  if (tid < kRows) srow[tid] = fmaxf(__uint_as_float(smax[tid]), 1e-30f);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    const int i = tid + t * kThreads;
+    if (i < tot) {
+      const int r = i / ne, e = i - r * ne;
+      const float inv = 1.0f / srow[r];
+      unsigned short* x = A0 + r * stride;
+      x[e] = to_bf16((float)v[t].x * inv);
+      x[ne + e] = to_bf16((float)v[t].y * inv);
+    }
+  }
+  // Boundary multi-index columns and the zero padding up to k1.
+  const int extra = net.k1 - 2 * ne;
+  for (int i = tid; i < kRows * extra; i += kThreads) {
+    const int r = i / extra, k = i - r * extra;
+    const long long row = row0 + r;
+    unsigned short val = 0;
+    if (k < D && row < R) {
+      const int b = (int)(row % nb);
+      val = to_bf16(bmi[(size_t)b * D + k]);
+    }
+    A0[r * stride + 2 * ne + k] = val;
   }
   __syncthreads();
   fused_layer(A0, stride, net.w1, net.k1, net.hid, net.b1, ToLds{A1, stride});
   __syncthreads();
   fused_layer(A1, stride, net.w2, net.hid, net.hid, net.b2, ToLds{A0, stride});
   __syncthreads();
-  fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3,
-              ToSlots<TI>{slab, rowbase, srow, ne});
+  fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3, ToSlots<TI>{slab, rowbase, srow});
 }
 
 }  // namespace closure
@@ -256,17 +301,30 @@ size_t FusedClosureLds(const FusedNet& net) {
   return (size_t)2 * closure::kRows * (net.width + 8) * sizeof(unsigned short);
 }
 
+bool FusedClosureFits(const FusedNet& net) {
+  // Both activation buffers in one CU's 160 KB of LDS, with the block's
+  // per-row tables; K steps in pairs of 32.
+  return FusedClosureLds(net) + closure::kRows * 24 <= (size_t)160 * 1024 &&
+         net.k1 % 64 == 0 && net.hid % 64 == 0;
+}
+
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
                               int members, int first, int nb, int ne, int D, const float* bmi,
                               int vbase, hipStream_t s) {
   const long long rows = (long long)members * nb;
   const unsigned blocks = (unsigned)((rows + closure::kRows - 1) / closure::kRows);
   const size_t lds = FusedClosureLds(net);
+  if (!FusedClosureFits(net) || ne > 64 || net.k1 % 64 != 0 || net.hid % 64 != 0)
+    return hipErrorInvalidConfiguration;
+  const void* k = fp32 ? reinterpret_cast<const void*>(&closure::fused<float>)
+                       : reinterpret_cast<const void*>(&closure::fused<double>);
+  hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
   if (fp32)
-    hipLaunchKernelGGL(closure::fused<float>, dim3(blocks), dim3(256), lds, s, net, (float*)slab,
-                       moff, members, first, nb, ne, D, bmi, vbase);
+    hipLaunchKernelGGL(closure::fused<float>, dim3(blocks), dim3(closure::kThreads), lds, s, net,
+                       (float*)slab, moff, members, first, nb, ne, D, bmi, vbase);
   else
-    hipLaunchKernelGGL(closure::fused<double>, dim3(blocks), dim3(256), lds, s, net,
+    hipLaunchKernelGGL(closure::fused<double>, dim3(blocks), dim3(closure::kThreads), lds, s, net,
                        (double*)slab, moff, members, first, nb, ne, D, bmi, vbase);
   return hipGetLastError();
 }

@@ -371,9 +371,13 @@ int Engine::GetMemberRho0(std::complex<double>* host) {
   const int NE = N() * N();
   int rc = kOk;
   if (members_ == 0) return GetNode0(host);
-  for (int m = 0; m < members_ && !rc; ++m)
-    rc = Check(hipMemcpyAsync(host + (size_t)m * NE, d_rho_ + (size_t)moff_[m] * NE * 2,
-                              NE * sizeof(double2), hipMemcpyDeviceToHost, stream_), "member rho0");
+  // One gather kernel and one copy (a copy per member costs a round trip each).
+  const size_t bytes = (size_t)members_ * NE * sizeof(double2);
+  if (!d_mrho_) rc = Check(hipMalloc((void**)&d_mrho_, bytes), "alloc member rho0");
+  if (!rc) rc = Check(LaunchGatherNode0(d_rho_, d_moff_, members_, NE, d_mrho_, stream_),
+                      "gather member rho0");
+  if (!rc) rc = Check(hipMemcpyAsync(host, d_mrho_, bytes, hipMemcpyDeviceToHost, stream_),
+                      "member rho0");
   return rc ? rc : Synchronize();
 }
 
@@ -386,7 +390,7 @@ Engine::~Engine() {
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
                   d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_cpack_, d_active_, d_kf_[0], d_kf_[1],
                   d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_rot_,
-                  d_ops_, d_alpha_, d_corr_, d_tlq_, d_tlsite_};
+                  d_ops_, d_alpha_, d_corr_, d_tlq_, d_tlsite_, d_mrho_};
   for (void* p : ptrs) Free(p);
   if (h_stage_) (void)hipHostFree(h_stage_);
   if (h_ops_) (void)hipHostFree(h_ops_);

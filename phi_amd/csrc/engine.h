@@ -306,6 +306,7 @@ class Engine {
   int BuildEdges(int closure_first, int closure_nb, int closure_vbase, int4** out);
   int UploadEdges(int closure_first, int closure_nb, int closure_vbase);
   int* d_moff_ = nullptr;
+  double* d_mrho_ = nullptr;  // node 0 of every member (GetMemberRho0)
   double2* d_Hm_ = nullptr;
   TileDesc* d_tiles_ = nullptr;
   int tiles_cap_ = 0;

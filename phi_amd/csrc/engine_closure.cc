@@ -138,7 +138,15 @@ int Engine::AttachClosure(Mlp* mlp) {
   int* dmoff = nullptr;
   int4* edge = nullptr;
   float *cbmi = nullptr, *cx = nullptr, *cs = nullptr, *cy = nullptr;
-  int rc = mlp->Pack(&fnet, &pack);
+  // The fused pass writes the network's outputs straight into the virtual
+  // slots, interleaved complex: packed output 2 (j ne + e) + part is the
+  // network's output j 2 ne + part ne + e (slot j, element e).
+  std::vector<int> perm((size_t)2 * NE * D);
+  for (int j = 0; j < D; ++j)
+    for (int e = 0; e < NE; ++e)
+      for (int part = 0; part < 2; ++part)
+        perm[(size_t)2 * (j * NE + e) + part] = j * 2 * NE + part * NE + e;
+  int rc = mlp->Pack(&fnet, &pack, perm.data());
   if (rc) {
     err_ = "closure network: " + mlp->last_error();
     return rc == kErrUnsupported ? kErrInvalidArgument : rc;
@@ -209,11 +217,12 @@ int Engine::AttachClosure(Mlp* mlp) {
 }
 
 // Predictions for the virtual slots of the stage input `in` (one fused
-// launch; the unfused path, kept for reference, runs the inputs kernel,
-// the network's GEMMs and the scatter kernel).
+// launch; the unfused path, kept for reference and for networks whose
+// activations do not fit one CU's LDS, runs the inputs kernel, the
+// network's GEMMs and the scatter kernel).
 int Engine::ClosurePredict(const void* in, bool in_fp32) {
   const int NE = h_.N * h_.N, D = h_.D;
-  if (closure_fused_)
+  if (closure_fused_ && FusedClosureFits(fused_) && NE <= 64)
     return Check(LaunchClosureFused(fused_, const_cast<void*>(in), in_fp32, d_moff_, members_,
                                     closure_first_, closure_nb_, NE, D, d_cbmi_, closure_vbase_,
                                     stream_), "closure fused");

@@ -110,9 +110,12 @@ hipError_t LaunchRhsGeneralDot(const RhsArgs& a, const double* x, double* y, int
                                int dot, const double* w, double* partial, hipStream_t s);
 hipError_t LaunchSnapshot(const double* src, double* hist, int* counter, int ne, int capacity,
                           hipStream_t s);
-// hist[slot] = mean over the members of their node 0 (members in order).
+// hist[slot] = mean over the members of their node 0 (fixed summation order).
 hipError_t LaunchSnapshotMean(const double* src, const int* member_offsets, int members,
                               double* hist, int* counter, int ne, int capacity, hipStream_t s);
+// dst (members x ne complex) = node 0 of every member.
+hipError_t LaunchGatherNode0(const double* src, const int* member_offsets, int members, int ne,
+                             double* dst, hipStream_t s);
 hipError_t LaunchFillU64(unsigned long long* p, unsigned long long v, int n, hipStream_t s);
 hipError_t LaunchCastToFloat(const double* src, float* dst, size_t n, hipStream_t s);
 // Basis change of every ADO of a slab: a_n <- R a_n R^dagger (R N x N,

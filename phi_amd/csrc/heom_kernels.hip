@@ -3,6 +3,7 @@
 // stage-kernel dispatch.  The stage kernels themselves are templates in
 // heom_device.h, instantiated per method and precision in heom_rk4_f64.hip,
 // heom_rk4_f32.hip, heom_rkf45_f64.hip and heom_rkf45_f32.hip.
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -204,25 +205,53 @@ __global__ void snapshot_node0(const double* __restrict__ src, double* __restric
 
 // As snapshot_node0 for an ensemble: the mean of every member's node 0,
 // summed in member order (deterministic).  One block.
-__global__ void snapshot_mean(const double* __restrict__ src, const int* __restrict__ moff,
-                              int members, double* __restrict__ hist, int* __restrict__ counter,
-                              int ne, int capacity) {
+// hist[slot] = mean over the members of their node 0 (slot = *counter,
+// left unchanged; bump_counter advances it).  One block per element: each
+// thread sums members tid, tid + blockDim.x, ... in order, then a fixed-order
+// tree over the block, so the sum is the same on every run.
+__global__ void __launch_bounds__(256) snapshot_mean(const double* __restrict__ src,
+                                                     const int* __restrict__ moff, int members,
+                                                     double* __restrict__ hist,
+                                                     const int* __restrict__ counter, int ne,
+                                                     int capacity) {
+  __shared__ double2 red[256];
   const int slot = *counter;
-  if (slot < capacity) {
-    const double2* s = reinterpret_cast<const double2*>(src);
-    const double w = 1.0 / members;
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      double x = 0, y = 0;
-      for (int m = 0; m < members; ++m) {
-        const double2 v = s[(size_t)moff[m] * ne + e];
-        x += v.x;
-        y += v.y;
-      }
-      reinterpret_cast<double2*>(hist)[(size_t)slot * ne + e] = make_double2(x * w, y * w);
-    }
+  const int e = blockIdx.x;
+  const double2* s = reinterpret_cast<const double2*>(src);
+  double x = 0, y = 0;
+  for (int m = threadIdx.x; m < members; m += blockDim.x) {
+    const double2 v = s[(size_t)moff[m] * ne + e];
+    x += v.x;
+    y += v.y;
   }
+  red[threadIdx.x] = make_double2(x, y);
   __syncthreads();
-  if (threadIdx.x == 0) *counter = slot + 1;
+  for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x].x += red[threadIdx.x + w].x;
+      red[threadIdx.x].y += red[threadIdx.x + w].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && slot < capacity) {
+    const double inv = 1.0 / members;
+    reinterpret_cast<double2*>(hist)[(size_t)slot * ne + e] =
+        make_double2(red[0].x * inv, red[0].y * inv);
+  }
+}
+
+__global__ void bump_counter(int* counter) { *counter += 1; }
+
+// dst[m] = node 0 of member m (members x ne complex), for one copy to the host.
+__global__ void gather_node0(const double* __restrict__ src, const int* __restrict__ moff,
+                             int members, int ne, double* __restrict__ dst) {
+  const long long n = (long long)members * ne;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int m = (int)(i / ne), e = (int)(i - (long long)m * ne);
+    reinterpret_cast<double2*>(dst)[i] =
+        reinterpret_cast<const double2*>(src)[(size_t)moff[m] * ne + e];
+  }
 }
 
 // One block per ADO (grid-stride over nodes): T = a R^dagger, then
@@ -677,8 +706,18 @@ hipError_t LaunchSnapshot(const double* src, double* hist, int* counter, int ne,
 
 hipError_t LaunchSnapshotMean(const double* src, const int* member_offsets, int members,
                               double* hist, int* counter, int ne, int capacity, hipStream_t s) {
-  hipLaunchKernelGGL(dev::snapshot_mean, dim3(1), dim3(256), 0, s, src, member_offsets, members,
+  hipLaunchKernelGGL(dev::snapshot_mean, dim3(ne), dim3(256), 0, s, src, member_offsets, members,
                      hist, counter, ne, capacity);
+  hipLaunchKernelGGL(dev::bump_counter, dim3(1), dim3(1), 0, s, counter);
+  return hipGetLastError();
+}
+
+hipError_t LaunchGatherNode0(const double* src, const int* member_offsets, int members, int ne,
+                             double* dst, hipStream_t s) {
+  const long long n = (long long)members * ne;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(dev::gather_node0, dim3(blocks), dim3(256), 0, s, src, member_offsets,
+                     members, ne, dst);
   return hipGetLastError();
 }
 

@@ -99,7 +99,7 @@ unsigned short Bf16(float f) {
 
 }  // namespace
 
-int Mlp::Pack(FusedNet* out, void** owned) {
+int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm) {
   *owned = nullptr;
   if (n_hid_ % 32 != 0) {
     err_ = "the fused closure pass needs a hidden width that is a multiple of 32";
@@ -122,13 +122,19 @@ int Mlp::Pack(FusedNet* out, void** owned) {
     for (int i = 0; i < n_in_; ++i) w[(size_t)o * k1 + i] = Bf16(p[off_[0] + (size_t)o * n_in_ + i]);
   for (int o = 0; o < hid; ++o)
     for (int i = 0; i < hid; ++i) w[n1 + (size_t)o * hid + i] = Bf16(p[off_[2] + (size_t)o * hid + i]);
-  for (int o = 0; o < n_out_; ++o)
+  for (int o = 0; o < n_out_; ++o) {
+    const int src = out_perm ? out_perm[o] : o;
+    if (src < 0 || src >= n_out_) {
+      err_ = "output permutation entry out of range";
+      return -3;
+    }
     for (int i = 0; i < hid; ++i)
-      w[n1 + n2 + (size_t)o * hid + i] = Bf16(p[off_[4] + (size_t)o * hid + i]);
+      w[n1 + n2 + (size_t)o * hid + i] = Bf16(p[off_[4] + (size_t)src * hid + i]);
+  }
   float* bias = reinterpret_cast<float*>(w.data() + nw);
   std::memcpy(bias, p.data() + off_[1], hid * sizeof(float));
   std::memcpy(bias + hid, p.data() + off_[3], hid * sizeof(float));
-  std::memcpy(bias + 2 * hid, p.data() + off_[5], n_out_ * sizeof(float));
+  for (int o = 0; o < n_out_; ++o) bias[2 * hid + o] = p[off_[5] + (out_perm ? out_perm[o] : o)];
   unsigned short* d = nullptr;
   rc = Check(hipMalloc((void**)&d, w.size() * sizeof(unsigned short)), "alloc pack");
   if (!rc) rc = Check(hipMemcpy(d, w.data(), w.size() * sizeof(unsigned short),

@@ -49,7 +49,9 @@ class Mlp {
   // A snapshot of the current parameters packed for the fused closure pass
   // (bfloat16 weights and float32 biases in one device buffer, *owned, which
   // the caller frees with hipFree; later training does not change it).
-  int Pack(struct FusedNet* out, void** owned);
+  // out_perm (n_out entries or null): packed output neuron o is neuron
+  // out_perm[o] of the network (a relabelling of the last layer's rows).
+  int Pack(struct FusedNet* out, void** owned, const int* out_perm = nullptr);
 
  private:
   Mlp() = default;
