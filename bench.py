@@ -302,6 +302,9 @@ def run_closure(P, np, steps, dt, train_members, test_members, log):
         "time_to_1ps_s_depth4": res["shallow"]["seconds"] * scale,
         "time_to_1ps_s_depth4_closure": res["closure"]["seconds"] * scale,
         "speedup_closure_vs_depth10": res["deep"]["seconds"] / res["closure"]["seconds"],
+        "closure_vs_depth4_time": res["closure"]["seconds"] / res["shallow"]["seconds"],
+        "fallback_threshold": getattr(net, "ymax", None),
+        "fallback_predictions": res["closure"]["fallen"],
         "max_population_error_vs_depth10": {"depth4": res["shallow"]["max_pop_err"],
                                             "depth4_closure": res["closure"]["max_pop_err"]},
         "mean_population_error_vs_depth10": {"depth4": res["shallow"]["mean_pop_err"],

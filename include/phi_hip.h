@@ -201,6 +201,14 @@ int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m);
  * the unfused kernels (inputs, the network's GEMMs, scatter); the network's
  * parameters are packed when the closure is attached. */
 int phi_hip_closure_fused(phi_ctx* c, int on);
+/* The closure's fallback to PHI's plain truncation (the dropped tier of
+ * LinkHierarchyNodes): before a stage uses them, the predictions of a
+ * boundary node whose outputs (next-tier entries over the node's largest
+ * entry) are not all finite and at most ymax in magnitude are replaced by
+ * zeros.  ymax > 0 sets the threshold (default 3e38: non-finite outputs
+ * only); *fallen (if not null) = boundary-node predictions replaced since
+ * the attach. */
+int phi_hip_closure_fallback(phi_ctx* c, double ymax, long long* fallen);
 int phi_hip_num_nodes(const phi_ctx* c);
 int phi_hip_num_modes(const phi_ctx* c);
 /* Multi-indices (num_nodes * D ints), PHI id order. */

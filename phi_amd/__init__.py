@@ -122,6 +122,8 @@ def lib():
                                                    ctypes.POINTER(ctypes.c_longlong)]),
         "phi_hip_closure_attach": (ctypes.c_int, [vp, vp]),
         "phi_hip_closure_fused": (ctypes.c_int, [vp, ctypes.c_int]),
+        "phi_hip_closure_fallback": (ctypes.c_int, [vp, ctypes.c_double,
+                                                    ctypes.POINTER(ctypes.c_longlong)]),
         "phi_hip_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, dp, ip]),
     }
     for name, (res, args) in sigs.items():
@@ -545,6 +547,14 @@ class Ensemble(Context):
     def set_closure_fused(self, on):
         """Closure predictions by the fused pass (True) or unfused kernels."""
         _check(lib().phi_hip_closure_fused(self._h, 1 if on else 0))
+
+    def closure_fallback(self, ymax=0.0):
+        """Set the closure's fallback threshold (ymax > 0; outputs above it,
+        or not finite, zero their boundary node's predictions) and return
+        how many boundary-node predictions fell back since the attach."""
+        n = ctypes.c_longlong()
+        _check(lib().phi_hip_closure_fallback(self._h, float(ymax), ctypes.byref(n)))
+        return n.value
 
 
 def _fp(a):

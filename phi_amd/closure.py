@@ -12,7 +12,9 @@ Pipeline (every step on the GPU through the C ABI):
             hierarchy and their next-tier neighbours are read out
             (Ensemble.closure_samples), a random fraction kept;
   train     Adam on the samples (Mlp.train, bf16 GEMM operands, float32
-            accumulation);
+            accumulation); the fallback threshold is set from the targets:
+            a prediction more than FALLBACK_FACTOR times the largest
+            training target falls back to plain truncation;
   evaluate  held-out Hamiltonians integrated plain-deep, plain-shallow and
             shallow + closure; populations compared with the deep run, and
             the time to 1 ps of each.
@@ -84,8 +86,12 @@ def init_flat(n_in, n_hidden, n_out, seed=0):
     return np.concatenate(parts).astype(np.float32)
 
 
+FALLBACK_FACTOR = 4.0
+
+
 def train(X, T, hidden=256, epochs=20, batch=512, lr=1e-3, seed=0, log=print):
     net = P.Mlp(X.shape[1], hidden, T.shape[1], max_batch=max(batch, 8192))
+    net.ymax = FALLBACK_FACTOR * float(np.abs(T).max())
     net.set_params(init_flat(X.shape[1], hidden, T.shape[1], seed))
     t0 = time.perf_counter()
     losses = net.train(X, T, batch=batch, epochs=epochs, lr=lr, seed=seed)
@@ -127,10 +133,14 @@ def evaluate(Hs, net, deep, shallow, steps, dt, every, precision="fp64", deep_pr
         ens = P.Ensemble(p, Hs, [lh] * len(Hs), precision=prec)
         if closed:
             ens.attach_closure(net)
+            if getattr(net, "ymax", None):
+                ens.closure_fallback(net.ymax)
         # time a copy of the run after autotuning on a scratch ensemble
         ens.autotune(dt, steps=3)
         pops, el = _run(ens, steps, dt, every)
         out[name] = {"pops": pops, "seconds": el, "nodes": ens.num_nodes}
+        if closed:
+            out[name]["fallen"] = ens.closure_fallback()
         log("closure eval: %s (depth %d%s, %s): %.3f s for %d steps, %d slab nodes"
             % (name, lh, ", closure" if closed else "", prec, el, steps, ens.num_nodes))
         ens.close()

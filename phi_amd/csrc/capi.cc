@@ -638,6 +638,14 @@ int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m) {
   });
 }
 
+int phi_hip_closure_fallback(phi_ctx* c, double ymax, long long* fallen) {
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->ClosureFallback(ymax, fallen);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
+}
+
 int phi_hip_closure_fused(phi_ctx* c, int on) {
   return Guarded(__func__, [&]() -> int {
     if (!c) return Err(PHI_HIP_EINVAL, "null argument");

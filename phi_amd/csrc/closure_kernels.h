@@ -25,10 +25,19 @@ struct FusedNet {
 size_t FusedClosureLds(const FusedNet& net);
 // Whether the fused pass's LDS fits one CU (hidden widths up to 256).
 bool FusedClosureFits(const FusedNet& net);
+// The fallback to plain truncation: a row whose outputs are not all finite
+// and at most ymax in magnitude (they are the children's entries over the
+// parent's largest entry) gets zero slots, i.e. PHI's truncation for that
+// boundary node; fallen counts such rows.
+struct ClosureGuard {
+  float ymax;
+  unsigned long long* fallen;
+};
+
 // Inputs, the three layers and the scatter in one launch (128 rows per block).
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
                               int members, int first, int nb, int ne, int D, const float* bmi,
-                              int vbase, hipStream_t s);
+                              int vbase, const ClosureGuard& guard, hipStream_t s);
 
 // Network inputs X (rows x (2 N^2 + D)) and scales S of the boundary nodes
 // first .. first + nb - 1 of every member, from a stage input in FP32 or
@@ -43,7 +52,7 @@ hipError_t LaunchClosureTargets(const double* rho, const int* moff, int members,
 // Network outputs Y -> virtual neighbour slots vbase + b*D + j of each member.
 hipError_t LaunchClosureScatter(const float* Y, const float* S, const int* moff, int members,
                                 int nb, int ne, int D, int vbase, void* out, bool out_fp32,
-                                hipStream_t s);
+                                const ClosureGuard& guard, hipStream_t s);
 
 }  // namespace phi
 

@@ -98,21 +98,29 @@ __global__ void __launch_bounds__(256) scatter(const float* __restrict__ Y,
                                                const float* __restrict__ S,
                                                const int* __restrict__ moff, int members,
                                                int nb, int ne, int D, int vbase,
-                                               TO* __restrict__ out) {
+                                               TO* __restrict__ out, ClosureGuard guard) {
   typedef typename C2T<TO>::type V;
+  __shared__ int bad;
   const long long row = blockIdx.x;
   if (row >= (long long)members * nb) return;
   const int m = (int)(row / nb), b = (int)(row - (long long)m * nb);
   const float s = S[row];
   const float* y = Y + row * (size_t)(2 * ne * D);
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * ne * D; e += blockDim.x)
+    if (!(fabsf(y[e]) <= guard.ymax)) bad = 1;
+  __syncthreads();
+  const float keep = bad ? 0.f : 1.f;
   V* dst = reinterpret_cast<V*>(out) + ((size_t)moff[m] + vbase + (size_t)b * D) * ne;
   for (int e = threadIdx.x; e < ne * D; e += blockDim.x) {
     const int j = e / ne, w = e - j * ne;
     V v;
-    v.x = (TO)(y[(size_t)j * 2 * ne + w] * s);
-    v.y = (TO)(y[(size_t)j * 2 * ne + ne + w] * s);
+    v.x = (TO)(keep * y[(size_t)j * 2 * ne + w] * s);
+    v.y = (TO)(keep * y[(size_t)j * 2 * ne + ne + w] * s);
     dst[(size_t)j * ne + w] = v;
   }
+  if (threadIdx.x == 0 && bad) atomicAdd(guard.fallen, 1ULL);
 }
 
 // --- Fused closure pass ------------------------------------------------------------
@@ -202,9 +210,13 @@ struct ToSlots {
   TO* slab;
   const long long* rowbase;  // per block row: first virtual slot's real offset, or -1
   const float* srow;
+  int* rowbad;               // per block row: an output failed the guard
+  float ymax;
   __device__ __forceinline__ void operator()(int row, int col, float v) const {
     const long long base = rowbase[row];
-    if (base >= 0) slab[base + col] = (TO)(v * srow[row]);
+    if (base < 0) return;
+    slab[base + col] = (TO)(v * srow[row]);
+    if (!(fabsf(v) <= ymax)) rowbad[row] = 1;
   }
 };
 
@@ -212,13 +224,15 @@ template <typename TI>
 __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
                                                   const int* __restrict__ moff, int members,
                                                   int first, int nb, int ne, int D,
-                                                  const float* __restrict__ bmi, int vbase) {
+                                                  const float* __restrict__ bmi, int vbase,
+                                                  ClosureGuard guard) {
   typedef typename C2T<TI>::type V;
   extern __shared__ __align__(16) unsigned short lds[];
   __shared__ float srow[kRows];
   __shared__ long long rowbase[kRows];
   __shared__ long long rowsrc[kRows];
   __shared__ unsigned smax[kRows];
+  __shared__ int rowbad[kRows];
   const int stride = net.width + 8;
   unsigned short* A0 = lds;
   unsigned short* A1 = lds + kRows * stride;
@@ -230,6 +244,7 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
   if (tid < kRows) {
     const long long row = row0 + tid;
     smax[tid] = 0u;
+    rowbad[tid] = 0;
     if (row < R) {
       const int m = (int)(row / nb), b = (int)(row - (long long)m * nb);
       const long long mo = moff[m];
@@ -292,7 +307,18 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
   __syncthreads();
   fused_layer(A1, stride, net.w2, net.hid, net.hid, net.b2, ToLds{A0, stride});
   __syncthreads();
-  fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3, ToSlots<TI>{slab, rowbase, srow});
+  fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3,
+              ToSlots<TI>{slab, rowbase, srow, rowbad, guard.ymax});
+  // Rows whose outputs failed the guard fall back to plain truncation: zero
+  // slots (a second write after every wavefront's first one).
+  __syncthreads();
+  int nbad = 0;
+  for (int r = 0; r < kRows; ++r) {
+    if (!rowbad[r]) continue;
+    ++nbad;
+    for (int col = tid; col < net.n_out; col += kThreads) slab[rowbase[r] + col] = (TI)0;
+  }
+  if (tid == 0 && nbad) atomicAdd(guard.fallen, (unsigned long long)nbad);
 }
 
 }  // namespace closure
@@ -310,7 +336,7 @@ bool FusedClosureFits(const FusedNet& net) {
 
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
                               int members, int first, int nb, int ne, int D, const float* bmi,
-                              int vbase, hipStream_t s) {
+                              int vbase, const ClosureGuard& guard, hipStream_t s) {
   const long long rows = (long long)members * nb;
   const unsigned blocks = (unsigned)((rows + closure::kRows - 1) / closure::kRows);
   const size_t lds = FusedClosureLds(net);
@@ -322,10 +348,10 @@ hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const 
   if (e != hipSuccess) return e;
   if (fp32)
     hipLaunchKernelGGL(closure::fused<float>, dim3(blocks), dim3(closure::kThreads), lds, s, net,
-                       (float*)slab, moff, members, first, nb, ne, D, bmi, vbase);
+                       (float*)slab, moff, members, first, nb, ne, D, bmi, vbase, guard);
   else
     hipLaunchKernelGGL(closure::fused<double>, dim3(blocks), dim3(closure::kThreads), lds, s, net,
-                       (double*)slab, moff, members, first, nb, ne, D, bmi, vbase);
+                       (double*)slab, moff, members, first, nb, ne, D, bmi, vbase, guard);
   return hipGetLastError();
 }
 
@@ -354,14 +380,14 @@ hipError_t LaunchClosureTargets(const double* rho, const int* moff, int members,
 
 hipError_t LaunchClosureScatter(const float* Y, const float* S, const int* moff, int members,
                                 int nb, int ne, int D, int vbase, void* out, bool out_fp32,
-                                hipStream_t s) {
+                                const ClosureGuard& guard, hipStream_t s) {
   const long long rows = (long long)members * nb;
   if (out_fp32)
     hipLaunchKernelGGL(closure::scatter<float>, dim3((unsigned)rows), dim3(256), 0, s, Y, S,
-                       moff, members, nb, ne, D, vbase, (float*)out);
+                       moff, members, nb, ne, D, vbase, (float*)out, guard);
   else
     hipLaunchKernelGGL(closure::scatter<double>, dim3((unsigned)rows), dim3(256), 0, s, Y, S,
-                       moff, members, nb, ne, D, vbase, (double*)out);
+                       moff, members, nb, ne, D, vbase, (double*)out, guard);
   return hipGetLastError();
 }
 

@@ -99,6 +99,10 @@ class Engine {
     closure_fused_ = v;
     DropGraph();
   }
+  // Fallback of the closure to plain truncation (ClosureGuard): outputs of
+  // magnitude above ymax (or not finite) zero their row's slots.  ymax <= 0
+  // leaves the threshold; *fallen = rows that fell back since the attach.
+  int ClosureFallback(double ymax, long long* fallen);
   bool fp32() const { return fp32_; }
   void set_fp32(bool v);
   // Stage-kernel shape: col_nm = kernel form (0 element, 1 matrix cores,
@@ -296,6 +300,8 @@ class Engine {
   class Mlp* closure_ = nullptr;
   bool closure_fused_ = true;
   FusedNet fused_{};
+  float closure_ymax_ = 3.0e38f;
+  unsigned long long* d_cfall_ = nullptr;
   int closure_first_ = 0, closure_nb_ = 0, closure_vbase_ = 0;
   float* d_cbmi_ = nullptr;
   float* d_cx_ = nullptr;

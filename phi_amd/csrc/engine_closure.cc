@@ -138,6 +138,7 @@ int Engine::AttachClosure(Mlp* mlp) {
   int* dmoff = nullptr;
   int4* edge = nullptr;
   float *cbmi = nullptr, *cx = nullptr, *cs = nullptr, *cy = nullptr;
+  unsigned long long* cfall = nullptr;
   // The fused pass writes the network's outputs straight into the virtual
   // slots, interleaved complex: packed output 2 (j ne + e) + part is the
   // network's output j 2 ne + part ne + e (slot j, element e).
@@ -167,9 +168,11 @@ int Engine::AttachClosure(Mlp* mlp) {
   if (!rc) rc = Check(hipMalloc((void**)&cx, (size_t)nrows * mlp->n_in() * 4), "alloc X");
   if (!rc) rc = Check(hipMalloc((void**)&cs, (size_t)nrows * 4), "alloc S");
   if (!rc) rc = Check(hipMalloc((void**)&cy, (size_t)nrows * mlp->n_out() * 4), "alloc Y");
+  if (!rc) rc = Check(hipMalloc((void**)&cfall, sizeof(unsigned long long)), "alloc fallback count");
+  if (!rc) rc = Check(hipMemsetAsync(cfall, 0, sizeof(unsigned long long), stream_), "zero count");
   if (!rc) rc = Synchronize();
   if (rc) {
-    void* ptrs[] = {pack, rho, dmoff, edge, cbmi, cx, cs, cy};
+    void* ptrs[] = {pack, rho, dmoff, edge, cbmi, cx, cs, cy, cfall};
     for (void* q : ptrs) FreeD(q);
     return rc;
   }
@@ -196,8 +199,9 @@ int Engine::AttachClosure(Mlp* mlp) {
   FreeD(d_edge_);
   d_edge_ = edge;
   args_.edge = d_edge_;
-  void* old[] = {d_cbmi_, d_cx_, d_cs_, d_cy_, d_cpack_};
+  void* old[] = {d_cbmi_, d_cx_, d_cs_, d_cy_, d_cpack_, d_cfall_};
   for (void* q : old) FreeD(q);
+  d_cfall_ = cfall;
   d_cbmi_ = cbmi;
   d_cx_ = cx;
   d_cs_ = cs;
@@ -222,10 +226,11 @@ int Engine::AttachClosure(Mlp* mlp) {
 // network's GEMMs and the scatter kernel).
 int Engine::ClosurePredict(const void* in, bool in_fp32) {
   const int NE = h_.N * h_.N, D = h_.D;
+  const ClosureGuard guard{closure_ymax_, d_cfall_};
   if (closure_fused_ && FusedClosureFits(fused_) && NE <= 64)
     return Check(LaunchClosureFused(fused_, const_cast<void*>(in), in_fp32, d_moff_, members_,
                                     closure_first_, closure_nb_, NE, D, d_cbmi_, closure_vbase_,
-                                    stream_), "closure fused");
+                                    guard, stream_), "closure fused");
   const long long nrows = (long long)members_ * closure_nb_;
   int rc = Check(LaunchClosureInputs(in, in_fp32, d_moff_, members_, closure_first_, closure_nb_,
                                      NE, D, d_cbmi_, d_cx_, d_cs_, stream_), "closure inputs");
@@ -235,8 +240,27 @@ int Engine::ClosurePredict(const void* in, bool in_fp32) {
   }
   if (!rc)
     rc = Check(LaunchClosureScatter(d_cy_, d_cs_, d_moff_, members_, closure_nb_, NE, D,
-                                    closure_vbase_, const_cast<void*>(in), in_fp32, stream_),
+                                    closure_vbase_, const_cast<void*>(in), in_fp32, guard,
+                                    stream_),
                "closure scatter");
+  return rc;
+}
+
+int Engine::ClosureFallback(double ymax, long long* fallen) {
+  if (!closure_) {
+    err_ = "no closure is attached";
+    return kErrInvalidArgument;
+  }
+  if (ymax > 0) {
+    closure_ymax_ = (float)std::min(ymax, 3.0e38);
+    DropGraph();  // the threshold is a launch argument
+  }
+  if (!fallen) return kOk;
+  unsigned long long n = 0;
+  int rc = Check(hipMemcpyAsync(&n, d_cfall_, sizeof(n), hipMemcpyDeviceToHost, stream_),
+                 "fallback count");
+  if (!rc) rc = Synchronize();
+  *fallen = (long long)n;
   return rc;
 }
 

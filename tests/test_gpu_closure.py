@@ -281,3 +281,42 @@ def test_failed_attach_leaves_the_ensemble_unchanged():
     assert off == ens.num_nodes
     net.close()
     ens.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_fallback_to_plain_truncation(fused):
+    """The closure's fallback: a network whose every output exceeds the
+    threshold (bias 5, ymax 1) has all its predictions replaced by zeros,
+    so the closed ensemble equals plain truncation (classic RK4 order both)
+    and every boundary-node prediction of every stage is counted; below the
+    threshold nothing falls back and the run differs from plain truncation."""
+    from phi_amd.configs.make_configs import c4_member
+    p = _c2_params()
+    dt = p.get_double("timestep")
+    Hs = np.array([c4_member(r)[0] for r in range(3)])
+    params = M.init_params(105, 256, 686, seed=0)
+    params[2] = (np.zeros_like(params[2][0]), np.full(686, 5.0, dtype=np.float32))
+    net = P.Mlp(105, 256, 686, max_batch=4096)
+    net.set_params(_flat(params))
+    lh, steps = 5, 20
+    plain = P.Ensemble(p, Hs, [lh] * 3)
+    plain.set_rk4_form("classic")
+    plain.rk4_steps(steps, dt)
+    boundary = int((plain.index().sum(axis=1) == lh - 1).sum())
+    for ymax, expect_fallen in ((1.0, 3 * boundary * 4 * steps), (10.0, 0)):
+        print("case: fallback fused=%s ymax=%g" % (fused, ymax), flush=True)
+        closed = P.Ensemble(p, Hs, [lh] * 3)
+        closed.attach_closure(net)
+        closed.set_closure_fused(fused)
+        assert closed.closure_fallback(ymax) == 0
+        closed.rk4_steps(steps, dt)
+        fallen = closed.closure_fallback()
+        d = np.abs(closed.member_rho0() - plain.member_rho0()).max()
+        print("  fallen %d (expected %d), max |closed - plain| = %.3e" % (fallen, expect_fallen, d))
+        assert fallen == expect_fallen
+        if expect_fallen:
+            assert d <= 1e-13
+        else:
+            assert d > 1e-10
+        closed.close()
+    plain.close()
