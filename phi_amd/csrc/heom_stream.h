@@ -85,61 +85,99 @@ __device__ __forceinline__ StreamLds<TB> stream_carve(unsigned char* smem, int N
 // -(i/hbar) H Y of one tile on the matrix cores, added into W.  Lane l of a
 // 16x16x4 MFMA holds A[l%16][l/16] and B[l/16][l%16]; the K dimension is
 // split so that lane group g = l/16 covers rows g*KS .. g*KS + KS - 1.
+// Work items are (16-column slice, 16-row tile of H) pairs dealt to the
+// waves round robin.  A wavefront's first two items have the same place in
+// every tile: their column, tile row and LDS offset are computed once per
+// kernel (GemmItems) and a tile only checks its column count.
+struct GemmItems {
+  int j[2];     // tile column of this lane
+  int mt[2];    // 16-row tile of H
+  int off[2];   // LDS offset of the column (Y and W share the layout)
+};
+
+__device__ __forceinline__ GemmItems stream_gemm_items(const RhsArgs& A) {
+  GemmItems g;
+  const int N = A.N, NP = N * (N + 1), MT = (N + 15) >> 4;
+  const int c16 = threadIdx.x & 15, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int item = wave + i * nwaves;
+    const int slice = item / MT;
+    g.mt[i] = item - slice * MT;
+    g.j[i] = slice * 16 + c16;
+    const int tj = fdiv(g.j[i], A.mN);
+    g.off[i] = tj * NP + (g.j[i] - tj * N) * (N + 1);
+  }
+  return g;
+}
+
 template <typename TB>
-__device__ __forceinline__ void stream_gemm(const RhsArgs& A, const StreamLds<TB>& s,
-                                            const typename C2<TB>::type* Y,
-                                            typename C2<TB>::type* W, int nt) {
+__device__ __forceinline__ void stream_gemm_item(const RhsArgs& A, const StreamLds<TB>& s,
+                                                 const typename C2<TB>::type* ycol,
+                                                 typename C2<TB>::type* wcol, bool jok, int mt) {
   typedef typename C2<TB>::type V;
   typedef Mfma<TB> M;
   const int N = A.N;
-  const int NP = N * (N + 1);
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
-  const int c16 = lane & 15;
   const int KS = (N + 3) >> 2;
+  const TB ih = (TB)A.inv_hbar;
+  const bool realh = A.real_h != 0;
+  typename M::acc xr = M::zero(), xi = M::zero();
+  for (int st = 0; st < KS; ++st) {
+    const int k = g * KS + st;
+    V y = mk<TB>(0, 0);
+    if (jok && k < N) y = ycol[k];
+    const TB hr = s.hfr[(mt * KS + st) * 64 + lane];
+    xr = M::mac(hr, y.x, xr);
+    xi = M::mac(hr, y.y, xi);
+    if (!realh) {
+      const TB hi = s.hfi[(mt * KS + st) * 64 + lane];
+      xr = M::mac(-hi, y.y, xr);
+      xi = M::mac(hi, y.x, xi);
+    }
+  }
+  if (jok) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * mt + M::row(g, r);
+      if (row < N) {
+        V w = wcol[row];
+        w.x += ih * xi[r];
+        w.y -= ih * xr[r];
+        wcol[row] = w;
+      }
+    }
+  }
+}
+
+template <typename TB>
+__device__ __forceinline__ void stream_gemm(const RhsArgs& A, const StreamLds<TB>& s,
+                                            const typename C2<TB>::type* Y,
+                                            typename C2<TB>::type* W, int nt,
+                                            const GemmItems& gi) {
+  const int N = A.N;
+  const int NP = N * (N + 1);
+  const int c16 = threadIdx.x & 15;
   const int MT = (N + 15) >> 4;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
-  const TB ih = (TB)A.inv_hbar;
-  const bool realh = A.real_h != 0;
   const int ncols = nt * N;
-  const int nslices = (ncols + 15) >> 4;
-  // Work items: (16-column slice, 16-row tile of H) pairs over the waves.
-  for (int item = wave; item < nslices * MT; item += nwaves) {
+  const int nitems = ((ncols + 15) >> 4) * MT;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (wave + i * nwaves < nitems)
+      stream_gemm_item<TB>(A, s, Y + gi.off[i], W + gi.off[i], gi.j[i] < ncols, gi.mt[i]);
+  }
+  // Further items (shapes with more than two per wavefront).
+  for (int item = wave + 2 * nwaves; item < nitems; item += nwaves) {
     const int slice = item / MT;
     const int mt = item - slice * MT;
     const int j = slice * 16 + c16;
     const bool jok = j < ncols;
     const int tj = jok ? fdiv(j, A.mN) : 0;
-    const int bj = j - tj * N;
-    const V* ycol = Y + tj * NP + bj * (N + 1);
-    typename M::acc xr = M::zero(), xi = M::zero();
-    for (int st = 0; st < KS; ++st) {
-      const int k = g * KS + st;
-      V y = mk<TB>(0, 0);
-      if (jok && k < N) y = ycol[k];
-      const TB hr = s.hfr[(mt * KS + st) * 64 + lane];
-      xr = M::mac(hr, y.x, xr);
-      xi = M::mac(hr, y.y, xi);
-      if (!realh) {
-        const TB hi = s.hfi[(mt * KS + st) * 64 + lane];
-        xr = M::mac(-hi, y.y, xr);
-        xi = M::mac(hi, y.x, xi);
-      }
-    }
-    if (jok) {
-      V* wcol = W + tj * NP + bj * (N + 1);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * mt + M::row(g, r);
-        if (row < N) {
-          V w = wcol[row];
-          w.x += ih * xi[r];
-          w.y -= ih * xr[r];
-          wcol[row] = w;
-        }
-      }
-    }
+    const int off = tj * NP + (j - tj * N) * (N + 1);
+    stream_gemm_item<TB>(A, s, Y + off, W + off, jok, mt);
   }
 }
 
@@ -398,6 +436,7 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     ot[i] = te * NP + a * (N + 1) + b;
   }
   double2 hp[2] = {make_double2(0., 0.), make_double2(0., 0.)};
+  const GemmItems gi = stream_gemm_items(A);
 
   for (int buf = 0; tile < tile_end; tile += step, buf ^= 1) {
     const int nt = ti.nt, base = ti.base;
@@ -469,7 +508,7 @@ __global__ void __launch_bounds__(EPT == 1 ? 1024 : 256)
     }
     __syncthreads();
     // (3) matrix-core product
-    stream_gemm<TB>(A, s, Yb, Wb, nt);
+    stream_gemm<TB>(A, s, Yb, Wb, nt, gi);
     __syncthreads();
     // (4) neighbour gathers of the next tile (its edge records arrived
     // during the product)
