@@ -1074,7 +1074,8 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   double res = std::sqrt(rn2.real());
   // The iterations keep their scalars on the device (BicgScalars): the dot
   // products land there, the update kernels derive beta, alpha and omega
-  // from them, a one-thread kernel closes each iteration.  The host checks
+  // from them, and the next iteration's p update closes each iteration (a
+  // one-block kernel closes the last of a batch).  The host checks
   // after every kBatch iterations; once converged (or broken down, or at
   // max_iter) the x/r update and the closing kernel do nothing, so x and
   // the count are those of stopping right there.
@@ -1082,7 +1083,9 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   BicgScalars* st = nullptr;
   double* hist = nullptr;
   double* part3 = nullptr;
-  if (!rc) rc = Check(hipMalloc((void**)&st, sizeof(BicgScalars)), "alloc bicgstab scalars");
+  // Two records alternate: iteration k works on st[k % 2], and closing it
+  // writes st[(k + 1) % 2] while other blocks may still read st[k % 2].
+  if (!rc) rc = Check(hipMalloc((void**)&st, 2 * sizeof(BicgScalars)), "alloc bicgstab scalars");
   // Partials: rhat.v (2 per block), t.s / t.t (3), |r|^2 / rhat.r (3).
   // When the matvec's LDS form fits, it writes the rhat.v and t.s / t.t
   // partials itself (one record per matvec block; mb blocks).
@@ -1113,10 +1116,14 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   };
   // rho for the first iteration; later ones come with the r update.
   if (!rc && !hs.done) rc = dev_dot(rh, r, &st->rho_new);
+  long long k = 0;  // iterations enqueued
   while (!rc && !hs.done) {
-    for (int b = 0; b < kBatch && !rc; ++b) {
-      // p = r + beta (p - omega v)
-      rc = Check(LaunchBicgUpdate(0, pp, n, r, pp, vv, st, nullptr, 0, stream_), "p update");
+    for (int b = 0; b < kBatch && !rc; ++b, ++k) {
+      BicgScalars* cur = st + (k & 1);
+      BicgScalars* prev = st + ((k + 1) & 1);
+      // (close the previous iteration, then) p = r + beta (p - omega v)
+      rc = Check(LaunchBicgP(b > 0, pp, n, r, vv, prev, cur, p_xr, kRedBlocks, hist, tol,
+                             max_iter, stream_), "p update");
       // v = A p with the rhat.v partials
       if (!rc && mb > 0)
         rc = Check(LaunchRhsGeneralDot(args_, pp, vv, 1, 1, rh, p_rv, stream_), "A p, rhat.v");
@@ -1125,8 +1132,7 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
         rc = Check(LaunchDotPartial(rh, vv, n, p_rv, kRedBlocks, stream_), "rhat.v");
       // s = r - alpha v (alpha from the rhat.v partials)
       if (!rc)
-        rc = Check(LaunchBicgUpdate(1, s, n, r, vv, nullptr, st, p_rv, nb_dot, stream_),
-                   "s update");
+        rc = Check(LaunchBicgS(s, n, r, vv, cur, p_rv, nb_dot, stream_), "s update");
       // t = A s with the t.s / t.t partials
       if (!rc && mb > 0)
         rc = Check(LaunchRhsGeneralDot(args_, s, t, 1, 2, s, p_ts, stream_), "A s, t.s, t.t");
@@ -1135,15 +1141,16 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
         rc = Check(LaunchBicgDotTS(t, s, n, p_ts, kRedBlocks, stream_), "t.s, t.t");
       // x = x + alpha p + omega s, r = s - omega t, partials of |r|^2, rhat.r
       if (!rc)
-        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, st, p_ts, nb_dot, p_xr, kRedBlocks,
+        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, cur, p_ts, nb_dot, p_xr, kRedBlocks,
                                 stream_),
                    "x/r update");
-      if (!rc)
-        rc = Check(LaunchBicgFinish(st, p_xr, kRedBlocks, hist, tol, max_iter, stream_),
-                   "finish");
     }
-    if (!rc) rc = Check(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, stream_),
-                        "bicgstab scalars");
+    // the batch's last iteration closes into the record the next batch starts from
+    if (!rc)
+      rc = Check(LaunchBicgFinish(st + ((k + 1) & 1), st + (k & 1), p_xr, kRedBlocks, hist, tol,
+                                  max_iter, stream_), "finish");
+    if (!rc) rc = Check(hipMemcpyAsync(&hs, st + (k & 1), sizeof(hs), hipMemcpyDeviceToHost,
+                                       stream_), "bicgstab scalars");
     if (!rc) rc = Synchronize();
   }
   const int it = hs.it;

@@ -146,13 +146,10 @@ struct BicgScalars {
 // Partials of conj(u) . v (2 doubles per block).
 hipError_t LaunchDotPartial(const double* u, const double* v, size_t n, double* partial,
                             int nblocks, hipStream_t s);
-// which = 0: y = x0 + beta x1 - beta omega x2 (p update; beta from the
-// previous iteration's scalars and rho_new); which = 1: y = x0 - alpha x1
-// (s update; rv = sum of the nb partials of rhat . v, kept in st,
+// s = r - alpha v (rv = sum of the nb partials of rhat . v, kept in st,
 // alpha = rho_new / rv).
-hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
-                            const double* x2, BicgScalars* st, const double* rv_part, int nb,
-                            hipStream_t s);
+hipError_t LaunchBicgS(double* y, size_t n, const double* r, const double* v, BicgScalars* st,
+                       const double* rv_part, int nb, hipStream_t s);
 // Partials of t . s (conjugating t) and |t|^2 (3 doubles per block).
 hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, double* partial3,
                            int nblocks, hipStream_t s);
@@ -163,11 +160,18 @@ hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
                         const double* t, const double* rhat, size_t n, BicgScalars* st,
                         const double* ts_part, int nb_ts, double* partial3, int nblocks,
                         hipStream_t s);
-// One block: st->rn2, st->rho_next from the x/r partials; alpha, omega,
-// rho_old <- this iteration's, rho_new <- rho_next; res = |r|; hist[it] =
-// res; done when !(res > tol), omega = 0 or it = max_iter.
-hipError_t LaunchBicgFinish(BicgScalars* st, const double* xr_part, int nblocks, double* hist,
-                            double tol, int max_iter, hipStream_t s);
+// Closing an iteration: from record prev and the x/r partials, the record
+// next of the following iteration (rn2, rho_next; alpha, omega, rho_old <-
+// this iteration's, rho_new <- rho_next; res = |r|; hist[it] = res; done
+// when !(res > tol), omega = 0 or it = max_iter).  One block.
+hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const double* xr_part,
+                            int nblocks, double* hist, double tol, int max_iter, hipStream_t s);
+// p = r + beta (p - omega v) with the scalars of record cur; close = true
+// first closes the previous iteration (record prev, the x/r partials) into
+// cur, as LaunchBicgFinish does.
+hipError_t LaunchBicgP(bool close, double* p, size_t n, const double* r, const double* v,
+                       const BicgScalars* prev, BicgScalars* cur, const double* xr_part,
+                       int nblocks, double* hist, double tol, int max_iter, hipStream_t s);
 
 }  // namespace phi
 

@@ -447,42 +447,26 @@ __device__ __forceinline__ void block_sum3(double a0, double a1, double a2,
   if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = s3[threadIdx.x][0];
 }
 
-// y = 1 x0 + c1 x1 (+ c2 x2), terms added left to right; y may be x1 (the
-// p update is in place), so no pointer is restrict.  WHICH 1 first reduces
-// the partials of rhat . v into rv (block 0 keeps it in st).
-template <int WHICH>
-__global__ void __launch_bounds__(256) bicg_update(double* y, size_t n, const double* x0,
-                                                   const double* x1, const double* x2,
-                                                   BicgScalars* st, const double* rv_part,
-                                                   int nb) {
-  double2 c1, c2 = make_double2(0., 0.);
-  if (WHICH == 0) {
-    const double2 beta = zmul(zdiv(st->rho_new, st->rho_old), zdiv(st->alpha, st->omega));
-    const double2 bo = zmul(beta, st->omega);
-    c1 = beta;
-    c2 = make_double2(-bo.x, -bo.y);
-  } else {
-    double pr[2];
-    reduce_partials<2>(rv_part, nb, pr);
-    const double2 rv = make_double2(pr[0], pr[1]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->rv = rv;
-    const double2 al = zdiv(st->rho_new, rv);
-    c1 = make_double2(-al.x, -al.y);
-  }
+// s = r - alpha v, after reducing the partials of rhat . v into rv (block 0
+// keeps it in st; alpha = rho_new / rv); terms added left to right.
+__global__ void __launch_bounds__(256) bicg_s(double* __restrict__ y, size_t n,
+                                              const double* __restrict__ x0,
+                                              const double* __restrict__ x1, BicgScalars* st,
+                                              const double* rv_part, int nb) {
+  double pr[2];
+  reduce_partials<2>(rv_part, nb, pr);
+  const double2 rv = make_double2(pr[0], pr[1]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rv = rv;
+  const double2 al = zdiv(st->rho_new, rv);
+  const double2 c1 = make_double2(-al.x, -al.y);
   const double2* a = reinterpret_cast<const double2*>(x0);
   const double2* b = reinterpret_cast<const double2*>(x1);
-  const double2* d = reinterpret_cast<const double2*>(x2);
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const double2 u = a[i], v = b[i];
     double ar = u.x, ai = u.y;
     ar += c1.x * v.x - c1.y * v.y;
     ai += c1.x * v.y + c1.y * v.x;
-    if (WHICH == 0) {
-      const double2 w = d[i];
-      ar += c2.x * w.x - c2.y * w.y;
-      ai += c2.x * w.y + c2.y * w.x;
-    }
     reinterpret_cast<double2*>(y)[i] = make_double2(ar, ai);
   }
 }
@@ -549,27 +533,78 @@ __global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* _
   block_sum3(acc, hr, hi, partial);
 }
 
-// One block: reduces the |r|^2 / rhat . r partials, then closes the
-// iteration.
-__global__ void __launch_bounds__(256) bicg_finish(BicgScalars* st, const double* xr_part, int nb,
-                                                   double* hist, double tol, int max_iter) {
-  if (st->done) return;  // uniform
+// Closes an iteration (every thread of every block computes the same
+// values; reduce_partials is called by all threads): from the record
+// `prev` of the iteration and the partials of |r|^2 and rhat . r its x/r
+// update wrote, the scalars that start the next iteration (alpha, omega,
+// rho_old = rho, rho = rhat . r, the residual, the count, the stop flag).
+__device__ __forceinline__ BicgScalars bicg_close(const BicgScalars* prev, const double* xr_part,
+                                                  int nb, double tol, int max_iter) {
+  BicgScalars c = *prev;
+  if (c.done) return c;  // uniform
   double q[3];
   reduce_partials<3>(xr_part, nb, q);
-  if (threadIdx.x != 0) return;
-  st->rn2 = make_double2(q[0], 0.);
-  st->rho_next = make_double2(q[1], q[2]);
-  const double2 al = zdiv(st->rho_new, st->rv), om = bicg_omega(st);
-  st->alpha = al;
-  st->omega = om;
-  st->rho_old = st->rho_new;
-  st->rho_new = st->rho_next;
-  const double res = sqrt(st->rn2.x);
-  st->res = res;
-  const int it = st->it + 1;
-  st->it = it;
-  hist[it] = res;
-  if (!(res > tol) || (om.x == 0 && om.y == 0) || it >= max_iter) st->done = 1;
+  c.rn2 = make_double2(q[0], 0.);
+  c.rho_next = make_double2(q[1], q[2]);
+  const double2 al = zdiv(c.rho_new, c.rv), om = bicg_omega(&c);
+  c.alpha = al;
+  c.omega = om;
+  c.rho_old = c.rho_new;
+  c.rho_new = c.rho_next;
+  c.res = sqrt(c.rn2.x);
+  c.it += 1;
+  if (!(c.res > tol) || (om.x == 0 && om.y == 0) || c.it >= max_iter) c.done = 1;
+  return c;
+}
+
+// One block: closes the iteration of record prev into record next (the last
+// iteration of a batch; within a batch the p update closes).
+__global__ void __launch_bounds__(256) bicg_finish(const BicgScalars* prev, BicgScalars* next,
+                                                   const double* xr_part, int nb, double* hist,
+                                                   double tol, int max_iter) {
+  const BicgScalars c = bicg_close(prev, xr_part, nb, tol, max_iter);
+  if (threadIdx.x == 0) {
+    if (!prev->done) hist[c.it] = c.res;
+    *next = c;
+  }
+}
+
+// p = r + beta (p - omega v) for the iteration of record cur.  CLOSE: first
+// close the previous iteration (record prev and its x/r partials) into cur
+// (block 0 writes it; every block computes the same scalars, and nothing
+// in this kernel reads cur).
+template <bool CLOSE>
+__global__ void __launch_bounds__(256) bicg_p(double* p, size_t n, const double* __restrict__ r,
+                                              const double* __restrict__ v,
+                                              const BicgScalars* prev, BicgScalars* cur,
+                                              const double* xr_part, int nb, double* hist,
+                                              double tol, int max_iter) {
+  BicgScalars c;
+  if (CLOSE) {
+    c = bicg_close(prev, xr_part, nb, tol, max_iter);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (!prev->done) hist[c.it] = c.res;
+      *cur = c;
+    }
+  } else {
+    c = *cur;
+  }
+  if (c.done) return;  // uniform
+  const double2 beta = zmul(zdiv(c.rho_new, c.rho_old), zdiv(c.alpha, c.omega));
+  const double2 bo = zmul(beta, c.omega);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double2 u = reinterpret_cast<const double2*>(r)[i];
+    const double2 w = reinterpret_cast<const double2*>(p)[i];
+    const double2 x = reinterpret_cast<const double2*>(v)[i];
+    // r + beta p - (beta omega) v, terms left to right
+    double ar = u.x, ai = u.y;
+    ar += beta.x * w.x - beta.y * w.y;
+    ai += beta.x * w.y + beta.y * w.x;
+    ar += -bo.x * x.x + bo.y * x.y;
+    ai += -bo.x * x.y - bo.y * x.x;
+    reinterpret_cast<double2*>(p)[i] = make_double2(ar, ai);
+  }
 }
 
 }  // namespace dev
@@ -758,17 +793,11 @@ hipError_t LaunchDotPartial(const double* u, const double* v, size_t n, double* 
   return hipGetLastError();
 }
 
-hipError_t LaunchBicgUpdate(int which, double* y, size_t n, const double* x0, const double* x1,
-                            const double* x2, BicgScalars* st, const double* rv_part, int nb,
-                            hipStream_t s) {
+hipError_t LaunchBicgS(double* y, size_t n, const double* r, const double* v, BicgScalars* st,
+                       const double* rv_part, int nb, hipStream_t s) {
   int blocks = (int)((n + 255) / 256);
   if (blocks > 8192) blocks = 8192;
-  if (which == 0)
-    hipLaunchKernelGGL(dev::bicg_update<0>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st,
-                       rv_part, nb);
-  else
-    hipLaunchKernelGGL(dev::bicg_update<1>, dim3(blocks), dim3(256), 0, s, y, n, x0, x1, x2, st,
-                       rv_part, nb);
+  hipLaunchKernelGGL(dev::bicg_s, dim3(blocks), dim3(256), 0, s, y, n, r, v, st, rv_part, nb);
   return hipGetLastError();
 }
 
@@ -787,10 +816,24 @@ hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
   return hipGetLastError();
 }
 
-hipError_t LaunchBicgFinish(BicgScalars* st, const double* xr_part, int nblocks, double* hist,
-                            double tol, int max_iter, hipStream_t s) {
-  hipLaunchKernelGGL(dev::bicg_finish, dim3(1), dim3(256), 0, s, st, xr_part, nblocks, hist, tol,
-                     max_iter);
+hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const double* xr_part,
+                            int nblocks, double* hist, double tol, int max_iter, hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_finish, dim3(1), dim3(256), 0, s, prev, next, xr_part, nblocks,
+                     hist, tol, max_iter);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBicgP(bool close, double* p, size_t n, const double* r, const double* v,
+                       const BicgScalars* prev, BicgScalars* cur, const double* xr_part,
+                       int nblocks, double* hist, double tol, int max_iter, hipStream_t s) {
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  if (close)
+    hipLaunchKernelGGL(dev::bicg_p<true>, dim3(blocks), dim3(256), 0, s, p, n, r, v, prev, cur,
+                       xr_part, nblocks, hist, tol, max_iter);
+  else
+    hipLaunchKernelGGL(dev::bicg_p<false>, dim3(blocks), dim3(256), 0, s, p, n, r, v, prev, cur,
+                       xr_part, nblocks, hist, tol, max_iter);
   return hipGetLastError();
 }
 
