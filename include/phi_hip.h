@@ -29,6 +29,8 @@
  *                                  1204-1466)
  *   phi_hip_rhs                    HierarchyUpdater::Get over all nodes
  *                                  (hierarchy_updater.h:36-52)
+ *   phi_hip_rhs_nodes              HierarchyUpdater::Get of given nodes on
+ *                                  the live hierarchy (same lines)
  *   phi_hip_rk4_steps              the body of Rk4Integrate's loop
  *                                  (hierarchy_integrator.cc:2677-2857)
  *   phi_hip_rkf45_attempt/_accept  one attempted step of Rkf45Integrate
@@ -232,6 +234,13 @@ int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n_complex);
  * no such limit).  Calls that overlap on one device share that array and
  * race: serialise them. */
 int phi_hip_rhs_noalloc(phi_ctx* c, const double* in, double* out, size_t n_complex);
+/* PHI's per-node HierarchyUpdater::Get(node, drho) (hierarchy_updater.h:36-52):
+ * drho[i] = the RHS of node nodes[i] (PHI id order) evaluated on the
+ * context's current state, as PHI's Get reads a node's live neighbours;
+ * count N x N blocks, column-major, n_complex = count * N^2.  The current
+ * flags (FilterTolerance), H and time-local matrices apply, as in
+ * phi_hip_rhs.  PHI_HIP_EINVAL for an id outside 0 .. num_nodes - 1. */
+int phi_hip_rhs_nodes(phi_ctx* c, const int* nodes, int count, double* drho, size_t n_complex);
 /* nsteps RK4 steps of size dt on the device state. */
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt);
 /* phi_hip_rk4_steps bracketed by two HIP events on the context's stream;

@@ -85,6 +85,7 @@ def lib():
         "phi_hip_get_rho0": (ctypes.c_int, [vp, dp, sz]),
         "phi_hip_rhs": (ctypes.c_int, [vp, dp, dp, sz]),
         "phi_hip_rhs_noalloc": (ctypes.c_int, [vp, dp, dp, sz]),
+        "phi_hip_rhs_nodes": (ctypes.c_int, [vp, ip, ctypes.c_int, dp, sz]),
         "phi_hip_rk4_steps": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double]),
         "phi_hip_rk4_stage_times": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
         "phi_hip_rk4_steps_timed": (ctypes.c_int, [vp, ctypes.c_longlong, ctypes.c_double, dp]),
@@ -400,6 +401,16 @@ class Context:
         out = np.zeros_like(f)
         _check(lib().phi_hip_rhs_noalloc(self._h, _dp(f), _dp(out), self.n_complex))
         return self._from_flat(out)
+
+    def rhs_nodes(self, nodes):
+        """PHI's per-node Get on the current state (phi_hip_rhs_nodes):
+        an array (len(nodes), N, N)."""
+        ids = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32).reshape(-1))
+        n = len(ids) * self.N * self.N
+        out = np.zeros(2 * n)
+        _check(lib().phi_hip_rhs_nodes(self._h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       len(ids), _dp(out), n))
+        return out.view(complex).reshape(len(ids), self.N, self.N).transpose(0, 2, 1).copy()
 
     def rk4_steps(self, nsteps, dt):
         _check(lib().phi_hip_rk4_steps(self._h, int(nsteps), float(dt)))

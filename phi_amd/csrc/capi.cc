@@ -702,6 +702,16 @@ int phi_hip_rhs(phi_ctx* c, const double* in, double* out, size_t n) {
   });
 }
 
+int phi_hip_rhs_nodes(phi_ctx* c, const int* nodes, int count, double* drho, size_t n) {
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !nodes || !drho || count < 1) return Err(PHI_HIP_EINVAL, "bad argument");
+    const size_t NE = (size_t)c->e->N() * c->e->N();
+    if (n != (size_t)count * NE) return Err(PHI_HIP_EINVAL, "output size is not count * N^2");
+    int rc = c->e->RhsNodes(nodes, count, reinterpret_cast<std::complex<double>*>(drho));
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
+}
+
 int phi_hip_rk4_steps(phi_ctx* c, long long nsteps, double dt) {
   return Guarded(__func__, [&]() -> int {
     if (!c || nsteps < 0) return Err(PHI_HIP_EINVAL, "bad argument");

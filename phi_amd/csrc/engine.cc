@@ -717,6 +717,38 @@ int Engine::RhsStaged(const std::complex<double>* x, std::complex<double>* y, do
   return rc;
 }
 
+int Engine::RhsNodes(const int* nodes, int count, std::complex<double>* y) {
+  if (members_) return Unsupported("the general RHS");
+  const int NE = h_.N * h_.N;
+  for (int i = 0; i < count; ++i)
+    if (nodes[i] < 0 || nodes[i] >= num_nodes()) {
+      err_ = "node id " + std::to_string(nodes[i]) + " outside 0.." +
+             std::to_string(num_nodes() - 1);
+      return kErrInvalidArgument;
+    }
+  const size_t bytes = state_elems() * sizeof(std::complex<double>);
+  const size_t out_bytes = (size_t)count * NE * sizeof(std::complex<double>);
+  double *dy = nullptr, *dout = nullptr;
+  int* dids = nullptr;
+  int rc = Check(hipMalloc((void**)&dy, bytes), "alloc rhs out");
+  if (!rc) rc = Check(hipMalloc((void**)&dout, out_bytes), "alloc node rhs");
+  if (!rc) rc = Check(hipMalloc((void**)&dids, count * sizeof(int)), "alloc node ids");
+  if (!rc)
+    rc = Check(hipMemcpyAsync(dids, nodes, count * sizeof(int), hipMemcpyHostToDevice, stream_),
+               "node ids upload");
+  // The general operator on the live state (current flags and H, as PHI's
+  // Get reads the node's linked neighbours), then the requested blocks.
+  if (!rc) rc = OperatorRhs(d_rho_, dy);
+  if (!rc) rc = Check(LaunchGatherNode0(dy, dids, count, NE, dout, stream_), "gather nodes");
+  if (!rc) rc = Check(hipMemcpyAsync(y, dout, out_bytes, hipMemcpyDeviceToHost, stream_),
+                      "node rhs download");
+  if (!rc) rc = Synchronize();
+  Free(dy);
+  Free(dout);
+  Free(dids);
+  return rc;
+}
+
 int Engine::StepForm() const {
   // The closed RHS is not linear: PHI's classic stage order then.
   if (closure_) return kRk4Classic;

@@ -131,6 +131,9 @@ class Engine {
   // state_elems() complex doubles on this engine's device); allocates
   // nothing.
   int RhsStaged(const std::complex<double>* x, std::complex<double>* y, double* dx, double* dy);
+  // PHI's per-node Get on the live hierarchy: the RHS of nodes[0..count-1]
+  // evaluated on the current device state, count N x N blocks into y.
+  int RhsNodes(const int* nodes, int count, std::complex<double>* y);
   // The device this engine's buffers and stream live on.
   int device() const { return device_; }
   // Same on device pointers (interleaved doubles).

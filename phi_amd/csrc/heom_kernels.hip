@@ -118,20 +118,26 @@ __global__ void __launch_bounds__(256) rhs_general(RhsArgs A, const double* __re
 }
 
 // partial[NC * block + c] = sum over the (256-thread) block of a[c], in a
-// fixed order.  Called by every thread of the block.
+// fixed order (shuffles within each wavefront, then the four wavefront sums
+// in order).  Called by every thread of the block.
 template <int NC>
 __device__ __forceinline__ void block_sum(const double* a, double* __restrict__ partial) {
-  __shared__ double sb[NC][256];
+  __shared__ double sw[4][NC];
+  double v[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) sb[c][threadIdx.x] = a[c];
+  for (int c = 0; c < NC; ++c) v[c] = a[c];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] += __shfl_xor(v[c], o, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sw[threadIdx.x >> 6][c] = v[c];
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) sb[c][threadIdx.x] += sb[c][threadIdx.x + s];
-    __syncthreads();
+  if ((int)threadIdx.x < NC) {
+    const int c = threadIdx.x;
+    partial[NC * blockIdx.x + c] = ((sw[0][c] + sw[1][c]) + sw[2][c]) + sw[3][c];
   }
-  if ((int)threadIdx.x < NC) partial[NC * blockIdx.x + threadIdx.x] = sb[threadIdx.x][0];
 }
 
 // The same with H, H^dagger and the ADOs of T nodes per block in LDS (one
@@ -405,46 +411,49 @@ __device__ __forceinline__ double2 bicg_omega(const BicgScalars* st) {
 
 // Sum of component c of nb partials (stride NC), fixed order; every
 // thread of the (256-thread) block gets the NC results.  Called by all.
+// Thread t sums records t, t + 256, ... (four loads in flight at a time),
+// each wavefront folds its 64 sums by shuffles, and the four wavefront sums
+// are added in wavefront order: the same order in every block, so every
+// block derives the same scalars from them.
 template <int NC>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ partial, int nb,
                                                 double* out) {
-  __shared__ double sp[NC][256];
-  __shared__ double so[NC];
+  __shared__ double sw[4][NC];
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) acc[c] = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x)
+  int i = threadIdx.x;
+  for (; i + 768 < nb; i += 1024) {
+    double v[4][NC];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[u][c] = partial[NC * (i + 256 * u) + c];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] += v[u][c];
+  }
+  for (; i < nb; i += 256)
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] += partial[NC * i + c];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) sp[c][threadIdx.x] = acc[c];
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
+  for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sp[c][threadIdx.x] += sp[c][threadIdx.x + s];
-    __syncthreads();
-  }
-  if ((int)threadIdx.x < NC) so[threadIdx.x] = sp[threadIdx.x][0];
+    for (int c = 0; c < NC; ++c) acc[c] += __shfl_xor(acc[c], o, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sw[threadIdx.x >> 6][c] = acc[c];
   __syncthreads();
 #pragma unroll
-  for (int c = 0; c < NC; ++c) out[c] = so[c];
+  for (int c = 0; c < NC; ++c) out[c] = ((sw[0][c] + sw[1][c]) + sw[2][c]) + sw[3][c];
 }
 
 // Block sum of three values per thread into partial[3 b + c].
 __device__ __forceinline__ void block_sum3(double a0, double a1, double a2,
                                            double* __restrict__ partial) {
-  __shared__ double s3[3][256];
-  s3[0][threadIdx.x] = a0;
-  s3[1][threadIdx.x] = a1;
-  s3[2][threadIdx.x] = a2;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int c = 0; c < 3; ++c) s3[c][threadIdx.x] += s3[c][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = s3[threadIdx.x][0];
+  const double a[3] = {a0, a1, a2};
+  block_sum<3>(a, partial);
 }
 
 // s = r - alpha v, after reducing the partials of rhat . v into rv (block 0
@@ -461,13 +470,21 @@ __global__ void __launch_bounds__(256) bicg_s(double* __restrict__ y, size_t n,
   const double2 c1 = make_double2(-al.x, -al.y);
   const double2* a = reinterpret_cast<const double2*>(x0);
   const double2* b = reinterpret_cast<const double2*>(x1);
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const double2 u = a[i], v = b[i];
-    double ar = u.x, ai = u.y;
-    ar += c1.x * v.x - c1.y * v.y;
-    ai += c1.x * v.y + c1.y * v.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // two elements per trip (a thread past n reads element n - 1, unused)
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+    const size_t j = i + stride < n ? i + stride : n - 1;
+    const double2 u0 = a[i], v0 = b[i], u1 = a[j], v1 = b[j];
+    double ar = u0.x, ai = u0.y;
+    ar += c1.x * v0.x - c1.y * v0.y;
+    ai += c1.x * v0.y + c1.y * v0.x;
     reinterpret_cast<double2*>(y)[i] = make_double2(ar, ai);
+    if (i + stride < n) {
+      double br = u1.x, bi = u1.y;
+      br += c1.x * v1.x - c1.y * v1.y;
+      bi += c1.x * v1.y + c1.y * v1.x;
+      reinterpret_cast<double2*>(y)[j] = make_double2(br, bi);
+    }
   }
 }
 
@@ -508,27 +525,37 @@ __global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* _
   const double2 al = zdiv(st->rho_new, st->rv);
   const double2 om = tt.x > 0 ? zdiv(ts, tt) : make_double2(0., 0.);
   double acc = 0, hr = 0, hi = 0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const double2 xv = reinterpret_cast<const double2*>(x)[i];
-    const double2 pv = reinterpret_cast<const double2*>(p)[i];
-    const double2 s = reinterpret_cast<const double2*>(sv)[i];
-    const double2 tv = reinterpret_cast<const double2*>(t)[i];
-    // x = 1 x + alpha p + omega s ; r = 1 s - omega t (left to right)
-    double xr = xv.x, xi = xv.y;
-    xr += al.x * pv.x - al.y * pv.y;
-    xi += al.x * pv.y + al.y * pv.x;
-    xr += om.x * s.x - om.y * s.y;
-    xi += om.x * s.y + om.y * s.x;
-    reinterpret_cast<double2*>(x)[i] = make_double2(xr, xi);
-    double rr = s.x, ri = s.y;
-    rr += -om.x * tv.x + om.y * tv.y;
-    ri += -om.x * tv.y - om.y * tv.x;
-    reinterpret_cast<double2*>(r)[i] = make_double2(rr, ri);
-    acc += rr * rr + ri * ri;
-    const double2 h = reinterpret_cast<const double2*>(rhat)[i];
-    hr += h.x * rr + h.y * ri;
-    hi += h.x * ri - h.y * rr;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // two elements per trip (a thread past n reads element n - 1, unused)
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+    const size_t j[2] = {i, i + stride < n ? i + stride : n - 1};
+    double2 xv[2], pv[2], s[2], tv[2], h[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      xv[e] = reinterpret_cast<const double2*>(x)[j[e]];
+      pv[e] = reinterpret_cast<const double2*>(p)[j[e]];
+      s[e] = reinterpret_cast<const double2*>(sv)[j[e]];
+      tv[e] = reinterpret_cast<const double2*>(t)[j[e]];
+      h[e] = reinterpret_cast<const double2*>(rhat)[j[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      if (e == 1 && !(i + stride < n)) break;
+      // x = 1 x + alpha p + omega s ; r = 1 s - omega t (left to right)
+      double xr = xv[e].x, xi = xv[e].y;
+      xr += al.x * pv[e].x - al.y * pv[e].y;
+      xi += al.x * pv[e].y + al.y * pv[e].x;
+      xr += om.x * s[e].x - om.y * s[e].y;
+      xi += om.x * s[e].y + om.y * s[e].x;
+      reinterpret_cast<double2*>(x)[j[e]] = make_double2(xr, xi);
+      double rr = s[e].x, ri = s[e].y;
+      rr += -om.x * tv[e].x + om.y * tv[e].y;
+      ri += -om.x * tv[e].y - om.y * tv[e].x;
+      reinterpret_cast<double2*>(r)[j[e]] = make_double2(rr, ri);
+      acc += rr * rr + ri * ri;
+      hr += h[e].x * rr + h[e].y * ri;
+      hi += h[e].x * ri - h[e].y * rr;
+    }
   }
   block_sum3(acc, hr, hi, partial);
 }
@@ -592,18 +619,27 @@ __global__ void __launch_bounds__(256) bicg_p(double* p, size_t n, const double*
   if (c.done) return;  // uniform
   const double2 beta = zmul(zdiv(c.rho_new, c.rho_old), zdiv(c.alpha, c.omega));
   const double2 bo = zmul(beta, c.omega);
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const double2 u = reinterpret_cast<const double2*>(r)[i];
-    const double2 w = reinterpret_cast<const double2*>(p)[i];
-    const double2 x = reinterpret_cast<const double2*>(v)[i];
-    // r + beta p - (beta omega) v, terms left to right
-    double ar = u.x, ai = u.y;
-    ar += beta.x * w.x - beta.y * w.y;
-    ai += beta.x * w.y + beta.y * w.x;
-    ar += -bo.x * x.x + bo.y * x.y;
-    ai += -bo.x * x.y - bo.y * x.x;
-    reinterpret_cast<double2*>(p)[i] = make_double2(ar, ai);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // two elements per trip (a thread past n reads element n - 1, unused)
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+    const size_t j[2] = {i, i + stride < n ? i + stride : n - 1};
+    double2 u[2], w[2], x[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      u[e] = reinterpret_cast<const double2*>(r)[j[e]];
+      w[e] = reinterpret_cast<const double2*>(p)[j[e]];
+      x[e] = reinterpret_cast<const double2*>(v)[j[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      // r + beta p - (beta omega) v, terms left to right
+      double ar = u[e].x, ai = u[e].y;
+      ar += beta.x * w[e].x - beta.y * w[e].y;
+      ai += beta.x * w[e].y + beta.y * w[e].x;
+      ar += -bo.x * x[e].x + bo.y * x[e].y;
+      ai += -bo.x * x[e].y - bo.y * x[e].x;
+      if (e == 0 || i + stride < n) reinterpret_cast<double2*>(p)[j[e]] = make_double2(ar, ai);
+    }
   }
 }
 
@@ -793,10 +829,15 @@ hipError_t LaunchDotPartial(const double* u, const double* v, size_t n, double* 
   return hipGetLastError();
 }
 
+// The BiCGSTAB update kernels run at most kBicgBlocks blocks (each block
+// first reduces the partials of the dot product it needs; fewer blocks
+// re-read them fewer times).
+constexpr int kBicgBlocks = 512;
+
 hipError_t LaunchBicgS(double* y, size_t n, const double* r, const double* v, BicgScalars* st,
                        const double* rv_part, int nb, hipStream_t s) {
-  int blocks = (int)((n + 255) / 256);
-  if (blocks > 8192) blocks = 8192;
+  int blocks = (int)((n + 511) / 512);
+  if (blocks > kBicgBlocks) blocks = kBicgBlocks;
   hipLaunchKernelGGL(dev::bicg_s, dim3(blocks), dim3(256), 0, s, y, n, r, v, st, rv_part, nb);
   return hipGetLastError();
 }
@@ -826,8 +867,8 @@ hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const do
 hipError_t LaunchBicgP(bool close, double* p, size_t n, const double* r, const double* v,
                        const BicgScalars* prev, BicgScalars* cur, const double* xr_part,
                        int nblocks, double* hist, double tol, int max_iter, hipStream_t s) {
-  int blocks = (int)((n + 255) / 256);
-  if (blocks > 8192) blocks = 8192;
+  int blocks = (int)((n + 511) / 512);
+  if (blocks > kBicgBlocks) blocks = kBicgBlocks;
   if (close)
     hipLaunchKernelGGL(dev::bicg_p<true>, dim3(blocks), dim3(256), 0, s, p, n, r, v, prev, cur,
                        xr_part, nblocks, hist, tol, max_iter);

@@ -111,6 +111,37 @@ def test_rhs_matches_oracle_c2_subset():
     assert np.abs(y_gpu - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
 
 
+def test_rhs_nodes_is_per_node_get_on_live_state():
+    """phi_hip_rhs_nodes (PHI's per-node Get) on the context's current state:
+    equal to the batched RHS of that state at the listed nodes and to the
+    oracle's Get of those nodes; ids outside the hierarchy are rejected."""
+    path = os.path.join(CFG, "c2.prm")
+    op = PO.read_command_line(path, "rk4")
+    op.hierarchy_truncation_level = 5
+    orc = HO.Oracle(HO.Problem.from_prm(op), threads=4)
+    p = P.PhiParameters.read(path, "rk4")
+    p.set_int("hierarchy_truncation_level", 5)
+    ctx = P.Context(p)
+    rng = np.random.default_rng(13)
+    x = rng.normal(size=(330, 7, 7)) + 1j * rng.normal(size=(330, 7, 7))
+    ctx.set_state(x)
+    ids = [0, 1, 7, 36, 329, 36, 120]
+    got = ctx.rhs_nodes(ids)
+    assert got.shape == (len(ids), 7, 7)
+    batched = ctx.rhs(x)
+    assert np.array_equal(got, batched[ids])
+    ref = orc.rhs(x)[ids]
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the live state: after RK4 steps the same call sees the new state
+    ctx.rk4_steps(3, op.timestep)
+    st = ctx.state()
+    assert np.array_equal(ctx.rhs_nodes([5, 0]), ctx.rhs(st)[[5, 0]])
+    for bad in ([330], [-1], [0, 400]):
+        with pytest.raises(P.PhiError):
+            ctx.rhs_nodes(bad)
+    ctx.close()
+
+
 def test_rk4_c1_full_run_fp64(tmp_path):
     """c1 (495 nodes, 1000 RK4 steps): every output time within 1e-10."""
     path = os.path.join(CFG, "c1.prm")
