@@ -8,7 +8,7 @@ namespace phi {
 
 // The closure network packed for the fused inference pass: weights as
 // bfloat16 rows (out x K, row-major; K = k1 for layer 1 and hid for layers
-// 2-3, rows padded to multiples of 64 with zeros), biases float32.
+// 2-3, rows padded to multiples of 256 with zeros), biases float32.
 struct FusedNet {
   const unsigned short* w1;
   const unsigned short* w2;

@@ -127,68 +127,107 @@ __global__ void __launch_bounds__(256) scatter(const float* __restrict__ Y,
 // One block of 512 threads (8 wavefronts) takes kRows = 128 rows end to
 // end: network inputs from the stage input (as `inputs`), the three layers
 // on the matrix cores (v_mfma_f32_16x16x32_bf16; activations in LDS as
-// bfloat16, weights packed as bfloat16 rows streamed from L2), and the
+// bfloat16, each hidden layer's output written over its input), and the
 // scatter of the outputs into the virtual slots (as `scatter`).  Same
 // arithmetic as the unfused path: operands rounded to bfloat16, float32
 // sums in the same K order, bias and GELU in float32.
 //
-// A layer's output tile of a wavefront is 64 rows x 64 columns (4 x 4 MFMA
-// tiles): wave w takes row half w & 1 and column chunks w >> 1, + 4, ...
-// Each weight fragment feeds 4 row tiles (128 rows share one read of the
-// weights), and the fragments of the next K step are loaded while the
-// current step's 16 products run.
+// A layer runs over groups of 256 output columns (the packed weights have
+// 256-row multiples, zero padded).  Within a group, wavefront w owns the
+// 64 x 64 output tile of row half w & 1 and column chunk w >> 1.  The
+// group's weights stream through LDS in K steps of 32: each thread loads
+// two 16-byte pieces of every step (coalesced, two steps ahead in
+// registers) and stores them in MFMA B-fragment order, piece (row, k-octet
+// o) at fragment (row / 16) * 64 + 16 o + row % 16, so a wavefront reads
+// each of its four fragments as one contiguous 1 KB block and every weight
+// is fetched once per block.
 constexpr int kRows = 128;
 constexpr int kThreads = 512;
+constexpr int kGroup = 256;
+constexpr int kStepK = 32;
+constexpr int kStageFrags = kGroup * kStepK / 8;  // 16-byte fragments per stage (16 KB)
+constexpr size_t kStageBytes = (size_t)2 * kStageFrags * 16;
 
 template <class EPI>
-__device__ __forceinline__ void fused_layer(const unsigned short* In, int stride_in,
+__device__ __forceinline__ void fused_layer(const unsigned short* act, int stride,
                                             const unsigned short* __restrict__ W, int K,
                                             int ncols, const float* __restrict__ bias,
-                                            const EPI& epi) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                                            bf16x8* stage, const EPI& epi) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, c = lane & 15;
-  const int rh = wave & 1;
-  for (int chunk = wave >> 1; chunk * 64 < ncols; chunk += 4) {
+  const int rh = wave & 1, cw = wave >> 1;
+  const int nsteps = K / kStepK;  // even: K is a multiple of 64
+  // This thread's two pieces of every stage.
+  int poff[2], pfrag[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int pc = tid + kThreads * j, row = pc >> 2, oct = pc & 3;
+    poff[j] = row * K + 8 * oct;
+    pfrag[j] = (row >> 4) * 64 + 16 * oct + (row & 15);
+  }
+  const unsigned short* arow = act + (64 * rh + c) * stride + 8 * q;
+  for (int g0 = 0; g0 < ncols; g0 += kGroup) {
+    const unsigned short* Wg = W + (size_t)g0 * K;
+    const bool busy = g0 + 64 * cw < ncols;  // uniform over the wavefront
     floatx4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[a][b] = (floatx4){0.f, 0.f, 0.f, 0.f};
-    const unsigned short* wrow = W + (size_t)(chunk * 64 + c) * K + 8 * q;
-    const unsigned short* arow = In + (64 * rh + c) * stride_in + 8 * q;
-    // Two weight buffers alternate over K steps of 32 (K is a multiple of
-    // 64): the next step's fragments load while this step's 16 products run.
-    bf16x8 b0[4], b1[4];
+    auto load = [&](bf16x8* r, int s) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) b0[b] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * b * K);
-    auto step = [&](int k0, const bf16x8* bv, bf16x8* bnext, int kn) {
+      for (int j = 0; j < 2; ++j)
+        r[j] = *reinterpret_cast<const bf16x8*>(Wg + poff[j] + s * kStepK);
+    };
+    auto put = [&](const bf16x8* r, int s) {
+      bf16x8* st = stage + (s & 1) * kStageFrags;
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
-        bnext[b] = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * b * K + kn);
-      bf16x8 av[4];
+      for (int j = 0; j < 2; ++j) st[pfrag[j]] = r[j];
+    };
+    auto compute = [&](int s) {
+      if (!busy) return;
+      const bf16x8* st = stage + (s & 1) * kStageFrags + 4 * cw * 64 + lane;
+      bf16x8 bv[4], av[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = st[64 * b];
 #pragma unroll
       for (int a = 0; a < 4; ++a)
-        av[a] = *reinterpret_cast<const bf16x8*>(arow + 16 * a * stride_in + k0);
+        av[a] = *reinterpret_cast<const bf16x8*>(arow + 16 * a * stride + s * kStepK);
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
     };
-    for (int k0 = 0; k0 < K; k0 += 64) {
-      step(k0, b0, b1, k0 + 32);
-      // (the last step reloads its own fragments: no branch around loads)
-      step(k0 + 32, b1, b0, k0 + 64 < K ? k0 + 64 : k0 + 32);
+    // Step s computes from stage s & 1 while step s + 1 waits in registers
+    // to be stored into the other stage and step s + 2 loads (steps past
+    // the end reload the last one: no branch around loads).
+    bf16x8 r0[2], r1[2];
+    load(r0, 0);
+    load(r1, 1);
+    put(r0, 0);
+    __syncthreads();
+    for (int s = 0; s < nsteps; s += 2) {
+      load(r0, min(s + 2, nsteps - 1));
+      compute(s);
+      put(r1, s + 1);
+      __syncthreads();
+      load(r1, min(s + 3, nsteps - 1));
+      compute(s + 1);
+      if (s + 2 < nsteps) put(r0, s + 2);
+      __syncthreads();
     }
+    if (busy) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int col = chunk * 64 + 16 * b + c;
-      if (col < ncols) {
-        const float bc = bias[col];
+      for (int b = 0; b < 4; ++b) {
+        const int col = g0 + 64 * cw + 16 * b + c;
+        if (col < ncols) {
+          const float bc = bias[col];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+          for (int a = 0; a < 4; ++a)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) epi(64 * rh + 16 * a + 4 * q + r, col, acc[a][b][r] + bc);
+            for (int r = 0; r < 4; ++r) epi(64 * rh + 16 * a + 4 * q + r, col, acc[a][b][r] + bc);
+        }
       }
     }
   }
@@ -234,8 +273,10 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
   __shared__ unsigned smax[kRows];
   __shared__ int rowbad[kRows];
   const int stride = net.width + 8;
+  // The activations (each layer's output over its input), then the two
+  // weight stages.
   unsigned short* A0 = lds;
-  unsigned short* A1 = lds + kRows * stride;
+  bf16x8* stage = reinterpret_cast<bf16x8*>(lds + kRows * stride);
   const long long R = (long long)members * nb;
   const long long row0 = (long long)blockIdx.x * kRows;
   const int tid = threadIdx.x;
@@ -303,11 +344,16 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
     A0[r * stride + 2 * ne + k] = val;
   }
   __syncthreads();
-  fused_layer(A0, stride, net.w1, net.k1, net.hid, net.b1, ToLds{A1, stride});
-  __syncthreads();
-  fused_layer(A1, stride, net.w2, net.hid, net.hid, net.b2, ToLds{A0, stride});
-  __syncthreads();
-  fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3,
+  // Hidden layers write their output over their input: every wavefront has
+  // finished reading it at the layer's last barrier.  One copy of the layer
+  // code serves both.
+#pragma unroll 1
+  for (int layer = 0; layer < 2; ++layer) {
+    fused_layer(A0, stride, layer ? net.w2 : net.w1, layer ? net.hid : net.k1, net.hid,
+                layer ? net.b2 : net.b1, stage, ToLds{A0, stride});
+    __syncthreads();
+  }
+  fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3, stage,
               ToSlots<TI>{slab, rowbase, srow, rowbad, guard.ymax});
   // Rows whose outputs failed the guard fall back to plain truncation: zero
   // slots (a second write after every wavefront's first one).
@@ -324,14 +370,16 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
 }  // namespace closure
 
 size_t FusedClosureLds(const FusedNet& net) {
-  return (size_t)2 * closure::kRows * (net.width + 8) * sizeof(unsigned short);
+  return (size_t)closure::kRows * (net.width + 8) * sizeof(unsigned short) +
+         closure::kStageBytes;
 }
 
 bool FusedClosureFits(const FusedNet& net) {
-  // Both activation buffers in one CU's 160 KB of LDS, with the block's
-  // per-row tables; K steps in pairs of 32.
-  return FusedClosureLds(net) + closure::kRows * 24 <= (size_t)160 * 1024 &&
-         net.k1 % 64 == 0 && net.hid % 64 == 0;
+  // The activations and two weight stages in one CU's 160 KB of LDS, with
+  // the block's per-row tables; K steps in pairs of 32; hidden layers of
+  // one column group (their output overwrites their input).
+  return FusedClosureLds(net) + closure::kRows * 32 <= (size_t)160 * 1024 &&
+         net.k1 % 64 == 0 && net.hid % 64 == 0 && net.hid <= closure::kGroup;
 }
 
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
@@ -340,8 +388,7 @@ hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const 
   const long long rows = (long long)members * nb;
   const unsigned blocks = (unsigned)((rows + closure::kRows - 1) / closure::kRows);
   const size_t lds = FusedClosureLds(net);
-  if (!FusedClosureFits(net) || ne > 64 || net.k1 % 64 != 0 || net.hid % 64 != 0)
-    return hipErrorInvalidConfiguration;
+  if (!FusedClosureFits(net) || ne > 64) return hipErrorInvalidConfiguration;
   const void* k = fp32 ? reinterpret_cast<const void*>(&closure::fused<float>)
                        : reinterpret_cast<const void*>(&closure::fused<double>);
   hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

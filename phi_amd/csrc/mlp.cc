@@ -110,9 +110,11 @@ int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm) {
   if (rc) return rc;
   const int k1 = (n_in_ + 31) / 32 * 32;
   const int hid = n_hid_;
-  auto rows64 = [](int n) { return (n + 63) / 64 * 64; };
-  const size_t n1 = (size_t)rows64(hid) * k1, n2 = (size_t)rows64(hid) * hid,
-               n3 = (size_t)rows64(n_out_) * hid;
+  // Rows padded to multiples of 256: the fused pass streams whole groups of
+  // 256 output columns.
+  auto rows256 = [](int n) { return (n + 255) / 256 * 256; };
+  const size_t n1 = (size_t)rows256(hid) * k1, n2 = (size_t)rows256(hid) * hid,
+               n3 = (size_t)rows256(n_out_) * hid;
   // One buffer: the bfloat16 weights, then (16-byte aligned) the float32
   // biases b1, b2, b3.
   const size_t nw = (n1 + n2 + n3 + 7) / 8 * 8;
