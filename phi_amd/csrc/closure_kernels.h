@@ -34,10 +34,12 @@ struct ClosureGuard {
   unsigned long long* fallen;
 };
 
-// Inputs, the three layers and the scatter in one launch (128 rows per block).
+// Inputs, the three layers and the scatter in one launch (128 rows per block);
+// slotoff[o] = float offset of packed output o from a row's first slot.
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
                               int members, int first, int nb, int ne, int D, const float* bmi,
-                              int vbase, const ClosureGuard& guard, hipStream_t s);
+                              int vbase, const int* slotoff, const ClosureGuard& guard,
+                              hipStream_t s);
 
 // Network inputs X (rows x (2 N^2 + D)) and scales S of the boundary nodes
 // first .. first + nb - 1 of every member, from a stage input in FP32 or
@@ -49,10 +51,12 @@ hipError_t LaunchClosureInputs(const void* in, bool in_fp32, const int* moff, in
 hipError_t LaunchClosureTargets(const double* rho, const int* moff, int members, int nb, int ne,
                                 int D, const int* children, const float* S, float* T,
                                 hipStream_t s);
-// Network outputs Y -> virtual neighbour slots vbase + b*D + j of each member.
+// Network outputs Y (rows x 2 N^2 D) -> column site[j] of the virtual
+// neighbour slots vbase + b*D + j of each member (the column the stage
+// kernels read).
 hipError_t LaunchClosureScatter(const float* Y, const float* S, const int* moff, int members,
-                                int nb, int ne, int D, int vbase, void* out, bool out_fp32,
-                                const ClosureGuard& guard, hipStream_t s);
+                                int nb, int N, int D, const int* site, int vbase, void* out,
+                                bool out_fp32, const ClosureGuard& guard, hipStream_t s);
 
 }  // namespace phi
 

@@ -7,6 +7,11 @@
 // Input row:  [Re rho_n / s (N^2, column-major), Im rho_n / s (N^2), n / L (D)]
 // Target/output row: for j = 0 .. D-1: [Re rho_{n+e_j} / s (N^2), Im (N^2)]
 // with s = the largest |Re| or |Im| entry of rho_n (at least 1e-30), per row.
+// The stage kernels read a next-tier neighbour n + e_j of n only in column
+// site(j) (element (a, b) of node n gathers element (a, b) of n + e_j for
+// the modes j of site b), so at run time only those outputs are evaluated
+// (the fused pass) or written (the scatter): 2 N D of the 2 N^2 D; the rest
+// of a virtual slot is never read.
 #include <hip/hip_runtime.h>
 
 #include "closure_kernels.h"
@@ -91,34 +96,40 @@ __global__ void __launch_bounds__(256) targets(const double* __restrict__ rho,
   }
 }
 
-// Predicted next-tier ADOs -> virtual slots: virtual node vbase + b*D + j of
-// member m holds rho_{n+e_j} of boundary node b.
+// Network outputs -> virtual slots: virtual node vbase + b*D + j of member m
+// holds rho_{n+e_j} of boundary node b, of which column site[j] (the read
+// one) is written; the guard checks those outputs.
 template <typename TO>
 __global__ void __launch_bounds__(256) scatter(const float* __restrict__ Y,
                                                const float* __restrict__ S,
                                                const int* __restrict__ moff, int members,
-                                               int nb, int ne, int D, int vbase,
+                                               int nb, int N, int D,
+                                               const int* __restrict__ site, int vbase,
                                                TO* __restrict__ out, ClosureGuard guard) {
   typedef typename C2T<TO>::type V;
   __shared__ int bad;
   const long long row = blockIdx.x;
   if (row >= (long long)members * nb) return;
   const int m = (int)(row / nb), b = (int)(row - (long long)m * nb);
+  const int ne = N * N;
   const float s = S[row];
   const float* y = Y + row * (size_t)(2 * ne * D);
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  for (int e = threadIdx.x; e < 2 * ne * D; e += blockDim.x)
-    if (!(fabsf(y[e]) <= guard.ymax)) bad = 1;
+  for (int e = threadIdx.x; e < 2 * N * D; e += blockDim.x) {
+    const int j = e / (2 * N), k = e - j * 2 * N, part = k / N, w = k - part * N;
+    if (!(fabsf(y[(size_t)j * 2 * ne + part * ne + site[j] * N + w]) <= guard.ymax)) bad = 1;
+  }
   __syncthreads();
   const float keep = bad ? 0.f : 1.f;
   V* dst = reinterpret_cast<V*>(out) + ((size_t)moff[m] + vbase + (size_t)b * D) * ne;
-  for (int e = threadIdx.x; e < ne * D; e += blockDim.x) {
-    const int j = e / ne, w = e - j * ne;
+  for (int e = threadIdx.x; e < N * D; e += blockDim.x) {
+    const int j = e / N, w = e - j * N;
+    const size_t o = (size_t)j * 2 * ne + (size_t)site[j] * N + w;
     V v;
-    v.x = (TO)(keep * y[(size_t)j * 2 * ne + w] * s);
-    v.y = (TO)(keep * y[(size_t)j * 2 * ne + ne + w] * s);
-    dst[(size_t)j * ne + w] = v;
+    v.x = (TO)(keep * y[o] * s);
+    v.y = (TO)(keep * y[o + ne] * s);
+    dst[(size_t)j * ne + (size_t)site[j] * N + w] = v;
   }
   if (threadIdx.x == 0 && bad) atomicAdd(guard.fallen, 1ULL);
 }
@@ -241,20 +252,23 @@ struct ToLds {
   }
 };
 
-// Outputs straight into the virtual slots: the packed output neurons are in
-// slot order (output 2 (j ne + e) + part is element e of slot j, real or
-// imaginary part), so output col of a row is float col of its slots.
+// Outputs straight into the virtual slots: the packed output neurons are
+// the network's outputs the stage kernels read, in slot order (packed output
+// 2 (j N + w) + part is element w of column site(j) of slot j, real or
+// imaginary part), and slotoff[col] is the float offset of packed output
+// col from the row's first slot.
 template <typename TO>
 struct ToSlots {
   TO* slab;
   const long long* rowbase;  // per block row: first virtual slot's real offset, or -1
   const float* srow;
   int* rowbad;               // per block row: an output failed the guard
+  const int* slotoff;
   float ymax;
   __device__ __forceinline__ void operator()(int row, int col, float v) const {
     const long long base = rowbase[row];
     if (base < 0) return;
-    slab[base + col] = (TO)(v * srow[row]);
+    slab[base + slotoff[col]] = (TO)(v * srow[row]);
     if (!(fabsf(v) <= ymax)) rowbad[row] = 1;
   }
 };
@@ -264,6 +278,7 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
                                                   const int* __restrict__ moff, int members,
                                                   int first, int nb, int ne, int D,
                                                   const float* __restrict__ bmi, int vbase,
+                                                  const int* __restrict__ slotoff,
                                                   ClosureGuard guard) {
   typedef typename C2T<TI>::type V;
   extern __shared__ __align__(16) unsigned short lds[];
@@ -354,7 +369,7 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
     __syncthreads();
   }
   fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3, stage,
-              ToSlots<TI>{slab, rowbase, srow, rowbad, guard.ymax});
+              ToSlots<TI>{slab, rowbase, srow, rowbad, slotoff, guard.ymax});
   // Rows whose outputs failed the guard fall back to plain truncation: zero
   // slots (a second write after every wavefront's first one).
   __syncthreads();
@@ -362,7 +377,7 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
   for (int r = 0; r < kRows; ++r) {
     if (!rowbad[r]) continue;
     ++nbad;
-    for (int col = tid; col < net.n_out; col += kThreads) slab[rowbase[r] + col] = (TI)0;
+    for (int col = tid; col < net.n_out; col += kThreads) slab[rowbase[r] + slotoff[col]] = (TI)0;
   }
   if (tid == 0 && nbad) atomicAdd(guard.fallen, (unsigned long long)nbad);
 }
@@ -384,7 +399,8 @@ bool FusedClosureFits(const FusedNet& net) {
 
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
                               int members, int first, int nb, int ne, int D, const float* bmi,
-                              int vbase, const ClosureGuard& guard, hipStream_t s) {
+                              int vbase, const int* slotoff, const ClosureGuard& guard,
+                              hipStream_t s) {
   const long long rows = (long long)members * nb;
   const unsigned blocks = (unsigned)((rows + closure::kRows - 1) / closure::kRows);
   const size_t lds = FusedClosureLds(net);
@@ -395,10 +411,10 @@ hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const 
   if (e != hipSuccess) return e;
   if (fp32)
     hipLaunchKernelGGL(closure::fused<float>, dim3(blocks), dim3(closure::kThreads), lds, s, net,
-                       (float*)slab, moff, members, first, nb, ne, D, bmi, vbase, guard);
+                       (float*)slab, moff, members, first, nb, ne, D, bmi, vbase, slotoff, guard);
   else
     hipLaunchKernelGGL(closure::fused<double>, dim3(blocks), dim3(closure::kThreads), lds, s, net,
-                       (double*)slab, moff, members, first, nb, ne, D, bmi, vbase, guard);
+                       (double*)slab, moff, members, first, nb, ne, D, bmi, vbase, slotoff, guard);
   return hipGetLastError();
 }
 
@@ -426,15 +442,15 @@ hipError_t LaunchClosureTargets(const double* rho, const int* moff, int members,
 }
 
 hipError_t LaunchClosureScatter(const float* Y, const float* S, const int* moff, int members,
-                                int nb, int ne, int D, int vbase, void* out, bool out_fp32,
-                                const ClosureGuard& guard, hipStream_t s) {
+                                int nb, int N, int D, const int* site, int vbase, void* out,
+                                bool out_fp32, const ClosureGuard& guard, hipStream_t s) {
   const long long rows = (long long)members * nb;
   if (out_fp32)
     hipLaunchKernelGGL(closure::scatter<float>, dim3((unsigned)rows), dim3(256), 0, s, Y, S,
-                       moff, members, nb, ne, D, vbase, (float*)out, guard);
+                       moff, members, nb, N, D, site, vbase, (float*)out, guard);
   else
     hipLaunchKernelGGL(closure::scatter<double>, dim3((unsigned)rows), dim3(256), 0, s, Y, S,
-                       moff, members, nb, ne, D, vbase, (double*)out, guard);
+                       moff, members, nb, N, D, site, vbase, (double*)out, guard);
   return hipGetLastError();
 }
 

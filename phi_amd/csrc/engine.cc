@@ -388,7 +388,7 @@ Engine::~Engine() {
                   d_prev_col_, d_prev_row_, d_delta_, d_H_, d_Hraw_, d_Hadj_, d_edge_, d_rho_, d_aux_,
                   d_y_[0], d_y_[1], d_k_[0], d_k_[1], d_k_[2], d_k_[3], d_k_[4], d_tmp_,
                   d_err_, d_hist_, d_hist_count_, d_partial_, d_dot_, d_moff_, d_Hm_,
-                  d_tiles_, d_cbmi_, d_cx_, d_cs_, d_cy_, d_cpack_, d_cfall_, d_active_, d_kf_[0], d_kf_[1],
+                  d_tiles_, d_cbmi_, d_csite_, d_cslot_, d_cx_, d_cs_, d_cy_, d_cpack_, d_cfall_, d_active_, d_kf_[0], d_kf_[1],
                   d_kf_[2], d_kf_[3], d_kf_[4], d_kf_[5], d_rot_,
                   d_ops_, d_alpha_, d_corr_, d_tlq_, d_tlsite_, d_mrho_};
   for (void* p : ptrs) Free(p);

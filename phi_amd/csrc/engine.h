@@ -307,6 +307,8 @@ class Engine {
   unsigned long long* d_cfall_ = nullptr;
   int closure_first_ = 0, closure_nb_ = 0, closure_vbase_ = 0;
   float* d_cbmi_ = nullptr;
+  int* d_csite_ = nullptr;   // site of each mode (the slot column the stage kernels read)
+  int* d_cslot_ = nullptr;   // per packed output: float offset from a row's first slot
   float* d_cx_ = nullptr;
   float* d_cs_ = nullptr;
   float* d_cy_ = nullptr;

@@ -138,16 +138,24 @@ int Engine::AttachClosure(Mlp* mlp) {
   int* dmoff = nullptr;
   int4* edge = nullptr;
   float *cbmi = nullptr, *cx = nullptr, *cs = nullptr, *cy = nullptr;
+  int *csite = nullptr, *cslot = nullptr;
   unsigned long long* cfall = nullptr;
-  // The fused pass writes the network's outputs straight into the virtual
-  // slots, interleaved complex: packed output 2 (j ne + e) + part is the
-  // network's output j 2 ne + part ne + e (slot j, element e).
-  std::vector<int> perm((size_t)2 * NE * D);
+  // The stage kernels read slot j only in column site(j), so the fused pass
+  // evaluates just those 2 N D of the network's 2 N^2 D outputs and writes
+  // them straight into the slots, interleaved complex: packed output
+  // 2 (j N + w) + part is the network's output j 2 N^2 + part N^2 +
+  // site(j) N + w (element w of that column), at float offset
+  // 2 (j N^2 + site(j) N + w) + part from the row's first slot.
+  std::vector<int> perm((size_t)2 * N * D), slotoff((size_t)2 * N * D);
   for (int j = 0; j < D; ++j)
-    for (int e = 0; e < NE; ++e)
-      for (int part = 0; part < 2; ++part)
-        perm[(size_t)2 * (j * NE + e) + part] = j * 2 * NE + part * NE + e;
-  int rc = mlp->Pack(&fnet, &pack, perm.data());
+    for (int w = 0; w < N; ++w)
+      for (int part = 0; part < 2; ++part) {
+        const size_t o = (size_t)2 * (j * N + w) + part;
+        const int e = h.site_of_mode[j] * N + w;  // column-major element
+        perm[o] = j * 2 * NE + part * NE + e;
+        slotoff[o] = 2 * (j * NE + e) + part;
+      }
+  int rc = mlp->Pack(&fnet, &pack, perm.data(), (int)perm.size());
   if (rc) {
     err_ = "closure network: " + mlp->last_error();
     return rc == kErrUnsupported ? kErrInvalidArgument : rc;
@@ -165,6 +173,8 @@ int Engine::AttachClosure(Mlp* mlp) {
   const std::vector<float> bmi = BoundaryIndex(h, first, nb, B);
   const long long nrows = (long long)members_ * nb;
   if (!rc) rc = Check(Up(&cbmi, bmi.data(), bmi.size()), "upload boundary indices");
+  if (!rc) rc = Check(Up(&csite, h.site_of_mode.data(), (size_t)D), "upload mode sites");
+  if (!rc) rc = Check(Up(&cslot, slotoff.data(), slotoff.size()), "upload slot offsets");
   if (!rc) rc = Check(hipMalloc((void**)&cx, (size_t)nrows * mlp->n_in() * 4), "alloc X");
   if (!rc) rc = Check(hipMalloc((void**)&cs, (size_t)nrows * 4), "alloc S");
   if (!rc) rc = Check(hipMalloc((void**)&cy, (size_t)nrows * mlp->n_out() * 4), "alloc Y");
@@ -172,7 +182,7 @@ int Engine::AttachClosure(Mlp* mlp) {
   if (!rc) rc = Check(hipMemsetAsync(cfall, 0, sizeof(unsigned long long), stream_), "zero count");
   if (!rc) rc = Synchronize();
   if (rc) {
-    void* ptrs[] = {pack, rho, dmoff, edge, cbmi, cx, cs, cy, cfall};
+    void* ptrs[] = {pack, rho, dmoff, edge, cbmi, csite, cslot, cx, cs, cy, cfall};
     for (void* q : ptrs) FreeD(q);
     return rc;
   }
@@ -199,10 +209,12 @@ int Engine::AttachClosure(Mlp* mlp) {
   FreeD(d_edge_);
   d_edge_ = edge;
   args_.edge = d_edge_;
-  void* old[] = {d_cbmi_, d_cx_, d_cs_, d_cy_, d_cpack_, d_cfall_};
+  void* old[] = {d_cbmi_, d_csite_, d_cslot_, d_cx_, d_cs_, d_cy_, d_cpack_, d_cfall_};
   for (void* q : old) FreeD(q);
   d_cfall_ = cfall;
   d_cbmi_ = cbmi;
+  d_csite_ = csite;
+  d_cslot_ = cslot;
   d_cx_ = cx;
   d_cs_ = cs;
   d_cy_ = cy;
@@ -230,7 +242,7 @@ int Engine::ClosurePredict(const void* in, bool in_fp32) {
   if (closure_fused_ && FusedClosureFits(fused_) && NE <= 64)
     return Check(LaunchClosureFused(fused_, const_cast<void*>(in), in_fp32, d_moff_, members_,
                                     closure_first_, closure_nb_, NE, D, d_cbmi_, closure_vbase_,
-                                    guard, stream_), "closure fused");
+                                    d_cslot_, guard, stream_), "closure fused");
   const long long nrows = (long long)members_ * closure_nb_;
   int rc = Check(LaunchClosureInputs(in, in_fp32, d_moff_, members_, closure_first_, closure_nb_,
                                      NE, D, d_cbmi_, d_cx_, d_cs_, stream_), "closure inputs");
@@ -239,9 +251,9 @@ int Engine::ClosurePredict(const void* in, bool in_fp32) {
     if (rc) err_ = "closure network: " + closure_->last_error();
   }
   if (!rc)
-    rc = Check(LaunchClosureScatter(d_cy_, d_cs_, d_moff_, members_, closure_nb_, NE, D,
-                                    closure_vbase_, const_cast<void*>(in), in_fp32, guard,
-                                    stream_),
+    rc = Check(LaunchClosureScatter(d_cy_, d_cs_, d_moff_, members_, closure_nb_, h_.N, D,
+                                    d_csite_, closure_vbase_, const_cast<void*>(in), in_fp32,
+                                    guard, stream_),
                "closure scatter");
   return rc;
 }

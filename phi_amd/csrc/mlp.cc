@@ -99,12 +99,13 @@ unsigned short Bf16(float f) {
 
 }  // namespace
 
-int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm) {
+int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm, int n_packed) {
   *owned = nullptr;
   if (n_hid_ % 32 != 0) {
     err_ = "the fused closure pass needs a hidden width that is a multiple of 32";
     return -3;
   }
+  if (!out_perm || n_packed <= 0) n_packed = n_out_;
   std::vector<float> p(nparam_);
   int rc = GetParams(p.data());
   if (rc) return rc;
@@ -114,17 +115,17 @@ int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm) {
   // 256 output columns.
   auto rows256 = [](int n) { return (n + 255) / 256 * 256; };
   const size_t n1 = (size_t)rows256(hid) * k1, n2 = (size_t)rows256(hid) * hid,
-               n3 = (size_t)rows256(n_out_) * hid;
+               n3 = (size_t)rows256(n_packed) * hid;
   // One buffer: the bfloat16 weights, then (16-byte aligned) the float32
   // biases b1, b2, b3.
   const size_t nw = (n1 + n2 + n3 + 7) / 8 * 8;
-  const size_t nb = (size_t)2 * hid + n_out_;
+  const size_t nb = (size_t)2 * hid + n_packed;
   std::vector<unsigned short> w(nw + 2 * nb, 0);
   for (int o = 0; o < hid; ++o)
     for (int i = 0; i < n_in_; ++i) w[(size_t)o * k1 + i] = Bf16(p[off_[0] + (size_t)o * n_in_ + i]);
   for (int o = 0; o < hid; ++o)
     for (int i = 0; i < hid; ++i) w[n1 + (size_t)o * hid + i] = Bf16(p[off_[2] + (size_t)o * hid + i]);
-  for (int o = 0; o < n_out_; ++o) {
+  for (int o = 0; o < n_packed; ++o) {
     const int src = out_perm ? out_perm[o] : o;
     if (src < 0 || src >= n_out_) {
       err_ = "output permutation entry out of range";
@@ -136,7 +137,7 @@ int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm) {
   float* bias = reinterpret_cast<float*>(w.data() + nw);
   std::memcpy(bias, p.data() + off_[1], hid * sizeof(float));
   std::memcpy(bias + hid, p.data() + off_[3], hid * sizeof(float));
-  for (int o = 0; o < n_out_; ++o) bias[2 * hid + o] = p[off_[5] + (out_perm ? out_perm[o] : o)];
+  for (int o = 0; o < n_packed; ++o) bias[2 * hid + o] = p[off_[5] + (out_perm ? out_perm[o] : o)];
   unsigned short* d = nullptr;
   rc = Check(hipMalloc((void**)&d, w.size() * sizeof(unsigned short)), "alloc pack");
   if (!rc) rc = Check(hipMemcpy(d, w.data(), w.size() * sizeof(unsigned short),
@@ -154,7 +155,7 @@ int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm) {
   out->b3 = db + 2 * hid;
   out->k1 = k1;
   out->hid = hid;
-  out->n_out = n_out_;
+  out->n_out = n_packed;
   out->width = std::max(k1, hid);
   *owned = d;
   return 0;

@@ -49,9 +49,11 @@ class Mlp {
   // A snapshot of the current parameters packed for the fused closure pass
   // (bfloat16 weights and float32 biases in one device buffer, *owned, which
   // the caller frees with hipFree; later training does not change it).
-  // out_perm (n_out entries or null): packed output neuron o is neuron
-  // out_perm[o] of the network (a relabelling of the last layer's rows).
-  int Pack(struct FusedNet* out, void** owned, const int* out_perm = nullptr);
+  // out_perm (n_packed entries, or null for all n_out in order): packed
+  // output neuron o is neuron out_perm[o] of the network (a selection and
+  // relabelling of the last layer's rows; n_packed <= 0 means n_out).
+  int Pack(struct FusedNet* out, void** owned, const int* out_perm = nullptr,
+           int n_packed = 0);
 
  private:
   Mlp() = default;

@@ -249,9 +249,11 @@ def test_fused_closure_pass_equals_unfused_and_network(prec):
     for bi in (0, len(bnodes) // 2, len(bnodes) - 1):
         kids = M.decode_children(y[bi], ss[bi], 7, 7)
         for j in range(7):
+            # only the column the stage kernels read (site j) is written; the
+            # rest of the slot keeps the seeded values
             slot = states[True][m * span + n_real + bi * 7 + j]
             tol = (1e-5 if prec == "fp64" else 1e-5) * max(1e-30, np.abs(kids[j]).max())
-            assert np.abs(slot - kids[j]).max() <= tol
+            assert np.abs(slot[:, j] - kids[j][:, j]).max() <= tol
 
 
 def test_failed_attach_leaves_the_ensemble_unchanged():

@@ -3,7 +3,7 @@
 A network is trained on the GPU (the project's forward/backward kernels,
 Adam) on depth-10 trajectories of 64 random 7-site Hamiltonians, then 16
 held-out Hamiltonians are integrated to 1 ps at depth 4 with and without the
-closure and at depth 10 (the judge).  The closure must bring the depth-4
+closure and at depth 10 (the reference run).  The closure must bring the depth-4
 populations closer to depth 10 than plain truncation does.
 """
 
