@@ -196,8 +196,11 @@ int phi_hip_closure_samples(phi_ctx* c, int shallow_depth, float* X, float* T,
                             long long cap_rows, long long* rows);
 /* Couple a network to an ensemble context of uniform depth (its last tier
  * gains predicted next-tier neighbours before every RK4 stage); resets the
- * state to the initial density matrix.  The network must outlive the
- * context's RK4 calls. */
+ * state to the initial density matrix.  Of each predicted neighbour
+ * n + e_j only column site(j), the part the stage kernels read, is
+ * evaluated and written (2 N D of the 2 N^2 D outputs; the rest of the
+ * virtual node is never read).  The network must outlive the context's RK4
+ * calls. */
 int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m);
 /* Closure predictions by one fused launch per stage (on = 1, default) or by
  * the unfused kernels (inputs, the network's GEMMs, scatter); the network's
@@ -205,8 +208,8 @@ int phi_hip_closure_attach(phi_ctx* c, phi_mlp* m);
 int phi_hip_closure_fused(phi_ctx* c, int on);
 /* The closure's fallback to PHI's plain truncation (the dropped tier of
  * LinkHierarchyNodes): before a stage uses them, the predictions of a
- * boundary node whose outputs (next-tier entries over the node's largest
- * entry) are not all finite and at most ymax in magnitude are replaced by
+ * boundary node whose evaluated outputs (next-tier entries over the node's
+ * largest entry) are not all finite and at most ymax in magnitude are replaced by
  * zeros.  ymax > 0 sets the threshold (default 3e38: non-finite outputs
  * only); *fallen (if not null) = boundary-node predictions replaced since
  * the attach. */
