@@ -1099,18 +1099,21 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
     rc = Check(hipMemcpyAsync(r, rr0, sizeof(rr0), hipMemcpyHostToDevice, stream_), "r0 set");
   }
   if (!rc) rc = Check(hipMemcpyAsync(rh, r, bytes, hipMemcpyDeviceToDevice, stream_), "rhat");
-  if (!rc) rc = Check(hipMemsetAsync(pp, 0, bytes, stream_), "p0");
-  if (!rc) rc = Check(hipMemsetAsync(vv, 0, bytes, stream_), "v0");
+  // p0 = r0 (the first iteration's p; later ones come with the x/r update)
+  if (!rc) rc = Check(hipMemcpyAsync(pp, r, bytes, hipMemcpyDeviceToDevice, stream_), "p0");
   Z rn2;
   if (!rc) rc = dot(r, r, &rn2);
   double res = std::sqrt(rn2.real());
   // The iterations keep their scalars on the device (BicgScalars): the dot
-  // products land there, the update kernels derive beta, alpha and omega
-  // from them, and the next iteration's p update closes each iteration (a
-  // one-block kernel closes the last of a batch).  The host checks
-  // after every kBatch iterations; once converged (or broken down, or at
-  // max_iter) the x/r update and the closing kernel do nothing, so x and
-  // the count are those of stopping right there.
+  // products land there, the update kernels derive alpha, omega and beta
+  // from them, and the next iteration's s update closes each iteration (a
+  // one-block kernel closes the last of a batch).  Four kernels per
+  // iteration: v = A p, s, t = A s (with rhat . t and rhat . s, so that
+  // rhat . r_new = rhat . s - omega rhat . t is known before r_new exists),
+  // and x, r and p in one pass.  The host checks after every kBatch
+  // iterations; once converged (or broken down, or at max_iter) the
+  // updates and the closing kernel do nothing, so x and the count are those
+  // of stopping right there.
   constexpr int kBatch = 16;
   BicgScalars* st = nullptr;
   double* hist = nullptr;
@@ -1118,18 +1121,18 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   // Two records alternate: iteration k works on st[k % 2], and closing it
   // writes st[(k + 1) % 2] while other blocks may still read st[k % 2].
   if (!rc) rc = Check(hipMalloc((void**)&st, 2 * sizeof(BicgScalars)), "alloc bicgstab scalars");
-  // Partials: rhat.v (2 per block), t.s / t.t (3), |r|^2 / rhat.r (3).
-  // When the matvec's LDS form fits, it writes the rhat.v and t.s / t.t
-  // partials itself (one record per matvec block; mb blocks).
+  // Partials: rhat.v (2 per block), t.s / t.t / rhat.t / rhat.s (7), |r|^2
+  // (1).  When the matvec's LDS form fits, it writes the rhat.v and the
+  // seven-value partials itself (one record per matvec block; mb blocks).
   constexpr int kRedBlocks = 512;
   const int mb = RhsGeneralDotBlocks(args_);
   const int nb_dot = mb > 0 ? mb : kRedBlocks;
   const size_t cap = (size_t)std::max(kRedBlocks, mb);
   if (!rc)
-    rc = Check(hipMalloc((void**)&part3, cap * 8 * sizeof(double)), "alloc bicgstab partials");
+    rc = Check(hipMalloc((void**)&part3, cap * 10 * sizeof(double)), "alloc bicgstab partials");
   double* p_rv = part3;
   double* p_ts = part3 + 2 * cap;
-  double* p_xr = part3 + 5 * cap;
+  double* p_xr = part3 + 9 * cap;
   if (!rc)
     rc = Check(hipMalloc((void**)&hist, (size_t)(std::max(max_iter, 0) + 1) * sizeof(double)),
                "alloc bicgstab history");
@@ -1153,29 +1156,30 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
     for (int b = 0; b < kBatch && !rc; ++b, ++k) {
       BicgScalars* cur = st + (k & 1);
       BicgScalars* prev = st + ((k + 1) & 1);
-      // (close the previous iteration, then) p = r + beta (p - omega v)
-      rc = Check(LaunchBicgP(b > 0, pp, n, r, vv, prev, cur, p_xr, kRedBlocks, hist, tol,
-                             max_iter, stream_), "p update");
       // v = A p with the rhat.v partials
-      if (!rc && mb > 0)
-        rc = Check(LaunchRhsGeneralDot(args_, pp, vv, 1, 1, rh, p_rv, stream_), "A p, rhat.v");
+      if (mb > 0)
+        rc = Check(LaunchRhsGeneralDot(args_, pp, vv, 1, 1, rh, nullptr, p_rv, stream_),
+                   "A p, rhat.v");
       if (!rc && mb == 0) rc = RhsDevice(pp, vv, 1);
       if (!rc && mb == 0)
         rc = Check(LaunchDotPartial(rh, vv, n, p_rv, kRedBlocks, stream_), "rhat.v");
-      // s = r - alpha v (alpha from the rhat.v partials)
+      // (close the previous iteration, then) s = r - alpha v
       if (!rc)
-        rc = Check(LaunchBicgS(s, n, r, vv, cur, p_rv, nb_dot, stream_), "s update");
-      // t = A s with the t.s / t.t partials
+        rc = Check(LaunchBicgS(b > 0, s, n, r, vv, prev, cur, p_rv, nb_dot, p_xr, kRedBlocks,
+                               hist, tol, max_iter, stream_), "s update");
+      // t = A s with the t.s, t.t, rhat.t, rhat.s partials
       if (!rc && mb > 0)
-        rc = Check(LaunchRhsGeneralDot(args_, s, t, 1, 2, s, p_ts, stream_), "A s, t.s, t.t");
+        rc = Check(LaunchRhsGeneralDot(args_, s, t, 1, 3, s, rh, p_ts, stream_),
+                   "A s, t.s, t.t, rhat.t, rhat.s");
       if (!rc && mb == 0) rc = RhsDevice(s, t, 1);
       if (!rc && mb == 0)
-        rc = Check(LaunchBicgDotTS(t, s, n, p_ts, kRedBlocks, stream_), "t.s, t.t");
-      // x = x + alpha p + omega s, r = s - omega t, partials of |r|^2, rhat.r
+        rc = Check(LaunchBicgDots(t, s, rh, n, p_ts, kRedBlocks, stream_), "t and s dots");
+      // x += alpha p + omega s, r = s - omega t, p = r + beta (p - omega v),
+      // partials of |r|^2
       if (!rc)
-        rc = Check(LaunchBicgXR(x, r, pp, s, t, rh, n, cur, p_ts, nb_dot, p_xr, kRedBlocks,
-                                stream_),
-                   "x/r update");
+        rc = Check(LaunchBicgXRP(x, r, pp, s, t, vv, n, cur, p_ts, nb_dot, p_xr, kRedBlocks,
+                                 stream_),
+                   "x/r/p update");
     }
     // the batch's last iteration closes into the record the next batch starts from
     if (!rc)

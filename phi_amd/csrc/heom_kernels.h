@@ -103,11 +103,12 @@ hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int tr
 // Blocks of LaunchRhsGeneralDot (one partial record each), 0 when its LDS
 // form does not fit (then it returns hipErrorInvalidValue).
 int RhsGeneralDotBlocks(const RhsArgs& a);
-// LaunchRhsGeneral (no active flags) fused with the next reduction of the
+// LaunchRhsGeneral (no active flags) fused with the next reductions of the
 // steady-state solver, per block: dot 1 = sum conj(w) y (2 doubles),
-// dot 2 = sum conj(y) w and |y|^2 (3 doubles).
+// dot 3 = sums of conj(y) w, |y|^2, conj(w2) y and conj(w2) w (7 doubles).
 hipError_t LaunchRhsGeneralDot(const RhsArgs& a, const double* x, double* y, int trace_row,
-                               int dot, const double* w, double* partial, hipStream_t s);
+                               int dot, const double* w, const double* w2, double* partial,
+                               hipStream_t s);
 hipError_t LaunchSnapshot(const double* src, double* hist, int* counter, int ne, int capacity,
                           hipStream_t s);
 // hist[slot] = mean over the members of their node 0 (fixed summation order).
@@ -129,8 +130,12 @@ hipError_t LaunchLincomb(double* y, size_t n, int nv, const double* const* x, co
                          hipStream_t s);
 
 // BiCGSTAB with its scalars on the device (no host round trip inside an
-// iteration).  The dot products land in this struct; the update kernels
-// derive their coefficients from it; the finish kernel closes an iteration.
+// iteration).  An iteration is four kernels: v = A p (with the partials of
+// rhat . v), the s update (which first closes the previous iteration), t =
+// A s (with the partials of t . s, |t|^2, rhat . t, rhat . s) and the
+// x/r/p update (with the partials of |r|^2).  The dot products land in this
+// struct; the update kernels derive their coefficients from it; the finish
+// kernel closes the last iteration of a batch.
 struct BicgScalars {
   double2 rho_old, alpha, omega;  // from the previous iteration (1, 1, 1 at start)
   double2 rho_new, rv, ts, tt, rn2;  // dot products of the current iteration
@@ -146,32 +151,31 @@ struct BicgScalars {
 // Partials of conj(u) . v (2 doubles per block).
 hipError_t LaunchDotPartial(const double* u, const double* v, size_t n, double* partial,
                             int nblocks, hipStream_t s);
-// s = r - alpha v (rv = sum of the nb partials of rhat . v, kept in st,
-// alpha = rho_new / rv).
-hipError_t LaunchBicgS(double* y, size_t n, const double* r, const double* v, BicgScalars* st,
-                       const double* rv_part, int nb, hipStream_t s);
-// Partials of t . s (conjugating t) and |t|^2 (3 doubles per block).
-hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, double* partial3,
-                           int nblocks, hipStream_t s);
-// Sums the t.s / t.t partials into st->ts, st->tt; x += alpha p + omega s,
-// r = s - omega t (omega = ts / tt, 0 if tt = 0), and partials of |r|^2
-// and rhat . r; nothing once st->done.
-hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
-                        const double* t, const double* rhat, size_t n, BicgScalars* st,
-                        const double* ts_part, int nb_ts, double* partial3, int nblocks,
-                        hipStream_t s);
-// Closing an iteration: from record prev and the x/r partials, the record
-// next of the following iteration (rn2, rho_next; alpha, omega, rho_old <-
-// this iteration's, rho_new <- rho_next; res = |r|; hist[it] = res; done
-// when !(res > tol), omega = 0 or it = max_iter).  One block.
+// s = r - alpha v for record cur (rv = sum of the nb partials of rhat . v,
+// kept in cur, alpha = rho_new / rv); close = true first closes the
+// previous iteration (record prev, its nbx |r|^2 partials) into cur, as
+// LaunchBicgFinish does.  Nothing once the (closed) record is done.
+hipError_t LaunchBicgS(bool close, double* y, size_t n, const double* r, const double* v,
+                       const BicgScalars* prev, BicgScalars* cur, const double* rv_part, int nb,
+                       const double* xr_part, int nbx, double* hist, double tol, int max_iter,
+                       hipStream_t s);
+// Partials of conj(t) s, |t|^2, conj(rhat) t, conj(rhat) s (7 doubles per
+// block): the matvec's dot 3 on its own.
+hipError_t LaunchBicgDots(const double* t, const double* sv, const double* rhat, size_t n,
+                          double* partial7, int nblocks, hipStream_t s);
+// From those partials: omega = ts / tt (0 if tt = 0), rho_next = rhat . s -
+// omega rhat . t (= rhat . r_new), beta; x += alpha p + omega s, r = s -
+// omega t, p = r + beta (p - omega v), partials of |r|^2 (1 double per
+// block); ts, tt, rho_next kept in st.  Nothing once st->done.
+hipError_t LaunchBicgXRP(double* x, double* r, double* p, const double* sv, const double* t,
+                         const double* v, size_t n, BicgScalars* st, const double* ts_part,
+                         int nb_ts, double* partial, int nblocks, hipStream_t s);
+// Closing an iteration: from record prev and the |r|^2 partials, the record
+// next of the following iteration (rn2; alpha, omega, rho_old <- this
+// iteration's, rho_new <- rho_next; res = |r|; hist[it] = res; done when
+// !(res > tol), omega = 0 or it = max_iter).  One block.
 hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const double* xr_part,
                             int nblocks, double* hist, double tol, int max_iter, hipStream_t s);
-// p = r + beta (p - omega v) with the scalars of record cur; close = true
-// first closes the previous iteration (record prev, the x/r partials) into
-// cur, as LaunchBicgFinish does.
-hipError_t LaunchBicgP(bool close, double* p, size_t n, const double* r, const double* v,
-                       const BicgScalars* prev, BicgScalars* cur, const double* xr_part,
-                       int nblocks, double* hist, double tol, int max_iter, hipStream_t s);
 
 }  // namespace phi
 

@@ -143,13 +143,14 @@ __device__ __forceinline__ void block_sum(const double* a, double* __restrict__ 
 // The same with H, H^dagger and the ADOs of T nodes per block in LDS (one
 // load of each per element instead of 2N): for N*N*(T+2) complex within
 // the LDS budget.  DOT fuses the solver's next reduction into the matvec:
-// 1 = the block's sum of conj(w) y (2 doubles per block), 2 = the block's
-// sums of conj(y) w and |y|^2 (3 doubles per block).
+// 1 = the block's sum of conj(w) y (2 doubles per block), 3 = the block's
+// sums of conj(y) w, |y|^2, conj(w2) y and conj(w2) w (7 doubles per block).
 template <int DOT>
 __global__ void __launch_bounds__(256) rhs_general_lds(RhsArgs A, const double* __restrict__ x,
                                                        double* __restrict__ y, int trace_row,
                                                        const unsigned char* __restrict__ act,
                                                        int T, const double* __restrict__ w,
+                                                       const double* __restrict__ w2,
                                                        double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = A.N, NE = A.NE;
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) rhs_general_lds(RhsArgs A, const double* 
     sH[e] = A.Hraw[e];
     sHd[e] = A.Hadj[e];
   }
-  double acc[3] = {0, 0, 0};
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int n0 = blockIdx.x * T; n0 < A.num_nodes; n0 += gridDim.x * T) {
     const int nt = min(T, A.num_nodes - n0);
     __syncthreads();
@@ -174,23 +175,29 @@ __global__ void __launch_bounds__(256) rhs_general_lds(RhsArgs A, const double* 
       const int a = r - b * N;
       const int n = n0 + te;
       const size_t g = (size_t)n * NE + r;
+      // the dot operands load while the element is evaluated
+      double2 q = make_double2(0., 0.), h = make_double2(0., 0.);
+      if constexpr (DOT != 0) q = reinterpret_cast<const double2*>(w)[g];
+      if constexpr (DOT == 3) h = reinterpret_cast<const double2*>(w2)[g];
       const double2 v = general_elem(A, x, sx + te * NE, sH, sHd, n, r, a, b, act,
                                      trace_row && g == 0);
       reinterpret_cast<double2*>(y)[g] = v;
       if constexpr (DOT == 1) {
-        const double2 q = reinterpret_cast<const double2*>(w)[g];
         acc[0] += q.x * v.x + q.y * v.y;
         acc[1] += q.x * v.y - q.y * v.x;
-      } else if constexpr (DOT == 2) {
-        const double2 q = reinterpret_cast<const double2*>(w)[g];
+      } else if constexpr (DOT == 3) {
         acc[0] += v.x * q.x + v.y * q.y;
         acc[1] += v.x * q.y - v.y * q.x;
         acc[2] += v.x * v.x + v.y * v.y;
+        acc[3] += h.x * v.x + h.y * v.y;
+        acc[4] += h.x * v.y - h.y * v.x;
+        acc[5] += h.x * q.x + h.y * q.y;
+        acc[6] += h.x * q.y - h.y * q.x;
       }
     }
   }
   if constexpr (DOT == 1) block_sum<2>(acc, partial);
-  if constexpr (DOT == 2) block_sum<3>(acc, partial);
+  if constexpr (DOT == 3) block_sum<7>(acc, partial);
 }
 
 // --- Small utility kernels ----------------------------------------------------
@@ -449,24 +456,64 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
   for (int c = 0; c < NC; ++c) out[c] = ((sw[0][c] + sw[1][c]) + sw[2][c]) + sw[3][c];
 }
 
-// Block sum of three values per thread into partial[3 b + c].
-__device__ __forceinline__ void block_sum3(double a0, double a1, double a2,
-                                           double* __restrict__ partial) {
-  const double a[3] = {a0, a1, a2};
-  block_sum<3>(a, partial);
+
+// Closes an iteration (every thread of every block computes the same
+// values; reduce_partials is called by all threads): from the record
+// `prev` of the iteration and the partials of |r|^2 its x/r/p update wrote,
+// the scalars that start the next iteration (alpha, omega, rho_old = rho,
+// rho = the rho_next that update derived, the residual, the count, the stop
+// flag).
+__device__ __forceinline__ BicgScalars bicg_close(const BicgScalars* prev, const double* xr_part,
+                                                  int nb, double tol, int max_iter) {
+  BicgScalars c = *prev;
+  if (c.done) return c;  // uniform
+  double q[1];
+  reduce_partials<1>(xr_part, nb, q);
+  c.rn2 = make_double2(q[0], 0.);
+  const double2 al = zdiv(c.rho_new, c.rv), om = bicg_omega(&c);
+  c.alpha = al;
+  c.omega = om;
+  c.rho_old = c.rho_new;
+  c.rho_new = c.rho_next;
+  c.res = sqrt(c.rn2.x);
+  c.it += 1;
+  if (!(c.res > tol) || (om.x == 0 && om.y == 0) || c.it >= max_iter) c.done = 1;
+  return c;
 }
 
-// s = r - alpha v, after reducing the partials of rhat . v into rv (block 0
-// keeps it in st; alpha = rho_new / rv); terms added left to right.
+// s = r - alpha v for the iteration of record cur, after reducing the
+// partials of rhat . v into rv (alpha = rho_new / rv; terms added left to
+// right).  CLOSE: first close the previous iteration (record prev and its
+// |r|^2 partials) into cur.  Block 0 writes cur (the closed scalars and rv);
+// every block computes them itself, and nothing in this kernel reads cur.
+template <bool CLOSE>
 __global__ void __launch_bounds__(256) bicg_s(double* __restrict__ y, size_t n,
                                               const double* __restrict__ x0,
-                                              const double* __restrict__ x1, BicgScalars* st,
-                                              const double* rv_part, int nb) {
+                                              const double* __restrict__ x1,
+                                              const BicgScalars* prev, BicgScalars* cur,
+                                              const double* rv_part, int nb,
+                                              const double* xr_part, int nbx, double* hist,
+                                              double tol, int max_iter) {
+  double2 rho;
+  int done;
+  if (CLOSE) {
+    const BicgScalars c = bicg_close(prev, xr_part, nbx, tol, max_iter);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (!prev->done) hist[c.it] = c.res;
+      *cur = c;
+    }
+    rho = c.rho_new;
+    done = c.done;
+  } else {
+    rho = cur->rho_new;
+    done = cur->done;
+  }
+  if (done) return;  // uniform
   double pr[2];
   reduce_partials<2>(rv_part, nb, pr);
   const double2 rv = make_double2(pr[0], pr[1]);
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->rv = rv;
-  const double2 al = zdiv(st->rho_new, rv);
+  if (blockIdx.x == 0 && threadIdx.x == 0) cur->rv = rv;
+  const double2 al = zdiv(rho, rv);
   const double2 c1 = make_double2(-al.x, -al.y);
   const double2* a = reinterpret_cast<const double2*>(x0);
   const double2* b = reinterpret_cast<const double2*>(x1);
@@ -488,55 +535,75 @@ __global__ void __launch_bounds__(256) bicg_s(double* __restrict__ y, size_t n,
   }
 }
 
-// ts = sum conj(t) s, tt = sum |t|^2
-__global__ void __launch_bounds__(256) bicg_dot_ts(const double* __restrict__ t,
-                                                   const double* __restrict__ sv, size_t n,
-                                                   double* __restrict__ partial) {
-  double a0 = 0, a1 = 0, a2 = 0;
+// The matvec's DOT 3 reductions on their own (when its LDS form does not
+// fit): conj(t) s, |t|^2, conj(rhat) t, conj(rhat) s.
+__global__ void __launch_bounds__(256) bicg_dots(const double* __restrict__ t,
+                                                 const double* __restrict__ sv,
+                                                 const double* __restrict__ rhat, size_t n,
+                                                 double* __restrict__ partial) {
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const double2 p = reinterpret_cast<const double2*>(t)[i];
     const double2 q = reinterpret_cast<const double2*>(sv)[i];
-    a0 += p.x * q.x + p.y * q.y;
-    a1 += p.x * q.y - p.y * q.x;
-    a2 += p.x * p.x + p.y * p.y;
+    const double2 h = reinterpret_cast<const double2*>(rhat)[i];
+    a[0] += p.x * q.x + p.y * q.y;
+    a[1] += p.x * q.y - p.y * q.x;
+    a[2] += p.x * p.x + p.y * p.y;
+    a[3] += h.x * p.x + h.y * p.y;
+    a[4] += h.x * p.y - h.y * p.x;
+    a[5] += h.x * q.x + h.y * q.y;
+    a[6] += h.x * q.y - h.y * q.x;
   }
-  block_sum3(a0, a1, a2, partial);
+  block_sum<7>(a, partial);
 }
 
-// Reduces the t.s / t.t partials (block 0 keeps ts and tt in st), then
-// x = x + alpha p + omega s, r = s - omega t and the partials of |r|^2 and
-// rhat . r.
-__global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* __restrict__ r,
-                                               const double* __restrict__ p,
-                                               const double* __restrict__ sv,
-                                               const double* __restrict__ t,
-                                               const double* __restrict__ rhat, size_t n,
-                                               BicgScalars* st, const double* __restrict__ ts_part,
-                                               int nb, double* __restrict__ partial) {
+// The x/r/p update of the iteration of record st, from the matvec's
+// partials of conj(t) s, |t|^2, conj(rhat) t and conj(rhat) s: omega = ts /
+// tt (0 if tt = 0), the next rho = conj(rhat) r_new = rhat.s - omega rhat.t
+// (linearity: no pass over r_new needed before p), beta = (rho_next / rho)
+// (alpha / omega); x += alpha p + omega s, r = s - omega t, p = r + beta
+// (p - omega v), and the partials of |r|^2.  Block 0 keeps ts, tt and
+// rho_next in st (other blocks read only rho_new, rv and done of it).
+// Nothing once st->done.
+__global__ void __launch_bounds__(256) bicg_xrp(double* __restrict__ x, double* __restrict__ r,
+                                                double* __restrict__ p,
+                                                const double* __restrict__ sv,
+                                                const double* __restrict__ t,
+                                                const double* __restrict__ v, size_t n,
+                                                BicgScalars* st, const double* __restrict__ ts_part,
+                                                int nb, double* __restrict__ partial) {
   if (st->done) return;  // uniform: every block returns
-  double q[3];
-  reduce_partials<3>(ts_part, nb, q);
+  double q[7];
+  reduce_partials<7>(ts_part, nb, q);
   const double2 ts = make_double2(q[0], q[1]), tt = make_double2(q[2], 0.);
+  const double2 rt = make_double2(q[3], q[4]), rs = make_double2(q[5], q[6]);
+  const double2 al = zdiv(st->rho_new, st->rv);
+  const double2 om = tt.x > 0 ? zdiv(ts, tt) : make_double2(0., 0.);
+  const double2 orr = zmul(om, rt);
+  const double2 rho_next = make_double2(rs.x - orr.x, rs.y - orr.y);
+  const double2 beta = (om.x == 0 && om.y == 0)
+                           ? make_double2(0., 0.)
+                           : zmul(zdiv(rho_next, st->rho_new), zdiv(al, om));
+  const double2 bo = zmul(beta, om);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->ts = ts;
     st->tt = tt;
+    st->rho_next = rho_next;
   }
-  const double2 al = zdiv(st->rho_new, st->rv);
-  const double2 om = tt.x > 0 ? zdiv(ts, tt) : make_double2(0., 0.);
-  double acc = 0, hr = 0, hi = 0;
+  double acc = 0;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   // two elements per trip (a thread past n reads element n - 1, unused)
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
     const size_t j[2] = {i, i + stride < n ? i + stride : n - 1};
-    double2 xv[2], pv[2], s[2], tv[2], h[2];
+    double2 xv[2], pv[2], s[2], tv[2], vv[2];
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       xv[e] = reinterpret_cast<const double2*>(x)[j[e]];
       pv[e] = reinterpret_cast<const double2*>(p)[j[e]];
       s[e] = reinterpret_cast<const double2*>(sv)[j[e]];
       tv[e] = reinterpret_cast<const double2*>(t)[j[e]];
-      h[e] = reinterpret_cast<const double2*>(rhat)[j[e]];
+      vv[e] = reinterpret_cast<const double2*>(v)[j[e]];
     }
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
@@ -553,39 +620,21 @@ __global__ void __launch_bounds__(256) bicg_xr(double* __restrict__ x, double* _
       ri += -om.x * tv[e].y - om.y * tv[e].x;
       reinterpret_cast<double2*>(r)[j[e]] = make_double2(rr, ri);
       acc += rr * rr + ri * ri;
-      hr += h[e].x * rr + h[e].y * ri;
-      hi += h[e].x * ri - h[e].y * rr;
+      // p = r + beta p - (beta omega) v, terms left to right
+      double ar = rr, ai = ri;
+      ar += beta.x * pv[e].x - beta.y * pv[e].y;
+      ai += beta.x * pv[e].y + beta.y * pv[e].x;
+      ar += -bo.x * vv[e].x + bo.y * vv[e].y;
+      ai += -bo.x * vv[e].y - bo.y * vv[e].x;
+      reinterpret_cast<double2*>(p)[j[e]] = make_double2(ar, ai);
     }
   }
-  block_sum3(acc, hr, hi, partial);
-}
-
-// Closes an iteration (every thread of every block computes the same
-// values; reduce_partials is called by all threads): from the record
-// `prev` of the iteration and the partials of |r|^2 and rhat . r its x/r
-// update wrote, the scalars that start the next iteration (alpha, omega,
-// rho_old = rho, rho = rhat . r, the residual, the count, the stop flag).
-__device__ __forceinline__ BicgScalars bicg_close(const BicgScalars* prev, const double* xr_part,
-                                                  int nb, double tol, int max_iter) {
-  BicgScalars c = *prev;
-  if (c.done) return c;  // uniform
-  double q[3];
-  reduce_partials<3>(xr_part, nb, q);
-  c.rn2 = make_double2(q[0], 0.);
-  c.rho_next = make_double2(q[1], q[2]);
-  const double2 al = zdiv(c.rho_new, c.rv), om = bicg_omega(&c);
-  c.alpha = al;
-  c.omega = om;
-  c.rho_old = c.rho_new;
-  c.rho_new = c.rho_next;
-  c.res = sqrt(c.rn2.x);
-  c.it += 1;
-  if (!(c.res > tol) || (om.x == 0 && om.y == 0) || c.it >= max_iter) c.done = 1;
-  return c;
+  const double a[1] = {acc};
+  block_sum<1>(a, partial);
 }
 
 // One block: closes the iteration of record prev into record next (the last
-// iteration of a batch; within a batch the p update closes).
+// iteration of a batch; within a batch the next s update closes).
 __global__ void __launch_bounds__(256) bicg_finish(const BicgScalars* prev, BicgScalars* next,
                                                    const double* xr_part, int nb, double* hist,
                                                    double tol, int max_iter) {
@@ -593,53 +642,6 @@ __global__ void __launch_bounds__(256) bicg_finish(const BicgScalars* prev, Bicg
   if (threadIdx.x == 0) {
     if (!prev->done) hist[c.it] = c.res;
     *next = c;
-  }
-}
-
-// p = r + beta (p - omega v) for the iteration of record cur.  CLOSE: first
-// close the previous iteration (record prev and its x/r partials) into cur
-// (block 0 writes it; every block computes the same scalars, and nothing
-// in this kernel reads cur).
-template <bool CLOSE>
-__global__ void __launch_bounds__(256) bicg_p(double* p, size_t n, const double* __restrict__ r,
-                                              const double* __restrict__ v,
-                                              const BicgScalars* prev, BicgScalars* cur,
-                                              const double* xr_part, int nb, double* hist,
-                                              double tol, int max_iter) {
-  BicgScalars c;
-  if (CLOSE) {
-    c = bicg_close(prev, xr_part, nb, tol, max_iter);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      if (!prev->done) hist[c.it] = c.res;
-      *cur = c;
-    }
-  } else {
-    c = *cur;
-  }
-  if (c.done) return;  // uniform
-  const double2 beta = zmul(zdiv(c.rho_new, c.rho_old), zdiv(c.alpha, c.omega));
-  const double2 bo = zmul(beta, c.omega);
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  // two elements per trip (a thread past n reads element n - 1, unused)
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
-    const size_t j[2] = {i, i + stride < n ? i + stride : n - 1};
-    double2 u[2], w[2], x[2];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      u[e] = reinterpret_cast<const double2*>(r)[j[e]];
-      w[e] = reinterpret_cast<const double2*>(p)[j[e]];
-      x[e] = reinterpret_cast<const double2*>(v)[j[e]];
-    }
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      // r + beta p - (beta omega) v, terms left to right
-      double ar = u[e].x, ai = u[e].y;
-      ar += beta.x * w[e].x - beta.y * w[e].y;
-      ai += beta.x * w[e].y + beta.y * w[e].x;
-      ar += -bo.x * x[e].x + bo.y * x[e].y;
-      ai += -bo.x * x[e].y - bo.y * x[e].x;
-      if (e == 0 || i + stride < n) reinterpret_cast<double2*>(p)[j[e]] = make_double2(ar, ai);
-    }
   }
 }
 
@@ -737,17 +739,18 @@ int RhsGeneralDotBlocks(const RhsArgs& a) {
 }
 
 hipError_t LaunchRhsGeneralDot(const RhsArgs& a, const double* x, double* y, int trace_row,
-                               int dot, const double* w, double* partial, hipStream_t s) {
+                               int dot, const double* w, const double* w2, double* partial,
+                               hipStream_t s) {
   int T, blocks;
   RhsLdsShape(a, &T, &blocks);
-  if (blocks == 0 || (dot != 1 && dot != 2)) return hipErrorInvalidValue;
+  if (blocks == 0 || (dot != 1 && dot != 3)) return hipErrorInvalidValue;
   const size_t sh = (size_t)(T + 2) * a.NE * sizeof(double2);
   if (dot == 1)
     hipLaunchKernelGGL(dev::rhs_general_lds<1>, dim3((unsigned)blocks), dim3(256), sh, s, a, x,
-                       y, trace_row, (const unsigned char*)nullptr, T, w, partial);
+                       y, trace_row, (const unsigned char*)nullptr, T, w, w2, partial);
   else
-    hipLaunchKernelGGL(dev::rhs_general_lds<2>, dim3((unsigned)blocks), dim3(256), sh, s, a, x,
-                       y, trace_row, (const unsigned char*)nullptr, T, w, partial);
+    hipLaunchKernelGGL(dev::rhs_general_lds<3>, dim3((unsigned)blocks), dim3(256), sh, s, a, x,
+                       y, trace_row, (const unsigned char*)nullptr, T, w, w2, partial);
   return hipGetLastError();
 }
 
@@ -759,7 +762,8 @@ hipError_t LaunchRhsGeneral(const RhsArgs& a, const double* x, double* y, int tr
   if (blocks > 0) {
     const size_t sh = (size_t)(T + 2) * a.NE * sizeof(double2);
     hipLaunchKernelGGL(dev::rhs_general_lds<0>, dim3((unsigned)blocks), dim3(256), sh, s, a, x,
-                       y, trace_row, active, T, (const double*)nullptr, (double*)nullptr);
+                       y, trace_row, active, T, (const double*)nullptr, (const double*)nullptr,
+                       (double*)nullptr);
     return hipGetLastError();
   }
   const size_t tot = (size_t)a.num_nodes * a.NE;
@@ -834,26 +838,32 @@ hipError_t LaunchDotPartial(const double* u, const double* v, size_t n, double* 
 // re-read them fewer times).
 constexpr int kBicgBlocks = 512;
 
-hipError_t LaunchBicgS(double* y, size_t n, const double* r, const double* v, BicgScalars* st,
-                       const double* rv_part, int nb, hipStream_t s) {
+hipError_t LaunchBicgS(bool close, double* y, size_t n, const double* r, const double* v,
+                       const BicgScalars* prev, BicgScalars* cur, const double* rv_part, int nb,
+                       const double* xr_part, int nbx, double* hist, double tol, int max_iter,
+                       hipStream_t s) {
   int blocks = (int)((n + 511) / 512);
   if (blocks > kBicgBlocks) blocks = kBicgBlocks;
-  hipLaunchKernelGGL(dev::bicg_s, dim3(blocks), dim3(256), 0, s, y, n, r, v, st, rv_part, nb);
+  if (close)
+    hipLaunchKernelGGL(dev::bicg_s<true>, dim3(blocks), dim3(256), 0, s, y, n, r, v, prev, cur,
+                       rv_part, nb, xr_part, nbx, hist, tol, max_iter);
+  else
+    hipLaunchKernelGGL(dev::bicg_s<false>, dim3(blocks), dim3(256), 0, s, y, n, r, v, prev, cur,
+                       rv_part, nb, xr_part, nbx, hist, tol, max_iter);
   return hipGetLastError();
 }
 
-hipError_t LaunchBicgDotTS(const double* t, const double* sv, size_t n, double* partial3,
-                           int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(dev::bicg_dot_ts, dim3(nblocks), dim3(256), 0, s, t, sv, n, partial3);
+hipError_t LaunchBicgDots(const double* t, const double* sv, const double* rhat, size_t n,
+                          double* partial7, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_dots, dim3(nblocks), dim3(256), 0, s, t, sv, rhat, n, partial7);
   return hipGetLastError();
 }
 
-hipError_t LaunchBicgXR(double* x, double* r, const double* p, const double* sv,
-                        const double* t, const double* rhat, size_t n, BicgScalars* st,
-                        const double* ts_part, int nb_ts, double* partial3, int nblocks,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(dev::bicg_xr, dim3(nblocks), dim3(256), 0, s, x, r, p, sv, t, rhat, n, st,
-                     ts_part, nb_ts, partial3);
+hipError_t LaunchBicgXRP(double* x, double* r, double* p, const double* sv, const double* t,
+                         const double* v, size_t n, BicgScalars* st, const double* ts_part,
+                         int nb_ts, double* partial, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicg_xrp, dim3(nblocks), dim3(256), 0, s, x, r, p, sv, t, v, n, st,
+                     ts_part, nb_ts, partial);
   return hipGetLastError();
 }
 
@@ -861,20 +871,6 @@ hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const do
                             int nblocks, double* hist, double tol, int max_iter, hipStream_t s) {
   hipLaunchKernelGGL(dev::bicg_finish, dim3(1), dim3(256), 0, s, prev, next, xr_part, nblocks,
                      hist, tol, max_iter);
-  return hipGetLastError();
-}
-
-hipError_t LaunchBicgP(bool close, double* p, size_t n, const double* r, const double* v,
-                       const BicgScalars* prev, BicgScalars* cur, const double* xr_part,
-                       int nblocks, double* hist, double tol, int max_iter, hipStream_t s) {
-  int blocks = (int)((n + 511) / 512);
-  if (blocks > kBicgBlocks) blocks = kBicgBlocks;
-  if (close)
-    hipLaunchKernelGGL(dev::bicg_p<true>, dim3(blocks), dim3(256), 0, s, p, n, r, v, prev, cur,
-                       xr_part, nblocks, hist, tol, max_iter);
-  else
-    hipLaunchKernelGGL(dev::bicg_p<false>, dim3(blocks), dim3(256), 0, s, p, n, r, v, prev, cur,
-                       xr_part, nblocks, hist, tol, max_iter);
   return hipGetLastError();
 }
 
