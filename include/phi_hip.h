@@ -184,9 +184,11 @@ int phi_hip_mlp_forward(phi_mlp* m, const float* x, int batch, float* y);
 int phi_hip_mlp_gradients(phi_mlp* m, const float* x, const float* t, int batch, double* loss,
                           float* grads, size_t n);
 /* Adam (beta 0.9 / 0.999, eps 1e-8) on `samples` rows of X / T, minibatches
- * of `batch` from a seeded shuffle each epoch; epoch_loss[e] = mean loss. */
+ * of `batch` from a seeded shuffle each epoch; epoch_loss[e] = mean loss.
+ * The learning rate goes from lr (first epoch) to lr_final (last epoch)
+ * geometrically, epoch by epoch; lr_final <= 0 keeps lr throughout. */
 int phi_hip_mlp_train(phi_mlp* m, const float* X, const float* T, long long samples, int batch,
-                      int epochs, float lr, unsigned seed, double* epoch_loss);
+                      int epochs, float lr, float lr_final, unsigned seed, double* epoch_loss);
 /* Training rows from a context's current state for hierarchies of depth
  * shallow_depth (PHI's HierarchyTruncation; the context is deeper): input
  * = [Re rho_n / s, Im rho_n / s, n / (shallow_depth - 1)], target = the D

@@ -118,7 +118,8 @@ def lib():
         "phi_hip_mlp_forward": (ctypes.c_int, [vp, fp, ctypes.c_int, fp]),
         "phi_hip_mlp_gradients": (ctypes.c_int, [vp, fp, fp, ctypes.c_int, dp, fp, sz]),
         "phi_hip_mlp_train": (ctypes.c_int, [vp, fp, fp, ctypes.c_longlong, ctypes.c_int,
-                                             ctypes.c_int, ctypes.c_float, ctypes.c_uint, dp]),
+                                             ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                             ctypes.c_uint, dp]),
         "phi_hip_closure_samples": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, ctypes.c_longlong,
                                                    ctypes.POINTER(ctypes.c_longlong)]),
         "phi_hip_closure_attach": (ctypes.c_int, [vp, vp]),
@@ -623,12 +624,15 @@ class Mlp:
                                            ctypes.byref(loss), _fp(g), g.size))
         return loss.value, g
 
-    def train(self, X, T, batch=256, epochs=1, lr=1e-3, seed=0):
+    def train(self, X, T, batch=256, epochs=1, lr=1e-3, seed=0, lr_final=0.0):
+        """Adam over `epochs` shuffled passes; the learning rate goes
+        geometrically from lr to lr_final (0: constant).  Per-epoch losses."""
         X = np.ascontiguousarray(np.asarray(X, dtype=np.float32))
         T = np.ascontiguousarray(np.asarray(T, dtype=np.float32))
         losses = np.zeros(epochs)
         _check(lib().phi_hip_mlp_train(self._h, _fp(X), _fp(T), X.shape[0], int(batch),
-                                       int(epochs), float(lr), int(seed), _dp(losses)))
+                                       int(epochs), float(lr), float(lr_final), int(seed),
+                                       _dp(losses)))
         return losses
 
 

@@ -613,10 +613,11 @@ int phi_hip_mlp_gradients(phi_mlp* m, const float* x, const float* t, int batch,
 }
 
 int phi_hip_mlp_train(phi_mlp* m, const float* X, const float* T, long long samples, int batch,
-                      int epochs, float lr, unsigned seed, double* epoch_loss) {
+                      int epochs, float lr, float lr_final, unsigned seed, double* epoch_loss) {
   return Guarded(__func__, [&]() -> int {
     if (!m || !X || !T) return Err(PHI_HIP_EINVAL, "null argument");
-    int rc = m->m->Train(X, T, samples, batch, epochs, lr, seed, epoch_loss);
+    if (!(lr > 0)) return Err(PHI_HIP_EINVAL, "learning rate must be positive");
+    int rc = m->m->Train(X, T, samples, batch, epochs, lr, lr_final, seed, epoch_loss);
     return rc ? Err(rc, m->m->last_error()) : Ok();
   });
 }

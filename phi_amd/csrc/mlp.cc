@@ -333,7 +333,7 @@ int Mlp::AdamStep(float lr) {
 }
 
 int Mlp::Train(const float* X, const float* T, long long samples, int batch, int epochs,
-               float lr, unsigned seed, double* epoch_loss) {
+               float lr, float lr_final, unsigned seed, double* epoch_loss) {
   if (samples < 1 || batch < 1 || batch > max_batch_ || epochs < 1) {
     err_ = "bad training sizes";
     return -1;
@@ -364,6 +364,9 @@ int Mlp::Train(const float* X, const float* T, long long samples, int batch, int
   std::vector<int> perm(samples);
   std::vector<double> part((size_t)nb * partial_blocks_);
   for (int ep = 0; ep < epochs && !rc; ++ep) {
+    const float lr_ep = (lr_final > 0 && epochs > 1)
+                            ? (float)(lr * std::pow((double)lr_final / lr, (double)ep / (epochs - 1)))
+                            : lr;
     std::iota(perm.begin(), perm.end(), 0);
     std::shuffle(perm.begin(), perm.end(), gen);
     rc = Check(hipMemcpyAsync(dIdx, perm.data(), perm.size() * sizeof(int),
@@ -373,7 +376,7 @@ int Mlp::Train(const float* X, const float* T, long long samples, int batch, int
       if (!rc) rc = Check(LaunchGatherRows(dT, dIdx + b * batch, batch, n_out_, d_t_, stream_),
                           "gather t");
       if (!rc) rc = BackwardBatch(d_x_, d_t_, batch, (int)b);
-      if (!rc) rc = AdamStep(lr);
+      if (!rc) rc = AdamStep(lr_ep);
     }
     if (!rc) rc = Check(hipMemcpyAsync(part.data(), d_partial_, part.size() * sizeof(double),
                                        hipMemcpyDeviceToHost, stream_), "epoch loss");

@@ -37,10 +37,11 @@ class Mlp {
   // Loss mean((f(x) - t)^2) and its gradient (flat, parameter layout).
   int Gradients(const float* x, const float* t, int batch, double* loss, float* grads);
   // Adam on host data: the samples are copied to the device once, then
-  // `epochs` passes over shuffled minibatches (seeded permutation).
-  // epoch_loss[e] = mean batch loss of epoch e.
+  // `epochs` passes over shuffled minibatches (seeded permutation), the
+  // learning rate going geometrically from lr to lr_final over the epochs
+  // (lr_final <= 0: constant).  epoch_loss[e] = mean batch loss of epoch e.
   int Train(const float* X, const float* T, long long samples, int batch, int epochs, float lr,
-            unsigned seed, double* epoch_loss);
+            float lr_final, unsigned seed, double* epoch_loss);
 
   // Forward on device arrays: x (batch x n_in) -> y (batch x n_out), on the
   // given stream; activations of the first max_batch rows live in this

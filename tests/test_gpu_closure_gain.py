@@ -1,7 +1,7 @@
 """What the trained closure network gains over plain truncation.
 
 A network is trained on the GPU (the project's forward/backward kernels,
-Adam) on depth-10 trajectories of 64 random 7-site Hamiltonians, then 16
+Adam) on depth-10 trajectories of 256 random 7-site Hamiltonians, then 32
 held-out Hamiltonians are integrated to 1 ps at depth 4 with and without the
 closure and at depth 10 (the reference run).  The closure must bring the depth-4
 populations closer to depth 10 than plain truncation does.
@@ -19,15 +19,19 @@ pytestmark = pytest.mark.gpu
 
 
 def test_trained_closure_beats_plain_truncation():
+    """The trained closure brings the depth-4 populations at least 20% closer
+    to depth 10 on average than plain truncation, and no further from it at
+    the worst point (the bench's configuration trains on 4096 trajectories;
+    this one on 256, evaluated on 32 held-out Hamiltonians)."""
     dt = 0.001
-    print("case: closure training on 64 depth-10 trajectories", flush=True)
-    X, T = C.generate_samples(C.hamiltonians(0, 64), deep=11, shallow=5, steps=1000, dt=dt,
-                              every=100, keep=0.5, seed=1, members=64)
-    assert len(X) > 10000
+    print("case: closure training on 256 depth-10 trajectories", flush=True)
+    X, T = C.generate_samples(C.hamiltonians(0, 256), deep=11, shallow=5, steps=1000, dt=dt,
+                              every=100, keep=0.25, seed=1, members=256)
+    assert len(X) > 50000
     net, losses = C.train(X, T, epochs=20, batch=512, lr=1e-3, seed=2)
     assert losses[-1] < losses[0]
-    print("case: closure evaluation on 16 held-out Hamiltonians", flush=True)
-    res = C.evaluate(C.hamiltonians(100000, 16), net, deep=11, shallow=5, steps=1000, dt=dt,
+    print("case: closure evaluation on 32 held-out Hamiltonians", flush=True)
+    res = C.evaluate(C.hamiltonians(100000, 32), net, deep=11, shallow=5, steps=1000, dt=dt,
                      every=10)
     plain, closed = res["shallow"], res["closure"]
     print("max population error vs depth 10: plain depth 4 %.3e, depth 4 + closure %.3e"
@@ -35,6 +39,6 @@ def test_trained_closure_beats_plain_truncation():
     print("mean population error vs depth 10: plain depth 4 %.3e, depth 4 + closure %.3e"
           % (plain["mean_pop_err"], closed["mean_pop_err"]))
     assert closed["max_pop_err"] <= 1e-3
-    assert closed["mean_pop_err"] < plain["mean_pop_err"]
-    assert closed["max_pop_err"] < plain["max_pop_err"]
+    assert closed["mean_pop_err"] <= 0.8 * plain["mean_pop_err"]
+    assert closed["max_pop_err"] <= plain["max_pop_err"]
     net.close()
