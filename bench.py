@@ -42,7 +42,9 @@ c5_closure   the learned truncation closure: trained on depth-10 runs,
          depth 4 + closure vs plain depth 4 vs plain depth 10 on held-out
          Hamiltonians (populations and time to 1 ps).
 --shape-ens/--shape-fp64/--shape-fp32/--shape-c3 pin a leg's launch shape
-instead of autotuning it (profiles of the shape an earlier run chose).
+instead of autotuning it (profiles of the shape an earlier run chose):
+form,tile,threads,blocks, optionally followed by threads,blocks of each of
+the four RK4 stages (the 12 values a leg's kernel_shape records).
 """
 
 import argparse
@@ -166,16 +168,31 @@ def launch_label(counts):
     return "hipGraph blocks (%d steps) + eager head/tail (%d steps)" % (g, e)
 
 
+def pin_shape(ctx, shape):
+    """Pin a launch shape: (form, tile, threads, blocks), optionally
+    followed by (threads, blocks) of each RK4 stage."""
+    ctx.set_launch(*shape[:4])
+    if len(shape) == 12:
+        for s in range(4):
+            ctx.set_stage_launch(s, shape[4 + 2 * s], shape[5 + 2 * s])
+
+
+def full_shape(ctx, shape):
+    """The common shape followed by (threads, blocks) of each RK4 stage."""
+    th, bl = ctx.stage_launch()
+    return tuple(shape[:4]) + tuple(v for s in range(4) for v in (th[s], bl[s]))
+
+
 def run_rk4(P, p, prec, steps, warmup, shape=None):
-    """RK4 timing of one configuration; `shape` = (form, tile, threads,
-    blocks) pins the launch shape, None autotunes it."""
+    """RK4 timing of one configuration; `shape` (pin_shape's form) pins
+    the launch shape, None autotunes it."""
     dt = p.get_double("timestep")
     ctx = P.Context(p, precision=prec)
     if shape is None:
         _, shape = ctx.autotune(dt, steps=5)
     else:
-        ctx.set_launch(*shape)
-        shape = tuple(shape)
+        pin_shape(ctx, shape)
+    shape = full_shape(ctx, shape)
     ctx.rk4_steps(warmup, dt)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -259,8 +276,8 @@ def run_ensemble(P, p, Hs, depths, prec, steps, warmup, dt, shape=None):
     if shape is None:
         _, shape = ens.autotune(dt, steps=3)
     else:
-        ens.set_launch(*shape)
-        shape = tuple(shape)
+        pin_shape(ens, shape)
+    shape = full_shape(ens, shape)
     ens.rk4_steps(warmup, dt)
     ens.synchronize()
     t0 = time.perf_counter()
@@ -329,11 +346,17 @@ def main():
     ap.add_argument("--calibration-copy", action="store_true",
                     help="first copy 1 GiB device to device (a plain copy of known size, "
                          "the reference point of the HBM byte counters under rocprofv3)")
-    shape = lambda s: tuple(int(v) for v in s.split(","))
+    def shape(s):
+        v = tuple(int(x) for x in s.split(","))
+        if len(v) not in (4, 12):
+            raise argparse.ArgumentTypeError("a shape has 4 or 12 values")
+        return v
+
     for leg in ("ens", "fp64", "fp32", "c3"):
         ap.add_argument("--shape-" + leg, type=shape, default=None,
-                        help="launch shape form,tile,threads,blocks of the %s leg instead of "
-                             "autotuning it (e.g. to profile the shape a previous run chose)" % leg)
+                        help="launch shape form,tile,threads,blocks[,threads,blocks per RK4 "
+                             "stage] of the %s leg instead of autotuning it (e.g. to profile "
+                             "the shape a previous run chose)" % leg)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this integrator runs on a single MI355X "

@@ -279,10 +279,22 @@ int phi_hip_synchronize(phi_ctx* c);
  * (v_mfma 16x16x4, N <= 32), 2 persistent pipelined matrix-core kernels
  * (N <= 32, at most two bath modes per site; the default where they
  * apply), -1 default; tile = nodes per block (0 = default); threads =
- * threads per block (64..256 in steps of 64, 0 = 256); blocks = grid of the
- * persistent kernels (0 = as many as are resident at once).  Results do
- * not depend on the shape beyond rounding. */
+ * threads per block, a multiple of 64 (one-shot forms: 64..256, 0 = 256;
+ * persistent form: up to 1024 with one element per thread, up to 256 with
+ * 2..4, 0 = one element per thread); blocks = grid of the persistent
+ * kernels (0 = as many as are resident at once).  Every RK4 stage takes
+ * this shape.  Results do not depend on the shape beyond rounding.  No
+ * counterpart in PHI, whose partitioning is PartitionHierarchy,
+ * hierarchy_integrator.cc:636-808. */
 int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks);
+/* Threads and blocks of RK4 stage `stage` (0..3) of the persistent kernels
+ * alone (same rules; the tile stays phi_hip_set_launch's), in both
+ * precisions; phi_hip_set_launch resets every stage to its shape.  The
+ * first and last stage move the FP64 state and can prefer another grid.
+ * No counterpart in PHI. */
+int phi_hip_set_stage_launch(phi_ctx* c, int stage, int threads, int blocks);
+/* threads[4], blocks[4] of the four RK4 stages in the context's precision. */
+int phi_hip_stage_launch(const phi_ctx* c, int* threads, int* blocks);
 /* Stage order of phi_hip_rk4_steps (the RK4 loop body of Rk4Integrate,
  * hierarchy_integrator.cc:2709-2825): 0 = PHI's k0..k3 form with an
  * accumulator, 1 = nested form rho + dt L (rho + dt/2 L (rho + dt/3 L (rho +
@@ -306,12 +318,15 @@ int phi_hip_launch_counts(const phi_ctx* c, long long* graph_steps, long long* e
 /* Autotuner of the stage-kernel shape (no counterpart in PHI, whose
  * partitioning is PartitionHierarchy, hierarchy_integrator.cc:636-808):
  * times `steps` RK4 steps of size dt for every candidate shape in the
- * context's precision and RK4 form, keeps the fastest, then (unless
+ * context's precision and RK4 form and keeps the fastest; with its tile,
+ * gives each RK4 stage in turn the threads and blocks that make whole
+ * steps fastest (phi_hip_set_stage_launch); then (unless
  * phi_hip_set_graphs was called) times 129-step runs with graph blocks and
  * with eager launches and keeps the faster, and leaves the state
  * unchanged.  Cached per device and problem shape for the process.
  * *best_ms = milliseconds per step; shape[4] = {gemm, tile, threads,
- * blocks} as in phi_hip_set_launch. */
+ * blocks} of the common shape as in phi_hip_set_launch
+ * (phi_hip_stage_launch gives the stages' own). */
 int phi_hip_autotune(phi_ctx* c, int steps, double dt, double* best_ms, int* shape);
 
 #ifdef __cplusplus

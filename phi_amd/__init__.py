@@ -96,6 +96,9 @@ def lib():
         "phi_hip_synchronize": (ctypes.c_int, [vp]),
         "phi_hip_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int]),
+        "phi_hip_set_stage_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_int]),
+        "phi_hip_stage_launch": (ctypes.c_int, [vp, ip, ip]),
         "phi_hip_set_rk4_form": (ctypes.c_int, [vp, ctypes.c_int]),
         "phi_hip_set_graphs": (ctypes.c_int, [vp, ctypes.c_int]),
         "phi_hip_get_graphs": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
@@ -459,12 +462,26 @@ class Context:
         _check(lib().phi_hip_synchronize(self._h))
 
     def set_launch(self, gemm=-1, tile=0, threads=0, blocks=0):
-        """Stage-kernel shape: gemm 0 = element path, 1 = matrix cores,
-        2 = persistent matrix-core kernels, -1 = default; tile = nodes per
-        block (0 = default); threads = threads per block (0 = 256); blocks =
-        grid of the persistent kernels (0 = all resident)."""
+        """Stage-kernel shape of every RK4 stage: gemm 0 = element path,
+        1 = matrix cores, 2 = persistent matrix-core kernels, -1 = default;
+        tile = nodes per block (0 = default); threads = threads per block
+        (0 = 256, or one element per thread for the persistent kernels);
+        blocks = grid of the persistent kernels (0 = all resident)."""
         _check(lib().phi_hip_set_launch(self._h, int(gemm), int(tile), int(threads),
                                         int(blocks)))
+
+    def set_stage_launch(self, stage, threads=0, blocks=0):
+        """Threads and blocks of RK4 stage `stage` (0..3) of the persistent
+        kernels alone; the tile stays set_launch's, which resets it."""
+        _check(lib().phi_hip_set_stage_launch(self._h, int(stage), int(threads), int(blocks)))
+
+    def stage_launch(self):
+        """((threads of stages 0..3), (blocks of stages 0..3)) in the
+        context's precision."""
+        th = (ctypes.c_int * 4)()
+        bl = (ctypes.c_int * 4)()
+        _check(lib().phi_hip_stage_launch(self._h, th, bl))
+        return tuple(th), tuple(bl)
 
     def set_graphs(self, on):
         """RK4 step blocks replayed from a captured hipGraph (True) or

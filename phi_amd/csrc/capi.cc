@@ -788,6 +788,22 @@ int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks) 
   });
 }
 
+int phi_hip_set_stage_launch(phi_ctx* c, int stage, int threads, int blocks) {
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->SetStageLaunch(stage, threads, blocks);
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
+}
+
+int phi_hip_stage_launch(const phi_ctx* c, int* threads, int* blocks) {
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !threads || !blocks) return Err(PHI_HIP_EINVAL, "null argument");
+    c->e->StageLaunch(threads, blocks);
+    return Ok();
+  });
+}
+
 int phi_hip_set_graphs(phi_ctx* c, int on) {
   return Guarded(__func__, [&]() -> int {
     if (!c) return Err(PHI_HIP_EINVAL, "null argument");

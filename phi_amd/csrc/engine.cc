@@ -401,10 +401,35 @@ Engine::~Engine() {
 
 void Engine::set_fp32(bool v) { fp32_ = v; }
 
+namespace {
+// Dynamic LDS a stage-kernel block may take.
+constexpr size_t kLdsBudget = 64 * 1024;
+}  // namespace
+
+int Engine::StreamThreads(int t, bool fp32, int threads, int* th) {
+  const int N = h_.N, NE = N * N;
+  const int n = threads > 0 ? threads : std::min(1024, (t * NE + 63) / 64 * 64);
+  const int ept = (t * NE + n - 1) / n;
+  if (ept > 4 || (ept > 1 && n > 256) ||
+      StreamTileLdsBytes(N, h_.D, h_.L, t, fp32) > kLdsBudget) {
+    err_ = "tile does not fit the persistent kernels (at most 4 elements per thread, "
+           "256 threads when more than one)";
+    return kErrInvalidArgument;
+  }
+  // H's MFMA A-fragments: ceil(N/16) row tiles x ceil(N/4) k-steps x 64 lanes
+  const int frags = ((N + 15) / 16) * ((N + 3) / 4) * 64;
+  if (members_ && 2 * n < frags) {
+    err_ = "an ensemble block needs at least half as many threads as H has MFMA "
+           "fragment elements";
+    return kErrInvalidArgument;
+  }
+  *th = n;
+  return kOk;
+}
+
 int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
   RhsArgs a = args_;
   const int N = h_.N, D = h_.D, L = h_.L, NE = N * N;
-  const size_t kLdsBudget = 64 * 1024;
   const bool stream_ok = N <= 32 && mps_ <= 2 && d_edge_ != nullptr;
   if (col_nm < 0) {
     // Default: the persistent matrix-core kernels where they apply, else
@@ -447,21 +472,8 @@ int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
       while (t > 1 && ((size_t)t * NE > 1024 ||
                        StreamTileLdsBytes(N, D, L, t, f == 1) > kLdsBudget))
         --t;
-      if (threads == 0) th = std::min(1024, (t * NE + 63) / 64 * 64);
-      const int ept = (t * NE + th - 1) / th;
-      if (ept > 4 || (ept > 1 && th > 256) ||
-          StreamTileLdsBytes(N, D, L, t, f == 1) > kLdsBudget) {
-        err_ = "tile does not fit the persistent kernels (at most 4 elements per thread, "
-               "256 threads when more than one)";
-        return kErrInvalidArgument;
-      }
-      // H's MFMA A-fragments: ceil(N/16) row tiles x ceil(N/4) k-steps x 64 lanes
-      const int frags = ((N + 15) / 16) * ((N + 3) / 4) * 64;
-      if (members_ && 2 * th < frags) {
-        err_ = "an ensemble block needs at least half as many threads as H has MFMA "
-               "fragment elements";
-        return kErrInvalidArgument;
-      }
+      const int rt = StreamThreads(t, f == 1, threads, &th);
+      if (rt) return rt;
       // Kernels without a persistent form (RKF45) run the one-shot
       // matrix-core kernels with 256 threads and their default tile.
       int t1 = std::max(1, std::min(64, 256 / NE));
@@ -485,7 +497,67 @@ int Engine::SetLaunch(int col_nm, int tile, int threads, int blocks) {
   a.col_nm = col_nm;
   a.tile = a.col_tile[0];
   args_ = a;
+  for (int f = 0; f < 2; ++f)
+    for (int s = 0; s < 4; ++s) {
+      stage_threads_[f][s] = a.col_threads[f];
+      stage_grid_[f][s] = blocks;
+    }
   return EnsureTiles();
+}
+
+int Engine::SetStageLaunch(int stage, int threads, int blocks) {
+  if (stage < 0 || stage > 3) {
+    err_ = "RK4 stage must be 0..3";
+    return kErrInvalidArgument;
+  }
+  if (args_.col_nm != 2) {
+    err_ = "per-stage shapes apply to the persistent kernels (form 2)";
+    return kErrInvalidArgument;
+  }
+  if (threads != 0 && (threads < 64 || threads > 1024 || threads % 64 != 0)) {
+    err_ = "threads per block must be a multiple of 64 up to 1024";
+    return kErrInvalidArgument;
+  }
+  if (blocks < 0) {
+    err_ = "blocks must be >= 0";
+    return kErrInvalidArgument;
+  }
+  int th[2];
+  for (int f = 0; f < 2; ++f) {
+    const int rt = StreamThreads(args_.col_tile[f], f == 1, threads, &th[f]);
+    if (rt) return rt;
+  }
+  for (int f = 0; f < 2; ++f) {
+    stage_threads_[f][stage] = th[f];
+    stage_grid_[f][stage] = blocks;
+  }
+  return kOk;
+}
+
+void Engine::StageLaunch(int threads[4], int blocks[4]) const {
+  const int f = fp32_ ? 1 : 0;
+  for (int s = 0; s < 4; ++s) {
+    threads[s] = args_.col_nm == 2 ? stage_threads_[f][s] : args_.col_threads[f];
+    blocks[s] = args_.col_nm == 2 ? stage_grid_[f][s] : args_.col_grid[f];
+  }
+}
+
+void Engine::SetStageShapes(const int threads[4], const int blocks[4]) {
+  const int f = fp32_ ? 1 : 0;
+  for (int s = 0; s < 4; ++s) {
+    stage_threads_[f][s] = threads[s];
+    stage_grid_[f][s] = blocks[s];
+  }
+}
+
+RhsArgs Engine::StageArgs(int s) const {
+  RhsArgs a = args_;
+  if (a.col_nm == 2)
+    for (int f = 0; f < 2; ++f) {
+      a.col_threads[f] = stage_threads_[f][s];
+      a.col_grid[f] = stage_grid_[f][s];
+    }
+  return a;
 }
 
 int Engine::Unsupported(const char* what) {
@@ -506,6 +578,7 @@ struct TuneKey {
 };
 struct TuneVal {
   int col_nm, tile, threads, blocks;
+  int stage_threads[4], stage_blocks[4];  // per RK4 stage (persistent kernels)
   double ms;
   bool graphs;
 };
@@ -524,6 +597,7 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   int rc = AllocBulk();
   if (rc) return rc;
   const int N = h_.N, NE = N * N;
+  const int f = fp32_ ? 1 : 0;
   int dev = 0;
   (void)hipGetDevice(&dev);
   const TuneKey key{dev, N, h_.D, h_.L, slab_nodes_, mps_, members_, fp32_, rk4_form_};
@@ -533,6 +607,7 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
     if (it != g_tune.end()) {
       const TuneVal& v = it->second;
       rc = SetLaunch(v.col_nm, v.tile, v.threads, v.blocks);
+      if (!rc) SetStageShapes(v.stage_threads, v.stage_blocks);
       if (!rc && !graphs_pinned_) {
         use_graphs_ = v.graphs;
         if (!use_graphs_) DropGraph();
@@ -549,17 +624,25 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   }
   int cus = 1;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  // Candidates: the persistent kernels over tiles of up to 1024 elements and
+  // Candidates: the persistent kernels over tiles of up to 1024 elements,
+  // one element per thread or 2..4 per thread with 64..256 threads, and
   // grids of 0 (all resident) or 1..4 blocks per CU; the one-shot forms as
   // a reference point.
   struct Cand { int col_nm, tile, threads, blocks; };
   std::vector<Cand> cands;
   const bool stream_ok = N <= 32 && mps_ <= 2 && d_edge_ != nullptr;
+  const int grids[] = {0, cus, 2 * cus, 3 * cus, 4 * cus};
   if (stream_ok) {
     const int tiles[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 48, 64, 96, 128, 192, 256};
     for (int t : tiles) {
       if ((long long)t * NE > 1024) break;
-      for (int b : {0, 1, 2, 3, 4}) cands.push_back({2, t, 0, b * cus});
+      std::vector<int> ths{0};
+      for (int th = 64; th <= 256; th += 64) {
+        const int ept = (t * NE + th - 1) / th;
+        if (ept >= 2 && ept <= 4) ths.push_back(th);
+      }
+      for (int th : ths)
+        for (int b : grids) cands.push_back({2, t, th, b});
     }
   }
   if (N <= 32) cands.push_back({1, 0, 0, 0});
@@ -573,30 +656,74 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) rc = Check(hipEventCreate(&e0), "event");
   if (!rc) rc = Check(hipEventCreate(&e1), "event");
+  // Milliseconds per step of `steps` eager steps after one untimed step.
+  auto time_steps = [&](float* per_step) {
+    int r = Rk4Step(dt, false);
+    if (!r) r = Check(hipEventRecord(e0, stream_), "record");
+    for (int s = 0; s < steps && !r; ++s) r = Rk4Step(dt, false);
+    if (!r) r = Check(hipEventRecord(e1, stream_), "record");
+    if (!r) r = Check(hipEventSynchronize(e1), "event sync");
+    float ms = 0;
+    if (!r) r = Check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    *per_step = ms / steps;
+    return r;
+  };
   const RhsArgs keep = args_;
+  int keep_th[2][4], keep_g[2][4];
+  std::memcpy(keep_th, stage_threads_, sizeof(keep_th));
+  std::memcpy(keep_g, stage_grid_, sizeof(keep_g));
   const bool keep_graphs = use_graphs_;
-  TuneVal best{keep.col_nm, keep.col_tile[fp32_ ? 1 : 0], keep.col_threads[fp32_ ? 1 : 0],
-               keep.col_grid[fp32_ ? 1 : 0], 1e300, use_graphs_};
+  TuneVal best{keep.col_nm, keep.col_tile[f], keep.col_threads[f], keep.col_grid[f],
+               {}, {}, 1e300, use_graphs_};
+  StageLaunch(best.stage_threads, best.stage_blocks);
   for (const Cand& c : cands) {
     if (rc) break;
     if (SetLaunch(c.col_nm, c.tile, c.threads, c.blocks) != kOk) continue;
-    rc = Rk4Step(dt, false);
-    if (!rc) rc = Check(hipEventRecord(e0, stream_), "record");
-    for (int s = 0; s < steps && !rc; ++s) rc = Rk4Step(dt, false);
-    if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");
-    if (!rc) rc = Check(hipEventSynchronize(e1), "event sync");
     float ms = 0;
-    if (!rc) rc = Check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-    if (!rc && ms / steps < best.ms) {
-      const int f = fp32_ ? 1 : 0;
-      best = TuneVal{c.col_nm, args_.col_tile[f], args_.col_threads[f], args_.col_grid[f],
-                     ms / steps, use_graphs_};
+    rc = time_steps(&ms);
+    if (!rc && ms < best.ms) {
+      best.col_nm = c.col_nm;
+      best.tile = args_.col_tile[f];
+      best.threads = args_.col_threads[f];
+      best.blocks = args_.col_grid[f];
+      StageLaunch(best.stage_threads, best.stage_blocks);
+      best.ms = ms;
     }
+  }
+  // Stages differ in traffic (the first and last move the FP64 state), so
+  // each stage in turn takes the threads and grid, with the chosen tile,
+  // that make whole steps fastest given the other stages' shapes (a change
+  // kept only when it gains more than 0.5%).
+  if (!rc && best.col_nm == 2) {
+    rc = SetLaunch(best.col_nm, best.tile, best.threads, best.blocks);
+    float cur = 0;
+    if (!rc) rc = time_steps(&cur);
+    for (int s = 0; s < 4 && !rc; ++s) {
+      for (int th = 0; th <= 256 && !rc; th += 64)
+        for (int b : grids) {
+          if (rc || SetStageLaunch(s, th, b) != kOk) continue;
+          int rth[4], rb[4];
+          StageLaunch(rth, rb);
+          if (rth[s] == best.stage_threads[s] && rb[s] == best.stage_blocks[s]) continue;
+          float ms = 0;
+          rc = time_steps(&ms);
+          if (!rc && ms < 0.995f * cur) {
+            cur = ms;
+            best.stage_threads[s] = rth[s];
+            best.stage_blocks[s] = rb[s];
+          }
+        }
+      SetStageShapes(best.stage_threads, best.stage_blocks);
+    }
+    if (!rc) best.ms = cur;
   }
   // Graph blocks or eager launches for the chosen shape: short stage kernels
   // can run faster eagerly (the graph's node-to-node latency), long ones
   // do not care, tiny ones need the graph when the host cannot keep up.
-  if (!rc && !graphs_pinned_) rc = SetLaunch(best.col_nm, best.tile, best.threads, best.blocks);
+  if (!rc && !graphs_pinned_) {
+    rc = SetLaunch(best.col_nm, best.tile, best.threads, best.blocks);
+    if (!rc) SetStageShapes(best.stage_threads, best.stage_blocks);
+  }
   if (!rc && !graphs_pinned_) {
     // One eager head step, then four graph blocks (Rk4Steps' layout).
     const long long n = 4LL * kGraphSteps + 1;
@@ -629,12 +756,15 @@ int Engine::Autotune(int steps, double dt, double* best_ms, int shape_out[4]) {
   if (!rc) rc = rc_restore ? rc_restore : rc_sync;
   if (rc) {
     args_ = keep;
+    std::memcpy(stage_threads_, keep_th, sizeof(keep_th));
+    std::memcpy(stage_grid_, keep_g, sizeof(keep_g));
     use_graphs_ = keep_graphs;
     DropGraph();
     return rc;
   }
   rc = SetLaunch(best.col_nm, best.tile, best.threads, best.blocks);
   if (rc) return rc;
+  SetStageShapes(best.stage_threads, best.stage_blocks);
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     g_tune[key] = best;
@@ -780,8 +910,8 @@ int Engine::Rk4Step(double dt, bool snapshot) {
   for (int s = 0; s < 4 && !rc; ++s) {
     if (closure_) rc = ClosurePredict(in[s], s > 0 && fp32_);
     if (!rc)
-      rc = Check(LaunchRk4Stage(args_, fp32_, s, form, in[s], d_rho_, aux, out[s], dt, stream_),
-                 "rk4 stage");
+      rc = Check(LaunchRk4Stage(StageArgs(s), fp32_, s, form, in[s], d_rho_, aux, out[s], dt,
+                                stream_), "rk4 stage");
   }
   if (!rc && snapshot) {
     // A single hierarchy records rho_0; an ensemble the member average.
@@ -806,7 +936,12 @@ void Engine::DropGraph() {
 int Engine::GraphBlock(int steps, int snap_every, double dt) {
   GraphKey key{steps, snap_every, dt, fp32_, StepForm(), args_.col_nm,
                args_.col_tile[0], args_.col_tile[1], args_.col_threads[0], args_.col_threads[1],
-               args_.col_grid[0], args_.col_grid[1], d_y_[0], d_rho_, d_tiles_};
+               args_.col_grid[0], args_.col_grid[1], d_y_[0], d_rho_, d_tiles_, {}};
+  for (int f = 0; f < 2; ++f)
+    for (int s = 0; s < 4; ++s) {
+      key.stage[f * 8 + s] = stage_threads_[f][s];
+      key.stage[f * 8 + 4 + s] = stage_grid_[f][s];
+    }
   if (graph_exec_ && key == graph_key_) return kOk;
   DropGraph();
   // The persistent kernels query their occupancy on first launch; callers
@@ -917,8 +1052,8 @@ int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   for (int s = 0; s < 4 && !rc; ++s) {
     for (long long i = 0; i <= nsteps && !rc; ++i) {
       if (i == 1) rc = Check(hipEventRecord(e0, stream_), "record");
-      if (!rc) rc = Check(LaunchRk4Stage(args_, fp32_, s, form, in[s], d_rho_, aux, out[s], dt,
-                                         stream_), "rk4 stage");
+      if (!rc) rc = Check(LaunchRk4Stage(StageArgs(s), fp32_, s, form, in[s], d_rho_, aux, out[s],
+                                         dt, stream_), "rk4 stage");
     }
     if (!rc) rc = Check(hipEventRecord(e1, stream_), "record");
     if (!rc) rc = Check(hipEventSynchronize(e1), "event sync");

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <complex>
 #include <string>
 #include <vector>
@@ -110,8 +111,13 @@ class Engine {
   // (0 = default for the LDS budget); threads = threads per block (0 =
   // 256); blocks = grid of the persistent kernels (0 = all resident).
   int SetLaunch(int col_nm, int tile, int threads, int blocks);
-  // RK4 stage order: true = nested (Taylor/Horner) form, false = PHI's
-  // classic k0..k3 form.  Both give PHI's step to rounding.
+  // Threads and grid of RK4 stage `stage` (0..3) of the persistent kernels
+  // alone, in both precisions, with the tile of the common shape (threads 0
+  // = one element per thread).  SetLaunch resets every stage to the common
+  // shape.
+  int SetStageLaunch(int stage, int threads, int blocks);
+  // Threads and grid of the four RK4 stages in the current precision.
+  void StageLaunch(int threads[4], int blocks[4]) const;
   // RK4 stage order (kRk4Classic, kRk4Nested, kRk4Increment).  The
   // increment form needs the persistent kernels; with the one-shot kernels
   // a step runs nested instead, and the closure always runs classic.
@@ -222,13 +228,15 @@ class Engine {
   long long last_graph_steps() const { return last_graph_steps_; }
   long long last_eager_steps() const { return last_eager_steps_; }
   // Autotuner: times `steps` RK4 steps (after one warm-up step) for each
-  // candidate shape of the persistent kernels (tile x grid) in the current
-  // precision and RK4 form, keeps the fastest, then (graphs not pinned)
-  // times graph blocks against eager launches for it, and restores the
-  // state.
+  // candidate shape of the persistent kernels (tile x threads x grid) in
+  // the current precision and RK4 form and keeps the fastest; then, with
+  // that tile, gives each stage in turn the threads and grid that make
+  // whole steps fastest; then (graphs not pinned) times graph blocks
+  // against eager launches, and restores the state.
   // Results are cached per (device, problem shape, precision, form) for the
   // process.  *best_ms = milliseconds per step of the winner; shape_out =
-  // {form, tile, threads, blocks}.
+  // {form, tile, threads, blocks} of the common shape (StageLaunch gives
+  // the stages' own).
   int Autotune(int steps, double dt, double* best_ms, int shape_out[4]);
 
  private:
@@ -243,6 +251,14 @@ class Engine {
   void StageBuffers(int form, const void* in[4], void* out[4], void** aux) const;
   int GraphBlock(int steps, int snap_every, double dt);
   void DropGraph();
+  // args_ with RK4 stage s's threads and grid.
+  RhsArgs StageArgs(int s) const;
+  // Stage shapes {threads, grid} of the current precision, unchecked (the
+  // autotuner's own, found valid for the current tile).
+  void SetStageShapes(const int threads[4], const int blocks[4]);
+  // Whether `threads` (0 = one element per thread) fit the persistent
+  // kernels with tile t of precision f; *th = the resolved count.
+  int StreamThreads(int t, bool fp32, int threads, int* th);
 
   struct GraphKey {
     int steps, snap_every;
@@ -252,11 +268,12 @@ class Engine {
     const void* y0;
     const void* rho;
     const void* tiles;
+    std::array<int, 16> stage;  // stage_threads_ and stage_grid_
     bool operator==(const GraphKey& o) const {
       return steps == o.steps && snap_every == o.snap_every && dt == o.dt && fp32 == o.fp32 &&
              form == o.form && col_nm == o.col_nm && tile0 == o.tile0 && tile1 == o.tile1 &&
              th0 == o.th0 && th1 == o.th1 && grid0 == o.grid0 && grid1 == o.grid1 &&
-             y0 == o.y0 && rho == o.rho && tiles == o.tiles;
+             y0 == o.y0 && rho == o.rho && tiles == o.tiles && stage == o.stage;
     }
   };
   static constexpr int kGraphSteps = 32;
@@ -268,6 +285,10 @@ class Engine {
 
   HierarchyTables h_;
   RhsArgs args_{};
+  // Threads and grid of each RK4 stage of the persistent kernels, per
+  // precision ([0] FP64, [1] FP32); the tile is args_.col_tile.
+  int stage_threads_[2][4] = {};
+  int stage_grid_[2][4] = {};
   bool fp32_ = false;
   int rk4_form_ = kRk4Increment;
   int device_ = 0;

@@ -123,8 +123,8 @@ int Engine::Rk4StepsTD(long long nsteps, const double* t_before, double dt, long
         ++stage_slot_;
       }
       if (!rc)
-        rc = Check(LaunchRk4Stage(args_, fp32_, st, kRk4Classic, in[st], d_rho_, d_aux_, out[st], dt,
-                                  stream_), "rk4 stage");
+        rc = Check(LaunchRk4Stage(StageArgs(st), fp32_, st, kRk4Classic, in[st], d_rho_, d_aux_,
+                                  out[st], dt, stream_), "rk4 stage");
     }
     if (!rc && snap_every > 0 && (step_offset + s) % snap_every == 0)
       rc = Check(LaunchSnapshot(d_rho_, d_hist_, d_hist_count_, N() * N(), hist_capacity_, stream_),

@@ -332,6 +332,48 @@ def test_rk4_forms_and_launch_shapes_match_oracle(cfg, lh, steps):
             assert err <= 1e-10, (form, gemm, tile, threads, blocks, err)
 
 
+@pytest.mark.parametrize("cfg,lh,tile,per_stage", [
+    ("c2", 5, 4, ((64, 0), (0, 7), (128, 0), (192, 300))),
+    ("c3", 3, 1, ((192, 0), (0, 512), (256, 0), (0, 300)))])
+def test_stage_launch_shapes_match_oracle(cfg, lh, tile, per_stage):
+    """Each RK4 stage with threads and grid of its own (one element per
+    thread and 2..4 per thread, fixed and resident grids) gives the oracle's
+    state within 1e-10 (FP64); set_launch resets the stages to its shape;
+    shapes the tile cannot take and other kernel forms are refused."""
+    op, p = _subset_params(cfg, lh)
+    orc = HO.Oracle(HO.Problem.from_prm(op), threads=8)
+    dt = op.timestep
+    steps = 20
+    orc.rk4(steps * dt * (1 - 1e-9), dt, 1.0)
+    ctx = P.Context(p)
+    ne = ctx.N * ctx.N
+    ctx.set_launch(2, tile, 0, 0)
+    for s, (th, b) in enumerate(per_stage):
+        ctx.set_stage_launch(s, th, b)
+    th, bl = ctx.stage_launch()
+    one_per_thread = (tile * ne + 63) // 64 * 64
+    assert th == tuple(t or one_per_thread for t, _ in per_stage)
+    assert bl == tuple(b for _, b in per_stage)
+    print("case", cfg, "tile", tile, "stages", th, bl, flush=True)
+    ctx.rk4_steps(steps, dt)
+    assert np.abs(ctx.state() - orc.state()).max() <= 1e-10
+    with pytest.raises(P.PhiError):
+        ctx.set_stage_launch(4, 0, 0)
+    with pytest.raises(P.PhiError):
+        ctx.set_stage_launch(0, 100, 0)
+    with pytest.raises(P.PhiError):
+        ctx.set_stage_launch(1, 0, -1)
+    if tile * ne > 4 * 64:
+        with pytest.raises(P.PhiError):  # more than 4 elements per thread
+            ctx.set_stage_launch(2, 64, 0)
+    ctx.set_launch(2, tile, 0, 0)
+    assert ctx.stage_launch() == ((one_per_thread,) * 4, (0,) * 4)
+    ctx.set_launch(1, 0, 0, 0)
+    with pytest.raises(P.PhiError):
+        ctx.set_stage_launch(0, 0, 0)
+    ctx.close()
+
+
 @pytest.mark.parametrize("cfg,steps", [("c2", 1000), ("c3", 1000)])
 def test_fp32_bulk_population_error_vs_fp64(cfg, steps):
     """FP32-bulk RK4 (FP32 RHS, FP64 state) against the FP64 path over 1 ps:
@@ -391,6 +433,8 @@ def test_autotune_keeps_state_and_parity():
     ctx2 = P.Context(p)
     ms2, shape2 = ctx2.autotune(dt, steps=3)
     assert shape2 == shape
+    print("stage shapes", ctx.stage_launch())
+    assert ctx2.stage_launch() == ctx.stage_launch()
     ctx.close()
     ctx2.close()
 
