@@ -15,22 +15,35 @@
 #include <hip/hip_runtime.h>
 
 #include "closure_kernels.h"
+#include "mlp_act.h"
 
 namespace phi {
 namespace closure {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ unsigned short to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
+using act::gelu;
+using act::to_bf16;
 
-__device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+// Largest x over the wavefront, in every lane: rotations by 1, 2, 4 and 8
+// within each row of 16 lanes (DPP row_ror, no LDS), then the four rows'
+// results read as scalars.
+__device__ __forceinline__ float wave_max(float x) {
+#define PHI_ROR(n)                                                                          \
+  x = fmaxf(x, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(                      \
+                   0, __builtin_bit_cast(int, x), 0x120 + (n), 0xf, 0xf, false)))
+  PHI_ROR(1);
+  PHI_ROR(2);
+  PHI_ROR(4);
+  PHI_ROR(8);
+#undef PHI_ROR
+  const int b = __builtin_bit_cast(int, x);
+  return fmaxf(fmaxf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 0)),
+                     __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 16))),
+               fmaxf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 32)),
+                     __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 48))));
 }
 
 template <typename TI>
@@ -143,15 +156,22 @@ __global__ void __launch_bounds__(256) scatter(const float* __restrict__ Y,
 // arithmetic as the unfused path: operands rounded to bfloat16, float32
 // sums in the same K order, bias and GELU in float32.
 //
-// A layer runs over groups of 256 output columns (the packed weights have
-// 256-row multiples, zero padded).  Within a group, wavefront w owns the
-// 64 x 64 output tile of row half w & 1 and column chunk w >> 1.  The
-// group's weights stream through LDS in K steps of 32: each thread loads
-// two 16-byte pieces of every step (coalesced, two steps ahead in
-// registers) and stores them in MFMA B-fragment order, piece (row, k-octet
-// o) at fragment (row / 16) * 64 + 16 o + row % 16, so a wavefront reads
-// each of its four fragments as one contiguous 1 KB block and every weight
-// is fetched once per block.
+// A layer runs over groups of output columns (the packed weights have
+// 256-row multiples, zero padded).  The 8 wavefronts split the block's 128
+// rows into RW row groups and a group's columns into 8 / RW chunks of 64:
+// RW = 2 (64 x 64 tiles, groups of 256 columns) for the hidden layers, RW =
+// 4 (32 x 64 tiles, groups of 128) for an output layer of at most 128
+// columns, so that no wavefront idles.  Tiles are computed transposed
+// (weights as the MFMA's A operand, activations as B): a lane's four
+// accumulator entries are then four consecutive output columns of one row,
+// stored as one 8-byte LDS write.  The group's weights stream through LDS
+// in K steps of 32: each thread loads 16-byte pieces of every step
+// (coalesced, two steps ahead in registers) and stores them in MFMA
+// fragment order, piece (row, k-octet o) at fragment (row / 16) * 64 +
+// 16 o + (row % 16 ^ Swz(o)), so a wavefront reads each of its four
+// fragments as one 1 KB block and every weight is fetched once per block.
+// Swz spreads the 16 pieces a group of 16 lanes stores (4 rows x 4 octets)
+// over distinct banks.
 constexpr int kRows = 128;
 constexpr int kThreads = 512;
 constexpr int kGroup = 256;
@@ -159,61 +179,80 @@ constexpr int kStepK = 32;
 constexpr int kStageFrags = kGroup * kStepK / 8;  // 16-byte fragments per stage (16 KB)
 constexpr size_t kStageBytes = (size_t)2 * kStageFrags * 16;
 
-template <class EPI>
+__device__ __forceinline__ int Swz(int oct) { return (4 * oct) ^ ((oct >> 1) << 1); }
+
+template <int RW, class EPI>
 __device__ __forceinline__ void fused_layer(const unsigned short* act, int stride,
                                             const unsigned short* __restrict__ W, int K,
                                             int ncols, const float* __restrict__ bias,
                                             bf16x8* stage, const EPI& epi) {
+  constexpr int kCW = 8 / RW;              // column chunks of 64
+  constexpr int kGrp = 64 * kCW;           // columns per group
+  constexpr int kNA = kRows / RW / 16;     // 16-row blocks per wavefront
+  constexpr int kPieces = kGrp * 4 / kThreads;  // 16-byte pieces per thread and step
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, c = lane & 15;
-  const int rh = wave & 1, cw = wave >> 1;
+  const int rq = wave % RW, cw = wave / RW;
   const int nsteps = K / kStepK;  // even: K is a multiple of 64
-  // This thread's two pieces of every stage.
-  int poff[2], pfrag[2];
+  // This thread's pieces of every stage.
+  int poff[kPieces], pfrag[kPieces];
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < kPieces; ++j) {
     const int pc = tid + kThreads * j, row = pc >> 2, oct = pc & 3;
     poff[j] = row * K + 8 * oct;
-    pfrag[j] = (row >> 4) * 64 + 16 * oct + (row & 15);
+    pfrag[j] = (row >> 4) * 64 + 16 * oct + ((row & 15) ^ Swz(oct));
   }
-  const unsigned short* arow = act + (64 * rh + c) * stride + 8 * q;
-  for (int g0 = 0; g0 < ncols; g0 += kGroup) {
+  const int wrow0 = (kRows / RW) * rq;
+  const unsigned short* arow = act + (wrow0 + c) * stride + 8 * q;
+  const int myfrag = 16 * q + (c ^ Swz(q));
+  for (int g0 = 0; g0 < ncols; g0 += kGrp) {
     const unsigned short* Wg = W + (size_t)g0 * K;
     const bool busy = g0 + 64 * cw < ncols;  // uniform over the wavefront
-    floatx4 acc[4][4];
+    // The epilogue's loads, issued before the K loop hides their latency
+    // and unconditional so that they need no wait of their own (columns
+    // past ncols read the zero padding of the packed biases and rows).
+    floatx4 bc[4];
+    typename EPI::Pre pre[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const int col0 = g0 + 64 * cw + 16 * b + 4 * q;
+      bc[b] = *reinterpret_cast<const floatx4*>(bias + col0);
+      pre[b] = epi.pre(col0);
+    }
+    floatx4 acc[kNA][4];
+#pragma unroll
+    for (int a = 0; a < kNA; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) acc[a][b] = (floatx4){0.f, 0.f, 0.f, 0.f};
     auto load = [&](bf16x8* r, int s) {
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < kPieces; ++j)
         r[j] = *reinterpret_cast<const bf16x8*>(Wg + poff[j] + s * kStepK);
     };
     auto put = [&](const bf16x8* r, int s) {
       bf16x8* st = stage + (s & 1) * kStageFrags;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) st[pfrag[j]] = r[j];
+      for (int j = 0; j < kPieces; ++j) st[pfrag[j]] = r[j];
     };
     auto compute = [&](int s) {
       if (!busy) return;
-      const bf16x8* st = stage + (s & 1) * kStageFrags + 4 * cw * 64 + lane;
-      bf16x8 bv[4], av[4];
+      const bf16x8* st = stage + (s & 1) * kStageFrags + 4 * cw * 64 + myfrag;
+      bf16x8 bv[4], av[kNA];
 #pragma unroll
       for (int b = 0; b < 4; ++b) bv[b] = st[64 * b];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < kNA; ++a)
         av[a] = *reinterpret_cast<const bf16x8*>(arow + 16 * a * stride + s * kStepK);
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < kNA; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[b], av[a], acc[a][b], 0, 0, 0);
     };
     // Step s computes from stage s & 1 while step s + 1 waits in registers
     // to be stored into the other stage and step s + 2 loads (steps past
     // the end reload the last one: no branch around loads).
-    bf16x8 r0[2], r1[2];
+    bf16x8 r0[kPieces], r1[kPieces];
     load(r0, 0);
     load(r1, 1);
     put(r0, 0);
@@ -229,26 +268,30 @@ __device__ __forceinline__ void fused_layer(const unsigned short* act, int strid
       __syncthreads();
     }
     if (busy) {
+      // Entry r of acc[a][b]: row wrow0 + 16 a + c, column col0 + r.
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const int col = g0 + 64 * cw + 16 * b + c;
-        if (col < ncols) {
-          const float bc = bias[col];
+        const int col0 = g0 + 64 * cw + 16 * b + 4 * q;
+        if (col0 < ncols) {
 #pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) epi(64 * rh + 16 * a + 4 * q + r, col, acc[a][b][r] + bc);
+          for (int a = 0; a < kNA; ++a) epi(wrow0 + 16 * a + c, col0, acc[a][b] + bc[b], pre[b]);
         }
       }
     }
   }
 }
 
+// Hidden layers: GELU, bfloat16, four columns in one 8-byte LDS write.
 struct ToLds {
+  struct Pre {};
   unsigned short* out;
   int stride;
-  __device__ __forceinline__ void operator()(int row, int col, float v) const {
-    out[row * stride + col] = to_bf16(gelu(v));
+  __device__ __forceinline__ Pre pre(int) const { return {}; }
+  __device__ __forceinline__ void operator()(int row, int col0, floatx4 v, Pre) const {
+    bf16x4 o;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = (__bf16)gelu(v[r]);
+    *reinterpret_cast<bf16x4*>(out + row * stride + col0) = o;
   }
 };
 
@@ -256,20 +299,42 @@ struct ToLds {
 // the network's outputs the stage kernels read, in slot order (packed output
 // 2 (j N + w) + part is element w of column site(j) of slot j, real or
 // imaginary part), and slotoff[col] is the float offset of packed output
-// col from the row's first slot.
+// col from the row's first slot, so an even column and the next one are the
+// real and imaginary part of one complex element (one vector store; the
+// packed count 2 N D is even).
 template <typename TO>
 struct ToSlots {
+  typedef typename C2T<TO>::type V;
   TO* slab;
   const long long* rowbase;  // per block row: first virtual slot's real offset, or -1
   const float* srow;
   int* rowbad;               // per block row: an output failed the guard
   const int* slotoff;
+  int ncols;
   float ymax;
-  __device__ __forceinline__ void operator()(int row, int col, float v) const {
+  // Slot offsets of the pairs (col0, col0 + 1) and (col0 + 2, col0 + 3)
+  // (indices clamped into the table; a pair past ncols is not stored).
+  typedef int2 Pre;
+  __device__ __forceinline__ Pre pre(int col0) const {
+    return int2{slotoff[min(col0, ncols - 2)], slotoff[min(col0 + 2, ncols - 2)]};
+  }
+  __device__ __forceinline__ void operator()(int row, int col0, floatx4 v, Pre off) const {
     const long long base = rowbase[row];
     if (base < 0) return;
-    slab[base + slotoff[col]] = (TO)(v * srow[row]);
-    if (!(fabsf(v) <= ymax)) rowbad[row] = 1;
+    const float s = srow[row];
+    bool bad = false;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int col = col0 + 2 * h;
+      if (col < ncols) {
+        V z;
+        z.x = (TO)(v[2 * h] * s);
+        z.y = (TO)(v[2 * h + 1] * s);
+        *reinterpret_cast<V*>(slab + base + (h ? off.y : off.x)) = z;
+        bad |= !(fabsf(v[2 * h]) <= ymax) || !(fabsf(v[2 * h + 1]) <= ymax);
+      }
+    }
+    if (bad) rowbad[row] = 1;
   }
 };
 
@@ -281,11 +346,12 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
                                                   const int* __restrict__ slotoff,
                                                   ClosureGuard guard) {
   typedef typename C2T<TI>::type V;
+  constexpr int kRowsPerWave = kRows / (kThreads / 64);
   extern __shared__ __align__(16) unsigned short lds[];
   __shared__ float srow[kRows];
   __shared__ long long rowbase[kRows];
   __shared__ long long rowsrc[kRows];
-  __shared__ unsigned smax[kRows];
+  __shared__ int rowb[kRows];
   __shared__ int rowbad[kRows];
   const int stride = net.width + 8;
   // The activations (each layer's output over its input), then the two
@@ -294,69 +360,57 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
   bf16x8* stage = reinterpret_cast<bf16x8*>(lds + kRows * stride);
   const long long R = (long long)members * nb;
   const long long row0 = (long long)blockIdx.x * kRows;
-  const int tid = threadIdx.x;
-  // Inputs, for all the block's rows at once (one trip to memory for the
-  // member offsets, one for the ADOs): each row's source node and slots.
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // Each row's source node, boundary node and slots.
   if (tid < kRows) {
     const long long row = row0 + tid;
-    smax[tid] = 0u;
     rowbad[tid] = 0;
     if (row < R) {
       const int m = (int)(row / nb), b = (int)(row - (long long)m * nb);
       const long long mo = moff[m];
       rowsrc[tid] = (mo + first + b) * ne;
       rowbase[tid] = 2 * ((mo + vbase + (long long)b * D) * ne);
+      rowb[tid] = b;
     } else {
       rowsrc[tid] = -1;
       rowbase[tid] = -1;
+      rowb[tid] = -1;
     }
   }
   __syncthreads();
-  // Elements (row, e) of the tile, several per thread: largest |Re|, |Im|
-  // of each row (non-negative floats order as their bits).
-  const int tot = kRows * ne;
-  constexpr int kPer = 16;  // enough for ne <= 64 (N <= 8)
-  V v[kPer];
+  // Inputs: wavefront w takes rows w * 16 .. w * 16 + 15, lane e element e
+  // of each (ne <= 64) and boundary multi-index entry e (D <= 64); a row's
+  // scale s (largest |Re|, |Im|) is a wavefront maximum.  Then the scaled
+  // elements and the multi-index and zero padding columns up to k1.  The
+  // loads are unconditional (lanes and rows without data read element 0 and
+  // drop it), so all of them are in flight at once: a load under a branch
+  // gets its own wait for memory.
+  V v[kRowsPerWave];
+  float bm[kRowsPerWave];
 #pragma unroll
-  for (int t = 0; t < kPer; ++t) {
-    const int i = tid + t * kThreads;
-    const int r = i / ne;
-    const long long src = i < tot ? rowsrc[r] : -1;
-    v[t] = src >= 0 ? reinterpret_cast<const V*>(slab)[src + (i - r * ne)] : V{};
+  for (int t = 0; t < kRowsPerWave; ++t) {
+    const int r = wave * kRowsPerWave + t;
+    const long long src = rowsrc[r];
+    const int b = rowb[r];
+    const bool has_v = src >= 0 && lane < ne, has_b = b >= 0 && lane < D;
+    const V x = reinterpret_cast<const V*>(slab)[has_v ? src + lane : 0];
+    const float y = bmi[has_b ? (size_t)b * D + lane : 0];
+    v[t] = has_v ? x : V{};
+    bm[t] = has_b ? y : 0.f;
   }
-#pragma unroll
-  for (int t = 0; t < kPer; ++t) {
-    const int i = tid + t * kThreads;
-    if (i < tot) {
-      const float mx = fmaxf(fabsf((float)v[t].x), fabsf((float)v[t].y));
-      atomicMax(&smax[i / ne], __float_as_uint(mx));
-    }
-  }
-  __syncthreads();
-  if (tid < kRows) srow[tid] = fmaxf(__uint_as_float(smax[tid]), 1e-30f);
-  __syncthreads();
-#pragma unroll
-  for (int t = 0; t < kPer; ++t) {
-    const int i = tid + t * kThreads;
-    if (i < tot) {
-      const int r = i / ne, e = i - r * ne;
-      const float inv = 1.0f / srow[r];
-      unsigned short* x = A0 + r * stride;
-      x[e] = to_bf16((float)v[t].x * inv);
-      x[ne + e] = to_bf16((float)v[t].y * inv);
-    }
-  }
-  // Boundary multi-index columns and the zero padding up to k1.
   const int extra = net.k1 - 2 * ne;
-  for (int i = tid; i < kRows * extra; i += kThreads) {
-    const int r = i / extra, k = i - r * extra;
-    const long long row = row0 + r;
-    unsigned short val = 0;
-    if (k < D && row < R) {
-      const int b = (int)(row % nb);
-      val = to_bf16(bmi[(size_t)b * D + k]);
+#pragma unroll
+  for (int t = 0; t < kRowsPerWave; ++t) {
+    const int r = wave * kRowsPerWave + t;
+    const float sc = fmaxf(wave_max(fmaxf(fabsf((float)v[t].x), fabsf((float)v[t].y))), 1e-30f);
+    const float inv = 1.0f / sc;
+    if (lane == 0) srow[r] = sc;
+    unsigned short* x = A0 + r * stride;
+    if (lane < ne) {
+      x[lane] = to_bf16((float)v[t].x * inv);
+      x[ne + lane] = to_bf16((float)v[t].y * inv);
     }
-    A0[r * stride + 2 * ne + k] = val;
+    for (int k = lane; k < extra; k += 64) x[2 * ne + k] = k < D ? to_bf16(bm[t]) : 0;
   }
   __syncthreads();
   // Hidden layers write their output over their input: every wavefront has
@@ -364,22 +418,26 @@ __global__ void __launch_bounds__(kThreads) fused(FusedNet net, TI* slab,
   // code serves both.
 #pragma unroll 1
   for (int layer = 0; layer < 2; ++layer) {
-    fused_layer(A0, stride, layer ? net.w2 : net.w1, layer ? net.hid : net.k1, net.hid,
-                layer ? net.b2 : net.b1, stage, ToLds{A0, stride});
+    fused_layer<2>(A0, stride, layer ? net.w2 : net.w1, layer ? net.hid : net.k1, net.hid,
+                   layer ? net.b2 : net.b1, stage, ToLds{A0, stride});
     __syncthreads();
   }
-  fused_layer(A0, stride, net.w3, net.hid, net.n_out, net.b3, stage,
-              ToSlots<TI>{slab, rowbase, srow, rowbad, slotoff, guard.ymax});
+  const ToSlots<TI> to_slots{slab, rowbase, srow, rowbad, slotoff, net.n_out, guard.ymax};
+  if (net.n_out <= 128)
+    fused_layer<4>(A0, stride, net.w3, net.hid, net.n_out, net.b3, stage, to_slots);
+  else
+    fused_layer<2>(A0, stride, net.w3, net.hid, net.n_out, net.b3, stage, to_slots);
   // Rows whose outputs failed the guard fall back to plain truncation: zero
   // slots (a second write after every wavefront's first one).
   __syncthreads();
-  int nbad = 0;
-  for (int r = 0; r < kRows; ++r) {
-    if (!rowbad[r]) continue;
-    ++nbad;
-    for (int col = tid; col < net.n_out; col += kThreads) slab[rowbase[r] + slotoff[col]] = (TI)0;
+  const int nbad = __syncthreads_count(tid < kRows && rowbad[tid]);
+  if (nbad) {
+    for (int i = tid; i < kRows * net.n_out; i += kThreads) {
+      const int r = i / net.n_out, col = i - r * net.n_out;
+      if (rowbad[r]) slab[rowbase[r] + slotoff[col]] = (TI)0;
+    }
+    if (tid == 0) atomicAdd(guard.fallen, (unsigned long long)nbad);
   }
-  if (tid == 0 && nbad) atomicAdd(guard.fallen, (unsigned long long)nbad);
 }
 
 }  // namespace closure
@@ -392,9 +450,11 @@ size_t FusedClosureLds(const FusedNet& net) {
 bool FusedClosureFits(const FusedNet& net) {
   // The activations and two weight stages in one CU's 160 KB of LDS, with
   // the block's per-row tables; K steps in pairs of 32; hidden layers of
-  // one column group (their output overwrites their input).
+  // one column group (their output overwrites their input); outputs in
+  // (real, imaginary) pairs.
   return FusedClosureLds(net) + closure::kRows * 32 <= (size_t)160 * 1024 &&
-         net.k1 % 64 == 0 && net.hid % 64 == 0 && net.hid <= closure::kGroup;
+         net.k1 % 64 == 0 && net.hid % 64 == 0 && net.hid <= closure::kGroup &&
+         net.n_out % 2 == 0;
 }
 
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
@@ -404,7 +464,7 @@ hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const 
   const long long rows = (long long)members * nb;
   const unsigned blocks = (unsigned)((rows + closure::kRows - 1) / closure::kRows);
   const size_t lds = FusedClosureLds(net);
-  if (!FusedClosureFits(net) || ne > 64) return hipErrorInvalidConfiguration;
+  if (!FusedClosureFits(net) || ne > 64 || D > 64) return hipErrorInvalidConfiguration;
   const void* k = fp32 ? reinterpret_cast<const void*>(&closure::fused<float>)
                        : reinterpret_cast<const void*>(&closure::fused<double>);
   hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

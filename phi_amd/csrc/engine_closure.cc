@@ -239,7 +239,7 @@ int Engine::AttachClosure(Mlp* mlp) {
 int Engine::ClosurePredict(const void* in, bool in_fp32) {
   const int NE = h_.N * h_.N, D = h_.D;
   const ClosureGuard guard{closure_ymax_, d_cfall_};
-  if (closure_fused_ && FusedClosureFits(fused_) && NE <= 64)
+  if (closure_fused_ && FusedClosureFits(fused_) && NE <= 64 && D <= 64)
     return Check(LaunchClosureFused(fused_, const_cast<void*>(in), in_fp32, d_moff_, members_,
                                     closure_first_, closure_nb_, NE, D, d_cbmi_, closure_vbase_,
                                     d_cslot_, guard, stream_), "closure fused");

@@ -117,9 +117,10 @@ int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm, int n_packed) {
   const size_t n1 = (size_t)rows256(hid) * k1, n2 = (size_t)rows256(hid) * hid,
                n3 = (size_t)rows256(n_packed) * hid;
   // One buffer: the bfloat16 weights, then (16-byte aligned) the float32
-  // biases b1, b2, b3.
+  // biases b1, b2, b3 (b3 zero padded like W3's rows: the fused pass reads
+  // the biases of 4 output columns at once).
   const size_t nw = (n1 + n2 + n3 + 7) / 8 * 8;
-  const size_t nb = (size_t)2 * hid + n_packed;
+  const size_t nb = (size_t)2 * hid + rows256(n_packed);
   std::vector<unsigned short> w(nw + 2 * nb, 0);
   for (int o = 0; o < hid; ++o)
     for (int i = 0; i < n_in_; ++i) w[(size_t)o * k1 + i] = Bf16(p[off_[0] + (size_t)o * n_in_ + i]);

@@ -10,6 +10,7 @@
 
 #include <cstdint>
 
+#include "mlp_act.h"
 #include "mlp_kernels.h"
 
 namespace phi {
@@ -18,22 +19,9 @@ namespace mlp {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-// float32 -> bfloat16 bits, round to nearest even (NaN kept quiet).
-__device__ __forceinline__ unsigned short to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-
-__device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
-}
-
-__device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * 0.39894228040143268f * expf(-0.5f * z * z);
-}
+using act::gelu;
+using act::gelu_grad;
+using act::to_bf16;
 
 // --- Strided GEMM -----------------------------------------------------------------
 // C(i, j) = sum_k A(i, k) B(k, j), A(i, k) = A[i*sa_i + k*sa_k],
