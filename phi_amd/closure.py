@@ -153,3 +153,46 @@ def evaluate(Hs, net, deep, shallow, steps, dt, every, precision="fp64", deep_pr
         out[name]["max_pop_err"] = float(np.abs(out[name]["pops"] - ref).max())
         out[name]["mean_pop_err"] = float(np.abs(out[name]["pops"] - ref).mean())
     return out
+
+
+def timing_run(members=256, shallow=5, steps=10, dt=0.001, precision="fp32", seed=3,
+               autotune=True, log=print):
+    """A depth-`shallow` ensemble of `members` held-out Hamiltonians with a
+    closure network of random weights attached, advanced `steps` RK4 steps
+    (after autotuning, or with the default launch shape): the launches of a
+    closure run (stage kernels and fused closure passes, one per stage) for
+    timing and profiling, with no accuracy meaning."""
+    p = base_params()
+    n_in, n_out = 2 * 49 + 7, 2 * 49 * 7
+    net = P.Mlp(n_in, 256, n_out, max_batch=8192)
+    net.set_params(init_flat(n_in, 256, n_out, seed))
+    ens = P.Ensemble(p, hamiltonians(100000, members), [shallow] * members, precision=precision)
+    ens.attach_closure(net)
+    if autotune:
+        ens.autotune(dt, steps=3)
+    ens.rk4_steps(2, dt)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.rk4_steps(steps, dt)
+    ens.synchronize()
+    el = time.perf_counter() - t0
+    log("closure timing run: %d members, depth %d, %s, %d slab nodes: %.4f ms/step"
+        % (members, shallow, precision, ens.num_nodes, 1e3 * el / steps))
+    ens.close()
+    net.close()
+    return el / steps
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(
+        description="closure timing run (random network weights; launches only)")
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--depth", type=int, default=5, help="PHI depth (HierarchyTruncation)")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default launch shape (short runs under hardware counters)")
+    a = ap.parse_args()
+    timing_run(a.members, a.depth, a.steps, precision=a.precision, autotune=not a.no_autotune)

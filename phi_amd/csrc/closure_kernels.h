@@ -16,7 +16,7 @@ struct FusedNet {
   const float* b1;
   const float* b2;
   const float* b3;
-  int k1;     // inputs rounded up to a multiple of 32
+  int k1;     // inputs rounded up to a multiple of 64
   int hid;    // hidden width, a multiple of 32
   int n_out;
   int width;  // max(k1, hid): row width of the LDS activations

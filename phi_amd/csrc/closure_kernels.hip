@@ -165,46 +165,50 @@ __global__ void __launch_bounds__(256) scatter(const float* __restrict__ Y,
 // (weights as the MFMA's A operand, activations as B): a lane's four
 // accumulator entries are then four consecutive output columns of one row,
 // stored as one 8-byte LDS write.  The group's weights stream through LDS
-// in K steps of 32: each thread loads 16-byte pieces of every step
-// (coalesced, two steps ahead in registers) and stores them in MFMA
-// fragment order, piece (row, k-octet o) at fragment (row / 16) * 64 +
-// 16 o + (row % 16 ^ Swz(o)), so a wavefront reads each of its four
-// fragments as one 1 KB block and every weight is fetched once per block.
-// Swz spreads the 16 pieces a group of 16 lanes stores (4 rows x 4 octets)
-// over distinct banks.
+// in K steps of 64 (two MFMA k-halves of 32, one barrier per step): each
+// thread loads 16-byte pieces of every step (coalesced, two steps ahead in
+// registers) and stores piece (row, k-octet o) at slot 16 o + (row % 16 ^
+// 2 o % 16) of its 16-row block of 128 slots, so that a wavefront reads each
+// of its fragments as one 1 KB block, the 16 pieces that 16 consecutive
+// lanes store (2 rows x 8 octets) fall on distinct banks, and every weight
+// is fetched once per block.
 constexpr int kRows = 128;
 constexpr int kThreads = 512;
 constexpr int kGroup = 256;
-constexpr int kStepK = 32;
-constexpr int kStageFrags = kGroup * kStepK / 8;  // 16-byte fragments per stage (16 KB)
+constexpr int kStepK = 64;
+constexpr int kOcts = kStepK / 8;                 // k-octets per row and step
+constexpr int kStageFrags = kGroup * kOcts;       // 16-byte fragments per stage (32 KB)
 constexpr size_t kStageBytes = (size_t)2 * kStageFrags * 16;
 
-__device__ __forceinline__ int Swz(int oct) { return (4 * oct) ^ ((oct >> 1) << 1); }
+__device__ __forceinline__ int Slot(int row, int oct) {
+  return (row >> 4) * (16 * kOcts) + 16 * oct + ((row & 15) ^ ((2 * oct) & 15));
+}
 
 template <int RW, class EPI>
 __device__ __forceinline__ void fused_layer(const unsigned short* act, int stride,
                                             const unsigned short* __restrict__ W, int K,
                                             int ncols, const float* __restrict__ bias,
                                             bf16x8* stage, const EPI& epi) {
-  constexpr int kCW = 8 / RW;              // column chunks of 64
-  constexpr int kGrp = 64 * kCW;           // columns per group
-  constexpr int kNA = kRows / RW / 16;     // 16-row blocks per wavefront
-  constexpr int kPieces = kGrp * 4 / kThreads;  // 16-byte pieces per thread and step
+  constexpr int kCW = 8 / RW;                           // column chunks of 64
+  constexpr int kGrp = 64 * kCW;                        // columns per group
+  constexpr int kNA = kRows / RW / 16;                  // 16-row blocks per wavefront
+  constexpr int kPieces = kGrp * kOcts / kThreads;      // 16-byte pieces per thread and step
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int q = lane >> 4, c = lane & 15;
   const int rq = wave % RW, cw = wave / RW;
-  const int nsteps = K / kStepK;  // even: K is a multiple of 64
+  const int nsteps = K / kStepK;  // K is a multiple of 64
   // This thread's pieces of every stage.
-  int poff[kPieces], pfrag[kPieces];
+  int poff[kPieces], pslot[kPieces];
 #pragma unroll
   for (int j = 0; j < kPieces; ++j) {
-    const int pc = tid + kThreads * j, row = pc >> 2, oct = pc & 3;
+    const int pc = tid + kThreads * j, row = pc / kOcts, oct = pc % kOcts;
     poff[j] = row * K + 8 * oct;
-    pfrag[j] = (row >> 4) * 64 + 16 * oct + ((row & 15) ^ Swz(oct));
+    pslot[j] = Slot(row, oct);
   }
   const int wrow0 = (kRows / RW) * rq;
   const unsigned short* arow = act + (wrow0 + c) * stride + 8 * q;
-  const int myfrag = 16 * q + (c ^ Swz(q));
+  // This lane's fragment of k-half h of a 16-row block: octet 4 h + q, row c.
+  const int myslot[2] = {Slot(c, q), Slot(c, 4 + q)};
   for (int g0 = 0; g0 < ncols; g0 += kGrp) {
     const unsigned short* Wg = W + (size_t)g0 * K;
     const bool busy = g0 + 64 * cw < ncols;  // uniform over the wavefront
@@ -232,40 +236,46 @@ __device__ __forceinline__ void fused_layer(const unsigned short* act, int strid
     auto put = [&](const bf16x8* r, int s) {
       bf16x8* st = stage + (s & 1) * kStageFrags;
 #pragma unroll
-      for (int j = 0; j < kPieces; ++j) st[pfrag[j]] = r[j];
+      for (int j = 0; j < kPieces; ++j) st[pslot[j]] = r[j];
     };
     auto compute = [&](int s) {
       if (!busy) return;
-      const bf16x8* st = stage + (s & 1) * kStageFrags + 4 * cw * 64 + myfrag;
-      bf16x8 bv[4], av[kNA];
+      const bf16x8* st = stage + (s & 1) * kStageFrags + 4 * cw * (16 * kOcts);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = st[64 * b];
+      for (int h = 0; h < 2; ++h) {
+        bf16x8 bv[4], av[kNA];
 #pragma unroll
-      for (int a = 0; a < kNA; ++a)
-        av[a] = *reinterpret_cast<const bf16x8*>(arow + 16 * a * stride + s * kStepK);
+        for (int b = 0; b < 4; ++b) bv[b] = st[b * (16 * kOcts) + myslot[h]];
 #pragma unroll
-      for (int a = 0; a < kNA; ++a)
+        for (int a = 0; a < kNA; ++a)
+          av[a] = *reinterpret_cast<const bf16x8*>(arow + 16 * a * stride + s * kStepK + 32 * h);
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[b], av[a], acc[a][b], 0, 0, 0);
+        for (int a = 0; a < kNA; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[b], av[a], acc[a][b], 0, 0, 0);
+      }
     };
     // Step s computes from stage s & 1 while step s + 1 waits in registers
     // to be stored into the other stage and step s + 2 loads (steps past
-    // the end reload the last one: no branch around loads).
+    // the end reload the last one: no branch around loads).  The step
+    // count is uniform over the block, so are the branches around barriers.
     bf16x8 r0[kPieces], r1[kPieces];
     load(r0, 0);
-    load(r1, 1);
+    load(r1, min(1, nsteps - 1));
     put(r0, 0);
     __syncthreads();
     for (int s = 0; s < nsteps; s += 2) {
       load(r0, min(s + 2, nsteps - 1));
       compute(s);
-      put(r1, s + 1);
+      if (s + 1 < nsteps) put(r1, s + 1);
       __syncthreads();
-      load(r1, min(s + 3, nsteps - 1));
-      compute(s + 1);
-      if (s + 2 < nsteps) put(r0, s + 2);
-      __syncthreads();
+      if (s + 1 < nsteps) {
+        load(r1, min(s + 3, nsteps - 1));
+        compute(s + 1);
+        if (s + 2 < nsteps) put(r0, s + 2);
+        __syncthreads();
+      }
     }
     if (busy) {
       // Entry r of acc[a][b]: row wrow0 + 16 a + c, column col0 + r.
@@ -449,12 +459,12 @@ size_t FusedClosureLds(const FusedNet& net) {
 
 bool FusedClosureFits(const FusedNet& net) {
   // The activations and two weight stages in one CU's 160 KB of LDS, with
-  // the block's per-row tables; K steps in pairs of 32; hidden layers of
-  // one column group (their output overwrites their input); outputs in
-  // (real, imaginary) pairs.
+  // the block's per-row tables; K steps of 64; hidden layers of one column
+  // group (their output overwrites their input); outputs in (real,
+  // imaginary) pairs.
   return FusedClosureLds(net) + closure::kRows * 32 <= (size_t)160 * 1024 &&
-         net.k1 % 64 == 0 && net.hid % 64 == 0 && net.hid <= closure::kGroup &&
-         net.n_out % 2 == 0;
+         net.k1 % closure::kStepK == 0 && net.hid % closure::kStepK == 0 &&
+         net.hid <= closure::kGroup && net.n_out % 2 == 0;
 }
 
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,

@@ -109,7 +109,7 @@ int Mlp::Pack(FusedNet* out, void** owned, const int* out_perm, int n_packed) {
   std::vector<float> p(nparam_);
   int rc = GetParams(p.data());
   if (rc) return rc;
-  const int k1 = (n_in_ + 31) / 32 * 32;
+  const int k1 = (n_in_ + 63) / 64 * 64;
   const int hid = n_hid_;
   // Rows padded to multiples of 256: the fused pass streams whole groups of
   // 256 output columns.
