@@ -880,8 +880,10 @@ int Engine::RhsNodes(const int* nodes, int count, std::complex<double>* y) {
 }
 
 int Engine::StepForm() const {
-  // The closed RHS is not linear: PHI's classic stage order then.
-  if (closure_) return kRk4Classic;
+  // The closed RHS is not linear: PHI's classic stage order then, with the
+  // bulk accumulator when the bulk is FP32 (the stage inputs are FP32 values
+  // either way).
+  if (closure_) return fp32_ && args_.col_nm == 2 ? kRk4ClassicBulk : kRk4Classic;
   if (rk4_form_ == kRk4Increment && args_.col_nm != 2) return kRk4Nested;
   return rk4_form_;
 }

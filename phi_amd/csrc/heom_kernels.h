@@ -88,7 +88,10 @@ size_t StreamTileLdsBytes(int N, int D, int L, int tile, bool fp32);
 // Stage orders of an RK4 step: PHI's classic order (FP64 accumulator), the
 // nested (Horner) order and its increment form (heom_stream.h; persistent
 // kernels only).
-enum { kRk4Classic = 0, kRk4Nested = 1, kRk4Increment = 2 };
+// kRk4ClassicBulk: PHI's classic order with a bulk-precision accumulator and
+// a bulk copy of rho (internal: the learned closure's FP32 runs, whose RHS
+// is not linear; persistent kernels only).
+enum { kRk4Classic = 0, kRk4Nested = 1, kRk4Increment = 2, kRk4ClassicBulk = 3 };
 // One RK4 stage.  aux: the classic form's FP64 accumulator, or the
 // increment form's K (bulk precision); unused by the nested form.
 hipError_t LaunchRk4Stage(const RhsArgs& a, bool fp32, int stage, int form, const void* in,

@@ -66,7 +66,8 @@ inline hipError_t LaunchStream(const RhsArgs& a0, const TI* in, const EPI& epi, 
 
 // RK4 stage through the persistent kernels (stage 0 reads the FP64 state).
 // form: kRk4Classic (aux = FP64 accumulator), kRk4Nested, kRk4Increment
-// (aux = K, bulk precision).
+// (aux = K, bulk precision), kRk4ClassicBulk (aux = bulk accumulator, then
+// the bulk copy of rho).
 template <typename TB>
 inline hipError_t LaunchRk4Stream(const RhsArgs& a, int stage, int form, const void* in,
                                   double* rho, void* aux, TB* yout, double dt, hipStream_t s) {
@@ -80,6 +81,17 @@ inline hipError_t LaunchRk4Stream(const RhsArgs& a, int stage, int form, const v
       case 1: return LaunchStream<TB, TB>(a, y, dev::EpiIncr<TB, 1>{rho, kk, yout, dt}, s);
       case 2: return LaunchStream<TB, TB>(a, y, dev::EpiIncr<TB, 2>{rho, kk, yout, dt}, s);
       default: return LaunchStream<TB, TB>(a, y, dev::EpiIncr<TB, 3>{rho, kk, yout, dt}, s);
+    }
+  }
+  if (form == kRk4ClassicBulk) {
+    // aux = accumulator (first half) and bulk copy of rho (second half)
+    TB* ab = (TB*)aux;
+    TB* rb = ab + 2 * (size_t)a.num_nodes * a.NE;
+    switch (stage) {
+      case 0: return LaunchStream<TB, double>(a, x, dev::EpiClassicBulk<TB, 0, true>{rho, ab, rb, yout, dt}, s);
+      case 1: return LaunchStream<TB, TB>(a, y, dev::EpiClassicBulk<TB, 1, false>{rho, ab, rb, yout, dt}, s);
+      case 2: return LaunchStream<TB, TB>(a, y, dev::EpiClassicBulk<TB, 2, false>{rho, ab, rb, yout, dt}, s);
+      default: return LaunchStream<TB, TB>(a, y, dev::EpiClassicBulk<TB, 3, false>{rho, ab, rb, yout, dt}, s);
     }
   }
   if (form == kRk4Nested) {
@@ -131,7 +143,7 @@ template <typename TB, typename TI>
 inline hipError_t LaunchRk4T(const RhsArgs& a0, int stage, int form, const TI* in, double* rho,
                              void* aux, TB* yout, double dt, hipStream_t s) {
   if (a0.col_nm == 2) return LaunchRk4Stream<TB>(a0, stage, form, in, rho, aux, yout, dt, s);
-  if (form == kRk4Increment) return hipErrorInvalidValue;
+  if (form == kRk4Increment || form == kRk4ClassicBulk) return hipErrorInvalidValue;
   const bool horner = form == kRk4Nested;
   double* acc = (double*)aux;
   int threads;

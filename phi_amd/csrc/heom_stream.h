@@ -656,6 +656,57 @@ struct EpiClassic {
   }
 };
 
+// Classic order with the accumulator and a copy of rho in the bulk
+// precision (kRk4ClassicBulk, the learned closure's FP32 runs): the stage
+// inputs are bulk values in PHI's classic form too, so forming them from
+// the bulk copy changes only their rounding, and the accumulator holds only
+// the step's increment; rho is read by the first and last stage only:
+//   s0: acc = dt/6 k, rb = rho, v = rho + dt/2 k     s1: acc += dt/3 k, v = rb + dt/2 k
+//   s2: acc += dt/3 k, v = rb + dt k                  s3: rho += acc + dt/6 k
+// Per element and step 2 x 16 + 14 x s bytes (s = bulk complex size) against
+// the classic form's 10 x 16 + 6 x s.
+template <typename TB, int STAGE, bool IN_STATE>
+struct EpiClassicBulk {
+  typedef typename C2<TB>::type V;
+  struct Pre { double2 x; V q, r; };
+  static constexpr bool kInIsState = IN_STATE;
+  static constexpr bool kReduce = false;
+  double* rho;
+  TB* acc;
+  TB* rb;
+  TB* yout;
+  double dt;
+  __device__ __forceinline__ void finalize(double) const {}
+  __device__ __forceinline__ Pre pre(size_t g) const {
+    Pre p;
+    if (STAGE == 0 || STAGE == 3) p.x = ld2_nt(rho, g);
+    if (STAGE == 1 || STAGE == 2) p.r = ldv_nt<TB>(rb, g);
+    if (STAGE >= 1) p.q = ldv_nt<TB>(acc, g);
+    return p;
+  }
+  __device__ __forceinline__ Pre from_state(double2 x) const {
+    Pre p;
+    p.x = x;
+    return p;
+  }
+  __device__ __forceinline__ void apply(size_t g, double2 k, const Pre& p, double*) const {
+    const TB kx = (TB)k.x, ky = (TB)k.y;
+    if (STAGE == 0) {
+      const TB c6 = (TB)(dt / 6);
+      stv_nt<TB>(acc, g, mk<TB>(c6 * kx, c6 * ky));
+      stv_nt<TB>(rb, g, mk<TB>((TB)p.x.x, (TB)p.x.y));
+      st_nt<TB>(yout, g, p.x.x + (dt / 2) * k.x, p.x.y + (dt / 2) * k.y);
+    } else if (STAGE <= 2) {
+      const TB c3 = (TB)(dt / 3), c = (TB)(STAGE == 1 ? dt / 2 : dt);
+      stv_nt<TB>(acc, g, mk<TB>(p.q.x + c3 * kx, p.q.y + c3 * ky));
+      stv_nt<TB>(yout, g, mk<TB>(p.r.x + c * kx, p.r.y + c * ky));
+    } else {
+      st_nt<double>(rho, g, p.x.x + ((double)p.q.x + (dt / 6) * k.x),
+                    p.x.y + ((double)p.q.y + (dt / 6) * k.y));
+    }
+  }
+};
+
 // --- RKF45 epilogue ------------------------------------------------------------
 // Stage s < 5 stores k_s (bulk precision) and the next stage input
 //   tmp = rho + sum_{j <= s} b[s][j] k_j

@@ -203,6 +203,36 @@ def test_closure_injects_predicted_neighbours():
     assert err <= 1e-3 * scale
 
 
+def test_fp32_closure_run_tracks_fp64():
+    """A closure run with FP32 bulk (PHI's classic RK4 order with the
+    accumulator and a copy of rho in FP32; the stage inputs are FP32 values
+    in either FP32 form) follows the FP64 closure run (FP64 accumulator and
+    state): populations within 1e-6, coherences reported, after 200 steps
+    of a network with small random outputs."""
+    from phi_amd.configs.make_configs import c4_member
+    p = _c2_params()
+    dt = p.get_double("timestep")
+    Hs = np.array([c4_member(r)[0] for r in range(4)])
+    params = M.init_params(105, 256, 686, seed=4)
+    params[2] = (params[2][0] * 0.05, params[2][1] * 0.05)
+    net = P.Mlp(105, 256, 686, max_batch=4096)
+    net.set_params(_flat(params))
+    out = {}
+    for prec in ("fp64", "fp32"):
+        print("case: closure run", prec, flush=True)
+        ens = P.Ensemble(p, Hs, [5] * 4, precision=prec)
+        ens.attach_closure(net)
+        ens.rk4_steps(200, dt)
+        out[prec] = ens.member_rho0()
+        ens.close()
+    d = out["fp32"] - out["fp64"]
+    pop = np.abs(np.real(np.einsum("mii->mi", d))).max()
+    coh = np.abs(d - np.einsum("mii,ij->mij", d, np.eye(7))).max()
+    print("closure fp32 vs fp64: population %.3e, coherence %.3e" % (pop, coh))
+    assert np.isfinite(out["fp64"]).all()
+    assert pop <= 1e-6
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_fused_closure_pass_equals_unfused_and_network(prec):
     """The fused closure launch (inputs + 3 layers + scatter) gives the same
