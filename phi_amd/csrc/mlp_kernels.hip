@@ -29,11 +29,16 @@ using act::to_bf16;
 // wavefronts of 32 x 32 (2 x 2 MFMA tiles of 16 x 16).  Lane l of
 // v_mfma_f32_16x16x32_bf16 holds A[l%16][8(l/16) .. 8(l/16)+7] and
 // B[8(l/16) .. +7][l%16]; output register r holds D[4(l/16) + r][l%16].
+// The next K step's operands load into registers while the current one is
+// multiplied from LDS (two LDS buffers, one barrier per step); the loads are
+// unconditional (an element outside the matrices reads element 0 and is
+// replaced by zero), so all of a step's loads are in flight at once.
 constexpr int kBM = 64, kBN = 64, kBK = 32, kPad = 8;
+constexpr int kPer = kBM * kBK / 256;  // elements per thread and operand
 
 __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
-  __shared__ __align__(16) unsigned short As[kBM][kBK + kPad];
-  __shared__ __align__(16) unsigned short Bs[kBN][kBK + kPad];  // Bs[j][k]
+  __shared__ __align__(16) unsigned short As[2][kBM][kBK + kPad];
+  __shared__ __align__(16) unsigned short Bs[2][kBN][kBK + kPad];  // Bs[j][k]
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
@@ -44,43 +49,56 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (floatx4){0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += kBK) {
-    // Stage the tiles (zero outside the matrices).
+  // This thread's elements of a step, (tile row, k) = (tid / 32 + 8 t,
+  // tid % 32) when the operand is contiguous along k, else (tid % 64,
+  // tid / 64 + 4 t): consecutive lanes read consecutive addresses either way.
+  const bool a_rows = g.sa_k != 1, b_rows = g.sb_k != 1;
+  int ar[kPer], ak[kPer], br[kPer], bk[kPer];
 #pragma unroll
-    for (int t = 0; t < (kBM * kBK) / 256; ++t) {
-      const int e = tid + t * 256;
-      const int r = e / kBK, kk = e % kBK;
-      const int i = i0 + r, k = k0 + kk;
-      float v = 0.f;
-      if (i < g.M && k < g.K) v = g.A[(long long)i * g.sa_i + (long long)k * g.sa_k];
-      As[r][kk] = to_bf16(v);
-    }
+  for (int t = 0; t < kPer; ++t) {
+    ar[t] = a_rows ? (tid & 63) : (tid >> 5) + 8 * t;
+    ak[t] = a_rows ? (tid >> 6) + 4 * t : (tid & (kBK - 1));
+    br[t] = b_rows ? (tid & 63) : (tid >> 5) + 8 * t;
+    bk[t] = b_rows ? (tid >> 6) + 4 * t : (tid & (kBK - 1));
+  }
+  float ra[kPer], rb[kPer];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int t = 0; t < (kBN * kBK) / 256; ++t) {
-      const int e = tid + t * 256;
-      const int r = e / kBK, kk = e % kBK;
-      const int j = j0 + r, k = k0 + kk;
-      float v = 0.f;
-      if (j < g.N && k < g.K) v = g.B[(long long)k * g.sb_k + (long long)j * g.sb_j];
-      Bs[r][kk] = to_bf16(v);
+    for (int t = 0; t < kPer; ++t) {
+      const int i = i0 + ar[t], ka = k0 + ak[t], j = j0 + br[t], kb = k0 + bk[t];
+      const bool oa = i < g.M && ka < g.K, ob = j < g.N && kb < g.K;
+      const float va = g.A[oa ? (long long)i * g.sa_i + (long long)ka * g.sa_k : 0];
+      const float vb = g.B[ob ? (long long)kb * g.sb_k + (long long)j * g.sb_j : 0];
+      ra[t] = oa ? va : 0.f;
+      rb[t] = ob ? vb : 0.f;
     }
-    __syncthreads();
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+      As[buf][ar[t]][ak[t]] = to_bf16(ra[t]);
+      Bs[buf][br[t]][bk[t]] = to_bf16(rb[t]);
+    }
+  };
+  load(0);
+  put(0);
+  __syncthreads();
+  const int nsteps = (g.K + kBK - 1) / kBK;
+  for (int st = 0; st < nsteps; ++st) {
+    const int cur = st & 1;
+    load((st + 1) * kBK);  // past the end: every element is outside, zero
     bf16x8 af[2], bfr[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-      const unsigned short* pa = &As[wm * 32 + a * 16 + c][8 * q];
-      const unsigned short* pb = &Bs[wn * 32 + a * 16 + c][8 * q];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        af[a][t] = __builtin_bit_cast(__bf16, pa[t]);
-        bfr[a][t] = __builtin_bit_cast(__bf16, pb[t]);
-      }
+      af[a] = *reinterpret_cast<const bf16x8*>(&As[cur][wm * 32 + a * 16 + c][8 * q]);
+      bfr[a] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn * 32 + a * 16 + c][8 * q]);
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    if (st + 1 < nsteps) put(cur ^ 1);
     __syncthreads();
   }
   // Epilogue.
@@ -116,14 +134,36 @@ __global__ void gelu_backward(const float* __restrict__ dH, const float* __restr
     dZ[i] = dH[i] * gelu_grad(Z[i]);
 }
 
-// out[j] = sum_i X[i*ld + j] over rows, in row order (deterministic).
-__global__ void column_sums(const float* __restrict__ X, int rows, int cols, long long ld,
-                            float* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cols) return;
+// out[j] = sum_i X[i*ld + j] over rows, in a fixed order (deterministic):
+// thread (g, c) of a block sums rows g, g + 16, ... of column
+// 16 blockIdx.x + c in row order (8 loads in flight), then the 16 row
+// groups' sums are added in group order: the same order on every launch.
+constexpr int kSumCols = 16, kSumGroups = 16;
+
+__global__ void __launch_bounds__(256) column_sums(const float* __restrict__ X, int rows,
+                                                   int cols, long long ld,
+                                                   float* __restrict__ out) {
+  __shared__ float part[kSumGroups][kSumCols];
+  const int c = threadIdx.x % kSumCols, grp = threadIdx.x / kSumCols;
+  const int j = blockIdx.x * kSumCols + c;
+  const int jc = j < cols ? j : cols - 1;  // loads stay in the matrix
   float s = 0.f;
-  for (int i = 0; i < rows; ++i) s += X[(long long)i * ld + j];
-  out[j] = s;
+  int i = grp;
+  for (; i + 7 * kSumGroups < rows; i += 8 * kSumGroups) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = X[(long long)(i + u * kSumGroups) * ld + jc];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; i < rows; i += kSumGroups) s += X[(long long)i * ld + jc];
+  part[grp][c] = s;
+  __syncthreads();
+  if (grp == 0 && j < cols) {
+    float t = part[0][c];
+    for (int q = 1; q < kSumGroups; ++q) t += part[q][c];
+    out[j] = t;
+  }
 }
 
 // dY = 2 (Y - T) / n; partial[block] = sum of (Y - T)^2 over the block's
@@ -196,8 +236,8 @@ hipError_t LaunchGeluBackward(const float* dH, const float* Z, float* dZ, long l
 
 hipError_t LaunchColumnSums(const float* X, int rows, int cols, long long ld, float* out,
                             hipStream_t s) {
-  hipLaunchKernelGGL(mlp::column_sums, dim3((cols + 255) / 256), dim3(256), 0, s, X, rows, cols,
-                     ld, out);
+  hipLaunchKernelGGL(mlp::column_sums, dim3((cols + mlp::kSumCols - 1) / mlp::kSumCols),
+                     dim3(256), 0, s, X, rows, cols, ld, out);
   return hipGetLastError();
 }
 
