@@ -183,6 +183,46 @@ def full_shape(ctx, shape):
     return tuple(shape[:4]) + tuple(v for s in range(4) for v in (th[s], bl[s]))
 
 
+def copy_reference_gbs(nbytes=1 << 30, reps=10):
+    """Device-to-device hipMemcpy rate of the HIP runtime the library
+    loaded (nbytes per copy, read + write counted, HIP events on the null
+    stream around `reps` copies after two untimed ones): the streaming rate
+    this GPU reaches in practice, reported beside the published peak.  None
+    when it cannot be measured."""
+    import ctypes
+    try:
+        path = next(l.split()[-1] for l in open("/proc/self/maps") if "libamdhip64" in l)
+        hip = ctypes.CDLL(path)
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        ms = ctypes.c_float()
+        ok = hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+        ok = ok and hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+        ok = ok and hip.hipMemset(a, 0, ctypes.c_size_t(nbytes)) == 0
+        ok = ok and hip.hipEventCreate(ctypes.byref(e0)) == 0
+        ok = ok and hip.hipEventCreate(ctypes.byref(e1)) == 0
+        d2d = 3  # hipMemcpyDeviceToDevice
+        for _ in range(2):
+            ok = ok and hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), d2d) == 0
+        ok = ok and hip.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            ok = ok and hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), d2d) == 0
+        ok = ok and hip.hipEventRecord(e1, None) == 0
+        ok = ok and hip.hipEventSynchronize(e1) == 0
+        ok = ok and hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        for e in (e0, e1):
+            if e.value:
+                hip.hipEventDestroy(e)
+        for m in (a, b):
+            if m.value:
+                hip.hipFree(m)
+        if not ok or ms.value <= 0:
+            return None
+        return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    except (OSError, StopIteration, AttributeError):
+        return None
+
+
 def run_rk4(P, p, prec, steps, warmup, shape=None):
     """RK4 timing of one configuration; `shape` (pin_shape's form) pins
     the launch shape, None autotunes it."""
@@ -436,6 +476,12 @@ def main():
                          "stage_ms": st_ms, "dominant_stage": dom,
                          "achieved": sb[dom] * nodes / (st_ms[dom] * 1e-3) / 1e9}},
     }
+    cref = copy_reference_gbs()
+    line["roofline"]["copy_reference"] = {
+        "GBps": cref, "frac_of_copy": achieved / cref if cref else None,
+        "what": "hipMemcpy device to device of 1 GiB, read + write counted: the streaming "
+                "rate this GPU reaches in practice (peak stays the published 8 TB/s)"}
+    log("bench: device copy reference %s GB/s" % ("%.0f" % cref if cref else "n/a"))
     if not args.no_ensemble_fp64:
         log("bench: ensemble fp64")
         e64 = run_ensemble(P, p, Hs, depths, "fp64", K, W, dt)
