@@ -659,11 +659,13 @@ int phi_hip_num_nodes(const phi_ctx* c) { return c ? c->e->num_nodes() : PHI_HIP
 int phi_hip_num_modes(const phi_ctx* c) { return c ? c->e->tables().D : PHI_HIP_EINVAL; }
 
 int phi_hip_get_index(const phi_ctx* c, int* out, size_t cap) {
-  if (!c || !out) return Err(PHI_HIP_EINVAL, "null argument");
-  const auto& occ = c->e->tables().occ;
-  if (cap < occ.size()) return Err(PHI_HIP_EINVAL, "buffer too small");
-  for (size_t i = 0; i < occ.size(); ++i) out[i] = occ[i];
-  return Ok();
+  return Guarded(__func__, [&]() -> int {
+    if (!c || !out) return Err(PHI_HIP_EINVAL, "null argument");
+    const auto& occ = c->e->tables().occ;
+    if (cap < occ.size()) return Err(PHI_HIP_EINVAL, "buffer too small");
+    for (size_t i = 0; i < occ.size(); ++i) out[i] = occ[i];
+    return Ok();
+  });
 }
 
 int phi_hip_set_state(phi_ctx* c, const double* s, size_t n) {
