@@ -433,6 +433,21 @@ int HierarchyIntegrator::RunRk4() {
     return info_.status;
   }
   if (p.verbose > -1) std::cout << "\tIntegrating using RK4.\n";
+  // A long run first takes the fastest launch shape of the stage kernels
+  // (the autotuner times about 50 steps and restores the state; results
+  // change only in rounding).  Time-dependent systems and the explicit-k
+  // paths (which the autotuner does not cover) keep theirs.
+  constexpr double kAutotuneMinSteps = 10000;
+  if (!(sh_ && sh_->time_dependent()) && end_time > p.start_time &&
+      (end_time - p.start_time) / dt >= kAutotuneMinSteps) {
+    double ms = 0;
+    int shape[4];
+    const int rc = engine_->Autotune(5, dt, &ms, shape);
+    if (rc < 0 && rc != kErrUnsupported) {
+      Fail(rc, engine_->last_error());
+      return rc;
+    }
+  }
   const double t0 = Now();
   const std::time_t w0 = std::time(nullptr);
   std::time_t last_restart = w0;

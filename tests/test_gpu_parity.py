@@ -172,6 +172,26 @@ def test_rk4_c1_full_run_fp64(tmp_path):
     assert text.startswith(hdr.replace("#Threads=1", "#Threads=%d" % op.num_threads))
 
 
+def test_long_rk4_run_autotunes_and_matches_oracle():
+    """An RK4 run of 10 000 steps (c1 over ten times its integration time)
+    takes the autotuned launch shape before it starts; its final state still
+    equals the oracle's within 1e-10."""
+    path = os.path.join(CFG, "c1.prm")
+    op, orc = _oracle_from_prm(path, threads=8)
+    t_end = 10 * op.integration_time
+    steps, _, _ = orc.rk4(t_end, op.timestep, op.output_minimum_timestep)
+    assert steps >= 10000
+    p = P.PhiParameters.read(path, "rk4")
+    p.set_double("integration_time", t_end)
+    p.set_string("output_filename", "")
+    p.set_int("verbose", -2)
+    print("case: c1 RK4 over %d steps" % steps, flush=True)
+    it = P.HierarchyIntegrator(p)
+    it.Launch()
+    assert it.info()["steps"] == steps
+    assert np.abs(it.state() - orc.state()).max() <= 1e-10
+
+
 def test_rk4_c2_short_fp64_and_fp32():
     """7-site exciton, depth 8 (6435 nodes): 40 RK4 steps, FP64 within 1e-10;
     FP32-bulk populations within 1e-6 of FP64."""
