@@ -464,7 +464,7 @@ bool FusedClosureFits(const FusedNet& net) {
   // imaginary) pairs.
   return FusedClosureLds(net) + closure::kRows * 32 <= (size_t)160 * 1024 &&
          net.k1 % closure::kStepK == 0 && net.hid % closure::kStepK == 0 &&
-         net.hid <= closure::kGroup && net.n_out % 2 == 0;
+         net.hid <= closure::kGroup && net.n_out >= 2 && net.n_out % 2 == 0;
 }
 
 hipError_t LaunchClosureFused(const FusedNet& net, void* slab, bool fp32, const int* moff,
