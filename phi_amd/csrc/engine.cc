@@ -1288,44 +1288,73 @@ int Engine::SteadyState(double tol, int max_iter, int* iters, double* resid,
   };
   // rho for the first iteration; later ones come with the r update.
   if (!rc && !hs.done) rc = dev_dot(rh, r, &st->rho_new);
-  long long k = 0;  // iterations enqueued
-  while (!rc && !hs.done) {
-    for (int b = 0; b < kBatch && !rc; ++b, ++k) {
+  // One batch: kBatch iterations, then the last one closes into the record
+  // the next batch starts from.  kBatch is even, so every batch launches the
+  // same kernels on the same buffers and records; in the matvec's LDS form
+  // (kernels only) the batch is captured once as a hipGraph and replayed,
+  // which removes most of the gaps between its 4 kBatch + 1 launches.
+  auto enqueue_batch = [&](long long k0) -> int {
+    int e = kOk;
+    long long k = k0;
+    for (int b = 0; b < kBatch && !e; ++b, ++k) {
       BicgScalars* cur = st + (k & 1);
       BicgScalars* prev = st + ((k + 1) & 1);
       // v = A p with the rhat.v partials
       if (mb > 0)
-        rc = Check(LaunchRhsGeneralDot(args_, pp, vv, 1, 1, rh, nullptr, p_rv, stream_),
-                   "A p, rhat.v");
-      if (!rc && mb == 0) rc = RhsDevice(pp, vv, 1);
-      if (!rc && mb == 0)
-        rc = Check(LaunchDotPartial(rh, vv, n, p_rv, kRedBlocks, stream_), "rhat.v");
+        e = Check(LaunchRhsGeneralDot(args_, pp, vv, 1, 1, rh, nullptr, p_rv, stream_),
+                  "A p, rhat.v");
+      if (!e && mb == 0) e = RhsDevice(pp, vv, 1);
+      if (!e && mb == 0)
+        e = Check(LaunchDotPartial(rh, vv, n, p_rv, kRedBlocks, stream_), "rhat.v");
       // (close the previous iteration, then) s = r - alpha v
-      if (!rc)
-        rc = Check(LaunchBicgS(b > 0, s, n, r, vv, prev, cur, p_rv, nb_dot, p_xr, kRedBlocks,
-                               hist, tol, max_iter, stream_), "s update");
+      if (!e)
+        e = Check(LaunchBicgS(b > 0, s, n, r, vv, prev, cur, p_rv, nb_dot, p_xr, kRedBlocks,
+                              hist, tol, max_iter, stream_), "s update");
       // t = A s with the t.s, t.t, rhat.t, rhat.s partials
-      if (!rc && mb > 0)
-        rc = Check(LaunchRhsGeneralDot(args_, s, t, 1, 3, s, rh, p_ts, stream_),
-                   "A s, t.s, t.t, rhat.t, rhat.s");
-      if (!rc && mb == 0) rc = RhsDevice(s, t, 1);
-      if (!rc && mb == 0)
-        rc = Check(LaunchBicgDots(t, s, rh, n, p_ts, kRedBlocks, stream_), "t and s dots");
+      if (!e && mb > 0)
+        e = Check(LaunchRhsGeneralDot(args_, s, t, 1, 3, s, rh, p_ts, stream_),
+                  "A s, t.s, t.t, rhat.t, rhat.s");
+      if (!e && mb == 0) e = RhsDevice(s, t, 1);
+      if (!e && mb == 0)
+        e = Check(LaunchBicgDots(t, s, rh, n, p_ts, kRedBlocks, stream_), "t and s dots");
       // x += alpha p + omega s, r = s - omega t, p = r + beta (p - omega v),
       // partials of |r|^2
-      if (!rc)
-        rc = Check(LaunchBicgXRP(x, r, pp, s, t, vv, n, cur, p_ts, nb_dot, p_xr, kRedBlocks,
-                                 stream_),
-                   "x/r/p update");
+      if (!e)
+        e = Check(LaunchBicgXRP(x, r, pp, s, t, vv, n, cur, p_ts, nb_dot, p_xr, kRedBlocks,
+                                stream_),
+                  "x/r/p update");
     }
-    // the batch's last iteration closes into the record the next batch starts from
+    if (!e)
+      e = Check(LaunchBicgFinish(st + ((k + 1) & 1), st + (k & 1), p_xr, kRedBlocks, hist, tol,
+                                 max_iter, stream_), "finish");
+    return e;
+  };
+  hipGraphExec_t batch_exec = nullptr;
+  long long k = 0;  // iterations enqueued
+  while (!rc && !hs.done) {
+    if (mb > 0 && use_graphs_ && !batch_exec) {
+      hipGraph_t g = nullptr;
+      rc = Check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+                 "begin bicgstab capture");
+      if (!rc) {
+        rc = enqueue_batch(k);
+        const int rc_end = Check(hipStreamEndCapture(stream_, &g), "end bicgstab capture");
+        if (!rc) rc = rc_end;
+      }
+      if (!rc) rc = Check(hipGraphInstantiate(&batch_exec, g, nullptr, nullptr, 0),
+                          "instantiate bicgstab batch");
+      if (g) (void)hipGraphDestroy(g);
+      if (rc) batch_exec = nullptr;
+    }
     if (!rc)
-      rc = Check(LaunchBicgFinish(st + ((k + 1) & 1), st + (k & 1), p_xr, kRedBlocks, hist, tol,
-                                  max_iter, stream_), "finish");
+      rc = batch_exec ? Check(hipGraphLaunch(batch_exec, stream_), "bicgstab batch")
+                      : enqueue_batch(k);
+    k += kBatch;
     if (!rc) rc = Check(hipMemcpyAsync(&hs, st + (k & 1), sizeof(hs), hipMemcpyDeviceToHost,
                                        stream_), "bicgstab scalars");
     if (!rc) rc = Synchronize();
   }
+  if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
   const int it = hs.it;
   if (!rc) res = hs.res;
   if (!rc && history) {

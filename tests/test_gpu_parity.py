@@ -262,6 +262,28 @@ def test_steady_state_c1_small():
     assert np.abs(Lx).max() < 1e-8
 
 
+def test_steady_state_graph_batches_equal_eager_launches():
+    """The BiCGSTAB batches replayed from a captured hipGraph (one capture,
+    many replays with the host's residual check between them) give the
+    kernel-by-kernel solve bit for bit: iterations, residual and solution
+    (7-site system, depth 4)."""
+    _, p = _subset_params("c2", 5)
+    dt = p.get_double("timestep")
+    out = {}
+    for graphs in (False, True):
+        print("case: steady state graphs=%s" % graphs, flush=True)
+        ctx = P.Context(p)
+        ctx.set_graphs(graphs)
+        ctx.rk4_steps(200, dt)
+        rc, iters, res = ctx.steady_state(1e-10, 20000)
+        out[graphs] = (rc, iters, res, ctx.state())
+        ctx.close()
+    print("iterations %d / %d" % (out[False][1], out[True][1]))
+    assert out[False][0] == 0 and out[False][1] > 3 * 16
+    assert out[True][:3] == out[False][:3]
+    assert np.array_equal(out[True][3], out[False][3])
+
+
 def test_steady_state_iterations_match_dense_bicgstab():
     """The device-scalar BiCGSTAB takes the textbook iterates (van der
     Vorst 1992): residuals after k = 1..12 iterations equal those of a dense
