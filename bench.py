@@ -219,7 +219,7 @@ def copy_reference_gbs(nbytes=1 << 30, reps=10):
         if not ok or ms.value <= 0:
             return None
         return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
-    except (OSError, StopIteration, AttributeError):
+    except (OSError, StopIteration, AttributeError, ctypes.ArgumentError):
         return None
 
 
