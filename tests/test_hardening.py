@@ -1,8 +1,9 @@
 """Error paths of the C ABI on the CPU (no GPU calls reach a kernel):
 over-size hierarchies are refused before anything is enumerated, run
 parameters a loop cannot finish with are refused before anything is
-allocated, short bath-index lists are refused, and malformed `.prm` files
-never crash the reader (its decisions equal the oracle's)."""
+allocated, short bath-index lists are refused, malformed `.prm` files
+never crash the reader (its decisions equal the oracle's), and null contexts
+are refused."""
 
 import os
 import random
@@ -156,3 +157,26 @@ def test_node_count_estimate_of_huge_hierarchies_saturates(tmp_path):
     assert p.get_int("matrix_count") == o.matrix_count == PO.INT_MAX
     assert PO.node_count_estimate(9, 4) == 495          # C(8+4, 4), the 2-site c1 case
     assert PO.node_count_estimate(5, 24) == 20475       # the 24-site ring at depth 4
+
+
+def test_context_entry_points_refuse_a_null_context():
+    """Entry points that take a context return PHI_HIP_EINVAL with a message
+    for a null one (checked before any device call), the launch-shape calls
+    among them."""
+    import ctypes
+    L = P.lib()
+    ip = (ctypes.c_int * 4)()
+    dp = ctypes.c_double()
+    calls = {
+        "phi_hip_set_launch": (None, 2, 0, 0, 0),
+        "phi_hip_set_stage_launch": (None, 0, 0, 0),
+        "phi_hip_stage_launch": (None, ip, ip),
+        "phi_hip_autotune": (None, 3, 0.001, ctypes.byref(dp), ip),
+        "phi_hip_get_index": (None, ip, 4),
+        "phi_hip_set_rk4_form": (None, 2),
+        "phi_hip_set_graphs": (None, 1),
+    }
+    for name, args in calls.items():
+        rc = getattr(L, name)(*args)
+        assert rc == -1, (name, rc)
+        assert L.phi_hip_last_error().decode(), name
