@@ -36,6 +36,8 @@
  *   phi_hip_rkf45_attempt/_accept  one attempted step of Rkf45Integrate
  *                                  (hierarchy_integrator.cc:1998-2438)
  *   phi_hip_steady_state           BicgstabSteadyState
+ *   phi_hip_steady_state_l         BicgstabLSteadyState
+ *                                  (hierarchy_integrator.cc:3112-3489)
  *                                  (hierarchy_integrator.cc:3495-3783)
  *   phi_hip_get_rho0               density_matrix()
  *                                  (hierarchy_integrator.cc:177-191)
@@ -274,6 +276,13 @@ int phi_hip_active_flags(phi_ctx* c, int* flags, int n, int* count);
 /* Steady state from the current state: BiCGSTAB on L x = 0 with the
  * (0,0) row of node 0 replaced by trace(rho_0) = 1. */
 int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, double* resid);
+/* The same system by BiCGSTAB(ell), ell = 1..8 (Sleijpen and Fokkema 1993):
+ * ell BiCG steps, then a minimal-residual polynomial of degree ell;
+ * *iters counts BiCG steps (two matvecs each); ell = 1 is BiCGSTAB.
+ * PHI counterpart: BicgstabLSteadyState (hierarchy_integrator.cc:
+ * 3112-3489), selected by PHI's bicgstab method with ell >= 1. */
+int phi_hip_steady_state_l(phi_ctx* c, int ell, double tol, int max_iter, int* iters,
+                           double* resid);
 int phi_hip_synchronize(phi_ctx* c);
 /* Shape of the stage kernels: gemm = 0 element path, 1 matrix cores
  * (v_mfma 16x16x4, N <= 32), 2 persistent pipelined matrix-core kernels

@@ -93,6 +93,8 @@ def lib():
         "phi_hip_rkf45_accept": (ctypes.c_int, [vp]),
         "phi_hip_active_flags": (ctypes.c_int, [vp, ip, ctypes.c_int, ip]),
         "phi_hip_steady_state": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int, ip, dp]),
+        "phi_hip_steady_state_l": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                  ip, dp]),
         "phi_hip_synchronize": (ctypes.c_int, [vp]),
         "phi_hip_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int]),
@@ -456,6 +458,15 @@ class Context:
         it = ctypes.c_int()
         res = ctypes.c_double()
         rc = lib().phi_hip_steady_state(self._h, tol, max_iter, ctypes.byref(it), ctypes.byref(res))
+        return rc, it.value, res.value
+
+    def steady_state_l(self, ell, tol, max_iter):
+        """BiCGSTAB(ell) on the steady-state system: (status, BiCG steps,
+        residual)."""
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        rc = lib().phi_hip_steady_state_l(self._h, int(ell), tol, max_iter, ctypes.byref(it),
+                                          ctypes.byref(res))
         return rc, it.value, res.value
 
     def synchronize(self):

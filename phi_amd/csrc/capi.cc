@@ -782,6 +782,16 @@ int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, doubl
   });
 }
 
+int phi_hip_steady_state_l(phi_ctx* c, int ell, double tol, int max_iter, int* iters,
+                           double* resid) {
+  return Guarded(__func__, [&]() -> int {
+    if (!c) return Err(PHI_HIP_EINVAL, "null argument");
+    int rc = c->e->SteadyStateL(ell, tol, max_iter, iters, resid, nullptr);
+    if (rc == phi::kErrNotConverged) return Err(rc, "BiCGSTAB(ell) did not converge");
+    return rc ? Err(rc, c->e->last_error()) : Ok();
+  });
+}
+
 int phi_hip_set_launch(phi_ctx* c, int gemm, int tile, int threads, int blocks) {
   return Guarded(__func__, [&]() -> int {
     if (!c) return Err(PHI_HIP_EINVAL, "null argument");

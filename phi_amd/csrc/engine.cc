@@ -1072,6 +1072,167 @@ int Engine::Rk4StageTimes(long long nsteps, double dt, double ms[4]) {
   return rc ? rc : (rc_restore ? rc_restore : rc_sync);
 }
 
+// BiCGSTAB(ell) (Sleijpen and Fokkema, ETNA 1, 1993, Algorithm 3.1) on the
+// steady-state system of SteadyState, complex, inner products (a, b) =
+// a^H b: each cycle takes ell BiCG steps (u_0..u_ell, r_0..r_ell and their
+// images under A), then minimises |r_0 - sum_j gamma_j r_j| over the
+// Krylov polynomial of degree ell (modified Gram-Schmidt on r_1..r_ell).
+// Scalars on the host: every inner product is one reduction and one
+// round trip.  PHI's BicgstabLSteadyState (hierarchy_integrator.cc:
+// 3112-3489) follows the same outline.
+int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* resid,
+                         std::vector<double>* history) {
+  if (members_) return Unsupported("the steady state");
+  if (bath_model_ != 0) {
+    err_ = "the steady state with diagonal/full coupling operators is not supported";
+    return kErrUnsupported;
+  }
+  if (tl_) {
+    err_ = "the steady state with time-local truncation is not supported";
+    return kErrUnsupported;
+  }
+  if (ell < 1 || ell > 8) {
+    err_ = "BiCGSTAB(ell) takes ell = 1 .. 8";
+    return kErrInvalidArgument;
+  }
+  typedef std::complex<double> Z;
+  const size_t n = state_elems();
+  const size_t bytes = n * 2 * sizeof(double);
+  // r_0..r_ell, u_0..u_ell, rtilde
+  std::vector<double*> vr(ell + 1, nullptr), vu(ell + 1, nullptr);
+  double* rt = nullptr;
+  int rc = kOk;
+  for (auto& p : vr)
+    if (!rc) rc = Check(hipMalloc((void**)&p, bytes), "alloc bicgstab(l)");
+  for (auto& p : vu)
+    if (!rc) rc = Check(hipMalloc((void**)&p, bytes), "alloc bicgstab(l)");
+  if (!rc) rc = Check(hipMalloc((void**)&rt, bytes), "alloc bicgstab(l)");
+  auto cleanup = [&]() {
+    for (auto p : vr) Free(p);
+    for (auto p : vu) Free(p);
+    Free(rt);
+  };
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  double* x = d_rho_;
+  auto dot = [&](const double* a, const double* b, Z* out) -> int {
+    int e = Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, d_dot_, stream_), "dot");
+    double h[2];
+    if (!e) e = Check(hipMemcpyAsync(h, d_dot_, sizeof(h), hipMemcpyDeviceToHost, stream_), "dot d2h");
+    if (!e) e = Synchronize();
+    *out = Z(h[0], h[1]);
+    return e;
+  };
+  // y = a + c b (y may be a)
+  auto axpy = [&](double* y, const double* a, Z c, const double* b) -> int {
+    const double* xs[2] = {a, b};
+    const double2 cs[2] = {make_double2(1., 0.), make_double2(c.real(), c.imag())};
+    return Check(LaunchLincomb(y, n, 2, xs, cs, stream_), "lincomb");
+  };
+  // r_0 = b - A x (b = e_0 of the trace row), rtilde = r_0, u_0 = 0
+  rc = RhsDevice(x, vu[0], 1);
+  if (!rc) {
+    const double* xs[1] = {vu[0]};
+    const double2 cs[1] = {make_double2(-1., 0.)};
+    rc = Check(LaunchLincomb(vr[0], n, 1, xs, cs, stream_), "lincomb");
+  }
+  double r00[2];
+  if (!rc) rc = Check(hipMemcpyAsync(r00, vr[0], sizeof(r00), hipMemcpyDeviceToHost, stream_), "r0");
+  if (!rc) rc = Synchronize();
+  if (!rc) {
+    r00[0] += 1.0;
+    rc = Check(hipMemcpyAsync(vr[0], r00, sizeof(r00), hipMemcpyHostToDevice, stream_), "r0 set");
+  }
+  if (!rc) rc = Check(hipMemcpyAsync(rt, vr[0], bytes, hipMemcpyDeviceToDevice, stream_), "rtilde");
+  if (!rc) rc = Check(hipMemsetAsync(vu[0], 0, bytes, stream_), "u0");
+  Z rn2;
+  if (!rc) rc = dot(vr[0], vr[0], &rn2);
+  double res = std::sqrt(rn2.real());
+  if (history && !rc) history->push_back(res);
+  Z rho0(1, 0), alpha(0, 0), omega(1, 0);
+  int it = 0;
+  bool broke = false;
+  std::vector<std::vector<Z>> tau(ell + 1, std::vector<Z>(ell + 1));
+  std::vector<double> sigma(ell + 1);
+  std::vector<Z> gp(ell + 1), g(ell + 1), gpp(ell + 1);
+  while (!rc && res > tol && it < max_iter && !broke) {
+    rho0 = -omega * rho0;
+    // BiCG part
+    for (int j = 0; j < ell && !rc && !broke; ++j) {
+      Z rho1;
+      rc = dot(rt, vr[j], &rho1);
+      if (rc) break;
+      if (rho0 == Z(0, 0)) { broke = true; break; }
+      const Z beta = alpha * rho1 / rho0;
+      rho0 = rho1;
+      for (int i = 0; i <= j && !rc; ++i) rc = axpy(vu[i], vr[i], -beta, vu[i]);
+      if (!rc) rc = RhsDevice(vu[j], vu[j + 1], 1);
+      Z gam;
+      if (!rc) rc = dot(rt, vu[j + 1], &gam);
+      if (rc) break;
+      if (gam == Z(0, 0)) { broke = true; break; }
+      alpha = rho0 / gam;
+      for (int i = 0; i <= j && !rc; ++i) rc = axpy(vr[i], vr[i], -alpha, vu[i + 1]);
+      if (!rc) rc = RhsDevice(vr[j], vr[j + 1], 1);
+      if (!rc) rc = axpy(x, x, alpha, vu[0]);
+    }
+    if (rc || broke) break;
+    // MR part (modified Gram-Schmidt)
+    for (int j = 1; j <= ell && !rc && !broke; ++j) {
+      for (int i = 1; i < j && !rc; ++i) {
+        Z d;
+        rc = dot(vr[i], vr[j], &d);
+        if (rc) break;
+        tau[i][j] = d / sigma[i];
+        rc = axpy(vr[j], vr[j], -tau[i][j], vr[i]);
+      }
+      Z sj, d0;
+      if (!rc) rc = dot(vr[j], vr[j], &sj);
+      if (!rc) rc = dot(vr[j], vr[0], &d0);
+      if (rc) break;
+      sigma[j] = sj.real();
+      if (!(sigma[j] > 0)) { broke = true; break; }
+      gp[j] = d0 / sigma[j];
+    }
+    if (rc || broke) break;
+    g[ell] = gp[ell];
+    omega = g[ell];
+    for (int j = ell - 1; j >= 1; --j) {
+      Z sum(0, 0);
+      for (int i = j + 1; i <= ell; ++i) sum += tau[j][i] * g[i];
+      g[j] = gp[j] - sum;
+    }
+    for (int j = 1; j <= ell - 1; ++j) {
+      Z sum(0, 0);
+      for (int i = j + 1; i <= ell - 1; ++i) sum += tau[j][i] * g[i + 1];
+      gpp[j] = g[j + 1] + sum;
+    }
+    rc = axpy(x, x, g[1], vr[0]);
+    if (!rc) rc = axpy(vr[0], vr[0], -gp[ell], vr[ell]);
+    if (!rc) rc = axpy(vu[0], vu[0], -g[ell], vu[ell]);
+    for (int j = 1; j <= ell - 1 && !rc; ++j) {
+      rc = axpy(vu[0], vu[0], -g[j], vu[j]);
+      if (!rc) rc = axpy(x, x, gpp[j], vr[j]);
+      if (!rc) rc = axpy(vr[0], vr[0], -gp[j], vr[j]);
+    }
+    if (!rc) rc = dot(vr[0], vr[0], &rn2);
+    if (rc) break;
+    it += ell;
+    res = std::sqrt(rn2.real());
+    if (history) history->push_back(res);
+    if (omega == Z(0, 0)) broke = true;
+  }
+  const int rc_sync = Synchronize();
+  cleanup();
+  if (!rc) rc = rc_sync;
+  if (iters) *iters = it;
+  if (resid) *resid = res;
+  if (rc) return rc;
+  return res <= tol ? kOk : kErrNotConverged;
+}
+
 int Engine::HistoryCount() {
   int n = 0;
   int rc = Check(hipMemcpyAsync(&n, d_hist_count_, sizeof(int), hipMemcpyDeviceToHost, stream_), "hist count");

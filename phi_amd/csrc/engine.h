@@ -212,6 +212,11 @@ class Engine {
   // current state.  Returns kOk or kErrNotConverged; *iters, *resid report.
   int SteadyState(double tol, int max_iter, int* iters, double* resid,
                   std::vector<double>* history);
+  // The same system by BiCGSTAB(ell) (ell = 1..8): ell BiCG steps, then a
+  // minimal-residual polynomial of degree ell; *iters counts BiCG steps
+  // (two matvecs each), history holds |r| after every ell of them.
+  int SteadyStateL(int ell, double tol, int max_iter, int* iters, double* resid,
+                   std::vector<double>* history);
 
   int Synchronize();
   std::string last_error() const { return err_; }

@@ -729,8 +729,13 @@ int HierarchyIntegrator::RunSteadyState() {
   double resid = 0;
   std::vector<double> hist;
   const double t0 = Now();
-  int rc = engine_->SteadyState(p.bicgstab_tolerance, p.bicgstab_max_iterations, &iters, &resid,
-                                &hist);
+  // PHI's bicgstab word with ell >= 1 selects BiCGSTAB(ell)
+  // (hierarchy_integrator.cc:3112-3489), bicgstabu and ell = 0 plain BiCGSTAB.
+  int rc = p.run_method == BICGSTABL && p.bicgstab_ell >= 1
+               ? engine_->SteadyStateL(p.bicgstab_ell, p.bicgstab_tolerance,
+                                       p.bicgstab_max_iterations, &iters, &resid, &hist)
+               : engine_->SteadyState(p.bicgstab_tolerance, p.bicgstab_max_iterations, &iters,
+                                      &resid, &hist);
   info_.seconds = Now() - t0;
   info_.bicgstab_iterations = iters;
   info_.bicgstab_residual = resid;

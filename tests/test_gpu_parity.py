@@ -335,6 +335,97 @@ def test_steady_state_iterations_match_dense_bicgstab():
         assert abs(got - res[k]) <= 1e-9 * res[0], (k, got, res[k])
 
 
+def test_steady_state_l_cycles_match_dense_bicgstab_l():
+    """BiCGSTAB(ell) on the device (ell = 1, 2, 4) takes the iterates of the
+    dense restatement of Sleijpen and Fokkema's algorithm on the oracle's
+    steady-state operator: residuals after 1, 2 and 4 cycles agree."""
+    from oracle import krylov_oracle as KO
+
+    op = PO.read_command_line(os.path.join(CFG, "c1.prm"), "bicgstab")
+    kw = dict(gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature, lh=3,
+              num_terms=op.num_matsubara_terms)
+    p = P.params_from_problem(op.time_indepenent_hamiltonian, op.initial_density_matrix,
+                              method="bicgstab", **kw)
+    orc = HO.Oracle(HO.Problem(op.time_indepenent_hamiltonian, op.initial_density_matrix, **kw))
+    ctx = P.Context(p)
+    x0 = ctx.state()
+    ctx.close()
+    m = x0.size
+    A = np.zeros((m, m), dtype=complex)
+    for c in range(m):
+        e = np.zeros(m, dtype=complex)
+        e[c] = 1
+        A[:, c] = orc.steady_matvec(e.reshape(x0.shape)).reshape(-1)
+    b = np.zeros(m, dtype=complex)
+    b[0] = 1
+    for ell in (1, 2, 4):
+        res, _ = KO.bicgstab_l(A, b, x0, ell, 4)
+        for k in (1, 2, 4):
+            print("case: bicgstab(%d) %d cycles" % (ell, k), flush=True)
+            ctx = P.Context(p)
+            rc, it, got = ctx.steady_state_l(ell, 1e-30, k * ell)
+            ctx.close()
+            assert it == k * ell
+            # r_1..r_ell are A^j r_0, whose Gram-Schmidt loses digits as
+            # |A|^ell grows: measured 1.4e-9 of |r_0| at ell = 4, 4 cycles.
+            tol = 1e-9 if ell <= 2 else 1e-8
+            assert abs(got - res[k]) <= tol * res[0], (ell, k, got, res[k])
+
+
+def test_steady_state_l_converges_to_the_bicgstab_fixed_point():
+    """BiCGSTAB(2) and BiCGSTAB(4) on the 7-site system at depth 4 reach the
+    tolerance with fewer matrix-vector products than BiCGSTAB needs, and all
+    three land on the same fixed point of the oracle's operator."""
+    op, p = _subset_params("c2", 5)
+    dt = p.get_double("timestep")
+    ctx = P.Context(p)
+    ctx.rk4_steps(200, dt)
+    start = ctx.state()
+    rc, it_u, res = ctx.steady_state(1e-10, 20000)
+    assert rc == 0, (it_u, res)
+    ref = ctx.state()
+    ctx.close()
+    orc = HO.Oracle(HO.Problem(op.time_indepenent_hamiltonian, op.initial_density_matrix,
+                               gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature,
+                               lh=5, num_terms=op.num_matsubara_terms))
+    for ell in (2, 4):
+        print("case: bicgstab(%d) c2 lh 5" % ell, flush=True)
+        ctx = P.Context(p)
+        ctx.set_state(start)
+        rc, it, res = ctx.steady_state_l(ell, 1e-10, 20000)
+        x = ctx.state()
+        ctx.close()
+        print("ell %d: %d BiCG steps (plain BiCGSTAB %d), residual %.3g" % (ell, it, it_u, res))
+        assert rc == 0 and res <= 1e-10, (it, res)
+        assert it <= it_u
+        assert abs(np.trace(x[0]) - 1) < 1e-9
+        assert np.abs(orc.rhs(x)).max() < 1e-7
+        assert np.abs(x[0] - ref[0]).max() < 1e-8
+
+
+def test_integrator_bicgstab_words_run_bicgstab_l_and_bicgstab():
+    """PHI's bicgstab method runs BiCGSTAB(ell) with the configured ell
+    (2 by default) and bicgstabu plain BiCGSTAB; both converge to the same
+    steady-state density matrix of the dimer at depth 4."""
+    op = PO.read_command_line(os.path.join(CFG, "c1.prm"), "bicgstab")
+    kw = dict(gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature, lh=4,
+              num_terms=op.num_matsubara_terms)
+    out = {}
+    for method in ("bicgstab", "bicgstabu"):
+        print("case: integrator %s" % method, flush=True)
+        p = P.params_from_problem(op.time_indepenent_hamiltonian, op.initial_density_matrix,
+                                  method=method, **kw)
+        p.set_double("bicgstab_tolerance", 1e-11)
+        it = P.HierarchyIntegrator(p)
+        it.Launch()
+        out[method] = (it.info(), it.density_matrix())
+    for method, (info, rho) in out.items():
+        assert info["bicgstab_residual"] <= 1e-11, (method, info)
+        assert abs(np.trace(rho) - 1) < 1e-9
+    assert int(out["bicgstab"][0]["bicgstab_iterations"]) % 2 == 0
+    assert np.abs(out["bicgstab"][1] - out["bicgstabu"][1]).max() < 1e-8
+
+
 def _subset_params(cfg, lh):
     path = os.path.join(CFG, cfg + ".prm")
     op = PO.read_command_line(path, "rk4")

@@ -238,3 +238,74 @@ def test_thread_count_does_not_change_results():
     a.rk4(0.05, 0.001, 1.0)
     b.rk4(0.05, 0.001, 1.0)
     assert np.array_equal(a.state(), b.state())
+
+
+def _random_system(m, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+    A = A / np.sqrt(m) + 3 * np.eye(m)
+    b = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    return A, b
+
+
+def test_krylov_oracle_bicgstab_l_with_ell_1_is_bicgstab():
+    """BiCGSTAB(1) is BiCGSTAB (Sleijpen and Fokkema 1993, section 3): the
+    dense restatements take the same residuals iteration by iteration."""
+    from oracle import krylov_oracle as KO
+
+    A, b = _random_system(40, 1)
+    x0 = np.zeros(40, dtype=complex)
+    r1, x1 = KO.bicgstab(A, b, x0, 10)
+    rl, xl = KO.bicgstab_l(A, b, x0, 1, 10)
+    assert np.allclose(r1, rl, rtol=1e-10, atol=1e-14 * r1[0])
+    assert np.allclose(x1, xl, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("ell", [1, 2, 3, 4])
+def test_krylov_oracle_converges_to_the_dense_solution(ell):
+    """Every ell solves a well-conditioned complex system to numpy's
+    solution, and the recursively updated residual it reports equals
+    |b - A x| recomputed from its x."""
+    from oracle import krylov_oracle as KO
+
+    A, b = _random_system(60, 2 + ell)
+    x0 = np.full(60, 0.1 + 0.2j)
+    res, x = KO.bicgstab_l(A, b, x0, ell, 60 // ell)
+    assert res[-1] < 1e-10 * res[0]
+    assert np.abs(x - np.linalg.solve(A, b)).max() < 1e-9
+    assert abs(np.linalg.norm(b - A @ x) - res[-1]) < 1e-11 * res[0]
+    r4, x4 = KO.bicgstab_l(A, b, x0, ell, 2)
+    assert abs(np.linalg.norm(b - A @ x4) - r4[-1]) < 1e-11 * r4[0]
+    rs, xs = KO.bicgstab(A, b, x0, 60)
+    assert np.abs(xs - np.linalg.solve(A, b)).max() < 1e-9
+
+
+@pytest.mark.parametrize("ell", [1, 2, 3, 5])
+def test_krylov_oracle_mr_part_is_the_least_squares_polynomial(ell):
+    """Sleijpen and Fokkema 1993, section 3: the minimal-residual part
+    leaves r_0 - sum_j c_j r_j with c the least-squares coefficients of r_0
+    against r_j = A^j r_0 (numpy.linalg.lstsq), applies the same polynomial
+    to u_0 (u_j = A^j u_0), and moves x so that b - A x is the new r_0."""
+    from oracle import krylov_oracle as KO
+
+    m = 30
+    A, b = _random_system(m, 20 + ell)
+    rng = np.random.default_rng(ell)
+    x = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    u0 = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    r = [b - A @ x]
+    u = [u0]
+    for _ in range(ell):
+        r.append(A @ r[-1])
+        u.append(A @ u[-1])
+    R = np.stack(r[1:], axis=1)
+    c = np.linalg.lstsq(R, r[0], rcond=None)[0]
+    want_r = r[0] - R @ c
+    want_u = u0 - np.stack(u[1:], axis=1) @ c
+    rr = [v.copy() for v in r]
+    uu = [v.copy() for v in u]
+    x2, omega = KO.mr_part(rr, uu, x, ell)
+    assert np.allclose(rr[0], want_r, atol=1e-10 * np.linalg.norm(r[0]))
+    assert np.allclose(uu[0], want_u, atol=1e-9 * np.linalg.norm(u0))
+    assert np.allclose(b - A @ x2, want_r, atol=1e-9 * np.linalg.norm(r[0]))
+    assert omega == pytest.approx(c[-1], rel=1e-8)
