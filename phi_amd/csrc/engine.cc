@@ -111,7 +111,7 @@ int Engine::Create(const PhiParameters& p, Engine** out, std::string* why) {
   if (!rc) rc = e->Check(hipMemset(e->d_hist_count_, 0, sizeof(int)), "zero hist count");
   e->red_blocks_ = 1024;
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_partial_, e->red_blocks_ * 2 * sizeof(double)), "alloc partial");
-  if (!rc) rc = e->Check(hipMalloc((void**)&e->d_dot_, 2 * sizeof(double)), "alloc dot");
+  if (!rc) rc = e->Check(hipMalloc((void**)&e->d_dot_, 4 * sizeof(double)), "alloc dot");
   if (!rc) rc = e->Check(hipMalloc((void**)&e->d_err_, sizeof(unsigned long long)), "alloc err");
   // Adaptive truncation: every node starts active (hierarchy_node.cc:16).
   // The vacuum updater ignores the flags.
@@ -1125,6 +1125,18 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
     *out = Z(h[0], h[1]);
     return e;
   };
+  // Two dot products, one host round trip: (a, b) and (c, d).
+  auto dot2 = [&](const double* a, const double* b, const double* c, const double* d, Z* o1,
+                  Z* o2) -> int {
+    int e = Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, d_dot_, stream_), "dot");
+    if (!e) e = Check(LaunchDot(c, d, n, 0, d_partial_, red_blocks_, d_dot_ + 2, stream_), "dot");
+    double h[4];
+    if (!e) e = Check(hipMemcpyAsync(h, d_dot_, sizeof(h), hipMemcpyDeviceToHost, stream_), "dot d2h");
+    if (!e) e = Synchronize();
+    *o1 = Z(h[0], h[1]);
+    *o2 = Z(h[2], h[3]);
+    return e;
+  };
   // y = a + c b (y may be a)
   auto axpy = [&](double* y, const double* a, Z c, const double* b) -> int {
     const double* xs[2] = {a, b};
@@ -1147,8 +1159,10 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
   }
   if (!rc) rc = Check(hipMemcpyAsync(rt, vr[0], bytes, hipMemcpyDeviceToDevice, stream_), "rtilde");
   if (!rc) rc = Check(hipMemsetAsync(vu[0], 0, bytes, stream_), "u0");
-  Z rn2;
-  if (!rc) rc = dot(vr[0], vr[0], &rn2);
+  // |r_0|^2 and the first BiCG step's (rtilde, r_0) together; likewise at
+  // the end of every cycle.
+  Z rn2, rho1_next;
+  if (!rc) rc = dot2(vr[0], vr[0], rt, vr[0], &rn2, &rho1_next);
   double res = std::sqrt(rn2.real());
   if (history && !rc) history->push_back(res);
   Z rho0(1, 0), alpha(0, 0), omega(1, 0);
@@ -1161,8 +1175,8 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
     rho0 = -omega * rho0;
     // BiCG part
     for (int j = 0; j < ell && !rc && !broke; ++j) {
-      Z rho1;
-      rc = dot(rt, vr[j], &rho1);
+      Z rho1 = rho1_next;
+      if (j > 0) rc = dot(rt, vr[j], &rho1);
       if (rc) break;
       if (rho0 == Z(0, 0)) { broke = true; break; }
       const Z beta = alpha * rho1 / rho0;
@@ -1189,8 +1203,7 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
         rc = axpy(vr[j], vr[j], -tau[i][j], vr[i]);
       }
       Z sj, d0;
-      if (!rc) rc = dot(vr[j], vr[j], &sj);
-      if (!rc) rc = dot(vr[j], vr[0], &d0);
+      if (!rc) rc = dot2(vr[j], vr[j], vr[j], vr[0], &sj, &d0);
       if (rc) break;
       sigma[j] = sj.real();
       if (!(sigma[j] > 0)) { broke = true; break; }
@@ -1217,7 +1230,7 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
       if (!rc) rc = axpy(x, x, gpp[j], vr[j]);
       if (!rc) rc = axpy(vr[0], vr[0], -gp[j], vr[j]);
     }
-    if (!rc) rc = dot(vr[0], vr[0], &rn2);
+    if (!rc) rc = dot2(vr[0], vr[0], rt, vr[0], &rn2, &rho1_next);
     if (rc) break;
     it += ell;
     res = std::sqrt(rn2.real());

@@ -393,10 +393,15 @@ __global__ void __launch_bounds__(256) zlincomb(double* __restrict__ y, size_t n
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     double ar = 0, ai = 0;
-    for (int t = 0; t < nv; ++t) {
-      const double2 v = reinterpret_cast<const double2*>(xs[t])[i];
-      ar += cs[t].x * v.x - cs[t].y * v.y;
-      ai += cs[t].x * v.y + cs[t].y * v.x;
+    // Fully unrolled so that xs and cs stay in registers (a loop bounded
+    // by nv indexes them at run time, which puts them in scratch memory).
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t < nv) {
+        const double2 v = reinterpret_cast<const double2*>(xs[t])[i];
+        ar += cs[t].x * v.x - cs[t].y * v.y;
+        ai += cs[t].x * v.y + cs[t].y * v.x;
+      }
     }
     reinterpret_cast<double2*>(y)[i] = make_double2(ar, ai);
   }
