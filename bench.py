@@ -37,7 +37,8 @@ c3_fp32  the 24-site ring at depth 4 (20475 nodes of 24x24), FP32-bulk RK4:
 c3_rkf45 the 24-site ring with RKF45 (abs tol 1e-8) to 1 ps, FP64 and FP32
          bulk: attempts, time to 1 ps, HBM fraction of N^2 (112 + 29 s) per
          attempted node-step, FP32-vs-FP64 population / coherence error.
-c2_steady_state  BiCGSTAB to residual 1e-10 from the 7-site state at 1 ps.
+c2_steady_state  BiCGSTAB to residual 1e-10 from the 7-site state at 1 ps,
+         and BiCGSTAB(2) from the same state.
 c5_closure   the learned truncation closure: trained on depth-10 runs,
          depth 4 + closure vs plain depth 4 vs plain depth 10 on held-out
          Hamiltonians (populations and time to 1 ps).
@@ -258,18 +259,29 @@ def run_steady_state(P, np, p, steps, tol=1e-10, max_iter=20000):
     ctx = P.Context(p)
     ctx.rk4_steps(steps, dt)
     ctx.synchronize()
+    start = ctx.state()
     t0 = time.perf_counter()
     rc, iters, resid = ctx.steady_state(tol, max_iter)
     el = time.perf_counter() - t0
     r0 = ctx.rho0()
     nodes = ctx.num_nodes
     N = r0.shape[0]
+    # BiCGSTAB(2) (the reference's default steady-state solver) from the
+    # same state
+    ctx.set_state(start)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rc_l, steps_l, resid_l = ctx.steady_state_l(2, tol, max_iter)
+    el_l = time.perf_counter() - t0
+    trace_l = float(np.trace(ctx.rho0()).real)
     ctx.close()
     # 24 FP64 vector passes per iteration: 384 N^2 B per node
     per_iter = nodes * N * N * 384
     return {"tolerance": tol, "status": rc, "iterations": iters, "residual": resid,
             "seconds": el, "trace": float(np.trace(r0).real),
-            "hbm_frac_iteration": (per_iter * iters / el / (HBM_PEAK_GBS * 1e9)) if el > 0 else None}
+            "hbm_frac_iteration": (per_iter * iters / el / (HBM_PEAK_GBS * 1e9)) if el > 0 else None,
+            "bicgstab_l2": {"status": rc_l, "bicg_steps": steps_l, "residual": resid_l,
+                            "seconds": el_l, "trace": trace_l}}
 
 
 def run_c3_rkf45(P, np, log):
