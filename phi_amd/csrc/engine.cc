@@ -1091,11 +1091,14 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
     err_ = "the steady state with time-local truncation is not supported";
     return kErrUnsupported;
   }
-  if (ell < 1 || ell > 8) {
+  if (ell < 1 || ell > kBicglMaxEll) {
     err_ = "BiCGSTAB(ell) takes ell = 1 .. 8";
     return kErrInvalidArgument;
   }
-  typedef std::complex<double> Z;
+  if (max_iter < 0) {
+    err_ = "BiCGSTAB(ell) takes max_iter >= 0";
+    return kErrInvalidArgument;
+  }
   const size_t n = state_elems();
   const size_t bytes = n * 2 * sizeof(double);
   // r_0..r_ell, u_0..u_ell, rtilde
@@ -1117,34 +1120,30 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
     return rc;
   }
   double* x = d_rho_;
-  auto dot = [&](const double* a, const double* b, Z* out) -> int {
-    int e = Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, d_dot_, stream_), "dot");
-    double h[2];
-    if (!e) e = Check(hipMemcpyAsync(h, d_dot_, sizeof(h), hipMemcpyDeviceToHost, stream_), "dot d2h");
-    if (!e) e = Synchronize();
-    *out = Z(h[0], h[1]);
-    return e;
+  // Scalars and the per-cycle residuals on the device.
+  BicglScalars* S = nullptr;
+  double* hist = nullptr;
+  // (the residual history keeps its first 2^20 cycles)
+  const int hist_cap = std::min(max_iter / ell + 2, 1 << 20);
+  rc = Check(hipMalloc((void**)&S, sizeof(BicglScalars)), "alloc bicgstab(l) scalars");
+  if (!rc) rc = Check(hipMalloc((void**)&hist, (size_t)hist_cap * sizeof(double)), "alloc hist");
+  auto dotk = [&](const double* a, const double* b, int k) -> int {
+    return Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, (double*)&S->dot[k], stream_),
+                 "dot");
   };
-  // Two dot products, one host round trip: (a, b) and (c, d).
-  auto dot2 = [&](const double* a, const double* b, const double* c, const double* d, Z* o1,
-                  Z* o2) -> int {
-    int e = Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, d_dot_, stream_), "dot");
-    if (!e) e = Check(LaunchDot(c, d, n, 0, d_partial_, red_blocks_, d_dot_ + 2, stream_), "dot");
-    double h[4];
-    if (!e) e = Check(hipMemcpyAsync(h, d_dot_, sizeof(h), hipMemcpyDeviceToHost, stream_), "dot d2h");
-    if (!e) e = Synchronize();
-    *o1 = Z(h[0], h[1]);
-    *o2 = Z(h[2], h[3]);
-    return e;
+  // dot product into S->dot[k], then the scalar step op in the same kernel
+  auto dot_step = [&](const double* a, const double* b, int k, int op, int i, int j) -> int {
+    return Check(LaunchBicglDotStep(a, b, n, d_partial_, red_blocks_, S, k, op, i, j, tol,
+                                    max_iter, hist, hist_cap, stream_),
+                 "bicgstab(l) step");
   };
-  // y = a + c b (y may be a)
-  auto axpy = [&](double* y, const double* a, Z c, const double* b) -> int {
-    const double* xs[2] = {a, b};
-    const double2 cs[2] = {make_double2(1., 0.), make_double2(c.real(), c.imag())};
-    return Check(LaunchLincomb(y, n, 2, xs, cs, stream_), "lincomb");
+  // y = a + sign * c * b, c a scalar of S
+  auto axpy = [&](double* y, const double* a, const double* b, const double2* c,
+                  double sign) -> int {
+    return Check(LaunchZaxpyDev(y, a, b, n, c, sign, &S->stop, stream_), "axpy");
   };
   // r_0 = b - A x (b = e_0 of the trace row), rtilde = r_0, u_0 = 0
-  rc = RhsDevice(x, vu[0], 1);
+  if (!rc) rc = RhsDevice(x, vu[0], 1);
   if (!rc) {
     const double* xs[1] = {vu[0]};
     const double2 cs[1] = {make_double2(-1., 0.)};
@@ -1159,91 +1158,94 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
   }
   if (!rc) rc = Check(hipMemcpyAsync(rt, vr[0], bytes, hipMemcpyDeviceToDevice, stream_), "rtilde");
   if (!rc) rc = Check(hipMemsetAsync(vu[0], 0, bytes, stream_), "u0");
-  // |r_0|^2 and the first BiCG step's (rtilde, r_0) together; likewise at
-  // the end of every cycle.
-  Z rn2, rho1_next;
-  if (!rc) rc = dot2(vr[0], vr[0], rt, vr[0], &rn2, &rho1_next);
-  double res = std::sqrt(rn2.real());
-  if (history && !rc) history->push_back(res);
-  Z rho0(1, 0), alpha(0, 0), omega(1, 0);
-  int it = 0;
-  bool broke = false;
-  std::vector<std::vector<Z>> tau(ell + 1, std::vector<Z>(ell + 1));
-  std::vector<double> sigma(ell + 1);
-  std::vector<Z> gp(ell + 1), g(ell + 1), gpp(ell + 1);
-  while (!rc && res > tol && it < max_iter && !broke) {
-    rho0 = -omega * rho0;
-    // BiCG part
-    for (int j = 0; j < ell && !rc && !broke; ++j) {
-      Z rho1 = rho1_next;
-      if (j > 0) rc = dot(rt, vr[j], &rho1);
-      if (rc) break;
-      if (rho0 == Z(0, 0)) { broke = true; break; }
-      const Z beta = alpha * rho1 / rho0;
-      rho0 = rho1;
-      for (int i = 0; i <= j && !rc; ++i) rc = axpy(vu[i], vr[i], -beta, vu[i]);
-      if (!rc) rc = RhsDevice(vu[j], vu[j + 1], 1);
-      Z gam;
-      if (!rc) rc = dot(rt, vu[j + 1], &gam);
-      if (rc) break;
-      if (gam == Z(0, 0)) { broke = true; break; }
-      alpha = rho0 / gam;
-      for (int i = 0; i <= j && !rc; ++i) rc = axpy(vr[i], vr[i], -alpha, vu[i + 1]);
-      if (!rc) rc = RhsDevice(vr[j], vr[j + 1], 1);
-      if (!rc) rc = axpy(x, x, alpha, vu[0]);
+  // |r_0|^2 into dot[2] and the first BiCG step's (rtilde, r_0) into dot[3];
+  // every cycle ends the same way.
+  if (!rc) rc = dotk(vr[0], vr[0], 2);
+  if (!rc) rc = dot_step(rt, vr[0], 3, kBlInit, 0, 0);
+  // One cycle: ell BiCG steps, then the minimal-residual part (the first
+  // step's beta comes from the previous kBlInit / kBlEnd).
+  auto enqueue_cycle = [&]() -> int {
+    int e = kOk;
+    for (int j = 0; j < ell && !e; ++j) {
+      if (j > 0) e = dot_step(rt, vr[j], 0, kBlBeta, 0, 0);
+      for (int i = 0; i <= j && !e; ++i) e = axpy(vu[i], vr[i], vu[i], &S->beta, -1.);
+      if (!e) e = RhsDevice(vu[j], vu[j + 1], 1);
+      if (!e) e = dot_step(rt, vu[j + 1], 0, kBlAlpha, 0, 0);
+      for (int i = 0; i <= j && !e; ++i) e = axpy(vr[i], vr[i], vu[i + 1], &S->alpha, -1.);
+      if (!e) e = RhsDevice(vr[j], vr[j + 1], 1);
+      if (!e) e = axpy(x, x, vu[0], &S->alpha, 1.);
     }
-    if (rc || broke) break;
-    // MR part (modified Gram-Schmidt)
-    for (int j = 1; j <= ell && !rc && !broke; ++j) {
-      for (int i = 1; i < j && !rc; ++i) {
-        Z d;
-        rc = dot(vr[i], vr[j], &d);
-        if (rc) break;
-        tau[i][j] = d / sigma[i];
-        rc = axpy(vr[j], vr[j], -tau[i][j], vr[i]);
+    // modified Gram-Schmidt
+    for (int j = 1; j <= ell && !e; ++j) {
+      for (int i = 1; i < j && !e; ++i) {
+        e = dot_step(vr[i], vr[j], 0, kBlTau, i, j);
+        if (!e) e = axpy(vr[j], vr[j], vr[i], &S->tau[i][j], -1.);
       }
-      Z sj, d0;
-      if (!rc) rc = dot2(vr[j], vr[j], vr[j], vr[0], &sj, &d0);
-      if (rc) break;
-      sigma[j] = sj.real();
-      if (!(sigma[j] > 0)) { broke = true; break; }
-      gp[j] = d0 / sigma[j];
+      if (!e) e = dotk(vr[j], vr[j], 0);
+      // (for j = ell this also solves for g, gpp and omega)
+      if (!e) e = dot_step(vr[j], vr[0], 1, kBlSigma, ell, j);
     }
-    if (rc || broke) break;
-    g[ell] = gp[ell];
-    omega = g[ell];
-    for (int j = ell - 1; j >= 1; --j) {
-      Z sum(0, 0);
-      for (int i = j + 1; i <= ell; ++i) sum += tau[j][i] * g[i];
-      g[j] = gp[j] - sum;
+    if (!e) e = axpy(x, x, vr[0], &S->g[1], 1.);
+    if (!e) e = axpy(vr[0], vr[0], vr[ell], &S->gp[ell], -1.);
+    if (!e) e = axpy(vu[0], vu[0], vu[ell], &S->g[ell], -1.);
+    for (int j = 1; j <= ell - 1 && !e; ++j) {
+      e = axpy(vu[0], vu[0], vu[j], &S->g[j], -1.);
+      if (!e) e = axpy(x, x, vr[j], &S->gpp[j], 1.);
+      if (!e) e = axpy(vr[0], vr[0], vr[j], &S->gp[j], -1.);
     }
-    for (int j = 1; j <= ell - 1; ++j) {
-      Z sum(0, 0);
-      for (int i = j + 1; i <= ell - 1; ++i) sum += tau[j][i] * g[i + 1];
-      gpp[j] = g[j + 1] + sum;
+    if (!e) e = dotk(vr[0], vr[0], 2);
+    if (!e) e = dot_step(rt, vr[0], 3, kBlEnd, ell, 0);
+    return e;
+  };
+  // Batches of cycles (about 16 BiCG steps) between the host's checks,
+  // captured once as a hipGraph and replayed; cycles after the stop are
+  // no-ops.
+  const int batch = std::max(1, 16 / ell);
+  auto enqueue_batch = [&]() -> int {
+    int e = kOk;
+    for (int b = 0; b < batch && !e; ++b) e = enqueue_cycle();
+    return e;
+  };
+  BicglScalars hs{};
+  if (!rc) rc = Check(hipMemcpyAsync(&hs, S, sizeof(hs), hipMemcpyDeviceToHost, stream_), "scalars");
+  if (!rc) rc = Synchronize();
+  hipGraphExec_t exec = nullptr;
+  while (!rc && !hs.stop) {
+    if (use_graphs_ && !exec) {
+      hipGraph_t g = nullptr;
+      rc = Check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+                 "begin bicgstab(l) capture");
+      if (!rc) {
+        rc = enqueue_batch();
+        const int rc_end = Check(hipStreamEndCapture(stream_, &g), "end bicgstab(l) capture");
+        if (!rc) rc = rc_end;
+      }
+      if (!rc) rc = Check(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0),
+                          "instantiate bicgstab(l) batch");
+      if (g) (void)hipGraphDestroy(g);
+      if (rc) exec = nullptr;
     }
-    rc = axpy(x, x, g[1], vr[0]);
-    if (!rc) rc = axpy(vr[0], vr[0], -gp[ell], vr[ell]);
-    if (!rc) rc = axpy(vu[0], vu[0], -g[ell], vu[ell]);
-    for (int j = 1; j <= ell - 1 && !rc; ++j) {
-      rc = axpy(vu[0], vu[0], -g[j], vu[j]);
-      if (!rc) rc = axpy(x, x, gpp[j], vr[j]);
-      if (!rc) rc = axpy(vr[0], vr[0], -gp[j], vr[j]);
-    }
-    if (!rc) rc = dot2(vr[0], vr[0], rt, vr[0], &rn2, &rho1_next);
-    if (rc) break;
-    it += ell;
-    res = std::sqrt(rn2.real());
-    if (history) history->push_back(res);
-    if (omega == Z(0, 0)) broke = true;
+    if (!rc) rc = exec ? Check(hipGraphLaunch(exec, stream_), "bicgstab(l) batch") : enqueue_batch();
+    if (!rc) rc = Check(hipMemcpyAsync(&hs, S, sizeof(hs), hipMemcpyDeviceToHost, stream_), "scalars");
+    if (!rc) rc = Synchronize();
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (!rc && history) {
+    std::vector<double> h(std::min(hs.cycles + 1, hist_cap));
+    rc = Check(hipMemcpyAsync(h.data(), hist, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              stream_), "residual history");
+    if (!rc) rc = Synchronize();
+    if (!rc) history->insert(history->end(), h.begin(), h.end());
   }
   const int rc_sync = Synchronize();
+  Free(S);
+  Free(hist);
   cleanup();
   if (!rc) rc = rc_sync;
-  if (iters) *iters = it;
-  if (resid) *resid = res;
+  if (iters) *iters = hs.it;
+  if (resid) *resid = hs.res;
   if (rc) return rc;
-  return res <= tol ? kOk : kErrNotConverged;
+  return hs.res <= tol ? kOk : kErrNotConverged;
 }
 
 int Engine::HistoryCount() {

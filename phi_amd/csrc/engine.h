@@ -214,7 +214,9 @@ class Engine {
                   std::vector<double>* history);
   // The same system by BiCGSTAB(ell) (ell = 1..8): ell BiCG steps, then a
   // minimal-residual polynomial of degree ell; *iters counts BiCG steps
-  // (two matvecs each), history holds |r| after every ell of them.
+  // (two matvecs each), history holds |r| after every ell of them.  The
+  // scalars stay on the device (BicglScalars); batches of cycles are
+  // replayed from a hipGraph when graphs are on.
   int SteadyStateL(int ell, double tol, int max_iter, int* iters, double* resid,
                    std::vector<double>* history);
 

@@ -180,6 +180,51 @@ hipError_t LaunchBicgXRP(double* x, double* r, double* p, const double* sv, cons
 hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const double* xr_part,
                             int nblocks, double* hist, double tol, int max_iter, hipStream_t s);
 
+// BiCGSTAB(ell) with its scalars on the device (Sleijpen and Fokkema 1993,
+// Algorithm 3.1).  Dot products land in dot[0..3] (LaunchDot into
+// &dot[k], or LaunchBicglDotStep, whose reduction kernel then runs a
+// scalar step on one thread to derive the scalars from them); the vector
+// updates read their coefficient from this record.  stop (breakdown,
+// convergence or the step cap) turns every later step and update into a
+// no-op, so whole cycles can be enqueued ahead of the host's check.
+constexpr int kBicglMaxEll = 8;
+struct BicglScalars {
+  double2 dot[4];
+  double2 rho0, alpha, omega, beta;
+  double2 tau[kBicglMaxEll + 1][kBicglMaxEll + 1];
+  double2 gp[kBicglMaxEll + 1], g[kBicglMaxEll + 1], gpp[kBicglMaxEll + 1];
+  double sigma[kBicglMaxEll + 1];
+  double res;   // |r_0| after the last finished cycle
+  int stop;     // broke || converged || it >= max_iter
+  int broke;
+  int it;       // BiCG steps of the finished cycles
+  int cycles;   // finished cycles
+};
+// Scalar steps (one thread):
+//   kBlInit   res = sqrt(dot[2]), rho0 = 1, alpha = 0, omega = 1, hist[0];
+//             then the first cycle's start
+//   kBlBeta   rho1 = dot[0]: beta = alpha rho1 / rho0, rho0 = rho1
+//   kBlAlpha  alpha = rho0 / dot[0]
+//   kBlTau    tau[i][j] = dot[0] / sigma[i]
+//   kBlSigma  sigma[j] = dot[0], gp[j] = dot[1] / sigma[j]; for j = i
+//             (= ell) also g, gpp from tau and gp, omega = g[ell]
+//   kBlEnd    res = sqrt(dot[2]), it += ell (= i), hist[cycles]; stop; then
+//             the next cycle's start
+// A cycle's start is rho0 = -omega rho0 and its first BiCG step's beta with
+// rho1 = dot[3] = (rtilde, r_0).  A zero rho0 or dot[0] (alpha) or a
+// non-positive sigma, or omega = 0 at the end, is a breakdown.
+enum { kBlInit = 0, kBlBeta, kBlAlpha, kBlTau, kBlSigma, kBlEnd };
+// LaunchDot's reduction of conj(u) . v into S->dot[k] followed, in the same
+// one-block kernel, by the scalar step op.
+hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double* partial,
+                              int nblocks, BicglScalars* S, int k, int op, int i, int j,
+                              double tol, int max_iter, double* hist, int hist_cap,
+                              hipStream_t s);
+// y = a + sign * (*c) * b elementwise over n complex values (y may alias a
+// or b); nothing when *stop.
+hipError_t LaunchZaxpyDev(double* y, const double* a, const double* b, size_t n,
+                          const double2* c, double sign, const int* stop, hipStream_t s);
+
 }  // namespace phi
 
 #endif  // PHI_AMD_HEOM_KERNELS_H_

@@ -358,8 +358,10 @@ __global__ void __launch_bounds__(256) dot_partial(const double* __restrict__ u,
   }
 }
 
-__global__ void __launch_bounds__(256) dot_final(const double* __restrict__ partial, int nb,
-                                                 double* __restrict__ out) {
+// Fixed-order sum of nb complex partials by a block of 256 threads; the
+// result is valid in thread 0.
+__device__ __forceinline__ void dot_reduce(const double* partial, int nb, double* re,
+                                           double* im) {
   __shared__ double sr[256], si[256];
   double ar = 0, ai = 0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
@@ -376,9 +378,17 @@ __global__ void __launch_bounds__(256) dot_final(const double* __restrict__ part
     }
     __syncthreads();
   }
+  *re = sr[0];
+  *im = si[0];
+}
+
+__global__ void __launch_bounds__(256) dot_final(const double* __restrict__ partial, int nb,
+                                                 double* __restrict__ out) {
+  double re, im;
+  dot_reduce(partial, nb, &re, &im);
   if (threadIdx.x == 0) {
-    out[0] = sr[0];
-    out[1] = si[0];
+    out[0] = re;
+    out[1] = im;
   }
 }
 
@@ -876,6 +886,168 @@ hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const do
                             int nblocks, double* hist, double tol, int max_iter, hipStream_t s) {
   hipLaunchKernelGGL(dev::bicg_finish, dim3(1), dim3(256), 0, s, prev, next, xr_part, nblocks,
                      hist, tol, max_iter);
+  return hipGetLastError();
+}
+
+namespace dev {
+
+__device__ __forceinline__ bool zzero(double2 a) { return a.x == 0 && a.y == 0; }
+
+// rho0 = -omega rho0 (start of a cycle), then the first BiCG step's beta
+// from rho1 = dot[3] = (rtilde, r_0).
+__device__ void bicgl_start(BicglScalars* S) {
+  const double2 m = zmul(S->omega, S->rho0);
+  S->rho0 = make_double2(-m.x, -m.y);
+  const double2 rho1 = S->dot[3];
+  if (zzero(S->rho0)) {
+    S->broke = 1;
+    S->stop = 1;
+    return;
+  }
+  S->beta = zdiv(zmul(S->alpha, rho1), S->rho0);
+  S->rho0 = rho1;
+}
+
+__device__ void bicgl_solve(BicglScalars* S, int ell) {
+  S->g[ell] = S->gp[ell];
+  S->omega = S->g[ell];
+  for (int a = ell - 1; a >= 1; --a) {
+    double2 sum = make_double2(0., 0.);
+    for (int b = a + 1; b <= ell; ++b) {
+      const double2 t = zmul(S->tau[a][b], S->g[b]);
+      sum.x += t.x;
+      sum.y += t.y;
+    }
+    S->g[a] = make_double2(S->gp[a].x - sum.x, S->gp[a].y - sum.y);
+  }
+  for (int a = 1; a <= ell - 1; ++a) {
+    double2 sum = make_double2(0., 0.);
+    for (int b = a + 1; b <= ell - 1; ++b) {
+      const double2 t = zmul(S->tau[a][b], S->g[b + 1]);
+      sum.x += t.x;
+      sum.y += t.y;
+    }
+    S->gpp[a] = make_double2(S->g[a + 1].x + sum.x, S->g[a + 1].y + sum.y);
+  }
+}
+
+// One scalar step (one thread).
+__device__ void bicgl_body(BicglScalars* S, int op, int i, int j, double tol, int max_iter,
+                           double* hist, int hist_cap) {
+  if (op == kBlInit) {
+    S->res = sqrt(S->dot[2].x);
+    S->rho0 = make_double2(1., 0.);
+    S->alpha = make_double2(0., 0.);
+    S->omega = make_double2(1., 0.);
+    S->broke = 0;
+    S->it = 0;
+    S->cycles = 0;
+    S->stop = !(S->res > tol) || max_iter <= 0;
+    if (hist_cap > 0) hist[0] = S->res;
+    if (!S->stop) bicgl_start(S);
+    return;
+  }
+  if (S->stop) return;
+  switch (op) {
+    case kBlBeta: {
+      const double2 rho1 = S->dot[0];
+      if (zzero(S->rho0)) {
+        S->broke = 1;
+        S->stop = 1;
+        break;
+      }
+      S->beta = zdiv(zmul(S->alpha, rho1), S->rho0);
+      S->rho0 = rho1;
+      break;
+    }
+    case kBlAlpha: {
+      const double2 gam = S->dot[0];
+      if (zzero(gam)) {
+        S->broke = 1;
+        S->stop = 1;
+        break;
+      }
+      S->alpha = zdiv(S->rho0, gam);
+      break;
+    }
+    case kBlTau: {
+      const double2 d = S->dot[0];
+      const double sg = S->sigma[i];
+      S->tau[i][j] = make_double2(d.x / sg, d.y / sg);
+      break;
+    }
+    case kBlSigma: {
+      const double sg = S->dot[0].x;
+      S->sigma[j] = sg;
+      if (!(sg > 0)) {
+        S->broke = 1;
+        S->stop = 1;
+        break;
+      }
+      const double2 d0 = S->dot[1];
+      S->gp[j] = make_double2(d0.x / sg, d0.y / sg);
+      if (j == i) bicgl_solve(S, i);
+      break;
+    }
+    case kBlEnd: {
+      S->res = sqrt(S->dot[2].x);
+      S->it += i;
+      S->cycles += 1;
+      if (S->cycles < hist_cap) hist[S->cycles] = S->res;
+      if (zzero(S->omega)) S->broke = 1;
+      S->stop = S->broke || !(S->res > tol) || S->it >= max_iter;
+      if (!S->stop) bicgl_start(S);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// dot_final into S->dot[k], then the scalar step op (thread 0).
+__global__ void __launch_bounds__(256) bicgl_dot_step(const double* __restrict__ partial, int nb,
+                                                      BicglScalars* S, int k, int op, int i,
+                                                      int j, double tol, int max_iter,
+                                                      double* hist, int hist_cap) {
+  double re, im;
+  dot_reduce(partial, nb, &re, &im);
+  if (threadIdx.x == 0) {
+    S->dot[k] = make_double2(re, im);
+    bicgl_body(S, op, i, j, tol, max_iter, hist, hist_cap);
+  }
+}
+
+__global__ void __launch_bounds__(256) zaxpy_dev(double* y, const double* a, const double* b,
+                                                 size_t n, const double2* c, double sign,
+                                                 const int* stop) {
+  if (*stop) return;  // uniform: every block returns
+  const double2 cc = make_double2(sign * c->x, sign * c->y);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double2 av = reinterpret_cast<const double2*>(a)[i];
+    const double2 bv = reinterpret_cast<const double2*>(b)[i];
+    reinterpret_cast<double2*>(y)[i] =
+        make_double2(av.x + (cc.x * bv.x - cc.y * bv.y), av.y + (cc.x * bv.y + cc.y * bv.x));
+  }
+}
+
+}  // namespace dev
+
+hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double* partial,
+                             int nblocks, BicglScalars* S, int k, int op, int i, int j, double tol,
+                             int max_iter, double* hist, int hist_cap, hipStream_t s) {
+  hipLaunchKernelGGL(dev::dot_partial, dim3(nblocks), dim3(256), 0, s, u, v, n, 0, partial);
+  hipLaunchKernelGGL(dev::bicgl_dot_step, dim3(1), dim3(256), 0, s, partial, nblocks, S, k, op,
+                     i, j, tol, max_iter, hist, hist_cap);
+  return hipGetLastError();
+}
+
+hipError_t LaunchZaxpyDev(double* y, const double* a, const double* b, size_t n,
+                          const double2* c, double sign, const int* stop, hipStream_t s) {
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(dev::zaxpy_dev, dim3(blocks), dim3(256), 0, s, y, a, b, n, c, sign, stop);
   return hipGetLastError();
 }
 

@@ -375,7 +375,8 @@ def test_steady_state_l_cycles_match_dense_bicgstab_l():
 def test_steady_state_l_converges_to_the_bicgstab_fixed_point():
     """BiCGSTAB(2) and BiCGSTAB(4) on the 7-site system at depth 4 reach the
     tolerance with fewer matrix-vector products than BiCGSTAB needs, and all
-    three land on the same fixed point of the oracle's operator."""
+    three land on the same fixed point of the oracle's operator; graph
+    replays of the batches of cycles equal eager launches bit for bit."""
     op, p = _subset_params("c2", 5)
     dt = p.get_double("timestep")
     ctx = P.Context(p)
@@ -389,12 +390,18 @@ def test_steady_state_l_converges_to_the_bicgstab_fixed_point():
                                gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature,
                                lh=5, num_terms=op.num_matsubara_terms))
     for ell in (2, 4):
-        print("case: bicgstab(%d) c2 lh 5" % ell, flush=True)
-        ctx = P.Context(p)
-        ctx.set_state(start)
-        rc, it, res = ctx.steady_state_l(ell, 1e-10, 20000)
-        x = ctx.state()
-        ctx.close()
+        runs = []
+        for graphs in (True, False):
+            print("case: bicgstab(%d) c2 lh 5 graphs=%s" % (ell, graphs), flush=True)
+            ctx = P.Context(p)
+            ctx.set_graphs(graphs)
+            ctx.set_state(start)
+            runs.append(ctx.steady_state_l(ell, 1e-10, 20000) + (ctx.state(),))
+            ctx.close()
+        # batches replayed from a hipGraph = eager launches, bit for bit
+        assert runs[0][:3] == runs[1][:3]
+        assert np.array_equal(runs[0][3], runs[1][3])
+        rc, it, res, x = runs[0]
         print("ell %d: %d BiCG steps (plain BiCGSTAB %d), residual %.3g" % (ell, it, it_u, res))
         assert rc == 0 and res <= 1e-10, (it, res)
         assert it <= it_u
