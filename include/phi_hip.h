@@ -279,6 +279,8 @@ int phi_hip_steady_state(phi_ctx* c, double tol, int max_iter, int* iters, doubl
 /* The same system by BiCGSTAB(ell), ell = 1..8 (Sleijpen and Fokkema 1993):
  * ell BiCG steps, then a minimal-residual polynomial of degree ell;
  * *iters counts BiCG steps (two matvecs each); ell = 1 is BiCGSTAB.
+ * PHI_HIP_EINVAL for ell outside 1..8 or max_iter < 0 (state untouched),
+ * PHI_HIP_ENOTCONVERGED when max_iter BiCG steps leave |r| above tol.
  * PHI counterpart: BicgstabLSteadyState (hierarchy_integrator.cc:
  * 3112-3489), selected by PHI's bicgstab method with ell >= 1. */
 int phi_hip_steady_state_l(phi_ctx* c, int ell, double tol, int max_iter, int* iters,

@@ -349,6 +349,12 @@ def test_steady_state_l_cycles_match_dense_bicgstab_l():
     orc = HO.Oracle(HO.Problem(op.time_indepenent_hamiltonian, op.initial_density_matrix, **kw))
     ctx = P.Context(p)
     x0 = ctx.state()
+    # ell outside 1..8 or a negative step cap: an error, the state untouched
+    for ell, cap in ((0, 10), (9, 10), (2, -1)):
+        rc, _, _ = ctx.steady_state_l(ell, 1e-10, cap)
+        assert rc != 0, (ell, cap)
+        assert P.lib().phi_hip_last_error().decode()
+    assert np.array_equal(ctx.state(), x0)
     ctx.close()
     m = x0.size
     A = np.zeros((m, m), dtype=complex)

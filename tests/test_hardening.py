@@ -175,6 +175,8 @@ def test_context_entry_points_refuse_a_null_context():
         "phi_hip_get_index": (None, ip, 4),
         "phi_hip_set_rk4_form": (None, 2),
         "phi_hip_set_graphs": (None, 1),
+        "phi_hip_steady_state": (None, 1e-10, 10, ip, ctypes.byref(dp)),
+        "phi_hip_steady_state_l": (None, 2, 1e-10, 10, ip, ctypes.byref(dp)),
     }
     for name, args in calls.items():
         rc = getattr(L, name)(*args)
