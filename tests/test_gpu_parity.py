@@ -379,9 +379,9 @@ def test_steady_state_l_cycles_match_dense_bicgstab_l():
 
 
 def test_steady_state_l_converges_to_the_bicgstab_fixed_point():
-    """BiCGSTAB(2) and BiCGSTAB(4) on the 7-site system at depth 4 reach the
-    tolerance with fewer matrix-vector products than BiCGSTAB needs, and all
-    three land on the same fixed point of the oracle's operator; graph
+    """BiCGSTAB(2), (4) and (8, the largest ell) on the 7-site system at depth
+    4 reach the tolerance with fewer matrix-vector products than BiCGSTAB
+    needs, and all land on the same fixed point of the oracle's operator; graph
     replays of the batches of cycles equal eager launches bit for bit."""
     op, p = _subset_params("c2", 5)
     dt = p.get_double("timestep")
@@ -395,7 +395,7 @@ def test_steady_state_l_converges_to_the_bicgstab_fixed_point():
     orc = HO.Oracle(HO.Problem(op.time_indepenent_hamiltonian, op.initial_density_matrix,
                                gamma=op.gamma, lambda_=op.lambda_, temperature=op.temperature,
                                lh=5, num_terms=op.num_matsubara_terms))
-    for ell in (2, 4):
+    for ell in (2, 4, 8):
         runs = []
         for graphs in (True, False):
             print("case: bicgstab(%d) c2 lh 5 graphs=%s" % (ell, graphs), flush=True)
