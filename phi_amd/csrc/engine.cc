@@ -1124,7 +1124,7 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
   BicglScalars* S = nullptr;
   double* hist = nullptr;
   // (the residual history keeps its first 2^20 cycles)
-  const int hist_cap = std::min(max_iter / ell + 2, 1 << 20);
+  const int hist_cap = (int)std::min<long long>((long long)max_iter / ell + 2, 1 << 20);
   rc = Check(hipMalloc((void**)&S, sizeof(BicglScalars)), "alloc bicgstab(l) scalars");
   if (!rc) rc = Check(hipMalloc((void**)&hist, (size_t)hist_cap * sizeof(double)), "alloc hist");
   auto dotk = [&](const double* a, const double* b, int k) -> int {
