@@ -1127,14 +1127,19 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
   const int hist_cap = (int)std::min<long long>((long long)max_iter / ell + 2, 1 << 20);
   rc = Check(hipMalloc((void**)&S, sizeof(BicglScalars)), "alloc bicgstab(l) scalars");
   if (!rc) rc = Check(hipMalloc((void**)&hist, (size_t)hist_cap * sizeof(double)), "alloc hist");
-  auto dotk = [&](const double* a, const double* b, int k) -> int {
-    return Check(LaunchDot(a, b, n, 0, d_partial_, red_blocks_, (double*)&S->dot[k], stream_),
-                 "dot");
-  };
   // dot product into S->dot[k], then the scalar step op in the same kernel
   auto dot_step = [&](const double* a, const double* b, int k, int op, int i, int j) -> int {
     return Check(LaunchBicglDotStep(a, b, n, d_partial_, red_blocks_, S, k, op, i, j, tol,
                                     max_iter, hist, hist_cap, stream_),
+                 "bicgstab(l) step");
+  };
+  // (a, b) into S->dot[k] and (c, d) into S->dot[k + 1] in one pass, then
+  // the step op; the partials of 4 doubles per block fill d_partial_
+  // (2 * red_blocks_ doubles) with red_blocks_ / 2 blocks.
+  auto dot2_step = [&](const double* a, const double* b, const double* c, const double* d, int k,
+                       int op, int i, int j) -> int {
+    return Check(LaunchBicglDot2Step(a, b, c, d, n, d_partial_, red_blocks_ / 2, S, k, op, i, j,
+                                     tol, max_iter, hist, hist_cap, stream_),
                  "bicgstab(l) step");
   };
   // y = a + sign * c * b, c a scalar of S
@@ -1160,8 +1165,7 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
   if (!rc) rc = Check(hipMemsetAsync(vu[0], 0, bytes, stream_), "u0");
   // |r_0|^2 into dot[2] and the first BiCG step's (rtilde, r_0) into dot[3];
   // every cycle ends the same way.
-  if (!rc) rc = dotk(vr[0], vr[0], 2);
-  if (!rc) rc = dot_step(rt, vr[0], 3, kBlInit, 0, 0);
+  if (!rc) rc = dot2_step(vr[0], vr[0], rt, vr[0], 2, kBlInit, 0, 0);
   // One cycle: ell BiCG steps, then the minimal-residual part (the first
   // step's beta comes from the previous kBlInit / kBlEnd).
   auto enqueue_cycle = [&]() -> int {
@@ -1181,9 +1185,8 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
         e = dot_step(vr[i], vr[j], 0, kBlTau, i, j);
         if (!e) e = axpy(vr[j], vr[j], vr[i], &S->tau[i][j], -1.);
       }
-      if (!e) e = dotk(vr[j], vr[j], 0);
       // (for j = ell this also solves for g, gpp and omega)
-      if (!e) e = dot_step(vr[j], vr[0], 1, kBlSigma, ell, j);
+      if (!e) e = dot2_step(vr[j], vr[j], vr[j], vr[0], 0, kBlSigma, ell, j);
     }
     if (!e) e = axpy(x, x, vr[0], &S->g[1], 1.);
     if (!e) e = axpy(vr[0], vr[0], vr[ell], &S->gp[ell], -1.);
@@ -1193,8 +1196,7 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
       if (!e) e = axpy(x, x, vr[j], &S->gpp[j], 1.);
       if (!e) e = axpy(vr[0], vr[0], vr[j], &S->gp[j], -1.);
     }
-    if (!e) e = dotk(vr[0], vr[0], 2);
-    if (!e) e = dot_step(rt, vr[0], 3, kBlEnd, ell, 0);
+    if (!e) e = dot2_step(vr[0], vr[0], rt, vr[0], 2, kBlEnd, ell, 0);
     return e;
   };
   // Batches of cycles (about 16 BiCG steps) between the host's checks,

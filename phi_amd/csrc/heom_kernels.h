@@ -181,9 +181,9 @@ hipError_t LaunchBicgFinish(const BicgScalars* prev, BicgScalars* next, const do
                             int nblocks, double* hist, double tol, int max_iter, hipStream_t s);
 
 // BiCGSTAB(ell) with its scalars on the device (Sleijpen and Fokkema 1993,
-// Algorithm 3.1).  Dot products land in dot[0..3] (LaunchDot into
-// &dot[k], or LaunchBicglDotStep, whose reduction kernel then runs a
-// scalar step on one thread to derive the scalars from them); the vector
+// Algorithm 3.1).  Dot products land in dot[0..3] through
+// LaunchBicglDotStep or LaunchBicglDot2Step, whose reduction kernel then
+// runs a scalar step on one thread to derive the scalars from them; the vector
 // updates read their coefficient from this record.  stop (breakdown,
 // convergence or the step cap) turns every later step and update into a
 // no-op, so whole cycles can be enqueued ahead of the host's check.
@@ -220,6 +220,13 @@ hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double
                               int nblocks, BicglScalars* S, int k, int op, int i, int j,
                               double tol, int max_iter, double* hist, int hist_cap,
                               hipStream_t s);
+// Two dot products in one pass, conj(a) . b into S->dot[k] and conj(c) . d
+// into S->dot[k + 1] (k = 0..2), then the scalar step op; partial holds 4
+// doubles per block (nblocks blocks).
+hipError_t LaunchBicglDot2Step(const double* a, const double* b, const double* c,
+                               const double* d, size_t n, double* partial, int nblocks,
+                               BicglScalars* S, int k, int op, int i, int j, double tol,
+                               int max_iter, double* hist, int hist_cap, hipStream_t s);
 // y = a + sign * (*c) * b elementwise over n complex values (y may alias a
 // or b); nothing when *stop.
 hipError_t LaunchZaxpyDev(double* y, const double* a, const double* b, size_t n,

@@ -1004,6 +1004,67 @@ __device__ void bicgl_body(BicglScalars* S, int op, int i, int j, double tol, in
   }
 }
 
+// Partials of two dot products in one pass: conj(a) . b and conj(c) . d,
+// 4 doubles per block.
+__global__ void __launch_bounds__(256) dot2_partial(const double* __restrict__ a,
+                                                    const double* __restrict__ b,
+                                                    const double* __restrict__ c,
+                                                    const double* __restrict__ d, size_t n,
+                                                    double* __restrict__ partial) {
+  __shared__ double sh[4][256];
+  double acc[4] = {0, 0, 0, 0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double2 p = reinterpret_cast<const double2*>(a)[i];
+    const double2 q = reinterpret_cast<const double2*>(b)[i];
+    const double2 u = reinterpret_cast<const double2*>(c)[i];
+    const double2 v = reinterpret_cast<const double2*>(d)[i];
+    acc[0] += p.x * q.x + p.y * q.y;
+    acc[1] += p.x * q.y - p.y * q.x;
+    acc[2] += u.x * v.x + u.y * v.y;
+    acc[3] += u.x * v.y - u.y * v.x;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partial[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Fixed-order sum of nb records of dot2_partial (one block of 256 threads)
+// into S->dot[k] and S->dot[k + 1], then the scalar step op (thread 0).
+__global__ void __launch_bounds__(256) bicgl_dot2_step(const double* __restrict__ partial, int nb,
+                                                       BicglScalars* S, int k, int op, int i,
+                                                       int j, double tol, int max_iter,
+                                                       double* hist, int hist_cap) {
+  __shared__ double sh[4][256];
+  double acc[4] = {0, 0, 0, 0};
+  for (int r = threadIdx.x; r < nb; r += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] += partial[4 * r + c];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    S->dot[k] = make_double2(sh[0][0], sh[1][0]);
+    S->dot[k + 1] = make_double2(sh[2][0], sh[3][0]);
+    bicgl_body(S, op, i, j, tol, max_iter, hist, hist_cap);
+  }
+}
+
 // dot_final into S->dot[k], then the scalar step op (thread 0).
 __global__ void __launch_bounds__(256) bicgl_dot_step(const double* __restrict__ partial, int nb,
                                                       BicglScalars* S, int k, int op, int i,
@@ -1038,6 +1099,17 @@ hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double
                              int max_iter, double* hist, int hist_cap, hipStream_t s) {
   hipLaunchKernelGGL(dev::dot_partial, dim3(nblocks), dim3(256), 0, s, u, v, n, 0, partial);
   hipLaunchKernelGGL(dev::bicgl_dot_step, dim3(1), dim3(256), 0, s, partial, nblocks, S, k, op,
+                     i, j, tol, max_iter, hist, hist_cap);
+  return hipGetLastError();
+}
+
+hipError_t LaunchBicglDot2Step(const double* a, const double* b, const double* c,
+                              const double* d, size_t n, double* partial, int nblocks,
+                              BicglScalars* S, int k, int op, int i, int j, double tol,
+                              int max_iter, double* hist, int hist_cap, hipStream_t s) {
+  if (k < 0 || k > 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dev::dot2_partial, dim3(nblocks), dim3(256), 0, s, a, b, c, d, n, partial);
+  hipLaunchKernelGGL(dev::bicgl_dot2_step, dim3(1), dim3(256), 0, s, partial, nblocks, S, k, op,
                      i, j, tol, max_iter, hist, hist_cap);
   return hipGetLastError();
 }
