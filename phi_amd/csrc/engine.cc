@@ -1147,6 +1147,12 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
                   double sign) -> int {
     return Check(LaunchZaxpyDev(y, a, b, n, c, sign, &S->stop, stream_), "axpy");
   };
+  auto pairs = [&](const ZPairs& pr, const double2* c) -> int {
+    return Check(LaunchZaxpyPairs(pr, n, c, &S->stop, stream_), "axpy pairs");
+  };
+  auto comb = [&](double* y, const ZTerms& t) -> int {
+    return Check(LaunchZcombDev(y, y, t, n, &S->stop, stream_), "lincomb terms");
+  };
   // r_0 = b - A x (b = e_0 of the trace row), rtilde = r_0, u_0 = 0
   if (!rc) rc = RhsDevice(x, vu[0], 1);
   if (!rc) {
@@ -1172,12 +1178,33 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
     int e = kOk;
     for (int j = 0; j < ell && !e; ++j) {
       if (j > 0) e = dot_step(rt, vr[j], 0, kBlBeta, 0, 0);
-      for (int i = 0; i <= j && !e; ++i) e = axpy(vu[i], vr[i], vu[i], &S->beta, -1.);
+      // u_i = r_i - beta u_i, i <= j
+      ZPairs up{};
+      for (int i = 0; i <= j; ++i) {
+        up.y[i] = vu[i];
+        up.a[i] = vr[i];
+        up.b[i] = vu[i];
+        up.sign[i] = -1.;
+      }
+      up.count = j + 1;
+      if (!e) e = pairs(up, &S->beta);
       if (!e) e = RhsDevice(vu[j], vu[j + 1], 1);
       if (!e) e = dot_step(rt, vu[j + 1], 0, kBlAlpha, 0, 0);
-      for (int i = 0; i <= j && !e; ++i) e = axpy(vr[i], vr[i], vu[i + 1], &S->alpha, -1.);
+      // r_i -= alpha u_{i+1}, i <= j, and x += alpha u_0
+      ZPairs rp{};
+      for (int i = 0; i <= j; ++i) {
+        rp.y[i] = vr[i];
+        rp.a[i] = vr[i];
+        rp.b[i] = vu[i + 1];
+        rp.sign[i] = -1.;
+      }
+      rp.y[j + 1] = x;
+      rp.a[j + 1] = x;
+      rp.b[j + 1] = vu[0];
+      rp.sign[j + 1] = 1.;
+      rp.count = j + 2;
+      if (!e) e = pairs(rp, &S->alpha);
       if (!e) e = RhsDevice(vr[j], vr[j + 1], 1);
-      if (!e) e = axpy(x, x, vu[0], &S->alpha, 1.);
     }
     // modified Gram-Schmidt
     for (int j = 1; j <= ell && !e; ++j) {
@@ -1188,14 +1215,21 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
       // (for j = ell this also solves for g, gpp and omega)
       if (!e) e = dot2_step(vr[j], vr[j], vr[j], vr[0], 0, kBlSigma, ell, j);
     }
-    if (!e) e = axpy(x, x, vr[0], &S->g[1], 1.);
-    if (!e) e = axpy(vr[0], vr[0], vr[ell], &S->gp[ell], -1.);
-    if (!e) e = axpy(vu[0], vu[0], vu[ell], &S->g[ell], -1.);
-    for (int j = 1; j <= ell - 1 && !e; ++j) {
-      e = axpy(vu[0], vu[0], vu[j], &S->g[j], -1.);
-      if (!e) e = axpy(x, x, vr[j], &S->gpp[j], 1.);
-      if (!e) e = axpy(vr[0], vr[0], vr[j], &S->gp[j], -1.);
+    // x += g_1 r_0 + sum gpp_j r_j (before r_0 changes), then
+    // r_0 -= gp_ell r_ell + sum gp_j r_j and u_0 -= g_ell u_ell + sum g_j u_j
+    ZTerms tx{}, tr{}, tu{};
+    tx.b[0] = vr[0], tx.c[0] = &S->g[1], tx.sign[0] = 1.;
+    tr.b[0] = vr[ell], tr.c[0] = &S->gp[ell], tr.sign[0] = -1.;
+    tu.b[0] = vu[ell], tu.c[0] = &S->g[ell], tu.sign[0] = -1.;
+    for (int j = 1; j <= ell - 1; ++j) {
+      tx.b[j] = vr[j], tx.c[j] = &S->gpp[j], tx.sign[j] = 1.;
+      tr.b[j] = vr[j], tr.c[j] = &S->gp[j], tr.sign[j] = -1.;
+      tu.b[j] = vu[j], tu.c[j] = &S->g[j], tu.sign[j] = -1.;
     }
+    tx.count = tr.count = tu.count = ell;
+    if (!e) e = comb(x, tx);
+    if (!e) e = comb(vr[0], tr);
+    if (!e) e = comb(vu[0], tu);
     if (!e) e = dot2_step(vr[0], vr[0], rt, vr[0], 2, kBlEnd, ell, 0);
     return e;
   };

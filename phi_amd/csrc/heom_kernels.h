@@ -227,6 +227,29 @@ hipError_t LaunchBicglDot2Step(const double* a, const double* b, const double* c
                                const double* d, size_t n, double* partial, int nblocks,
                                BicglScalars* S, int k, int op, int i, int j, double tol,
                                int max_iter, double* hist, int hist_cap, hipStream_t s);
+// Several updates with one coefficient, one pass: y_p = a_p + sign_p *
+// (*c) * b_p for p < count (count <= kBicglMaxEll + 1; no y_p may be an
+// input of another pair); nothing when *stop.
+struct ZPairs {
+  double* y[kBicglMaxEll + 1];
+  const double* a[kBicglMaxEll + 1];
+  const double* b[kBicglMaxEll + 1];
+  double sign[kBicglMaxEll + 1];
+  int count;
+};
+hipError_t LaunchZaxpyPairs(const ZPairs& pr, size_t n, const double2* c, const int* stop,
+                            hipStream_t s);
+// One output from several terms: y = a + sum_t sign_t * (*c_t) * b_t over
+// t < count (count <= kBicglMaxEll; y may alias a, no b_t); nothing when
+// *stop.
+struct ZTerms {
+  const double* b[kBicglMaxEll];
+  const double2* c[kBicglMaxEll];
+  double sign[kBicglMaxEll];
+  int count;
+};
+hipError_t LaunchZcombDev(double* y, const double* a, const ZTerms& t, size_t n, const int* stop,
+                          hipStream_t s);
 // y = a + sign * (*c) * b elementwise over n complex values (y may alias a
 // or b); nothing when *stop.
 hipError_t LaunchZaxpyDev(double* y, const double* a, const double* b, size_t n,

@@ -1092,7 +1092,84 @@ __global__ void __launch_bounds__(256) zaxpy_dev(double* y, const double* a, con
   }
 }
 
+__global__ void __launch_bounds__(256) zaxpy_pairs(ZPairs pr, size_t n, const double2* c,
+                                                   const int* stop) {
+  if (*stop) return;  // uniform: every block returns
+  const double2 cc = *c;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int p = 0; p < kBicglMaxEll + 1; ++p) {
+      if (p < pr.count) {
+        const double2 av = reinterpret_cast<const double2*>(pr.a[p])[i];
+        const double2 bv = reinterpret_cast<const double2*>(pr.b[p])[i];
+        const double2 k = make_double2(pr.sign[p] * cc.x, pr.sign[p] * cc.y);
+        reinterpret_cast<double2*>(pr.y[p])[i] =
+            make_double2(av.x + (k.x * bv.x - k.y * bv.y), av.y + (k.x * bv.y + k.y * bv.x));
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) zcomb_dev(double* y, const double* a, ZTerms t, size_t n,
+                                                 const int* stop) {
+  if (*stop) return;  // uniform: every block returns
+  double2 k[kBicglMaxEll];
+#pragma unroll
+  for (int q = 0; q < kBicglMaxEll; ++q)
+    k[q] = q < t.count ? make_double2(t.sign[q] * t.c[q]->x, t.sign[q] * t.c[q]->y)
+                       : make_double2(0., 0.);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double2 acc = reinterpret_cast<const double2*>(a)[i];
+#pragma unroll
+    for (int q = 0; q < kBicglMaxEll; ++q) {
+      if (q < t.count) {
+        const double2 bv = reinterpret_cast<const double2*>(t.b[q])[i];
+        acc.x += k[q].x * bv.x - k[q].y * bv.y;
+        acc.y += k[q].x * bv.y + k[q].y * bv.x;
+      }
+    }
+    reinterpret_cast<double2*>(y)[i] = acc;
+  }
+}
+
 }  // namespace dev
+
+static int ZBlocks(size_t n) {
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  return blocks < 1 ? 1 : blocks;
+}
+
+hipError_t LaunchZaxpyPairs(const ZPairs& pr, size_t n, const double2* c, const int* stop,
+                            hipStream_t s) {
+  if (pr.count < 1 || pr.count > kBicglMaxEll + 1) return hipErrorInvalidValue;
+  // unused slots repeat slot 0 (never stored to), so every pointer is valid
+  ZPairs q = pr;
+  for (int p = q.count; p < kBicglMaxEll + 1; ++p) {
+    q.y[p] = q.y[0];
+    q.a[p] = q.a[0];
+    q.b[p] = q.b[0];
+    q.sign[p] = 0;
+  }
+  hipLaunchKernelGGL(dev::zaxpy_pairs, dim3(ZBlocks(n)), dim3(256), 0, s, q, n, c, stop);
+  return hipGetLastError();
+}
+
+hipError_t LaunchZcombDev(double* y, const double* a, const ZTerms& t, size_t n, const int* stop,
+                          hipStream_t s) {
+  if (t.count < 1 || t.count > kBicglMaxEll) return hipErrorInvalidValue;
+  // unused slots repeat slot 0 with weight 0, so every pointer is valid
+  ZTerms q = t;
+  for (int k = q.count; k < kBicglMaxEll; ++k) {
+    q.b[k] = q.b[0];
+    q.c[k] = q.c[0];
+    q.sign[k] = 0;
+  }
+  hipLaunchKernelGGL(dev::zcomb_dev, dim3(ZBlocks(n)), dim3(256), 0, s, y, a, q, n, stop);
+  return hipGetLastError();
+}
 
 hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double* partial,
                              int nblocks, BicglScalars* S, int k, int op, int i, int j, double tol,
