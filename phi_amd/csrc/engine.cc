@@ -1147,6 +1147,26 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
                   double sign) -> int {
     return Check(LaunchZaxpyDev(y, a, b, n, c, sign, &S->stop, stream_), "axpy");
   };
+  // y = A x with (rtilde, y) into S->dot[0], then the step op: the matvec
+  // writes the dot product's partials itself when its LDS form fits.
+  const int mb = RhsGeneralDotBlocks(args_);
+  double* pmv = nullptr;
+  if (!rc && mb > 0)
+    rc = Check(hipMalloc((void**)&pmv, (size_t)mb * 2 * sizeof(double)), "alloc matvec partials");
+  auto matvec_dot = [&](const double* xin, double* y, int op) -> int {
+    if (mb > 0) {
+      int e = Check(LaunchRhsGeneralDot(args_, xin, y, 1, 1, rt, nullptr, pmv, stream_),
+                    "rhs_general dot");
+      if (!e)
+        e = Check(LaunchBicglReduceStep(pmv, mb, S, 0, op, 0, 0, tol, max_iter, hist, hist_cap,
+                                        stream_),
+                  "bicgstab(l) step");
+      return e;
+    }
+    int e = RhsDevice(xin, y, 1);
+    if (!e) e = dot_step(rt, y, 0, op, 0, 0);
+    return e;
+  };
   auto pairs = [&](const ZPairs& pr, const double2* c) -> int {
     return Check(LaunchZaxpyPairs(pr, n, c, &S->stop, stream_), "axpy pairs");
   };
@@ -1177,7 +1197,8 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
   auto enqueue_cycle = [&]() -> int {
     int e = kOk;
     for (int j = 0; j < ell && !e; ++j) {
-      if (j > 0) e = dot_step(rt, vr[j], 0, kBlBeta, 0, 0);
+      // (beta: from kBlInit / kBlEnd for j = 0, else from the matvec that
+      // made r_j)
       // u_i = r_i - beta u_i, i <= j
       ZPairs up{};
       for (int i = 0; i <= j; ++i) {
@@ -1188,8 +1209,7 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
       }
       up.count = j + 1;
       if (!e) e = pairs(up, &S->beta);
-      if (!e) e = RhsDevice(vu[j], vu[j + 1], 1);
-      if (!e) e = dot_step(rt, vu[j + 1], 0, kBlAlpha, 0, 0);
+      if (!e) e = matvec_dot(vu[j], vu[j + 1], kBlAlpha);
       // r_i -= alpha u_{i+1}, i <= j, and x += alpha u_0
       ZPairs rp{};
       for (int i = 0; i <= j; ++i) {
@@ -1204,7 +1224,9 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
       rp.sign[j + 1] = 1.;
       rp.count = j + 2;
       if (!e) e = pairs(rp, &S->alpha);
-      if (!e) e = RhsDevice(vr[j], vr[j + 1], 1);
+      // r_{j+1} = A r_j, and the next BiCG step's beta from (rtilde, r_{j+1})
+      if (!e)
+        e = j + 1 < ell ? matvec_dot(vr[j], vr[j + 1], kBlBeta) : RhsDevice(vr[j], vr[j + 1], 1);
     }
     // modified Gram-Schmidt
     for (int j = 1; j <= ell && !e; ++j) {
@@ -1276,6 +1298,7 @@ int Engine::SteadyStateL(int ell, double tol, int max_iter, int* iters, double* 
   const int rc_sync = Synchronize();
   Free(S);
   Free(hist);
+  Free(pmv);
   cleanup();
   if (!rc) rc = rc_sync;
   if (iters) *iters = hs.it;

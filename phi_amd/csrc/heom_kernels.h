@@ -220,6 +220,11 @@ hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double
                               int nblocks, BicglScalars* S, int k, int op, int i, int j,
                               double tol, int max_iter, double* hist, int hist_cap,
                               hipStream_t s);
+// The reduction of nb complex partials (as LaunchRhsGeneralDot's dot 1
+// writes them) into S->dot[k], then the scalar step op.
+hipError_t LaunchBicglReduceStep(const double* partial, int nb, BicglScalars* S, int k, int op,
+                                 int i, int j, double tol, int max_iter, double* hist,
+                                 int hist_cap, hipStream_t s);
 // Two dot products in one pass, conj(a) . b into S->dot[k] and conj(c) . d
 // into S->dot[k + 1] (k = 0..2), then the scalar step op; partial holds 4
 // doubles per block (nblocks blocks).

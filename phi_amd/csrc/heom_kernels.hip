@@ -1180,6 +1180,14 @@ hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double
   return hipGetLastError();
 }
 
+hipError_t LaunchBicglReduceStep(const double* partial, int nb, BicglScalars* S, int k, int op,
+                                int i, int j, double tol, int max_iter, double* hist,
+                                int hist_cap, hipStream_t s) {
+  hipLaunchKernelGGL(dev::bicgl_dot_step, dim3(1), dim3(256), 0, s, partial, nb, S, k, op, i, j,
+                     tol, max_iter, hist, hist_cap);
+  return hipGetLastError();
+}
+
 hipError_t LaunchBicglDot2Step(const double* a, const double* b, const double* c,
                               const double* d, size_t n, double* partial, int nblocks,
                               BicglScalars* S, int k, int op, int i, int j, double tol,
