@@ -1174,6 +1174,7 @@ hipError_t LaunchZcombDev(double* y, const double* a, const ZTerms& t, size_t n,
 hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double* partial,
                              int nblocks, BicglScalars* S, int k, int op, int i, int j, double tol,
                              int max_iter, double* hist, int hist_cap, hipStream_t s) {
+  if (k < 0 || k > 3) return hipErrorInvalidValue;
   hipLaunchKernelGGL(dev::dot_partial, dim3(nblocks), dim3(256), 0, s, u, v, n, 0, partial);
   hipLaunchKernelGGL(dev::bicgl_dot_step, dim3(1), dim3(256), 0, s, partial, nblocks, S, k, op,
                      i, j, tol, max_iter, hist, hist_cap);
@@ -1183,6 +1184,7 @@ hipError_t LaunchBicglDotStep(const double* u, const double* v, size_t n, double
 hipError_t LaunchBicglReduceStep(const double* partial, int nb, BicglScalars* S, int k, int op,
                                 int i, int j, double tol, int max_iter, double* hist,
                                 int hist_cap, hipStream_t s) {
+  if (k < 0 || k > 3) return hipErrorInvalidValue;
   hipLaunchKernelGGL(dev::bicgl_dot_step, dim3(1), dim3(256), 0, s, partial, nb, S, k, op, i, j,
                      tol, max_iter, hist, hist_cap);
   return hipGetLastError();
